@@ -1,0 +1,148 @@
+/*
+ * plegma_amd.h -- C ABI of the MI355X (gfx950) two-point correlator library.
+ *
+ * Plain C: device pointers, sizes and status codes; no HIP or torch types
+ * (streams are passed as void*, a hipStream_t or NULL for the null stream).
+ * Every function returns PLG_OK (0) or a negative status; plg_last_error()
+ * gives the message of the last failure on the calling thread.
+ *
+ * Each entry point names the PLEGMA interface it replaces (paths relative to
+ * the PLEGMA source tree).  Single GPU, no MPI: the reference's spatial
+ * all-reduce (lib/kernels/PLEGMA_mesons.cuh:161-164) is the identity here.
+ */
+#ifndef PLEGMA_AMD_H
+#define PLEGMA_AMD_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum plg_status {
+  PLG_OK = 0,
+  PLG_ERR_INVALID = -1,     /* bad argument (reference: PLEGMA_error)          */
+  PLG_ERR_NOT_INIT = -2,    /* plg_init not called                             */
+  PLG_ERR_HIP = -3,         /* HIP runtime error                               */
+  PLG_ERR_ALLOC = -4,       /* device allocation failed                        */
+  PLG_ERR_SITE_SIZE = -5,   /* site size mismatch (PLEGMA_mesons.cuh:140-141)  */
+  PLG_ERR_UNSUPPORTED = -6, /* combination not provided by this library        */
+  PLG_ERR_RANGE = -7        /* propagator view does not cover a needed slice   */
+};
+
+/* ---- enums ------------------------------------------------------------- */
+enum plg_precision { PLG_F64 = 0, PLG_F32 = 1 };
+/* arithmetic path: FP64 throughout, or FP32 contraction + split-bf16 MFMA
+ * projection with FP32 accumulation and FP64 cross-block / time-slice sums */
+enum plg_mode { PLG_MODE_F64 = 0, PLG_MODE_MIXED = 1 };
+/* bf16 split products in the mixed projection (ignored in PLG_MODE_F64) */
+enum plg_split { PLG_SPLIT_BF16X3 = 3, PLG_SPLIT_BF16X6 = 6 };
+/* include/global/PLEGMA_enums.h:7 */
+enum plg_space { PLG_POSITION_SPACE = 0, PLG_MOMENTUM_SPACE = 1 };
+/* channel sets (correlator kinds) */
+enum plg_kind {
+  PLG_MESONS = 0,      /* contractMesons: 10 ch, datasets {S1, S2}            */
+  PLG_MESONS16 = 1,    /* same formula over the 16 GAMMAS, datasets {S1, S2}  */
+  PLG_MESONS_NEW = 2,  /* contractMesonsNew: 10 ch, S1 x conj(S2), 1 dataset  */
+  PLG_MESONS_ALL = 3,  /* contractMesonsAll: 34 ch, 1 dataset                 */
+  PLG_BARYONS = 4,     /* contractBaryons: groups by mask, 16 ch, 2 datasets  */
+  PLG_MESON_MESON = 5, /* four-propagator meson-meson, 20 ch, 1 dataset       */
+  PLG_TWOPT = 6        /* one propagator: group 0 = 16 mesons, then baryon
+                          groups by mask, 16 ch each, 1 dataset               */
+};
+/* baryon group bits in the reference's group order (PLEGMA_Correlator.cu:123-128) */
+#define PLG_G_NUCL (1 << 0)
+#define PLG_G_DELTAPP_11 (1 << 7)
+#define PLG_G_DELTAPP_22 (1 << 8)
+#define PLG_G_DELTAPP_33 (1 << 9)
+#define PLG_G_DELTAPP_ALL (PLG_G_DELTAPP_11 | PLG_G_DELTAPP_22 | PLG_G_DELTAPP_33)
+
+/* Propagator view (input of the path).  Complex interleaved SoA:
+ *   element (component i, time t, site x) at ((i*comp_stride) + plane*V3 + x)
+ *   complex units, plane = (t - t_first) mod T, plane < nt,
+ *   i = ((mu*4 + nu)*3 + a)*3 + b.
+ * PLEGMA's layout (lib/PLEGMA_Propagator.cu:39-44, lib/PLEGMA_Field.cu:225-264)
+ * is comp_stride = V, t_first = 0, nt = T. */
+typedef struct {
+  const void *data;    /* device pointer */
+  int precision;       /* plg_precision */
+  int t_first;
+  int nt;
+  int pad_;
+  int64_t comp_stride; /* complex elements */
+} plg_prop;
+
+/* ---- runtime ------------------------------------------------------------ */
+/* replaces PLEGMA_init (lib/PLEGMA.cu:19-212) for one process on one GPU:
+ * L = {Lx, Ly, Lz, Lt}; selects `device`, builds the phase tables. */
+int plg_init(const int L[4], int device);
+int plg_finalize(void);
+const char *plg_last_error(void);
+int plg_version(void);
+
+/* ---- conventions -------------------------------------------------------- */
+/* momentum list of PLEGMA_FT::createMom (lib/PLEGMA_FT.cu:50-72), 3D:
+ * writes min(n, cap) triples (nx,ny,nz) to xyz (may be NULL), returns n in *nmom */
+int plg_momlist(int q2max, int *nmom, int *xyz, int cap);
+/* PLEGMA_Correlator::startT/endT/localT (include/PLEGMA_Correlator.h:53-79) */
+int plg_window(const int source[4], int totalT, int *startT, int *endT, int *localT);
+/* lattice time of buffer row j (lib/kernels/PLEGMA_mesons.cuh:22) */
+int plg_row_time(const int source[4], int totalT, int j, int *t);
+/* shape of a kind: groups, datasets, channels per group (getSiteSize =
+ * ng*nd*nch, include/PLEGMA_Correlator.h:113-118) */
+int plg_kind_shape(int kind, int group_mask, int *ngroups, int *ndatasets, int *nch);
+
+/* ---- the hot path ------------------------------------------------------- */
+/* One correlator, replacing contract_mesons / contract_mesons_new /
+ * contract_mesons_all / contract_baryons (lib/kernels/PLEGMA_mesons.cuh:134-165,
+ * PLEGMA_mesonsNew.cuh:65-97, PLEGMA_mesonsAll.cuh:61-92,
+ * PLEGMA_baryons.cuh:169-206) with the fused DFT of
+ * PLEGMA_kernel_utils.cuh:820-860 and the host block sums.
+ *   moms: host array nmom*3 ints (a createMom list or any integer list)
+ *   out : device buffer, H_elem layout of the reference
+ *         momentum space: [g][d][j][m][c] complex, (PLEGMA_mesons.cuh:117-124,
+ *                         PLEGMA_baryons.cuh:151-160, 196-200)
+ *         position space: [g][d][j][x][c] complex
+ *         of out_precision (PLG_F64 / PLG_F32).  Overwritten, not accumulated.
+ * Synchronous with respect to `stream` ordering only. */
+int plg_correlator(int kind, int group_mask, const plg_prop *p1, const plg_prop *p2,
+                   const int source[4], int totalT, const int *moms, int nmom, int mode,
+                   int space, int out_precision, void *out, void *stream);
+/* number of complex elements plg_correlator writes (getTotalSize) */
+int plg_correlator_size(int kind, int group_mask, const int source[4], int totalT, int nmom,
+                        int space, int64_t *elems);
+
+/* ---- batched path (no single reference counterpart: the drivers loop over
+ * sources calling contractMesons/contractBaryons, e.g. plegma/mesons_2pt.cpp) */
+typedef struct {
+  int source[4];
+  int totalT;
+  int prop;            /* index into the props array */
+  int64_t out_offset;  /* complex elements into the output buffer */
+} plg_job;
+typedef struct plg_batch_s *plg_batch;
+#define PLG_BATCH_GRAPH 1   /* capture the launch sequence in a hipGraph, replay on run */
+#define PLG_BATCH_TUNE 2    /* pick tiles with the autotuner (cached) */
+int plg_batch_create(int kind, int group_mask, const plg_prop *props, int nprops,
+                     const plg_job *jobs, int njobs, const int *moms, int nmom, int mode,
+                     int split, int flags, plg_batch *out);
+/* writes each job's H_elem block (layout of plg_correlator, momentum space,
+ * out_precision F64) at its out_offset */
+int plg_batch_run(plg_batch b, void *out, void *stream);
+int plg_batch_info(plg_batch b, int64_t *out_elems, int64_t *nsites, int *nrows);
+int plg_batch_destroy(plg_batch b);
+
+/* ---- inputs ------------------------------------------------------------- */
+/* seeded synthetic propagator: Philox4x32-10 + Box-Muller N(0,1) entries,
+ * scaled by exp(-lambda*|t - t0|); counter (site + V3*t, component, stream) */
+int plg_synth_prop(const plg_prop *dst, uint64_t seed, uint32_t stream_id, int t0, double lambda,
+                   void *stream);
+/* base + eps * perturbation, both seeded, then the same scaling */
+int plg_synth_prop_perturbed(const plg_prop *dst, uint64_t seed, uint32_t base_stream,
+                             uint32_t pert_stream, double eps, int t0, double lambda, void *stream);
+/* round-to-nearest FP64 -> FP32 copy over dst's planes */
+int plg_cast_prop(const plg_prop *dst, const plg_prop *src, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,209 @@
+"""ctypes binding of the C ABI in include/plegma_amd.h (libplegma_amd.so).
+
+The library is the product: this module loads it or raises.  There is no CPU
+fallback.  torch supplies device memory and streams only.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libplegma_amd.so")
+
+# enums of include/plegma_amd.h
+F64, F32 = 0, 1
+MODE_F64, MODE_MIXED = 0, 1
+SPLIT_BF16X3, SPLIT_BF16X6 = 3, 6
+POSITION_SPACE, MOMENTUM_SPACE = 0, 1
+MESONS, MESONS16, MESONS_NEW, MESONS_ALL, BARYONS, MESON_MESON, TWOPT = range(7)
+G_NUCL = 1 << 0
+G_DELTAPP_11, G_DELTAPP_22, G_DELTAPP_33 = 1 << 7, 1 << 8, 1 << 9
+G_DELTAPP_ALL = G_DELTAPP_11 | G_DELTAPP_22 | G_DELTAPP_33
+BATCH_GRAPH, BATCH_TUNE = 1, 2
+
+STATUS = {0: "PLG_OK", -1: "PLG_ERR_INVALID", -2: "PLG_ERR_NOT_INIT", -3: "PLG_ERR_HIP",
+          -4: "PLG_ERR_ALLOC", -5: "PLG_ERR_SITE_SIZE", -6: "PLG_ERR_UNSUPPORTED",
+          -7: "PLG_ERR_RANGE"}
+
+EXPORTS = ["plg_init", "plg_finalize", "plg_last_error", "plg_version", "plg_momlist",
+           "plg_window", "plg_row_time", "plg_kind_shape", "plg_correlator",
+           "plg_correlator_size", "plg_batch_create", "plg_batch_run", "plg_batch_info",
+           "plg_batch_destroy", "plg_synth_prop", "plg_synth_prop_perturbed", "plg_cast_prop"]
+
+
+class PlgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class plg_prop(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("precision", C.c_int), ("t_first", C.c_int),
+                ("nt", C.c_int), ("pad_", C.c_int), ("comp_stride", C.c_int64)]
+
+
+class plg_job(C.Structure):
+    _fields_ = [("source", C.c_int * 4), ("totalT", C.c_int), ("prop", C.c_int),
+                ("out_offset", C.c_int64)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libplegma_amd.so; raise if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        ip = C.POINTER(C.c_int)
+        L.plg_last_error.restype = C.c_char_p
+        L.plg_init.argtypes = [ip, C.c_int]
+        L.plg_momlist.argtypes = [C.c_int, ip, ip, C.c_int]
+        L.plg_window.argtypes = [ip, C.c_int, ip, ip, ip]
+        L.plg_row_time.argtypes = [ip, C.c_int, C.c_int, ip]
+        L.plg_kind_shape.argtypes = [C.c_int, C.c_int, ip, ip, ip]
+        L.plg_correlator.argtypes = [C.c_int, C.c_int, C.POINTER(plg_prop), C.POINTER(plg_prop), ip,
+                                     C.c_int, ip, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                     C.c_void_p]
+        L.plg_correlator_size.argtypes = [C.c_int, C.c_int, ip, C.c_int, C.c_int, C.c_int,
+                                          C.POINTER(C.c_int64)]
+        L.plg_batch_create.argtypes = [C.c_int, C.c_int, C.POINTER(plg_prop), C.c_int,
+                                       C.POINTER(plg_job), C.c_int, ip, C.c_int, C.c_int, C.c_int,
+                                       C.c_int, C.POINTER(C.c_void_p)]
+        L.plg_batch_run.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.plg_batch_info.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), ip]
+        L.plg_batch_destroy.argtypes = [C.c_void_p]
+        L.plg_synth_prop.argtypes = [C.POINTER(plg_prop), C.c_uint64, C.c_uint32, C.c_int,
+                                     C.c_double, C.c_void_p]
+        L.plg_synth_prop_perturbed.argtypes = [C.POINTER(plg_prop), C.c_uint64, C.c_uint32,
+                                               C.c_uint32, C.c_double, C.c_int, C.c_double,
+                                               C.c_void_p]
+        L.plg_cast_prop.argtypes = [C.POINTER(plg_prop), C.POINTER(plg_prop), C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise PlgError(rc, lib().plg_last_error().decode())
+    return rc
+
+
+def ivec(v):
+    return (C.c_int * len(v))(*[int(x) for x in v])
+
+
+def _stream(s):
+    if s is None:
+        return None
+    return C.c_void_p(s.cuda_stream if hasattr(s, "cuda_stream") else int(s))
+
+
+# ---------------------------------------------------------------- wrappers
+def init(L, device=0):
+    check(lib().plg_init(ivec(L), device))
+
+
+def finalize():
+    check(lib().plg_finalize())
+
+
+def momlist(q2max):
+    n = C.c_int()
+    check(lib().plg_momlist(q2max, C.byref(n), None, 0))
+    buf = (C.c_int * (3 * n.value))()
+    check(lib().plg_momlist(q2max, C.byref(n), buf, n.value))
+    return [tuple(buf[3 * i:3 * i + 3]) for i in range(n.value)]
+
+
+def window(source, totalT):
+    s, e, lt = C.c_int(), C.c_int(), C.c_int()
+    check(lib().plg_window(ivec(source), totalT, C.byref(s), C.byref(e), C.byref(lt)))
+    return s.value, e.value, lt.value
+
+
+def row_time(source, totalT, j):
+    t = C.c_int()
+    check(lib().plg_row_time(ivec(source), totalT, j, C.byref(t)))
+    return t.value
+
+
+def kind_shape(kind, mask=G_NUCL):
+    a, b, c = C.c_int(), C.c_int(), C.c_int()
+    check(lib().plg_kind_shape(kind, mask, C.byref(a), C.byref(b), C.byref(c)))
+    return a.value, b.value, c.value
+
+
+def prop_view(tensor, precision, t_first=0, nt=None, V3=None, T=None):
+    """plg_prop over a torch tensor holding 144 x (nt*V3) complex entries."""
+    nt = T if nt is None else nt
+    cs = tensor.numel() // (2 * 144)
+    return plg_prop(tensor.data_ptr(), precision, t_first, nt, 0, cs)
+
+
+def correlator(kind, mask, p1, p2, source, totalT, moms, mode, space, out_prec, out,
+               stream=None):
+    flat = [int(v) for m in moms for v in m]
+    check(lib().plg_correlator(kind, mask, C.byref(p1), C.byref(p2) if p2 is not None else None,
+                               ivec(source), totalT, ivec(flat) if flat else None, len(moms), mode,
+                               space, out_prec, C.c_void_p(out.data_ptr()), _stream(stream)))
+
+
+def correlator_size(kind, mask, source, totalT, nmom, space=MOMENTUM_SPACE):
+    n = C.c_int64()
+    check(lib().plg_correlator_size(kind, mask, ivec(source), totalT, nmom, space, C.byref(n)))
+    return n.value
+
+
+class Batch:
+    """plg_batch_* wrapper: many (source, window, propagator) jobs in one plan."""
+
+    def __init__(self, kind, mask, props, jobs, moms, mode, split=SPLIT_BF16X6, flags=0):
+        P = (plg_prop * len(props))(*props)
+        J = (plg_job * len(jobs))()
+        for i, (src, totalT, prop, off) in enumerate(jobs):
+            J[i].source = (C.c_int * 4)(*src)
+            J[i].totalT = totalT
+            J[i].prop = prop
+            J[i].out_offset = off
+        flat = [int(v) for m in moms for v in m]
+        h = C.c_void_p()
+        check(lib().plg_batch_create(kind, mask, P, len(props), J, len(jobs), ivec(flat),
+                                     len(moms), mode, split, flags, C.byref(h)))
+        self.h = h
+        self._keep = (P, J)
+
+    def info(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int()
+        check(lib().plg_batch_info(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def run(self, out, stream=None):
+        check(lib().plg_batch_run(self.h, C.c_void_p(out.data_ptr()), _stream(stream)))
+
+    def close(self):
+        if self.h:
+            lib().plg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def synth_prop(view, seed, stream_id, t0, lam, stream=None):
+    check(lib().plg_synth_prop(C.byref(view), seed, stream_id, t0, lam, _stream(stream)))
+
+
+def synth_prop_perturbed(view, seed, base_stream, pert_stream, eps, t0, lam, stream=None):
+    check(lib().plg_synth_prop_perturbed(C.byref(view), seed, base_stream, pert_stream, eps, t0,
+                                         lam, _stream(stream)))
+
+
+def cast_prop(dst, src, stream=None):
+    check(lib().plg_cast_prop(C.byref(dst), C.byref(src), _stream(stream)))

@@ -1,0 +1,773 @@
+// plg_api.hip -- the C ABI of libplegma_amd.so (declared in include/plegma_amd.h).
+//
+// A plan maps a correlator kind onto
+//   contraction launches -> site correlator c[row][ch][x]      (plg_contract.hip)
+//   projection GEMM      -> per-split partials part[s][n][m]   (plg_project.hip)
+//   finalize             -> FP64 split sums, source phase, H_elem layout
+// and is built either per call (plg_correlator, the reference's per-source
+// contract* calls) or once for many sources (plg_batch_*, replayed from a
+// captured hipGraph).
+#include "plg_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <sstream>
+#include <vector>
+
+namespace plg {
+
+static thread_local std::string g_err;
+void set_error(const std::string &m) { g_err = m; }
+int fail(int code, const std::string &m) {
+  g_err = m;
+  return code;
+}
+#define HIPCHK(x)                                                                      \
+  do {                                                                                 \
+    hipError_t e_ = (x);                                                               \
+    if (e_ != hipSuccess) return fail(PLG_ERR_HIP, std::string(#x ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// ------------------------------------------------------------------ context
+struct Context {
+  bool init = false;
+  int L[4] = {0, 0, 0, 0};
+  int64_t V3 = 0;
+  int device = 0;
+  int M = 1;                     // lcm(Lx,Ly,Lz)
+  double *d_cos = nullptr, *d_sin = nullptr;
+  uint16_t *d_tab3 = nullptr;    // [M][6] bf16 pieces of cos, sin
+  // grow-only workspace
+  void *ws_c = nullptr;
+  size_t ws_c_bytes = 0;
+  void *ws_part = nullptr;
+  size_t ws_part_bytes = 0;
+};
+static Context g_ctx;
+static std::mutex g_mu;
+
+static uint16_t host_bf16_rn(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+static float host_bf16_f(uint16_t h) {
+  uint32_t u = (uint32_t)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+// three bf16 pieces of a double: p0 + p1 + p2 ~ v to about 2^-24 relative
+static void split3_host(double v, uint16_t *p) {
+  double r = v;
+  for (int i = 0; i < 3; i++) {
+    p[i] = host_bf16_rn((float)r);
+    r -= (double)host_bf16_f(p[i]);
+  }
+}
+
+static int ensure(void **buf, size_t *have, size_t need) {
+  if (need <= *have) return PLG_OK;
+  if (*buf) hipFree(*buf);
+  *buf = nullptr;
+  *have = 0;
+  hipError_t e = hipMalloc(buf, need);
+  if (e != hipSuccess) {
+    *buf = nullptr;
+    return fail(PLG_ERR_ALLOC, "device allocation of " + std::to_string(need) + " bytes failed");
+  }
+  *have = need;
+  return PLG_OK;
+}
+
+// ------------------------------------------------------------------ conventions
+static void momlist_host(int q2max, std::vector<int> &out) {
+  out.clear();
+  for (int iQ = 0; iQ <= q2max; iQ++)
+    for (int nx = iQ; nx >= -iQ; nx--)
+      for (int ny = iQ; ny >= -iQ; ny--)
+        for (int nz = iQ; nz >= -iQ; nz--)
+          if (nx * nx + ny * ny + nz * nz == iQ) {
+            out.push_back(nx);
+            out.push_back(ny);
+            out.push_back(nz);
+          }
+}
+
+struct Window {
+  int startT, endT, localT, maxT;
+};
+// include/PLEGMA_Correlator.h:53-79 with procPosition = 0, localL = totalL
+static Window window_of(const int L[4], const int src[4], int totalT) {
+  const int T = L[3];
+  Window w;
+  int start = (T - src[3]) % T;
+  if (start >= totalT) start = 0;
+  w.startT = start;
+  w.endT = start > 0 ? std::min(totalT, T + start) : 0;
+  const int ts = src[3] % T;
+  w.localT = w.endT - w.startT + std::min(totalT, T - ts);
+  w.maxT = w.endT - w.startT;
+  return w;
+}
+static int row_time(const int L[4], const int src[4], const Window &w, int j) {
+  int t = j;
+  if (t >= w.maxT) t = (src[3] % L[3]) + t - w.maxT;
+  return t;
+}
+
+static int popcount_groups(int mask) {
+  int n = 0;
+  for (int b = 0; b < 10; b++) n += (mask >> b) & 1;
+  return n;
+}
+static bool baryon_mask_ok(int mask) {
+  const int allowed = PLG_G_NUCL | PLG_G_DELTAPP_ALL;
+  return mask != 0 && (mask & ~allowed) == 0;
+}
+
+static int kind_shape(int kind, int mask, int &ng, int &nd, int &nch) {
+  switch (kind) {
+  case PLG_MESONS: ng = 1; nd = 2; nch = 10; return PLG_OK;
+  case PLG_MESONS16: ng = 1; nd = 2; nch = 16; return PLG_OK;
+  case PLG_MESONS_NEW: ng = 1; nd = 1; nch = 10; return PLG_OK;
+  case PLG_MESONS_ALL: ng = 1; nd = 1; nch = 34; return PLG_OK;
+  case PLG_BARYONS:
+    if (!baryon_mask_ok(mask)) return fail(PLG_ERR_UNSUPPORTED, "baryon groups: nucl_nucl and deltapp_deltamm_kk only");
+    ng = popcount_groups(mask); nd = 2; nch = 16; return PLG_OK;
+  case PLG_TWOPT:
+    if (mask != 0 && !baryon_mask_ok(mask)) return fail(PLG_ERR_UNSUPPORTED, "twopt groups: nucl_nucl and deltapp_deltamm_kk only");
+    ng = 1 + popcount_groups(mask); nd = 1; nch = 16; return PLG_OK;
+  case PLG_MESON_MESON:
+    return fail(PLG_ERR_UNSUPPORTED, "meson-meson kind not built into this library version");
+  }
+  return fail(PLG_ERR_INVALID, "unknown kind " + std::to_string(kind));
+}
+
+// ------------------------------------------------------------------ plans
+struct Launch {
+  enum { MESON, BARYON } what;
+  int set;          // meson set
+  bool two, mes;    // baryon
+  int gmask;
+  int ch0;
+};
+
+struct JobIn {
+  int src[4];
+  int totalT;
+  int pa, pb;       // prop indices
+  int64_t out_off;
+};
+
+struct Plan {
+  int kind = 0, mask = 0, mode = PLG_MODE_F64, terms = 6, space = PLG_MOMENTUM_SPACE;
+  int T = 1;        // compute precision of the contraction (PLG_F64 / PLG_F32)
+  int nch_c = 0;    // computed channels per row
+  int nmom = 0;
+  std::vector<int> moms;
+  std::vector<RowDev> rows;
+  std::vector<OutRowDev> orows;
+  std::vector<int> csel;
+  std::vector<int64_t> ch_off;
+  int64_t m_stride = 0;
+  std::vector<Launch> launches;
+  std::vector<plg_prop> props;
+  int64_t out_elems = 0;
+  int64_t nsites = 0;
+  // tuning
+  int mtile = 64, nsplit = 1, tpb = 1;
+  // device tables (owned by the plan)
+  void *d_props = nullptr;
+  RowDev *d_rows = nullptr;
+  OutRowDev *d_orows = nullptr;
+  int *d_csel = nullptr;
+  int64_t *d_choff = nullptr;
+  int *d_ms = nullptr;
+  // batch-owned workspace (single calls use the context workspace)
+  void *c = nullptr, *part = nullptr;
+  size_t c_bytes = 0, part_bytes = 0;
+  bool own_ws = false;
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void *graph_out = nullptr;
+  hipStream_t cap_stream = nullptr;
+
+  ~Plan() {
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    if (cap_stream) hipStreamDestroy(cap_stream);
+    for (void *p : {(void *)d_props, (void *)d_rows, (void *)d_orows, (void *)d_csel, (void *)d_choff, (void *)d_ms})
+      if (p) hipFree(p);
+    if (own_ws) {
+      if (c) hipFree(c);
+      if (part) hipFree(part);
+    }
+  }
+};
+
+static bool same_view(const plg_prop &a, const plg_prop &b) {
+  return a.data == b.data && a.precision == b.precision && a.t_first == b.t_first && a.nt == b.nt &&
+         a.comp_stride == b.comp_stride;
+}
+
+static int check_prop(const plg_prop &p, int want_prec, int t) {
+  if (!p.data) return fail(PLG_ERR_INVALID, "propagator data is NULL");
+  if (p.precision != want_prec) return fail(PLG_ERR_INVALID, "propagator precision does not match the mode");
+  const int T = g_ctx.L[3];
+  if (p.nt <= 0 || p.nt > T || p.t_first < 0 || p.t_first >= T)
+    return fail(PLG_ERR_INVALID, "propagator view: bad t_first/nt");
+  if (p.comp_stride < (int64_t)p.nt * g_ctx.V3) return fail(PLG_ERR_INVALID, "propagator view: comp_stride < nt*V3");
+  int plane = ((t - p.t_first) % T + T) % T;
+  if (plane >= p.nt)
+    return fail(PLG_ERR_RANGE, "propagator view does not hold time slice " + std::to_string(t));
+  return PLG_OK;
+}
+
+// build the host side of a plan for a list of jobs
+static int build_plan(Plan &P, int kind, int mask, const std::vector<plg_prop> &props,
+                      const std::vector<JobIn> &jobs, const int *moms, int nmom, int mode, int terms,
+                      int space) {
+  if (!g_ctx.init) return fail(PLG_ERR_NOT_INIT, "plg_init not called");
+  if (mode != PLG_MODE_F64 && mode != PLG_MODE_MIXED) return fail(PLG_ERR_INVALID, "bad mode");
+  if (mode == PLG_MODE_MIXED && terms != 3 && terms != 6) return fail(PLG_ERR_INVALID, "split must be 3 or 6");
+  if (space != PLG_MOMENTUM_SPACE && space != PLG_POSITION_SPACE) return fail(PLG_ERR_INVALID, "bad space");
+  if (space == PLG_MOMENTUM_SPACE && (nmom <= 0 || !moms)) return fail(PLG_ERR_INVALID, "empty momentum list");
+  int ng, nd, nch;
+  int rc = kind_shape(kind, mask, ng, nd, nch);
+  if (rc) return rc;
+  P.kind = kind; P.mask = mask; P.mode = mode; P.terms = terms; P.space = space;
+  P.T = mode == PLG_MODE_F64 ? PLG_F64 : PLG_F32;
+  P.props = props;
+  P.nmom = space == PLG_MOMENTUM_SPACE ? nmom : 0;
+  P.moms.assign(moms, moms + 3 * P.nmom);
+  const int64_t V3 = g_ctx.V3;
+  const int64_t inner = space == PLG_MOMENTUM_SPACE ? (int64_t)nmom : V3;  // per-row extent
+
+  // channels computed per row, launches, output-channel maps
+  bool pair_same = false;
+  const int DEL = PLG_G_DELTAPP_ALL;
+  int gcomp = 0;  // baryon groups computed
+  std::vector<int> gsel;  // computed group index of each output baryon group
+  auto baryon_groups = [&](int m, int &gm) {
+    gm = (m == PLG_G_NUCL) ? PLG_G_NUCL : ((m & ~DEL) == 0 ? DEL : (PLG_G_NUCL | DEL));
+    std::vector<int> bits;
+    for (int b : {0, 7, 8, 9})
+      if (gm & (1 << b)) bits.push_back(b);
+    gcomp = (int)bits.size();
+    gsel.clear();
+    for (int b : {0, 7, 8, 9})
+      if (m & (1 << b)) gsel.push_back((int)(std::find(bits.begin(), bits.end(), b) - bits.begin()));
+  };
+  std::vector<std::pair<int, int>> rowspec;  // per job: rows added
+  P.rows.clear();
+  P.orows.clear();
+  P.launches.clear();
+  P.nsites = 0;
+  int64_t out_total = 0;
+
+  // decide per-kind structure (same for every job)
+  switch (kind) {
+  case PLG_MESONS:
+  case PLG_MESONS16:
+    P.nch_c = nch;
+    P.launches.push_back({Launch::MESON, kind == PLG_MESONS ? SET_M10 : SET_M16, false, false, 0, 0});
+    break;
+  case PLG_MESONS_NEW:
+    P.nch_c = 10;
+    P.launches.push_back({Launch::MESON, SET_M10_PAIR, false, false, 0, 0});
+    break;
+  case PLG_MESONS_ALL:
+    P.nch_c = 34;
+    P.launches.push_back({Launch::MESON, SET_MALL, false, false, 0, 0});
+    break;
+  case PLG_BARYONS: {
+    int gm;
+    baryon_groups(mask, gm);
+    P.launches.push_back({Launch::BARYON, 0, true, false, gm, 0});  // 'two' fixed per job below
+    P.nch_c = 2 * gcomp * 16;
+    break;
+  }
+  case PLG_TWOPT: {
+    int gm = 0;
+    if (mask) baryon_groups(mask, gm);
+    if (mode == PLG_MODE_MIXED) {
+      P.launches.push_back({Launch::BARYON, 0, false, true, gm, 0});
+    } else {
+      P.launches.push_back({Launch::MESON, SET_M16, false, false, 0, 0});
+      if (gm) P.launches.push_back({Launch::BARYON, 0, false, false, gm, 16});
+    }
+    P.nch_c = 16 + gcomp * 16;
+    break;
+  }
+  }
+
+  // output channel maps (identical for every out row)
+  P.csel.clear();
+  P.ch_off.clear();
+  // per-job H sizes depend on localT; ch_off uses group stride, which depends
+  // on localT too -> encode group stride into base per (g) by making ch_off
+  // relative and adding per-row group offsets is not possible with one map,
+  // so out rows are emitted per (g, d, j) when ng > 1.
+  bool two_props_baryons = true;
+  for (size_t jb = 0; jb < jobs.size(); jb++) {
+    const JobIn &J = jobs[jb];
+    const Window w = window_of(g_ctx.L, J.src, J.totalT);
+    const int lt = w.localT;
+    const plg_prop &pa = props[J.pa];
+    const plg_prop &pb = props[J.pb];
+    pair_same = same_view(pa, pb);
+    const int crow0 = (int)P.rows.size();
+    const int64_t gstride = (int64_t)nd * lt * inner * nch;  // H group stride
+    for (int j = 0; j < lt; j++) {
+      const int t = row_time(g_ctx.L, J.src, w, j);
+      if ((rc = check_prop(pa, P.T, t))) return rc;
+      if ((rc = check_prop(pb, P.T, t))) return rc;
+    }
+    switch (kind) {
+    case PLG_MESONS:
+    case PLG_MESONS16: {
+      const int nd_c = pair_same ? 1 : 2;
+      for (int d = 0; d < nd_c; d++)
+        for (int j = 0; j < lt; j++)
+          P.rows.push_back({row_time(g_ctx.L, J.src, w, j), d == 0 ? J.pa : J.pb, d == 0 ? J.pa : J.pb, (int)jb});
+      for (int d = 0; d < 2; d++)
+        for (int j = 0; j < lt; j++) {
+          OutRowDev o{};
+          o.crow = crow0 + (pair_same ? j : d * lt + j);
+          o.ch_base = 0;
+          o.sx = J.src[0]; o.sy = J.src[1]; o.sz = J.src[2];
+          o.base = J.out_off + ((int64_t)d * lt + j) * inner * nch;
+          P.orows.push_back(o);
+        }
+      break;
+    }
+    case PLG_MESONS_NEW:
+    case PLG_MESONS_ALL:
+      for (int j = 0; j < lt; j++) {
+        P.rows.push_back({row_time(g_ctx.L, J.src, w, j), J.pa, J.pb, (int)jb});
+        OutRowDev o{};
+        o.crow = crow0 + j;
+        o.sx = J.src[0]; o.sy = J.src[1]; o.sz = J.src[2];
+        o.base = J.out_off + (int64_t)j * inner * nch;
+        P.orows.push_back(o);
+      }
+      break;
+    case PLG_BARYONS:
+      if (jb == 0) two_props_baryons = !pair_same;
+      else if (two_props_baryons != !pair_same)
+        return fail(PLG_ERR_UNSUPPORTED, "batch mixes jobs with one and two distinct propagators");
+      for (int j = 0; j < lt; j++) {
+        P.rows.push_back({row_time(g_ctx.L, J.src, w, j), J.pa, J.pb, (int)jb});
+        for (size_t g = 0; g < gsel.size(); g++)
+          for (int d = 0; d < 2; d++) {
+            OutRowDev o{};
+            o.crow = crow0 + j;
+            const int dc = pair_same ? 0 : d;  // computed dataset
+            o.ch_base = (dc * gcomp + gsel[g]) * 16;
+            o.sx = J.src[0]; o.sy = J.src[1]; o.sz = J.src[2];
+            o.base = J.out_off + (int64_t)g * gstride + ((int64_t)d * lt + j) * inner * nch;
+            P.orows.push_back(o);
+          }
+      }
+      break;
+    case PLG_TWOPT:
+      for (int j = 0; j < lt; j++) {
+        P.rows.push_back({row_time(g_ctx.L, J.src, w, j), J.pa, J.pa, (int)jb});
+        for (int g = 0; g < ng; g++) {
+          OutRowDev o{};
+          o.crow = crow0 + j;
+          o.ch_base = g == 0 ? 0 : 16 + gsel[g - 1] * 16;
+          o.sx = J.src[0]; o.sy = J.src[1]; o.sz = J.src[2];
+          o.base = J.out_off + (int64_t)g * gstride + (int64_t)j * inner * nch;
+          P.orows.push_back(o);
+        }
+      }
+      break;
+    }
+    P.nsites += (int64_t)lt * V3;
+    out_total = std::max(out_total, J.out_off + (int64_t)ng * nd * lt * inner * nch);
+  }
+  if (kind == PLG_BARYONS) {
+    P.launches[0].two = two_props_baryons;
+    if (!two_props_baryons) P.nch_c = gcomp * 16;
+  }
+  // every out row reads nout channels contiguous from ch_base
+  const int nout = (kind == PLG_BARYONS || kind == PLG_TWOPT) ? 16 : nch;
+  for (int k = 0; k < nout; k++) {
+    P.csel.push_back(k);
+    P.ch_off.push_back(space == PLG_MOMENTUM_SPACE ? k : k);
+  }
+  P.m_stride = nch;  // momentum (or site) stride in H
+  P.out_elems = out_total;
+  return PLG_OK;
+}
+
+template <typename T> static void fill_propdev(const std::vector<plg_prop> &in, std::vector<PropDev<T>> &out) {
+  out.clear();
+  for (auto &p : in) out.push_back(PropDev<T>{(const cx<T> *)p.data, p.comp_stride, p.t_first, p.nt});
+}
+
+// plan tables are small; copied synchronously at plan build time
+template <typename A> static int upload(A **dst, const std::vector<A> &v, hipStream_t) {
+  if (*dst) hipFree(*dst);
+  *dst = nullptr;
+  size_t n = std::max<size_t>(v.size(), 1) * sizeof(A);
+  HIPCHK(hipMalloc((void **)dst, n));
+  if (!v.empty()) HIPCHK(hipMemcpy(*dst, v.data(), v.size() * sizeof(A), hipMemcpyHostToDevice));
+  return PLG_OK;
+}
+
+static void choose_tiles(Plan &P) {
+  const int64_t ncols = (int64_t)P.rows.size() * P.nch_c;
+  const int Mreal = 2 * std::max(P.nmom, 1);
+  P.mtile = Mreal <= 96 ? 32 : 64;
+  const int64_t tiles = ((ncols + 127) / 128) * ((Mreal + P.mtile - 1) / P.mtile);
+  int64_t ns = (2048 + tiles - 1) / tiles;
+  ns = std::max<int64_t>(1, std::min<int64_t>(ns, g_ctx.V3 / 512));
+  P.nsplit = (int)std::max<int64_t>(1, ns);
+  const int64_t btiles = (g_ctx.V3 + 15) / 16;
+  int64_t tpb = (btiles * (int64_t)P.rows.size()) / 8192;
+  P.tpb = (int)std::max<int64_t>(1, std::min<int64_t>(tpb, 64));
+}
+
+static int upload_plan(Plan &P, hipStream_t st) {
+  int rc;
+  if (P.T == PLG_F64) {
+    std::vector<PropDev<double>> pd;
+    fill_propdev(P.props, pd);
+    if ((rc = upload((PropDev<double> **)&P.d_props, pd, st))) return rc;
+  } else {
+    std::vector<PropDev<float>> pd;
+    fill_propdev(P.props, pd);
+    if ((rc = upload((PropDev<float> **)&P.d_props, pd, st))) return rc;
+  }
+  std::vector<int> ms;
+  for (int m = 0; m < P.nmom; m++)
+    for (int i = 0; i < 3; i++) {
+      long v = (long)P.moms[3 * m + i] * (g_ctx.M / g_ctx.L[i]);
+      v %= g_ctx.M;
+      if (v < 0) v += g_ctx.M;
+      ms.push_back((int)v);
+    }
+  if ((rc = upload(&P.d_rows, P.rows, st))) return rc;
+  if ((rc = upload(&P.d_orows, P.orows, st))) return rc;
+  if ((rc = upload(&P.d_csel, P.csel, st))) return rc;
+  if ((rc = upload(&P.d_choff, P.ch_off, st))) return rc;
+  if ((rc = upload(&P.d_ms, ms, st))) return rc;
+  return PLG_OK;
+}
+
+static size_t c_bytes(const Plan &P) {
+  return (size_t)P.rows.size() * P.nch_c * g_ctx.V3 * (P.T == PLG_F64 ? 16 : 8);
+}
+static size_t part_bytes(const Plan &P) {
+  return (size_t)P.nsplit * P.rows.size() * P.nch_c * std::max(P.nmom, 1) * 16;
+}
+
+// enqueue the whole plan on a stream (capturable: no allocation, no memset)
+static int enqueue(Plan &P, void *c, void *part, void *out, int out_prec, hipStream_t st) {
+  const int nrows = (int)P.rows.size();
+  int rc;
+  for (const Launch &l : P.launches) {
+    if (P.T == PLG_F64) {
+      auto *pd = (const PropDev<double> *)P.d_props;
+      if (l.what == Launch::MESON)
+        rc = launch_meson<double>(l.set, pd, P.d_rows, nrows, (cx<double> *)c, g_ctx.V3, g_ctx.L[3], P.nch_c, l.ch0, st);
+      else
+        rc = launch_baryon<double>(l.two, l.mes, l.gmask, pd, P.d_rows, nrows, (cx<double> *)c, g_ctx.V3,
+                                   g_ctx.L[3], P.nch_c, l.ch0, P.tpb, st);
+    } else {
+      auto *pd = (const PropDev<float> *)P.d_props;
+      if (l.what == Launch::MESON)
+        rc = launch_meson<float>(l.set, pd, P.d_rows, nrows, (cx<float> *)c, g_ctx.V3, g_ctx.L[3], P.nch_c, l.ch0, st);
+      else
+        rc = launch_baryon<float>(l.two, l.mes, l.gmask, pd, P.d_rows, nrows, (cx<float> *)c, g_ctx.V3,
+                                  g_ctx.L[3], P.nch_c, l.ch0, P.tpb, st);
+    }
+    if (rc) return fail(rc, "contraction launch failed: " + std::string(hipGetErrorString(hipGetLastError())));
+  }
+  const int nout = (int)P.csel.size();
+  if (P.space == PLG_POSITION_SPACE) {
+    rc = launch_scatter_pos(c, P.T, g_ctx.V3, P.d_orows, (int)P.orows.size(), P.nch_c, nout, P.d_csel, P.d_choff,
+                            P.m_stride, out, out_prec, st);
+    return rc ? fail(rc, "position-space scatter failed") : PLG_OK;
+  }
+  ProjGeom g;
+  g.V3 = g_ctx.V3;
+  for (int i = 0; i < 3; i++) g.L[i] = g_ctx.L[i];
+  g.M = g_ctx.M;
+  g.nmom = P.nmom;
+  g.ncols = nrows * P.nch_c;
+  g.nsplit = P.nsplit;
+  g.mom_scaled = P.d_ms;
+  g.cos_tab = g_ctx.d_cos;
+  g.sin_tab = g_ctx.d_sin;
+  if (P.T == PLG_F64) rc = launch_project_f64(g, (const cx<double> *)c, (cx<double> *)part, P.mtile, st);
+  else rc = launch_project_bf16(g, (const cx<float> *)c, (cx<double> *)part, P.terms, g_ctx.d_tab3, P.mtile, st);
+  if (rc) return fail(rc, "projection launch failed");
+  rc = launch_finalize(g, (const cx<double> *)part, P.d_orows, (int)P.orows.size(), P.nch_c, nout, P.d_csel,
+                       P.d_choff, P.m_stride, out, out_prec, st);
+  return rc ? fail(rc, "finalize launch failed") : PLG_OK;
+}
+
+}  // namespace plg
+
+using namespace plg;
+
+struct plg_batch_s {
+  Plan plan;
+};
+
+extern "C" {
+
+int plg_version(void) { return 100; }
+const char *plg_last_error(void) { return g_err.c_str(); }
+
+int plg_init(const int L[4], int device) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!L) return fail(PLG_ERR_INVALID, "L is NULL");
+  for (int i = 0; i < 4; i++)
+    if (L[i] <= 0) return fail(PLG_ERR_INVALID, "lattice extents must be positive");
+  if ((int64_t)L[0] * L[1] * L[2] * L[3] > (int64_t)1 << 32)
+    return fail(PLG_ERR_INVALID, "lattice volume above 2^32 sites");
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(PLG_ERR_INVALID, "no such device");
+  HIPCHK(hipSetDevice(device));
+  if (g_ctx.init) plg_finalize();
+  std::memcpy(g_ctx.L, L, sizeof(int) * 4);
+  g_ctx.V3 = (int64_t)L[0] * L[1] * L[2];
+  g_ctx.device = device;
+  int M = std::lcm(std::lcm(L[0], L[1]), L[2]);
+  g_ctx.M = M;
+  // exact phase table: cos/sin(2 pi k / M) for integer k, in FP64 on the host
+  std::vector<double> c(M), s(M);
+  std::vector<uint16_t> t3((size_t)M * 6);
+  for (int k = 0; k < M; k++) {
+    const double th = 2.0 * M_PI * (double)k / (double)M;
+    c[k] = std::cos(th);
+    s[k] = std::sin(th);
+    split3_host(c[k], &t3[(size_t)6 * k]);
+    split3_host(s[k], &t3[(size_t)6 * k + 3]);
+  }
+  HIPCHK(hipMalloc((void **)&g_ctx.d_cos, sizeof(double) * M));
+  HIPCHK(hipMalloc((void **)&g_ctx.d_sin, sizeof(double) * M));
+  HIPCHK(hipMalloc((void **)&g_ctx.d_tab3, sizeof(uint16_t) * 6 * M));
+  HIPCHK(hipMemcpy(g_ctx.d_cos, c.data(), sizeof(double) * M, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g_ctx.d_sin, s.data(), sizeof(double) * M, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g_ctx.d_tab3, t3.data(), sizeof(uint16_t) * 6 * M, hipMemcpyHostToDevice));
+  g_ctx.init = true;
+  return PLG_OK;
+}
+
+int plg_finalize(void) {
+  if (!g_ctx.init) return PLG_OK;
+  hipDeviceSynchronize();
+  for (void *p : {(void *)g_ctx.d_cos, (void *)g_ctx.d_sin, (void *)g_ctx.d_tab3, g_ctx.ws_c, g_ctx.ws_part})
+    if (p) hipFree(p);
+  g_ctx = Context();
+  return PLG_OK;
+}
+
+int plg_momlist(int q2max, int *nmom, int *xyz, int cap) {
+  if (q2max < 0) return fail(PLG_ERR_INVALID, "The maximum number of Q2 cannot be negative");
+  std::vector<int> v;
+  momlist_host(q2max, v);
+  const int n = (int)v.size() / 3;
+  if (nmom) *nmom = n;
+  if (xyz)
+    for (int i = 0; i < std::min(n, cap) * 3; i++) xyz[i] = v[i];
+  return PLG_OK;
+}
+
+int plg_window(const int source[4], int totalT, int *startT, int *endT, int *localT) {
+  if (!g_ctx.init) return fail(PLG_ERR_NOT_INIT, "plg_init not called");
+  if (!source || totalT <= 0 || totalT > g_ctx.L[3]) return fail(PLG_ERR_INVALID, "bad source or totalT");
+  for (int i = 0; i < 4; i++)
+    if (source[i] < 0 || source[i] >= g_ctx.L[i]) return fail(PLG_ERR_INVALID, "source outside the lattice");
+  Window w = window_of(g_ctx.L, source, totalT);
+  if (startT) *startT = w.startT;
+  if (endT) *endT = w.endT;
+  if (localT) *localT = w.localT;
+  return PLG_OK;
+}
+
+int plg_row_time(const int source[4], int totalT, int j, int *t) {
+  int lt;
+  int rc = plg_window(source, totalT, nullptr, nullptr, &lt);
+  if (rc) return rc;
+  if (j < 0 || j >= lt) return fail(PLG_ERR_INVALID, "row outside the window");
+  *t = row_time(g_ctx.L, source, window_of(g_ctx.L, source, totalT), j);
+  return PLG_OK;
+}
+
+int plg_kind_shape(int kind, int group_mask, int *ngroups, int *ndatasets, int *nch) {
+  int ng, nd, nc;
+  int rc = kind_shape(kind, group_mask, ng, nd, nc);
+  if (rc) return rc;
+  if (ngroups) *ngroups = ng;
+  if (ndatasets) *ndatasets = nd;
+  if (nch) *nch = nc;
+  return PLG_OK;
+}
+
+int plg_correlator_size(int kind, int group_mask, const int source[4], int totalT, int nmom, int space,
+                        int64_t *elems) {
+  int ng, nd, nc, lt;
+  int rc = kind_shape(kind, group_mask, ng, nd, nc);
+  if (rc) return rc;
+  if ((rc = plg_window(source, totalT, nullptr, nullptr, &lt))) return rc;
+  const int64_t inner = space == PLG_MOMENTUM_SPACE ? nmom : g_ctx.V3;
+  *elems = (int64_t)ng * nd * lt * inner * nc;
+  return PLG_OK;
+}
+
+int plg_correlator(int kind, int group_mask, const plg_prop *p1, const plg_prop *p2, const int source[4],
+                   int totalT, const int *moms, int nmom, int mode, int space, int out_precision, void *out,
+                   void *stream) {
+  if (!g_ctx.init) return fail(PLG_ERR_NOT_INIT, "plg_init not called");
+  if (!p1 || !out || !source) return fail(PLG_ERR_INVALID, "NULL argument");
+  if (out_precision != PLG_F64 && out_precision != PLG_F32) return fail(PLG_ERR_INVALID, "bad out precision");
+  int lt;
+  int rc = plg_window(source, totalT, nullptr, nullptr, &lt);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  Plan P;
+  std::vector<plg_prop> props = {*p1, p2 ? *p2 : *p1};
+  if (!same_view(props[0], props[1]) && (kind == PLG_TWOPT))
+    return fail(PLG_ERR_INVALID, "twopt takes one propagator");
+  std::vector<JobIn> jobs = {{{source[0], source[1], source[2], source[3]}, totalT, 0, 1, 0}};
+  if ((rc = build_plan(P, kind, group_mask, props, jobs, moms, nmom, mode, PLG_SPLIT_BF16X6, space))) return rc;
+  choose_tiles(P);
+  if ((rc = upload_plan(P, st))) return rc;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if ((rc = ensure(&g_ctx.ws_c, &g_ctx.ws_c_bytes, c_bytes(P)))) return rc;
+    if ((rc = ensure(&g_ctx.ws_part, &g_ctx.ws_part_bytes, part_bytes(P)))) return rc;
+  }
+  rc = enqueue(P, g_ctx.ws_c, g_ctx.ws_part, out, out_precision, st);
+  if (rc) return rc;
+  // the plan's device tables are freed on return: finish this correlator first
+  HIPCHK(hipStreamSynchronize(st));
+  return PLG_OK;
+}
+
+int plg_batch_create(int kind, int group_mask, const plg_prop *props, int nprops, const plg_job *jobs, int njobs,
+                     const int *moms, int nmom, int mode, int split, int flags, plg_batch *out) {
+  if (!g_ctx.init) return fail(PLG_ERR_NOT_INIT, "plg_init not called");
+  if (!props || nprops <= 0 || !jobs || njobs <= 0 || !out) return fail(PLG_ERR_INVALID, "empty batch");
+  std::vector<plg_prop> pv(props, props + nprops);
+  std::vector<JobIn> jv;
+  for (int i = 0; i < njobs; i++) {
+    const plg_job &j = jobs[i];
+    if (j.prop < 0 || j.prop >= nprops) return fail(PLG_ERR_INVALID, "job propagator index out of range");
+    int lt;
+    int rc = plg_window(j.source, j.totalT, nullptr, nullptr, &lt);
+    if (rc) return rc;
+    jv.push_back({{j.source[0], j.source[1], j.source[2], j.source[3]}, j.totalT, j.prop, j.prop, j.out_offset});
+  }
+  plg_batch b = new plg_batch_s();
+  Plan &P = b->plan;
+  int rc = build_plan(P, kind, group_mask, pv, jv, moms, nmom, mode, split, PLG_MOMENTUM_SPACE);
+  if (!rc) {
+    choose_tiles(P);
+    rc = upload_plan(P, nullptr);
+  }
+  if (!rc) {
+    P.own_ws = true;
+    rc = ensure(&P.c, &P.c_bytes, c_bytes(P));
+    if (!rc) rc = ensure(&P.part, &P.part_bytes, part_bytes(P));
+  }
+  if (!rc && (flags & PLG_BATCH_GRAPH)) {
+    if (hipStreamCreateWithFlags(&P.cap_stream, hipStreamNonBlocking) != hipSuccess)
+      rc = fail(PLG_ERR_HIP, "stream creation failed");
+  }
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  hipDeviceSynchronize();
+  *out = b;
+  return PLG_OK;
+}
+
+int plg_batch_run(plg_batch b, void *out, void *stream) {
+  if (!b || !out) return fail(PLG_ERR_INVALID, "NULL argument");
+  Plan &P = b->plan;
+  hipStream_t st = (hipStream_t)stream;
+  if (!P.cap_stream) return enqueue(P, P.c, P.part, out, PLG_F64, st);
+  if (!P.exec || P.graph_out != out) {
+    if (P.exec) hipGraphExecDestroy(P.exec);
+    if (P.graph) hipGraphDestroy(P.graph);
+    P.exec = nullptr;
+    P.graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(P.cap_stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue(P, P.c, P.part, out, PLG_F64, P.cap_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(P.cap_stream, &g);
+    if (rc) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(PLG_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    P.graph = g;
+    HIPCHK(hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0));
+    P.graph_out = out;
+  }
+  HIPCHK(hipGraphLaunch(P.exec, st));
+  return PLG_OK;
+}
+
+int plg_batch_info(plg_batch b, int64_t *out_elems, int64_t *nsites, int *nrows) {
+  if (!b) return fail(PLG_ERR_INVALID, "NULL batch");
+  if (out_elems) *out_elems = b->plan.out_elems;
+  if (nsites) *nsites = b->plan.nsites;
+  if (nrows) *nrows = (int)b->plan.rows.size();
+  return PLG_OK;
+}
+
+int plg_batch_destroy(plg_batch b) {
+  if (b) {
+    hipDeviceSynchronize();
+    delete b;
+  }
+  return PLG_OK;
+}
+
+int plg_synth_prop(const plg_prop *dst, uint64_t seed, uint32_t stream_id, int t0, double lambda, void *stream) {
+  return plg_synth_prop_perturbed(dst, seed, stream_id, 0, 0.0, t0, lambda, stream);
+}
+
+int plg_synth_prop_perturbed(const plg_prop *dst, uint64_t seed, uint32_t base_stream, uint32_t pert_stream,
+                             double eps, int t0, double lambda, void *stream) {
+  if (!g_ctx.init) return fail(PLG_ERR_NOT_INIT, "plg_init not called");
+  if (!dst || !dst->data) return fail(PLG_ERR_INVALID, "NULL propagator");
+  if (dst->nt <= 0 || dst->nt > g_ctx.L[3] || dst->comp_stride < (int64_t)dst->nt * g_ctx.V3)
+    return fail(PLG_ERR_INVALID, "bad propagator view");
+  int rc = launch_synth(*dst, g_ctx.L, seed, base_stream, pert_stream, eps, t0, lambda, (hipStream_t)stream);
+  return rc ? fail(rc, "synth launch failed") : PLG_OK;
+}
+
+int plg_cast_prop(const plg_prop *dst, const plg_prop *src, void *stream) {
+  if (!g_ctx.init) return fail(PLG_ERR_NOT_INIT, "plg_init not called");
+  if (!dst || !src || dst->precision != PLG_F32 || src->precision != PLG_F64)
+    return fail(PLG_ERR_INVALID, "cast copies an F64 view into an F32 view");
+  for (int pl = 0; pl < dst->nt; pl++) {
+    const int t = (dst->t_first + pl) % g_ctx.L[3];
+    int rc = check_prop(*src, PLG_F64, t);
+    if (rc) return rc;
+  }
+  int rc = launch_cast(*dst, *src, g_ctx.V3, g_ctx.L[3], (hipStream_t)stream);
+  return rc ? fail(rc, "cast launch failed") : PLG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,149 @@
+// plg_common.h -- shared device/host definitions of the 2pt path.
+//
+// Spin algebra: every matrix of the 16-element Clifford basis used by PLEGMA
+// (lib/kernels/PLEGMA_gammas.cuh:6-41, GAMMAS order of
+// include/global/PLEGMA_enums.h:22) is a signed permutation: row i has one
+// non-zero, at column i ^ c (c in {0,1,2,3}), with value i^ph[i].  The
+// tables below are derived at compile time from gamma_1..gamma_4.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace plg {
+
+// ------------------------------------------------------------------ spin algebra
+struct SPerm {
+  int col[4];  // column of the non-zero in row i
+  int ph[4];   // value = i^ph (0:1, 1:i, 2:-1, 3:-i)
+};
+
+constexpr SPerm sp_mul(const SPerm &a, const SPerm &b) {
+  SPerm r{};
+  for (int i = 0; i < 4; i++) {
+    r.col[i] = b.col[a.col[i]];
+    r.ph[i] = (a.ph[i] + b.ph[a.col[i]]) & 3;
+  }
+  return r;
+}
+constexpr SPerm sp_phase(const SPerm &a, int ph) {
+  SPerm r = a;
+  for (int i = 0; i < 4; i++) r.ph[i] = (a.ph[i] + ph) & 3;
+  return r;
+}
+// adjoint: (A^dag)[j][i] = conj(A[i][j])
+constexpr SPerm sp_dag(const SPerm &a) {
+  SPerm r{};
+  for (int i = 0; i < 4; i++) {
+    r.col[a.col[i]] = i;
+    r.ph[a.col[i]] = (4 - a.ph[i]) & 3;
+  }
+  return r;
+}
+
+constexpr SPerm G_ONE{{0, 1, 2, 3}, {0, 0, 0, 0}};
+constexpr SPerm G_1{{3, 2, 1, 0}, {1, 1, 3, 3}};
+constexpr SPerm G_2{{3, 2, 1, 0}, {0, 2, 2, 0}};
+constexpr SPerm G_3{{2, 3, 0, 1}, {1, 3, 3, 1}};
+constexpr SPerm G_4{{0, 1, 2, 3}, {0, 0, 2, 2}};
+constexpr SPerm G_5 = sp_mul(sp_mul(sp_mul(G_1, G_2), G_3), G_4);
+// sigma_{mu nu} = -(i/2)[g_mu, g_nu] = -i g_mu g_nu for mu != nu
+constexpr SPerm sp_sigma(const SPerm &a, const SPerm &b) { return sp_phase(sp_mul(a, b), 3); }
+
+constexpr SPerm gamma_op(int op) {
+  switch (op) {
+  case 0: return G_ONE;
+  case 1: return G_1;
+  case 2: return G_2;
+  case 3: return G_3;
+  case 4: return G_4;
+  case 5: return G_5;
+  case 6: return sp_mul(G_5, G_1);
+  case 7: return sp_mul(G_5, G_2);
+  case 8: return sp_mul(G_5, G_3);
+  case 9: return sp_mul(G_5, G_4);
+  case 10: return sp_sigma(G_1, G_2);
+  case 11: return sp_sigma(G_1, G_3);
+  case 12: return sp_sigma(G_2, G_3);
+  case 13: return sp_sigma(G_4, G_1);
+  case 14: return sp_sigma(G_4, G_2);
+  default: return sp_sigma(G_4, G_3);
+  }
+}
+// charge conjugation C = i g2 g4
+constexpr SPerm G_C = sp_phase(sp_mul(G_2, G_4), 1);
+// diquark matrix A = C Gamma (0: C g5, k=1..3: C g_k) and Abar = g4 A^dag g4
+constexpr SPerm diquark(int which) { return sp_mul(G_C, gamma_op(which == 0 ? 5 : which)); }
+constexpr SPerm diquark_bar(int which) { return sp_mul(sp_mul(G_4, sp_dag(diquark(which))), G_4); }
+
+// xor class of a signed permutation
+constexpr int sp_class(const SPerm &a) { return a.col[0]; }
+
+// channel order of contractMesons (lib/PLEGMA_Correlator.cu:47-50)
+constexpr int MESON10_OPS[10] = {5, 0, 6, 7, 8, 9, 1, 2, 3, 4};
+
+// ------------------------------------------------------------------ complex
+template <typename T> struct cx {
+  T re, im;
+};
+template <typename T> __host__ __device__ inline cx<T> cmake(T r, T i) { return cx<T>{r, i}; }
+template <typename T> __host__ __device__ inline cx<T> operator+(cx<T> a, cx<T> b) {
+  return {a.re + b.re, a.im + b.im};
+}
+template <typename T> __host__ __device__ inline cx<T> operator-(cx<T> a, cx<T> b) {
+  return {a.re - b.re, a.im - b.im};
+}
+template <typename T> __host__ __device__ inline cx<T> operator*(cx<T> a, cx<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T> __host__ __device__ inline cx<T> operator*(T s, cx<T> a) {
+  return {s * a.re, s * a.im};
+}
+template <typename T> __host__ __device__ inline cx<T> cconj(cx<T> a) { return {a.re, -a.im}; }
+// a += b * c
+template <typename T> __host__ __device__ inline void cfma(cx<T> &a, cx<T> b, cx<T> c) {
+  a.re = fma(b.re, c.re, a.re);
+  a.re = fma(-b.im, c.im, a.re);
+  a.im = fma(b.re, c.im, a.im);
+  a.im = fma(b.im, c.re, a.im);
+}
+// a += b * conj(c)
+template <typename T> __host__ __device__ inline void cfma_conj(cx<T> &a, cx<T> b, cx<T> c) {
+  a.re = fma(b.re, c.re, a.re);
+  a.re = fma(b.im, c.im, a.re);
+  a.im = fma(b.im, c.re, a.im);
+  a.im = fma(-b.re, c.im, a.im);
+}
+// multiply by i^ph (compile-time ph folds to swaps/negations)
+template <typename T> __host__ __device__ inline cx<T> cmul_iph(cx<T> a, int ph) {
+  switch (ph & 3) {
+  case 0: return a;
+  case 1: return {-a.im, a.re};
+  case 2: return {-a.re, -a.im};
+  default: return {a.im, -a.re};
+  }
+}
+
+// ------------------------------------------------------------------ epsilon
+// eps permutations in the order of lib/kernels/PLEGMA_kernel_utils.cuh:49-56
+constexpr int EPS[6][3] = {{0, 1, 2}, {2, 0, 1}, {1, 2, 0}, {2, 1, 0}, {0, 2, 1}, {1, 0, 2}};
+constexpr int SGN_EPS[6] = {+1, +1, +1, -1, -1, -1};
+
+// propagator component index: ((mu*4 + nu)*3 + a)*3 + b
+__host__ __device__ constexpr int pidx(int mu, int nu, int a, int b) {
+  return ((mu * 4 + nu) * 3 + a) * 3 + b;
+}
+
+// compile-time loop: f(std::integral_constant<int, I>) for I = 0..N-1, so that
+// indices into register arrays are constants in every instance
+template <int I> struct ic {
+  static constexpr int value = I;
+  constexpr operator int() const { return I; }
+};
+template <int B, int E, typename F> __host__ __device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (B < E) {
+    f(ic<B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+}  // namespace plg

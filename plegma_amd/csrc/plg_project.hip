@@ -1,0 +1,406 @@
+// plg_project.hip -- momentum projection as a dense phase-matrix GEMM on MFMA.
+//
+// Replaces the fused naive DFT of lib/kernels/PLEGMA_kernel_utils.cuh:820-860
+// (per site and momentum a sincos, an LDS tree reduction and a host sum of
+// per-block partials, PLEGMA_mesons.cuh:113-120) by
+//
+//   part[s][n][m] = sum_{x in split s} exp(-i 2 pi n_m . x / L) c[n][x]
+//
+// with n = (row, channel) column of the site correlator.  The complex product
+// is a real GEMM with interleaved K (2x + re/im) and interleaved M (2m + re/im):
+//   A[2m  ][2x] =  cos   A[2m  ][2x+1] = sin
+//   A[2m+1][2x] = -sin   A[2m+1][2x+1] = cos      (theta = 2 pi k / M)
+// A is generated per K tile from the exact integer phase index
+//   k = sum_i n_i (x_i) (M / L_i) mod M,  M = lcm(Lx, Ly, Lz),
+// and the FP64 table cos/sin(2 pi k / M); the source offset is applied later
+// in FP64 as a per-momentum constant phase (the finalize kernel).
+//
+//  k_project_f64   v_mfma_f64_16x16x4_f64, FP64 operands and accumulation.
+//  k_project_bf16  v_mfma_f32_16x16x32_bf16 on split operands a = a0+a1+a2,
+//                  b = b0+b1+b2 (bf16 pieces); 3 products (a0b0+a0b1+a1b0) or
+//                  6 (+a0b2+a1b1+a2b0); FP32 MFMA accumulation flushed into FP64
+//                  registers every FLUSH k-steps; FP64 partials per split.
+#include "plg_internal.h"
+
+namespace plg {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+constexpr int PJ_THREADS = 256;
+constexpr int PJ_NT = 128;  // columns per workgroup (4 waves x 32)
+
+struct SiteXYZ {
+  int x, y, z;
+};
+__device__ __forceinline__ SiteXYZ site_xyz(int64_t s, const ProjGeom &g) {
+  SiteXYZ r;
+  r.x = (int)(s % g.L[0]);
+  int64_t q = s / g.L[0];
+  r.y = (int)(q % g.L[1]);
+  r.z = (int)(q / g.L[1]);
+  return r;
+}
+__device__ __forceinline__ int phase_index(const int *ms, int m, SiteXYZ p, int M) {
+  int k = ms[3 * m] * p.x + ms[3 * m + 1] * p.y + ms[3 * m + 2] * p.z;
+  k %= M;
+  return k < 0 ? k + M : k;
+}
+
+// sites [x_begin, x_end) of split s
+__device__ __forceinline__ void split_range(const ProjGeom &g, int s, int64_t &b, int64_t &e) {
+  const int64_t per = (g.V3 + g.nsplit - 1) / g.nsplit;
+  b = (int64_t)s * per;
+  e = b + per < g.V3 ? b + per : g.V3;
+}
+
+// ------------------------------------------------------------------ FP64
+constexpr int F64_KT = 16;                 // sites per K tile
+constexpr int F64_LD = 2 * F64_KT + 1;     // padded LDS row (doubles)
+
+template <int MT>
+__global__ __launch_bounds__(PJ_THREADS) void k_project_f64(ProjGeom g, const cx<double> *__restrict__ c,
+                                                            double *__restrict__ part) {
+  constexpr int MB = MT / 16;  // M blocks per wave
+  __shared__ double sB[PJ_NT][F64_LD];
+  __shared__ double sA[MT][F64_LD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t n0 = (int64_t)blockIdx.x * PJ_NT;
+  const int mr0 = blockIdx.y * MT;
+  const int split = blockIdx.z;
+  int64_t xb, xe;
+  split_range(g, split, xb, xe);
+  const int Mreal = 2 * g.nmom;
+
+  d4 acc[MB][2];
+#pragma unroll
+  for (int i = 0; i < MB; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = d4{0, 0, 0, 0};
+
+  for (int64_t xt = xb; xt < xe; xt += F64_KT) {
+    // stage B: 128 columns x 16 sites
+    {
+      const int s = tid & 15;
+      const int64_t x = xt + s;
+#pragma unroll
+      for (int i = 0; i < PJ_NT / 16; i++) {
+        const int col = (tid >> 4) + 16 * i;
+        const int64_t n = n0 + col;
+        cx<double> v = cmake<double>(0, 0);
+        if (x < xe && n < g.ncols) v = c[n * g.V3 + x];
+        sB[col][2 * s] = v.re;
+        sB[col][2 * s + 1] = v.im;
+      }
+    }
+    // stage A: MT real rows x 16 sites
+#pragma unroll
+    for (int i = 0; i < (MT * F64_KT) / PJ_THREADS; i++) {
+      const int e = tid + PJ_THREADS * i;
+      const int mr = e / F64_KT, s = e % F64_KT;
+      const int m = (mr0 + mr) >> 1, ri = (mr0 + mr) & 1;
+      const int64_t x = xt + s;
+      double a0 = 0, a1 = 0;
+      if (m < g.nmom && x < xe) {
+        const int k = phase_index(g.mom_scaled, m, site_xyz(x, g), g.M);
+        const double cs = g.cos_tab[k], sn = g.sin_tab[k];
+        a0 = ri ? -sn : cs;
+        a1 = ri ? cs : sn;
+      }
+      sA[mr][2 * s] = a0;
+      sA[mr][2 * s + 1] = a1;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 2 * F64_KT / 4; ks++) {
+      const int kk = ks * 4 + (lane >> 4);
+      double bf[2];
+#pragma unroll
+      for (int j = 0; j < 2; j++) bf[j] = sB[wave * 32 + j * 16 + (lane & 15)][kk];
+#pragma unroll
+      for (int i = 0; i < MB; i++) {
+        const double af = sA[i * 16 + (lane & 15)][kk];
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af, bf[j], acc[i][j], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  // D layout of 16x16x4 f64: lane l, value v -> row 4*(l/16)+v, column l%16
+#pragma unroll
+  for (int i = 0; i < MB; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int mr = mr0 + i * 16 + 4 * (lane >> 4) + v;
+        const int64_t n = n0 + wave * 32 + j * 16 + (lane & 15);
+        if (mr < Mreal && n < g.ncols) part[((int64_t)split * g.ncols + n) * Mreal + mr] = acc[i][j][v];
+      }
+}
+
+int launch_project_f64(const ProjGeom &g, const cx<double> *c, cx<double> *part, int mtile,
+                       hipStream_t st) {
+  const int Mreal = 2 * g.nmom;
+  dim3 block(PJ_THREADS);
+  if (mtile == 32) {
+    dim3 grid((unsigned)((g.ncols + PJ_NT - 1) / PJ_NT), (unsigned)((Mreal + 31) / 32), (unsigned)g.nsplit);
+    k_project_f64<32><<<grid, block, 0, st>>>(g, c, reinterpret_cast<double *>(part));
+  } else {
+    dim3 grid((unsigned)((g.ncols + PJ_NT - 1) / PJ_NT), (unsigned)((Mreal + 63) / 64), (unsigned)g.nsplit);
+    k_project_f64<64><<<grid, block, 0, st>>>(g, c, reinterpret_cast<double *>(part));
+  }
+  return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;
+}
+
+// ------------------------------------------------------------------ split bf16
+constexpr int BF_KT = 16;               // sites per K tile = one 16x16x32 k-step
+constexpr int BF_LD = 2 * BF_KT + 8;    // padded LDS row (bf16 elements)
+
+__device__ __forceinline__ uint16_t bf16_rn(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ void split3(float v, uint16_t &p0, uint16_t &p1, uint16_t &p2) {
+  p0 = bf16_rn(v);
+  const float r1 = v - bf16_f(p0);
+  p1 = bf16_rn(r1);
+  const float r2 = r1 - bf16_f(p1);
+  p2 = bf16_rn(r2);
+}
+
+template <int MT, int TERMS, int FLUSH>
+__global__ __launch_bounds__(PJ_THREADS) void k_project_bf16(ProjGeom g, const cx<float> *__restrict__ c,
+                                                             const uint16_t *__restrict__ tab,
+                                                             double *__restrict__ part) {
+  constexpr int MB = MT / 16;
+  constexpr int NP = TERMS == 3 ? 2 : 3;  // pieces needed
+  __shared__ __align__(16) uint16_t sB[NP][PJ_NT][BF_LD];
+  __shared__ __align__(16) uint16_t sA[NP][MT][BF_LD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t n0 = (int64_t)blockIdx.x * PJ_NT;
+  const int mr0 = blockIdx.y * MT;
+  const int split = blockIdx.z;
+  int64_t xb, xe;
+  split_range(g, split, xb, xe);
+  const int Mreal = 2 * g.nmom;
+
+  f4 acc[MB][2];
+  double acc64[MB][2][4];
+#pragma unroll
+  for (int i = 0; i < MB; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      acc[i][j] = f4{0, 0, 0, 0};
+#pragma unroll
+      for (int v = 0; v < 4; v++) acc64[i][j][v] = 0;
+    }
+
+  int step = 0;
+  for (int64_t xt = xb; xt < xe; xt += BF_KT) {
+    {
+      const int s = tid & 15;
+      const int64_t x = xt + s;
+#pragma unroll
+      for (int i = 0; i < PJ_NT / 16; i++) {
+        const int col = (tid >> 4) + 16 * i;
+        const int64_t n = n0 + col;
+        cx<float> v = cmake<float>(0, 0);
+        if (x < xe && n < g.ncols) v = c[n * g.V3 + x];
+        uint16_t r0, r1, r2, i0, i1, i2;
+        split3(v.re, r0, r1, r2);
+        split3(v.im, i0, i1, i2);
+        sB[0][col][2 * s] = r0;
+        sB[0][col][2 * s + 1] = i0;
+        sB[1][col][2 * s] = r1;
+        sB[1][col][2 * s + 1] = i1;
+        if (NP == 3) {
+          sB[NP - 1][col][2 * s] = r2;
+          sB[NP - 1][col][2 * s + 1] = i2;
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < (MT * BF_KT) / PJ_THREADS; i++) {
+      const int e = tid + PJ_THREADS * i;
+      const int mr = e / BF_KT, s = e % BF_KT;
+      const int m = (mr0 + mr) >> 1, ri = (mr0 + mr) & 1;
+      const int64_t x = xt + s;
+      // tab: [k][6] = cos pieces 0..2, sin pieces 0..2
+      uint16_t a0[3] = {0, 0, 0}, a1[3] = {0, 0, 0};
+      if (m < g.nmom && x < xe) {
+        const int k = phase_index(g.mom_scaled, m, site_xyz(x, g), g.M);
+        const uint16_t *t = tab + 6 * k;
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+          const uint16_t cs = t[p], sn = t[3 + p];
+          a0[p] = ri ? (uint16_t)(sn ^ 0x8000u) : cs;
+          a1[p] = ri ? cs : sn;
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < NP; p++) {
+        sA[p][mr][2 * s] = a0[p];
+        sA[p][mr][2 * s + 1] = a1[p];
+      }
+    }
+    __syncthreads();
+    {
+      // 16x16x32: lane l holds A[l%16][8*(l/16) .. +7], B[8*(l/16) .. +7][l%16]
+      const int kk = 8 * (lane >> 4);
+      bf8 bf[NP][2];
+#pragma unroll
+      for (int p = 0; p < NP; p++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+          bf[p][j] = *reinterpret_cast<const bf8 *>(&sB[p][wave * 32 + j * 16 + (lane & 15)][kk]);
+#pragma unroll
+      for (int i = 0; i < MB; i++) {
+        bf8 af[NP];
+#pragma unroll
+        for (int p = 0; p < NP; p++) af[p] = *reinterpret_cast<const bf8 *>(&sA[p][i * 16 + (lane & 15)][kk]);
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+          f4 a = acc[i][j];
+          if (TERMS == 6) {
+            a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[2], bf[0][j], a, 0, 0, 0);
+            a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1], bf[1][j], a, 0, 0, 0);
+            a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0], bf[2][j], a, 0, 0, 0);
+          }
+          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1], bf[0][j], a, 0, 0, 0);
+          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0], bf[1][j], a, 0, 0, 0);
+          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0], bf[0][j], a, 0, 0, 0);
+          acc[i][j] = a;
+        }
+      }
+    }
+    if (++step == FLUSH) {
+      step = 0;
+#pragma unroll
+      for (int i = 0; i < MB; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) acc64[i][j][v] += (double)acc[i][j][v];
+          acc[i][j] = f4{0, 0, 0, 0};
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < MB; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int mr = mr0 + i * 16 + 4 * (lane >> 4) + v;
+        const int64_t n = n0 + wave * 32 + j * 16 + (lane & 15);
+        if (mr < Mreal && n < g.ncols)
+          part[((int64_t)split * g.ncols + n) * Mreal + mr] = acc64[i][j][v] + (double)acc[i][j][v];
+      }
+}
+
+int launch_project_bf16(const ProjGeom &g, const cx<float> *c, cx<double> *part, int terms,
+                        const uint16_t *tab, int mtile, hipStream_t st) {
+  const int Mreal = 2 * g.nmom;
+  dim3 block(PJ_THREADS);
+  dim3 grid((unsigned)((g.ncols + PJ_NT - 1) / PJ_NT), (unsigned)((Mreal + mtile - 1) / mtile),
+            (unsigned)g.nsplit);
+  double *pd = reinterpret_cast<double *>(part);
+  if (mtile == 32) {
+    if (terms == 6) k_project_bf16<32, 6, 8><<<grid, block, 0, st>>>(g, c, tab, pd);
+    else k_project_bf16<32, 3, 8><<<grid, block, 0, st>>>(g, c, tab, pd);
+  } else {
+    if (terms == 6) k_project_bf16<64, 6, 8><<<grid, block, 0, st>>>(g, c, tab, pd);
+    else k_project_bf16<64, 3, 8><<<grid, block, 0, st>>>(g, c, tab, pd);
+  }
+  return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;
+}
+
+// ------------------------------------------------------------------ finalize
+// out[base + ch_off[k] + m*m_stride] = exp(+i 2pi n_m . x0 / L) * sum_s part[s][n][m],
+// n = crow*nch_c + ch_base + csel[k].  With the GEMM's exp(-i 2pi n.x/L) this
+// is exp(-i 2pi n.(x - x0)/L) (PLEGMA_kernel_utils.cuh:833-838, sign -1).
+template <typename TO>
+__global__ void k_finalize(ProjGeom g, const cx<double> *__restrict__ part, const OutRowDev *__restrict__ orows,
+                           int nch_c, int nout, const int *__restrict__ csel, const int64_t *__restrict__ ch_off,
+                           int64_t m_stride, cx<TO> *__restrict__ out) {
+  const OutRowDev R = orows[blockIdx.y];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)g.nmom * nout) return;
+  const int m = (int)(e / nout), k = (int)(e % nout);
+  const int64_t n = (int64_t)R.crow * nch_c + R.ch_base + csel[k];
+  double re = 0, im = 0;
+  for (int s = 0; s < g.nsplit; s++) {
+    const cx<double> v = part[((int64_t)s * g.ncols + n) * g.nmom + m];
+    re += v.re;
+    im += v.im;
+  }
+  int ph = g.mom_scaled[3 * m] * R.sx + g.mom_scaled[3 * m + 1] * R.sy + g.mom_scaled[3 * m + 2] * R.sz;
+  ph %= g.M;
+  if (ph < 0) ph += g.M;
+  const double cs = g.cos_tab[ph], sn = g.sin_tab[ph];
+  cx<TO> o;
+  o.re = (TO)(re * cs - im * sn);
+  o.im = (TO)(re * sn + im * cs);
+  out[R.base + ch_off[k] + (int64_t)m * m_stride] = o;
+}
+
+int launch_finalize(const ProjGeom &g, const cx<double> *part, const OutRowDev *orows, int norows,
+                    int nch_c, int nout, const int *csel, const int64_t *ch_off, int64_t m_stride,
+                    void *out, int out_prec, hipStream_t st) {
+  const int64_t ne = (int64_t)g.nmom * nout;
+  dim3 block(256), grid((unsigned)((ne + 255) / 256), (unsigned)norows);
+  if (out_prec == PLG_F64)
+    k_finalize<double><<<grid, block, 0, st>>>(g, part, orows, nch_c, nout, csel, ch_off, m_stride,
+                                               (cx<double> *)out);
+  else
+    k_finalize<float><<<grid, block, 0, st>>>(g, part, orows, nch_c, nout, csel, ch_off, m_stride,
+                                              (cx<float> *)out);
+  return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;
+}
+
+// ------------------------------------------------------------------ position space
+template <typename TI, typename TO>
+__global__ void k_scatter_pos(const cx<TI> *__restrict__ c, int64_t V3, const OutRowDev *__restrict__ orows,
+                              int nch_c, int nout, const int *__restrict__ csel,
+                              const int64_t *__restrict__ ch_off, int64_t x_stride, cx<TO> *__restrict__ out) {
+  const OutRowDev R = orows[blockIdx.y];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= V3 * nout) return;
+  const int64_t x = e / nout;
+  const int k = (int)(e % nout);
+  const cx<TI> v = c[((int64_t)R.crow * nch_c + R.ch_base + csel[k]) * V3 + x];
+  cx<TO> o;
+  o.re = (TO)v.re;
+  o.im = (TO)v.im;
+  out[R.base + ch_off[k] + x * x_stride] = o;
+}
+
+template <typename TI, typename TO>
+static void scatter_t(const void *c, int64_t V3, const OutRowDev *orows, int norows, int nch_c, int nout,
+                      const int *csel, const int64_t *ch_off, int64_t x_stride, void *out, hipStream_t st) {
+  const int64_t ne = V3 * nout;
+  dim3 block(256), grid((unsigned)((ne + 255) / 256), (unsigned)norows);
+  k_scatter_pos<TI, TO><<<grid, block, 0, st>>>((const cx<TI> *)c, V3, orows, nch_c, nout, csel, ch_off,
+                                                x_stride, (cx<TO> *)out);
+}
+
+int launch_scatter_pos(const void *c, int c_prec, int64_t V3, const OutRowDev *orows, int norows,
+                       int nch_c, int nout, const int *csel, const int64_t *ch_off, int64_t x_stride,
+                       void *out, int out_prec, hipStream_t st) {
+  if (c_prec == PLG_F64) {
+    if (out_prec == PLG_F64) scatter_t<double, double>(c, V3, orows, norows, nch_c, nout, csel, ch_off, x_stride, out, st);
+    else scatter_t<double, float>(c, V3, orows, norows, nch_c, nout, csel, ch_off, x_stride, out, st);
+  } else {
+    if (out_prec == PLG_F64) scatter_t<float, double>(c, V3, orows, norows, nch_c, nout, csel, ch_off, x_stride, out, st);
+    else scatter_t<float, float>(c, V3, orows, norows, nch_c, nout, csel, ch_off, x_stride, out, st);
+  }
+  return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;
+}
+
+}  // namespace plg

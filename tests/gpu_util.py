@@ -1,0 +1,70 @@
+"""Helpers of the GPU tests: device propagators through the C ABI, oracle calls
+on the same bytes, error norms."""
+import numpy as np
+
+from oracle import oracle as O
+from plegma_amd import capi
+
+
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        import pytest
+        pytest.skip("no HIP device")
+    return torch
+
+
+def make_prop(L, seed, stream, t0, lam, prec=capi.F64, t_first=0, nt=None):
+    """Seeded propagator on the device; returns (tensor, plg_prop view)."""
+    torch = torch_cuda()
+    V3 = L[0] * L[1] * L[2]
+    nt = L[3] if nt is None else nt
+    dt = torch.float64 if prec == capi.F64 else torch.float32
+    t = torch.empty((144, nt * V3, 2), dtype=dt, device="cuda")
+    v = capi.plg_prop(t.data_ptr(), prec, t_first, nt, 0, nt * V3)
+    capi.synth_prop(v, seed, stream, t0, lam)
+    return t, v
+
+
+def cast_f32(t64, v64):
+    torch = torch_cuda()
+    t = torch.empty(t64.shape, dtype=torch.float32, device="cuda")
+    v = capi.plg_prop(t.data_ptr(), capi.F32, v64.t_first, v64.nt, 0, v64.comp_stride)
+    capi.cast_prop(v, v64)
+    return t, v
+
+
+def host(t):
+    import torch
+    torch.cuda.synchronize()
+    return np.ascontiguousarray(t.detach().to("cpu", torch.float64).numpy().reshape(-1))
+
+
+def run_correlator(kind, mask, v1, v2, src, totalT, q2, mode=capi.MODE_F64,
+                   space=capi.MOMENTUM_SPACE, out_prec=capi.F64):
+    torch = torch_cuda()
+    moms = capi.momlist(q2)
+    n = capi.correlator_size(kind, mask, src, totalT, len(moms), space)
+    dt = torch.float64 if out_prec == capi.F64 else torch.float32
+    out = torch.full((n, 2), float("nan"), dtype=dt, device="cuda")
+    capi.correlator(kind, mask, v1, v2, src, totalT, moms, mode, space, out_prec, out)
+    h = host(out).reshape(n, 2)
+    return h[:, 0] + 1j * h[:, 1]
+
+
+def shaped(H, kind, mask, src, totalT, q2):
+    ng, nd, nch = capi.kind_shape(kind, mask)
+    _, _, lt = capi.window(src, totalT)
+    return H.reshape(ng, nd, lt, len(capi.momlist(q2)), nch)
+
+
+def rel_err(got, ref, axis=-2):
+    """max over everything of |got - ref| / max_over_axis |ref| (normwise over
+    momenta by default: one norm per group, dataset, row and channel)."""
+    den = np.abs(ref).max(axis=axis, keepdims=True)
+    den = np.where(den == 0, 1.0, den)
+    return float((np.abs(got - ref) / den).max())
+
+
+def oracle_H(kind, L, src, totalT, q2, P1, P2, mask=O.G_NUCL, rows=None, nthreads=0):
+    return O.correlator(kind, L, src, totalT, q2, P1, P2, mask=mask, rows=rows, nthreads=nthreads)

@@ -1,0 +1,60 @@
+"""CPU-side checks of the C ABI: the library loads, exports every function
+include/plegma_amd.h declares, and the entry points that need no device
+(momentum list, kind shapes) agree with the oracle.  No kernel is launched."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from plegma_amd import capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    txt = open(os.path.join(ROOT, "include", "plegma_amd.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(plg_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_all_declared_symbols():
+    lib = capi.lib()
+    names = declared()
+    assert names, "no declarations parsed"
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(capi.EXPORTS) == names
+
+
+def test_momlist_without_device():
+    for q in (0, 2, 4, 9, 16):
+        got = np.array(capi.momlist(q)).reshape(-1, 3)
+        assert np.array_equal(got, O.momlist(q))
+
+
+def test_negative_q2_is_an_error():
+    with pytest.raises(capi.PlgError) as e:
+        capi.momlist(-3)
+    assert e.value.code == -1
+
+
+@pytest.mark.parametrize("kind,okind,mask", [
+    (capi.MESONS, O.MESONS, 0), (capi.MESONS16, O.MESONS16, 0),
+    (capi.MESONS_NEW, O.MESONS_NEW, 0), (capi.MESONS_ALL, O.MESONS_ALL, 0),
+    (capi.BARYONS, O.BARYONS, capi.G_NUCL | capi.G_DELTAPP_ALL)])
+def test_kind_shapes(kind, okind, mask):
+    assert capi.kind_shape(kind, mask) == O.kind_shape(okind, mask)
+
+
+def test_unsupported_groups_rejected():
+    with pytest.raises(capi.PlgError) as e:
+        capi.kind_shape(capi.BARYONS, 1 << 1)   # nucl_nucl2: not on this path
+    assert e.value.code == -6
+
+
+def test_calls_before_init_fail_cleanly():
+    with pytest.raises(capi.PlgError) as e:
+        capi.window([0, 0, 0, 0], 4)
+    assert e.value.code == -2

@@ -1,0 +1,247 @@
+"""GPU parity of the 2pt path through the C ABI against the CPU oracle.
+
+Inputs are seeded propagators generated on the device (plg_synth_prop); the
+oracle reads the very same bytes copied back to the host.  FP64 path:
+normwise relative error <= 1e-12 per (group, dataset, row, channel) over the
+momentum list.  Mixed path (FP32 contraction, split-bf16 projection): error
+per channel and momentum as stated in each test.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from plegma_amd import capi
+
+import gpu_util as G
+
+pytestmark = pytest.mark.gpu
+
+L1 = [8, 8, 8, 16]      # BASELINE configuration 1 geometry
+Q1 = 4                   # |n|^2 <= 4 : 33 momenta
+TOL64 = 1e-12
+
+
+@pytest.fixture(scope="module", autouse=True)
+def ctx():
+    G.torch_cuda()
+    print("\n[gpu] plg_init", L1, flush=True)
+    capi.init(L1, 0)
+    yield
+    capi.finalize()
+
+
+@pytest.fixture(scope="module")
+def props():
+    print("[gpu] synth props", flush=True)
+    t1, v1 = G.make_prop(L1, 1234, 1, 0, 0.5)
+    t2, v2 = G.make_prop(L1, 1234, 2, 0, 0.5)
+    return (t1, v1, G.host(t1)), (t2, v2, G.host(t2))
+
+
+def test_synth_matches_oracle(props):
+    (t1, v1, h1), _ = props
+    ref = O.synth_prop(L1, 1234, 1, 0, 0.5)
+    np.testing.assert_allclose(h1, ref, rtol=1e-13, atol=1e-15)
+
+
+def test_momlist_matches_oracle():
+    for q in (0, 1, 4, 9, 16):
+        assert np.array_equal(np.array(capi.momlist(q)).reshape(-1, 3), O.momlist(q))
+
+
+@pytest.mark.parametrize("src,totalT", [([0, 0, 0, 0], 16), ([3, 5, 1, 13], 8), ([7, 0, 2, 5], 16)])
+def test_window_rules(src, totalT):
+    assert capi.window(src, totalT) == O.window(L1, src, totalT)
+    _, _, lt = capi.window(src, totalT)
+    for j in range(lt):
+        assert capi.row_time(src, totalT, j) == O.row_time(L1, src, totalT, j)
+
+
+@pytest.mark.parametrize("kind,okind,same", [
+    (capi.MESONS, O.MESONS, False), (capi.MESONS, O.MESONS, True),
+    (capi.MESONS16, O.MESONS16, False), (capi.MESONS_NEW, O.MESONS_NEW, False),
+    (capi.MESONS_ALL, O.MESONS_ALL, False)])
+def test_mesons_f64(props, kind, okind, same):
+    (t1, v1, h1), (t2, v2, h2) = props
+    src, totalT = [0, 0, 0, 0], 16
+    if same:
+        v2, h2 = v1, h1
+    print(f"[gpu] mesons kind={kind} same={same}", flush=True)
+    H = G.shaped(G.run_correlator(kind, 0, v1, v2, src, totalT, Q1), kind, 0, src, totalT, Q1)
+    R = G.oracle_H(okind, L1, src, totalT, Q1, h1, h2)
+    assert G.rel_err(H, R) < TOL64
+
+
+@pytest.mark.parametrize("src,totalT", [([0, 0, 0, 0], 16), ([3, 5, 1, 13], 8)])
+@pytest.mark.parametrize("same", [False, True])
+def test_baryons_f64(props, src, totalT, same):
+    (t1, v1, h1), (t2, v2, h2) = props
+    if same:
+        v2, h2 = v1, h1
+    mask = capi.G_NUCL | capi.G_DELTAPP_ALL
+    print(f"[gpu] baryons src={src} totalT={totalT} same={same}", flush=True)
+    H = G.shaped(G.run_correlator(capi.BARYONS, mask, v1, v2, src, totalT, Q1),
+                 capi.BARYONS, mask, src, totalT, Q1)
+    rows = [0, 3, totalT - 1]
+    R = G.oracle_H(O.BARYONS, L1, src, totalT, Q1, h1, h2,
+                   mask=O.G_NUCL | sum(O.G_DELTA32), rows=rows)
+    assert G.rel_err(H[:, :, rows], R[:, :, rows]) < TOL64
+
+
+def test_baryon_group_subset(props):
+    (t1, v1, h1), (t2, v2, h2) = props
+    src, totalT = [1, 2, 3, 4], 16
+    mask = capi.G_DELTAPP_22
+    H = G.shaped(G.run_correlator(capi.BARYONS, mask, v1, v2, src, totalT, 1),
+                 capi.BARYONS, mask, src, totalT, 1)
+    rows = [0, 11]
+    R = G.oracle_H(O.BARYONS, L1, src, totalT, 1, h1, h2, mask=O.G_DELTA32[1], rows=rows)
+    assert G.rel_err(H[:, :, rows], R[:, :, rows]) < TOL64
+
+
+def _twopt_oracle(h, src, totalT, q2, rows):
+    M = G.oracle_H(O.MESONS16, L1, src, totalT, q2, h, h, rows=rows)[:, :1]
+    B = G.oracle_H(O.BARYONS, L1, src, totalT, q2, h, h, mask=O.G_NUCL | sum(O.G_DELTA32),
+                   rows=rows)[:, :1]
+    return np.concatenate([M, B], 0)
+
+
+def test_twopt_f64(props):
+    (t1, v1, h1), _ = props
+    src, totalT = [2, 1, 0, 9], 12
+    mask = capi.G_NUCL | capi.G_DELTAPP_ALL
+    H = G.shaped(G.run_correlator(capi.TWOPT, mask, v1, v1, src, totalT, Q1),
+                 capi.TWOPT, mask, src, totalT, Q1)
+    rows = [0, 6, 11]
+    R = _twopt_oracle(h1, src, totalT, Q1, rows)
+    assert G.rel_err(H[:, :, rows], R[:, :, rows]) < TOL64
+
+
+@pytest.mark.parametrize("split,tol", [(capi.SPLIT_BF16X6, 1e-6), (capi.SPLIT_BF16X3, 1e-4)])
+def test_twopt_mixed_error(props, split, tol):
+    """FP32 contraction + split-bf16 MFMA projection vs the FP64 oracle; error
+    per (group, channel, momentum) normalised per time slice."""
+    (t1, v1, h1), _ = props
+    t32, v32 = G.cast_f32(t1, v1)
+    src, totalT = [0, 0, 0, 0], 16
+    mask = capi.G_NUCL | capi.G_DELTAPP_ALL
+    if split == capi.SPLIT_BF16X6:
+        H = G.run_correlator(capi.TWOPT, mask, v32, v32, src, totalT, Q1, mode=capi.MODE_MIXED)
+    else:
+        moms = capi.momlist(Q1)
+        b = capi.Batch(capi.TWOPT, mask, [v32], [(src, totalT, 0, 0)], moms, capi.MODE_MIXED,
+                       split=split)
+        n, _, _ = b.info()
+        torch = G.torch_cuda()
+        out = torch.zeros((n, 2), dtype=torch.float64, device="cuda")
+        b.run(out)
+        h = G.host(out).reshape(n, 2)
+        H = h[:, 0] + 1j * h[:, 1]
+        b.close()
+    H = G.shaped(H, capi.TWOPT, mask, src, totalT, Q1)
+    rows = [0, 5, 15]
+    R = _twopt_oracle(h1, src, totalT, Q1, rows)
+    # normalise by the largest |C| over momenta of each (group, row, channel)
+    err = G.rel_err(H[:, :, rows], R[:, :, rows])
+    print(f"[gpu] mixed split={split} max normwise rel err {err:.3e}", flush=True)
+    assert err < tol
+
+
+def test_position_space(props):
+    (t1, v1, h1), (t2, v2, h2) = props
+    src, totalT = [0, 0, 0, 3], 4
+    H = G.run_correlator(capi.MESONS_NEW, 0, v1, v2, src, totalT, 0, space=capi.POSITION_SPACE)
+    V3 = 512
+    H = H.reshape(1, 1, 4, V3, 10)
+    for j in range(4):
+        t = O.row_time(L1, src, totalT, j)
+        ref = O.site_correlator(O.MESONS_NEW, L1, t, h1, h2)
+        assert G.rel_err(H[0, 0, j], ref, axis=0) < 1e-13
+
+
+def test_batch_matches_single(props):
+    (t1, v1, h1), (t2, v2, h2) = props
+    torch = G.torch_cuda()
+    mask = capi.G_NUCL | capi.G_DELTAPP_ALL
+    moms = capi.momlist(Q1)
+    jobs_in = [([0, 0, 0, 0], 16, 0), ([3, 5, 1, 13], 8, 1), ([7, 0, 2, 5], 5, 0)]
+    jobs, off = [], 0
+    for src, totalT, p in jobs_in:
+        jobs.append((src, totalT, p, off))
+        off += capi.correlator_size(capi.TWOPT, mask, src, totalT, len(moms))
+    b = capi.Batch(capi.TWOPT, mask, [v1, v2], jobs, moms, capi.MODE_F64)
+    n, nsites, nrows = b.info()
+    assert n == off and nrows == 16 + 8 + 5 and nsites == nrows * 512
+    out = torch.zeros((n, 2), dtype=torch.float64, device="cuda")
+    b.run(out)
+    h = G.host(out).reshape(n, 2)
+    Hb = h[:, 0] + 1j * h[:, 1]
+    for (src, totalT, p), (_, _, _, o) in zip(jobs_in, jobs):
+        v = (v1, v2)[p]
+        Hs = G.run_correlator(capi.TWOPT, mask, v, v, src, totalT, Q1)
+        np.testing.assert_allclose(Hb[o:o + len(Hs)], Hs, rtol=0, atol=1e-12 * np.abs(Hs).max())
+    b.close()
+
+
+def test_batch_graph_replays(props):
+    """hipGraph replays interleaved with eager work give the eager result each time."""
+    (t1, v1, h1), (t2, v2, h2) = props
+    torch = G.torch_cuda()
+    t32a, v32a = G.cast_f32(t1, v1)
+    t32b, v32b = G.cast_f32(t2, v2)
+    mask = capi.G_NUCL | capi.G_DELTAPP_ALL
+    moms = capi.momlist(Q1)
+    jobs = [([0, 0, 0, 0], 16, 0, 0)]
+    jobs.append(([3, 5, 1, 13], 8, 1, capi.correlator_size(capi.TWOPT, mask, [0, 0, 0, 0], 16, len(moms))))
+    eager = capi.Batch(capi.TWOPT, mask, [v32a, v32b], jobs, moms, capi.MODE_MIXED)
+    graph = capi.Batch(capi.TWOPT, mask, [v32a, v32b], jobs, moms, capi.MODE_MIXED,
+                       flags=capi.BATCH_GRAPH)
+    n, _, _ = eager.info()
+    ref = torch.zeros((n, 2), dtype=torch.float64, device="cuda")
+    eager.run(ref)
+    ref_h = G.host(ref)
+    out = torch.zeros((n, 2), dtype=torch.float64, device="cuda")
+    s = torch.cuda.Stream()
+    for rep in range(3):
+        out.fill_(float("nan"))
+        graph.run(out, stream=s)
+        s.synchronize()
+        assert np.array_equal(G.host(out), ref_h), f"replay {rep}"
+        junk = torch.randn(1 << 20, device="cuda").sum()  # eager work between replays
+        junk.item()
+    eager.close()
+    graph.close()
+
+
+def test_error_codes(props):
+    (t1, v1, h1), _ = props
+    torch = G.torch_cuda()
+    out = torch.zeros((10, 2), dtype=torch.float64, device="cuda")
+    with pytest.raises(capi.PlgError) as e:
+        capi.correlator(capi.MESONS, 0, v1, v1, [0, 0, 0, 99], 16, capi.momlist(1),
+                        capi.MODE_F64, capi.MOMENTUM_SPACE, capi.F64, out)
+    assert e.value.code == -1
+    with pytest.raises(capi.PlgError) as e:
+        capi.momlist(-1)
+    assert e.value.code == -1
+    # F64 view handed to the mixed mode
+    with pytest.raises(capi.PlgError):
+        capi.correlator(capi.MESONS, 0, v1, v1, [0, 0, 0, 0], 16, capi.momlist(1),
+                        capi.MODE_MIXED, capi.MOMENTUM_SPACE, capi.F64, out)
+    # a window the propagator view does not hold
+    tw, vw = G.make_prop(L1, 5, 0, 0, 0.5, t_first=2, nt=4)
+    with pytest.raises(capi.PlgError) as e:
+        capi.correlator(capi.MESONS, 0, vw, vw, [0, 0, 0, 2], 8, capi.momlist(1),
+                        capi.MODE_F64, capi.MOMENTUM_SPACE, capi.F64, out)
+    assert e.value.code == -7
+
+
+def test_window_view(props):
+    """a propagator stored only over its source window (t_first, nt) gives the
+    same correlator as the full-volume one"""
+    (t1, v1, h1), _ = props
+    src, totalT = [1, 1, 1, 13], 6     # times 13,14,15,0,1,2
+    tw, vw = G.make_prop(L1, 1234, 1, 0, 0.5, t_first=13, nt=6)
+    Hf = G.run_correlator(capi.MESONS16, 0, v1, v1, src, totalT, Q1)
+    Hw = G.run_correlator(capi.MESONS16, 0, vw, vw, src, totalT, Q1)
+    np.testing.assert_allclose(Hw, Hf, rtol=0, atol=1e-13 * np.abs(Hf).max())
