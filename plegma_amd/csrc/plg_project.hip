@@ -128,14 +128,14 @@ __global__ __launch_bounds__(PJ_THREADS) void k_project_f64(ProjGeom g, const cx
     }
     __syncthreads();
   }
-  // D layout of 16x16x4 f64: lane l, value v -> row 4*(l/16)+v, column l%16
+  // D layout of 16x16x4 f64 on gfx950: lane l, value v -> row l/16 + 4v, column l%16
 #pragma unroll
   for (int i = 0; i < MB; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int mr = mr0 + i * 16 + 4 * (lane >> 4) + v;
+        const int mr = mr0 + i * 16 + (lane >> 4) + 4 * v;
         const int64_t n = n0 + wave * 32 + j * 16 + (lane & 15);
         if (mr < Mreal && n < g.ncols) part[((int64_t)split * g.ncols + n) * Mreal + mr] = acc[i][j][v];
       }
