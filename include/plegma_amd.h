@@ -129,6 +129,10 @@ int plg_batch_create(int kind, int group_mask, const plg_prop *props, int nprops
  * out_precision F64) at its out_offset */
 int plg_batch_run(plg_batch b, void *out, void *stream);
 int plg_batch_info(plg_batch b, int64_t *out_elems, int64_t *nsites, int *nrows);
+/* one eager pass with a HIP event after every stage on `stream`: per-stage
+ * milliseconds and stage names (32 chars each) for up to `cap` stages */
+int plg_batch_profile(plg_batch b, void *out, void *stream, int cap, int *nstages, double *ms,
+                      char *names);
 int plg_batch_destroy(plg_batch b);
 
 /* ---- inputs ------------------------------------------------------------- */

@@ -17,7 +17,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
 
-MESONS, MESONS16, MESONS_NEW, MESONS_ALL, BARYONS, MESON_MESON = range(6)
+MESONS, MESONS16, MESONS_NEW, MESONS_ALL, BARYONS, MESON_MESON, TWOPT = range(7)
 G_NUCL = 1 << 0
 G_DELTA12 = (1 << 4, 1 << 5, 1 << 6)
 G_DELTA32 = (1 << 7, 1 << 8, 1 << 9)

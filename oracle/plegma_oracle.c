@@ -532,6 +532,12 @@ int orc_kind_nch(int kind, int mask, int *nd, int *ng) {
     *nd = 2; *ng = g; return 16;
   }
   case ORC_MESON_MESON: *nd = 1; *ng = 1; return NMM_MESON + NMM_MM;
+  case ORC_TWOPT: {
+    int g = 1;
+    for (int b = 0; b < 10; b++)
+      if (mask & (1 << b)) g++;
+    *nd = 1; *ng = g; return 16;
+  }
   }
   return -1;
 }
@@ -562,6 +568,24 @@ static int site_accum(int kind, int mask, const double *S1, const double *S2, do
   case ORC_MESONS_NEW: orc_site_mesons_new(S1, S2, acc); return 10;
   case ORC_MESONS_ALL: orc_site_mesons_all(S1, S2, acc); return 34;
   case ORC_MESON_MESON: orc_site_meson_meson(S1, S2, acc); return NMM_MESON + NMM_MM;
+  case ORC_TWOPT: {
+    /* the reference bodies on one propagator, first dataset only:
+     * group 0 = mesons (GAMMAS order), then each selected baryon group */
+    cplx S[NPROP], r[16];
+    load_site(S1, S);
+    for (int ip = 0; ip < 16; ip++) store(meson_channel(ip, S, S), acc + 2 * ip);
+    int g = 1;
+    for (int b = 0; b < 10; b++) {
+      if (!(mask & (1 << b))) continue;
+      if (b == 0) nton_body(S, S, r);
+      else if (b >= 4 && b <= 6) delta12_body(b - 4, S, S, r);
+      else if (b >= 7 && b <= 9) delta32_body(b - 7, S, r);
+      else return -1;
+      for (int c = 0; c < 16; c++) store(r[c], acc + 2 * (16 * g + c));
+      g++;
+    }
+    return 16 * g;
+  }
   case ORC_BARYONS: {
     /* reference runs one pass per group (PLEGMA_baryons.cuh:192-203);
      * accum per group is [dataset][16].  Here: acc[(d*ng + g)*16 + c]. */

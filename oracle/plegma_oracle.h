@@ -34,7 +34,8 @@ enum {
   ORC_MESONS_NEW = 2,  /* contractMesonsNew: 10 ch, S1 x conj(S2), 1 dataset */
   ORC_MESONS_ALL = 3,  /* contractMesonsAll: 34 ch, 1 dataset               */
   ORC_BARYONS = 4,     /* contractBaryons: groups selected by mask, 16 ch, 2 datasets */
-  ORC_MESON_MESON = 5  /* four-propagator meson-meson (defined by formula)   */
+  ORC_MESON_MESON = 5, /* four-propagator meson-meson (defined by formula)   */
+  ORC_TWOPT = 6        /* one propagator: 16 mesons then baryon groups, 1 dataset */
 };
 /* baryon group bits, in the reference's group order */
 enum {

@@ -29,7 +29,7 @@ STATUS = {0: "PLG_OK", -1: "PLG_ERR_INVALID", -2: "PLG_ERR_NOT_INIT", -3: "PLG_E
 EXPORTS = ["plg_init", "plg_finalize", "plg_last_error", "plg_version", "plg_momlist",
            "plg_window", "plg_row_time", "plg_kind_shape", "plg_correlator",
            "plg_correlator_size", "plg_batch_create", "plg_batch_run", "plg_batch_info",
-           "plg_batch_destroy", "plg_synth_prop", "plg_synth_prop_perturbed", "plg_cast_prop"]
+           "plg_batch_destroy", "plg_batch_profile", "plg_synth_prop", "plg_synth_prop_perturbed", "plg_cast_prop"]
 
 
 class PlgError(RuntimeError):
@@ -76,6 +76,8 @@ def lib():
         L.plg_batch_run.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.plg_batch_info.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), ip]
         L.plg_batch_destroy.argtypes = [C.c_void_p]
+        L.plg_batch_profile.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, ip,
+                                        C.POINTER(C.c_double), C.c_char_p]
         L.plg_synth_prop.argtypes = [C.POINTER(plg_prop), C.c_uint64, C.c_uint32, C.c_int,
                                      C.c_double, C.c_void_p]
         L.plg_synth_prop_perturbed.argtypes = [C.POINTER(plg_prop), C.c_uint64, C.c_uint32,
@@ -180,6 +182,16 @@ class Batch:
         a, b, c = C.c_int64(), C.c_int64(), C.c_int()
         check(lib().plg_batch_info(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
+
+    def profile(self, out, stream=None, cap=16):
+        """[(stage name, milliseconds)] of one eager pass with events between stages"""
+        n = C.c_int()
+        ms = (C.c_double * cap)()
+        names = C.create_string_buffer(32 * cap)
+        check(lib().plg_batch_profile(self.h, C.c_void_p(out.data_ptr()), _stream(stream), cap,
+                                      C.byref(n), ms, names))
+        raw = names.raw
+        return [(raw[32 * i:32 * i + 32].split(b"\0")[0].decode(), ms[i]) for i in range(n.value)]
 
     def run(self, out, stream=None):
         check(lib().plg_batch_run(self.h, C.c_void_p(out.data_ptr()), _stream(stream)))

@@ -474,9 +474,30 @@ static size_t part_bytes(const Plan &P) {
 }
 
 // enqueue the whole plan on a stream (capturable: no allocation, no memset)
-static int enqueue(Plan &P, void *c, void *part, void *out, int out_prec, hipStream_t st) {
+// optional stage profiling: an event is recorded after every stage
+struct StageLog {
+  std::vector<hipEvent_t> ev;
+  std::vector<std::string> name;
+  void mark(const std::string &n, hipStream_t st) {
+    hipEvent_t e;
+    hipEventCreate(&e);
+    hipEventRecord(e, st);
+    ev.push_back(e);
+    name.push_back(n);
+  }
+};
+
+static std::string launch_name(const Plan &P, const Launch &l) {
+  std::string t = P.T == PLG_F64 ? "f64" : "f32";
+  if (l.what == Launch::MESON) return "contract_mesons_" + t;
+  return std::string(l.mes ? "contract_twopt_" : "contract_baryons_") + t;
+}
+
+static int enqueue(Plan &P, void *c, void *part, void *out, int out_prec, hipStream_t st,
+                   StageLog *log = nullptr) {
   const int nrows = (int)P.rows.size();
   int rc;
+  if (log) log->mark("start", st);
   for (const Launch &l : P.launches) {
     if (P.T == PLG_F64) {
       auto *pd = (const PropDev<double> *)P.d_props;
@@ -494,6 +515,7 @@ static int enqueue(Plan &P, void *c, void *part, void *out, int out_prec, hipStr
                                   g_ctx.L[3], P.nch_c, l.ch0, P.tpb, st);
     }
     if (rc) return fail(rc, "contraction launch failed: " + std::string(hipGetErrorString(hipGetLastError())));
+    if (log) log->mark(launch_name(P, l), st);
   }
   const int nout = (int)P.csel.size();
   if (P.space == PLG_POSITION_SPACE) {
@@ -514,8 +536,10 @@ static int enqueue(Plan &P, void *c, void *part, void *out, int out_prec, hipStr
   if (P.T == PLG_F64) rc = launch_project_f64(g, (const cx<double> *)c, (cx<double> *)part, P.mtile, st);
   else rc = launch_project_bf16(g, (const cx<float> *)c, (cx<double> *)part, P.terms, g_ctx.d_tab3, P.mtile, st);
   if (rc) return fail(rc, "projection launch failed");
+  if (log) log->mark(P.T == PLG_F64 ? "project_f64" : (P.terms == 6 ? "project_bf16x6" : "project_bf16x3"), st);
   rc = launch_finalize(g, (const cx<double> *)part, P.d_orows, (int)P.orows.size(), P.nch_c, nout, P.d_csel,
                        P.d_choff, P.m_stride, out, out_prec, st);
+  if (log) log->mark("finalize", st);
   return rc ? fail(rc, "finalize launch failed") : PLG_OK;
 }
 
@@ -725,6 +749,30 @@ int plg_batch_run(plg_batch b, void *out, void *stream) {
   }
   HIPCHK(hipGraphLaunch(P.exec, st));
   return PLG_OK;
+}
+
+int plg_batch_profile(plg_batch b, void *out, void *stream, int cap, int *nstages, double *ms, char *names) {
+  if (!b || !out || !nstages) return fail(PLG_ERR_INVALID, "NULL argument");
+  Plan &P = b->plan;
+  hipStream_t st = (hipStream_t)stream;
+  StageLog log;
+  int rc = enqueue(P, P.c, P.part, out, PLG_F64, st, &log);
+  if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(PLG_ERR_HIP, "profile sync failed");
+  const int n = (int)log.ev.size() - 1;
+  if (!rc) {
+    *nstages = n;
+    for (int i = 0; i < n && i < cap; i++) {
+      float t = 0;
+      hipEventElapsedTime(&t, log.ev[i], log.ev[i + 1]);
+      if (ms) ms[i] = t;
+      if (names) {
+        std::memset(names + 32 * i, 0, 32);
+        std::strncpy(names + 32 * i, log.name[i + 1].c_str(), 31);
+      }
+    }
+  }
+  for (hipEvent_t e : log.ev) hipEventDestroy(e);
+  return rc;
 }
 
 int plg_batch_info(plg_batch b, int64_t *out_elems, int64_t *nsites, int *nrows) {
