@@ -126,10 +126,12 @@ def main():
         t64 = torch.empty((144, w * V3, 2), dtype=torch.float64, device=dev)
         a = capi.plg_prop(t64.data_ptr(), capi.F64, x0[3], w, 0, w * V3)
         capi.synth_prop(a, SEED, s, x0[3], LAMBDA)
+        # FP32 copy with per-time-slice power-of-two exponents (exact rescaling)
         t32 = torch.empty((144, w * V3, 2), dtype=torch.float32, device=dev)
-        b = capi.plg_prop(t32.data_ptr(), capi.F32, x0[3], w, 0, w * V3)
+        e32 = torch.empty(w, dtype=torch.int32, device=dev)
+        b = capi.plg_prop(t32.data_ptr(), capi.F32, x0[3], w, 0, w * V3, e32.data_ptr())
         capi.cast_prop(b, a)
-        keep += [t64, t32]
+        keep += [t64, t32, e32]
         v64.append(a)
         v32.append(b)
     torch.cuda.synchronize()

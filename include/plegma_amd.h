@@ -61,7 +61,11 @@ enum plg_kind {
  *   complex units, plane = (t - t_first) mod T, plane < nt,
  *   i = ((mu*4 + nu)*3 + a)*3 + b.
  * PLEGMA's layout (lib/PLEGMA_Propagator.cu:39-44, lib/PLEGMA_Field.cu:225-264)
- * is comp_stride = V, t_first = 0, nt = T. */
+ * is comp_stride = V, t_first = 0, nt = T, plane_exp = NULL.
+ * plane_exp (optional, device, nt ints): the stored entries of plane p are the
+ * propagator times 2^-plane_exp[p] (an exact power-of-two rescaling that keeps
+ * FP32 data of fast-decaying propagators inside the normal range).  Results
+ * are returned unscaled. */
 typedef struct {
   const void *data;    /* device pointer */
   int precision;       /* plg_precision */
@@ -69,6 +73,7 @@ typedef struct {
   int nt;
   int pad_;
   int64_t comp_stride; /* complex elements */
+  const int *plane_exp;
 } plg_prop;
 
 /* ---- runtime ------------------------------------------------------------ */
@@ -137,13 +142,17 @@ int plg_batch_destroy(plg_batch b);
 
 /* ---- inputs ------------------------------------------------------------- */
 /* seeded synthetic propagator: Philox4x32-10 + Box-Muller N(0,1) entries,
- * scaled by exp(-lambda*|t - t0|); counter (site + V3*t, component, stream) */
+ * scaled by exp(-lambda*|t - t0|); counter (site + V3*t, component, stream).
+ * With dst->plane_exp set, e = round(-lambda |t - t0| / ln 2) is written per
+ * plane and entries are stored times 2^-e. */
 int plg_synth_prop(const plg_prop *dst, uint64_t seed, uint32_t stream_id, int t0, double lambda,
                    void *stream);
 /* base + eps * perturbation, both seeded, then the same scaling */
 int plg_synth_prop_perturbed(const plg_prop *dst, uint64_t seed, uint32_t base_stream,
                              uint32_t pert_stream, double eps, int t0, double lambda, void *stream);
-/* round-to-nearest FP64 -> FP32 copy over dst's planes */
+/* round-to-nearest FP64 -> FP32 copy over dst's planes; when dst->plane_exp
+ * is set, it is written first (e = floor(log2 max |entry|) of each plane) and
+ * the copy stores entry * 2^-e */
 int plg_cast_prop(const plg_prop *dst, const plg_prop *src, void *stream);
 
 #ifdef __cplusplus

@@ -40,7 +40,8 @@ class PlgError(RuntimeError):
 
 class plg_prop(C.Structure):
     _fields_ = [("data", C.c_void_p), ("precision", C.c_int), ("t_first", C.c_int),
-                ("nt", C.c_int), ("pad_", C.c_int), ("comp_stride", C.c_int64)]
+                ("nt", C.c_int), ("pad_", C.c_int), ("comp_stride", C.c_int64),
+                ("plane_exp", C.c_void_p)]
 
 
 class plg_job(C.Structure):

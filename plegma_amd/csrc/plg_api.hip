@@ -267,7 +267,20 @@ static int build_plan(Plan &P, int kind, int mask, const std::vector<plg_prop> &
     for (int b : {0, 7, 8, 9})
       if (m & (1 << b)) gsel.push_back((int)(std::find(bits.begin(), bits.end(), b) - bits.begin()));
   };
-  std::vector<std::pair<int, int>> rowspec;  // per job: rows added
+  std::vector<int> obits;  // reference group bit of each output baryon group
+  for (int b : {0, 7, 8, 9})
+    if (mask & (1 << b)) obits.push_back(b);
+  // result scale of an out row: the contraction is multilinear in the
+  // propagators, ka/kb = how many factors of each view it holds
+  auto set_scale = [&](OutRowDev &o, const plg_prop &a, const plg_prop &b, int t, int ka, int kb) {
+    const int T = g_ctx.L[3];
+    o.ea = ka ? a.plane_exp : nullptr;
+    o.eb = kb ? b.plane_exp : nullptr;
+    o.pla = ((t - a.t_first) % T + T) % T;
+    o.plb = ((t - b.t_first) % T + T) % T;
+    o.ka = ka;
+    o.kb = kb;
+  };
   P.rows.clear();
   P.orows.clear();
   P.launches.clear();
@@ -346,6 +359,8 @@ static int build_plan(Plan &P, int kind, int mask, const std::vector<plg_prop> &
           o.ch_base = 0;
           o.sx = J.src[0]; o.sy = J.src[1]; o.sz = J.src[2];
           o.base = J.out_off + ((int64_t)d * lt + j) * inner * nch;
+          const plg_prop &pd = d == 0 ? pa : pb;
+          set_scale(o, pd, pd, row_time(g_ctx.L, J.src, w, j), 2, 0);
           P.orows.push_back(o);
         }
       break;
@@ -358,6 +373,7 @@ static int build_plan(Plan &P, int kind, int mask, const std::vector<plg_prop> &
         o.crow = crow0 + j;
         o.sx = J.src[0]; o.sy = J.src[1]; o.sz = J.src[2];
         o.base = J.out_off + (int64_t)j * inner * nch;
+        set_scale(o, pa, pb, row_time(g_ctx.L, J.src, w, j), 1, 1);
         P.orows.push_back(o);
       }
       break;
@@ -375,6 +391,12 @@ static int build_plan(Plan &P, int kind, int mask, const std::vector<plg_prop> &
             o.ch_base = (dc * gcomp + gsel[g]) * 16;
             o.sx = J.src[0]; o.sy = J.src[1]; o.sz = J.src[2];
             o.base = J.out_off + (int64_t)g * gstride + ((int64_t)d * lt + j) * inner * nch;
+            // proton of dataset d: the d-th propagator twice and the other once;
+            // Delta of dataset d: the d-th propagator three times
+            const bool nucl = obits[g] == 0;
+            const int k1 = nucl ? (d == 0 ? 2 : 1) : (d == 0 ? 3 : 0);
+            const int k2 = nucl ? (d == 0 ? 1 : 2) : (d == 0 ? 0 : 3);
+            set_scale(o, pa, pb, row_time(g_ctx.L, J.src, w, j), k1, k2);
             P.orows.push_back(o);
           }
       }
@@ -388,6 +410,7 @@ static int build_plan(Plan &P, int kind, int mask, const std::vector<plg_prop> &
           o.ch_base = g == 0 ? 0 : 16 + gsel[g - 1] * 16;
           o.sx = J.src[0]; o.sy = J.src[1]; o.sz = J.src[2];
           o.base = J.out_off + (int64_t)g * gstride + (int64_t)j * inner * nch;
+          set_scale(o, pa, pa, row_time(g_ctx.L, J.src, w, j), g == 0 ? 2 : 3, 0);
           P.orows.push_back(o);
         }
       }

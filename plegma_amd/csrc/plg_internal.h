@@ -39,7 +39,17 @@ struct OutRowDev {
   int sx, sy, sz;
   int pad;
   int64_t base;  // H offset of (m = 0, k = 0)
+  // result scale 2^(ka*ea[pla] + kb*eb[plb]) undoes the propagators' plane
+  // exponents (ea/eb NULL: no scaling)
+  const int *ea, *eb;
+  int pla, plb, ka, kb;
 };
+__device__ __forceinline__ double row_scale(const OutRowDev &R) {
+  int e = 0;
+  if (R.ea) e += R.ka * R.ea[R.pla];
+  if (R.eb) e += R.kb * R.eb[R.plb];
+  return ldexp(1.0, e);
+}
 
 template <typename T>
 int launch_meson(int set, const PropDev<T> *props, const RowDev *rows, int nrows, cx<T> *out,

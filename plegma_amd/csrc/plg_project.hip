@@ -343,7 +343,8 @@ __global__ void k_finalize(ProjGeom g, const cx<double> *__restrict__ part, cons
   int ph = g.mom_scaled[3 * m] * R.sx + g.mom_scaled[3 * m + 1] * R.sy + g.mom_scaled[3 * m + 2] * R.sz;
   ph %= g.M;
   if (ph < 0) ph += g.M;
-  const double cs = g.cos_tab[ph], sn = g.sin_tab[ph];
+  const double sc = row_scale(R);
+  const double cs = g.cos_tab[ph] * sc, sn = g.sin_tab[ph] * sc;
   cx<TO> o;
   o.re = (TO)(re * cs - im * sn);
   o.im = (TO)(re * sn + im * cs);
@@ -375,9 +376,10 @@ __global__ void k_scatter_pos(const cx<TI> *__restrict__ c, int64_t V3, const Ou
   const int64_t x = e / nout;
   const int k = (int)(e % nout);
   const cx<TI> v = c[((int64_t)R.crow * nch_c + R.ch_base + csel[k]) * V3 + x];
+  const double sc = row_scale(R);
   cx<TO> o;
-  o.re = (TO)v.re;
-  o.im = (TO)v.im;
+  o.re = (TO)(v.re * sc);
+  o.im = (TO)(v.im * sc);
   out[R.base + ch_off[k] + x * x_stride] = o;
 }
 

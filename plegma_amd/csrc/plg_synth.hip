@@ -41,15 +41,27 @@ __device__ __forceinline__ void normal_pair(uint64_t seed, uint32_t stream, uint
   im = rad * sn;
 }
 
+// plane exponent of the analytic envelope exp(-lambda |t - t0|)
+__device__ __forceinline__ int envelope_exp(int t, int t0, double lambda) {
+  return (int)rint(-lambda * fabs((double)(t - t0)) / 0.6931471805599453);
+}
+
 template <typename T>
 __global__ void k_synth(cx<T> *__restrict__ data, int64_t cs, int t_first, int nt, int64_t V3, int T_lat,
-                        uint64_t seed, uint32_t sa, uint32_t sb, double eps, int t0, double lambda) {
+                        uint64_t seed, uint32_t sa, uint32_t sb, double eps, int t0, double lambda,
+                        int *__restrict__ plane_exp) {
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int pl = blockIdx.y;
   const int comp = blockIdx.z;
   if (x >= V3) return;
   const int t = (t_first + pl) % T_lat;
-  const double scale = exp(-lambda * fabs((double)(t - t0)));
+  double scale = exp(-lambda * fabs((double)(t - t0)));
+  if (plane_exp) {
+    const int e = envelope_exp(t, t0, lambda);
+    if (x == 0 && comp == 0) plane_exp[pl] = e;
+    // exp(-lambda|dt|) * 2^-e, evaluated without forming the tiny factor
+    scale = exp(-lambda * fabs((double)(t - t0)) - (double)e * 0.6931471805599453);
+  }
   const uint32_t site = (uint32_t)(x + V3 * t);
   double re, im;
   normal_pair(seed, sa, site, (uint32_t)comp, re, im);
@@ -69,32 +81,75 @@ int launch_synth(const plg_prop &dst, const int L[4], uint64_t seed, uint32_t sa
                  int t0, double lambda, hipStream_t st) {
   const int64_t V3 = (int64_t)L[0] * L[1] * L[2];
   dim3 block(256), grid((unsigned)((V3 + 255) / 256), (unsigned)dst.nt, 144);
+  int *pe = const_cast<int *>(dst.plane_exp);
   if (dst.precision == PLG_F64)
     k_synth<double><<<grid, block, 0, st>>>((cx<double> *)dst.data, dst.comp_stride, dst.t_first, dst.nt, V3,
-                                            L[3], seed, sa, sb, eps, t0, lambda);
+                                            L[3], seed, sa, sb, eps, t0, lambda, pe);
   else
     k_synth<float><<<grid, block, 0, st>>>((cx<float> *)dst.data, dst.comp_stride, dst.t_first, dst.nt, V3,
-                                           L[3], seed, sa, sb, eps, t0, lambda);
+                                           L[3], seed, sa, sb, eps, t0, lambda, pe);
   return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;
+}
+
+// per-plane exponent floor(log2 max|entry|) of an FP64 view's planes (the
+// planes of dst), by a max-reduction over components and sites
+__global__ void k_fill_int(int *p, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_plane_exp(int *__restrict__ pexp, int dtf, const cx<double> *__restrict__ src, int64_t scs,
+                            int stf, int64_t V3, int T_lat) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int pl = blockIdx.y, comp = blockIdx.z;
+  int e = INT_MIN;
+  if (x < V3) {
+    const int t = (dtf + pl) % T_lat;
+    int spl = t - stf;
+    spl += spl < 0 ? T_lat : 0;
+    const cx<double> v = src[(int64_t)comp * scs + (int64_t)spl * V3 + x];
+    const double a = fmax(fabs(v.re), fabs(v.im));
+    if (a > 0) e = ilogb(a);
+  }
+  // wave reduction, then one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) e = max(e, __shfl_xor(e, off));
+  if ((threadIdx.x & 63) == 0 && e != INT_MIN) atomicMax(&pexp[pl], e);
 }
 
 // dst (F32) <- src (F64), round to nearest, over dst's planes
 __global__ void k_cast(cx<float> *__restrict__ dst, int64_t dcs, int dtf, const cx<double> *__restrict__ src,
-                       int64_t scs, int stf, int64_t V3, int T_lat) {
+                       int64_t scs, int stf, int64_t V3, int T_lat, int *__restrict__ pexp) {
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int pl = blockIdx.y, comp = blockIdx.z;
   if (x >= V3) return;
   const int t = (dtf + pl) % T_lat;
   int spl = t - stf;
   spl += spl < 0 ? T_lat : 0;
-  const cx<double> v = src[(int64_t)comp * scs + (int64_t)spl * V3 + x];
+  cx<double> v = src[(int64_t)comp * scs + (int64_t)spl * V3 + x];
+  if (pexp) {
+    int e = pexp[pl];
+    if (e == INT_MIN) e = 0;          // all-zero plane
+    v.re = ldexp(v.re, -e);           // exact power-of-two rescaling
+    v.im = ldexp(v.im, -e);
+  }
   dst[(int64_t)comp * dcs + (int64_t)pl * V3 + x] = cmake<float>((float)v.re, (float)v.im);
+}
+
+__global__ void k_fix_zero_exp(int *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && p[i] == INT_MIN) p[i] = 0;
 }
 
 int launch_cast(const plg_prop &dst, const plg_prop &src, int64_t V3, int T_lat, hipStream_t st) {
   dim3 block(256), grid((unsigned)((V3 + 255) / 256), (unsigned)dst.nt, 144);
+  int *pe = const_cast<int *>(dst.plane_exp);
+  if (pe) {
+    k_fill_int<<<(dst.nt + 255) / 256, 256, 0, st>>>(pe, dst.nt, INT_MIN);
+    k_plane_exp<<<grid, block, 0, st>>>(pe, dst.t_first, (const cx<double> *)src.data, src.comp_stride,
+                                        src.t_first, V3, T_lat);
+  }
   k_cast<<<grid, block, 0, st>>>((cx<float> *)dst.data, dst.comp_stride, dst.t_first, (const cx<double> *)src.data,
-                                 src.comp_stride, src.t_first, V3, T_lat);
+                                 src.comp_stride, src.t_first, V3, T_lat, pe);
+  if (pe) k_fix_zero_exp<<<(dst.nt + 255) / 256, 256, 0, st>>>(pe, dst.nt);
   return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;
 }
 

@@ -26,12 +26,17 @@ def make_prop(L, seed, stream, t0, lam, prec=capi.F64, t_first=0, nt=None):
     return t, v
 
 
-def cast_f32(t64, v64):
+def cast_f32(t64, v64, scaled=False):
+    """FP32 copy; with scaled=True the copy carries per-plane exponents."""
     torch = torch_cuda()
     t = torch.empty(t64.shape, dtype=torch.float32, device="cuda")
     v = capi.plg_prop(t.data_ptr(), capi.F32, v64.t_first, v64.nt, 0, v64.comp_stride)
+    e = None
+    if scaled:
+        e = torch.empty(v64.nt, dtype=torch.int32, device="cuda")
+        v.plane_exp = e.data_ptr()
     capi.cast_prop(v, v64)
-    return t, v
+    return (t, e), v
 
 
 def host(t):
