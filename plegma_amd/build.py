@@ -16,7 +16,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libplegma_amd.so")
-SOURCES = ["plg_api.hip", "plg_contract.hip", "plg_project.hip", "plg_synth.hip"]
+SOURCES = ["plg_api.hip", "plg_contract.hip", "plg_project.hip", "plg_project_bf16.hip",
+           "plg_synth.hip"]
 ARCH = "gfx950"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

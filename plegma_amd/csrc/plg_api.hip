@@ -42,7 +42,7 @@ struct Context {
   int device = 0;
   int M = 1;                     // lcm(Lx,Ly,Lz)
   double *d_cos = nullptr, *d_sin = nullptr;
-  uint16_t *d_tab3 = nullptr;    // [M][6] bf16 pieces of cos, sin
+  uint32_t *d_wtab = nullptr;    // [M][6] packed bf16 operand pairs (see plg_init)
   // grow-only workspace
   void *ws_c = nullptr;
   size_t ws_c_bytes = 0;
@@ -183,7 +183,7 @@ struct Plan {
   int64_t out_elems = 0;
   int64_t nsites = 0;
   // tuning
-  int mtile = 64, nsplit = 1, tpb = 1;
+  int mtile = 64, mbw = 9, nsplit = 1, tpb = 1;
   // device tables (owned by the plan)
   void *d_props = nullptr;
   RowDev *d_rows = nullptr;
@@ -452,8 +452,15 @@ template <typename A> static int upload(A **dst, const std::vector<A> &v, hipStr
 static void choose_tiles(Plan &P) {
   const int64_t ncols = (int64_t)P.rows.size() * P.nch_c;
   const int Mreal = 2 * std::max(P.nmom, 1);
-  P.mtile = Mreal <= 96 ? 32 : 64;
-  const int64_t tiles = ((ncols + 127) / 128) * ((Mreal + P.mtile - 1) / P.mtile);
+  int64_t tiles;
+  if (P.T == PLG_F64) {
+    P.mtile = Mreal <= 96 ? 32 : 64;
+    tiles = ((ncols + 127) / 128) * ((Mreal + P.mtile - 1) / P.mtile);
+  } else {
+    P.mbw = pick_mbw(std::max(P.nmom, 1));
+    const int nblk = (Mreal + 15) / 16;
+    tiles = ((ncols + 31) / 32) * ((nblk + 4 * P.mbw - 1) / (4 * P.mbw));
+  }
   int64_t ns = (2048 + tiles - 1) / tiles;
   ns = std::max<int64_t>(1, std::min<int64_t>(ns, g_ctx.V3 / 512));
   P.nsplit = (int)std::max<int64_t>(1, ns);
@@ -557,7 +564,7 @@ static int enqueue(Plan &P, void *c, void *part, void *out, int out_prec, hipStr
   g.cos_tab = g_ctx.d_cos;
   g.sin_tab = g_ctx.d_sin;
   if (P.T == PLG_F64) rc = launch_project_f64(g, (const cx<double> *)c, (cx<double> *)part, P.mtile, st);
-  else rc = launch_project_bf16(g, (const cx<float> *)c, (cx<double> *)part, P.terms, g_ctx.d_tab3, P.mtile, st);
+  else rc = launch_project_bf16(g, (const cx<float> *)c, (cx<double> *)part, P.terms, g_ctx.d_wtab, P.mbw, st);
   if (rc) return fail(rc, "projection launch failed");
   if (log) log->mark(P.T == PLG_F64 ? "project_f64" : (P.terms == 6 ? "project_bf16x6" : "project_bf16x3"), st);
   rc = launch_finalize(g, (const cx<double> *)part, P.d_orows, (int)P.orows.size(), P.nch_c, nout, P.d_csel,
@@ -597,21 +604,28 @@ int plg_init(const int L[4], int device) {
   int M = std::lcm(std::lcm(L[0], L[1]), L[2]);
   g_ctx.M = M;
   // exact phase table: cos/sin(2 pi k / M) for integer k, in FP64 on the host
+  // and its bf16 pieces packed as (re, im) operand pairs of the two real rows
+  // of a momentum: W[k][p] = cos_p | sin_p << 16, W[k][3+p] = -sin_p | cos_p << 16
   std::vector<double> c(M), s(M);
-  std::vector<uint16_t> t3((size_t)M * 6);
+  std::vector<uint32_t> wt((size_t)M * 6);
   for (int k = 0; k < M; k++) {
     const double th = 2.0 * M_PI * (double)k / (double)M;
     c[k] = std::cos(th);
     s[k] = std::sin(th);
-    split3_host(c[k], &t3[(size_t)6 * k]);
-    split3_host(s[k], &t3[(size_t)6 * k + 3]);
+    uint16_t cp[3], sp[3];
+    split3_host(c[k], cp);
+    split3_host(s[k], sp);
+    for (int p = 0; p < 3; p++) {
+      wt[(size_t)6 * k + p] = (uint32_t)cp[p] | ((uint32_t)sp[p] << 16);
+      wt[(size_t)6 * k + 3 + p] = (uint32_t)(sp[p] ^ 0x8000u) | ((uint32_t)cp[p] << 16);
+    }
   }
   HIPCHK(hipMalloc((void **)&g_ctx.d_cos, sizeof(double) * M));
   HIPCHK(hipMalloc((void **)&g_ctx.d_sin, sizeof(double) * M));
-  HIPCHK(hipMalloc((void **)&g_ctx.d_tab3, sizeof(uint16_t) * 6 * M));
+  HIPCHK(hipMalloc((void **)&g_ctx.d_wtab, sizeof(uint32_t) * 6 * M));
   HIPCHK(hipMemcpy(g_ctx.d_cos, c.data(), sizeof(double) * M, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(g_ctx.d_sin, s.data(), sizeof(double) * M, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(g_ctx.d_tab3, t3.data(), sizeof(uint16_t) * 6 * M, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g_ctx.d_wtab, wt.data(), sizeof(uint32_t) * 6 * M, hipMemcpyHostToDevice));
   g_ctx.init = true;
   return PLG_OK;
 }
@@ -619,7 +633,7 @@ int plg_init(const int L[4], int device) {
 int plg_finalize(void) {
   if (!g_ctx.init) return PLG_OK;
   hipDeviceSynchronize();
-  for (void *p : {(void *)g_ctx.d_cos, (void *)g_ctx.d_sin, (void *)g_ctx.d_tab3, g_ctx.ws_c, g_ctx.ws_part})
+  for (void *p : {(void *)g_ctx.d_cos, (void *)g_ctx.d_sin, (void *)g_ctx.d_wtab, g_ctx.ws_c, g_ctx.ws_part})
     if (p) hipFree(p);
   g_ctx = Context();
   return PLG_OK;

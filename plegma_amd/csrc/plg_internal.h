@@ -74,9 +74,11 @@ struct ProjGeom {
 };
 int launch_project_f64(const ProjGeom &g, const cx<double> *c, cx<double> *part, int mtile,
                        hipStream_t st);
+// wtab: M x 6 packed words (cos_p | sin_p << 16, -sin_p | cos_p << 16) for
+// the three bf16 pieces p; mbw: 16-row blocks per wave (1, 2, 3, 5, 9)
 int launch_project_bf16(const ProjGeom &g, const cx<float> *c, cx<double> *part, int split_terms,
-                        const uint16_t *tab_split /* M*2*3 bf16: cos,sin x3 pieces */,
-                        int mtile, hipStream_t st);
+                        const uint32_t *wtab, int mbw, hipStream_t st);
+int pick_mbw(int nmom);
 // sum over splits, source phase, scatter into the H layout
 // nch_c: channels per computed row; nout: output channels per out row
 int launch_finalize(const ProjGeom &g, const cx<double> *part, const OutRowDev *orows, int norows,

@@ -1,4 +1,5 @@
-// plg_project.hip -- momentum projection as a dense phase-matrix GEMM on MFMA.
+// plg_project.hip -- momentum projection as a dense phase-matrix GEMM on MFMA
+// (FP64 path), and the finalize / position-space scatter kernels.
 //
 // Replaces the fused naive DFT of lib/kernels/PLEGMA_kernel_utils.cuh:820-860
 // (per site and momentum a sincos, an LDS tree reduction and a host sum of
@@ -11,22 +12,17 @@
 //   A[2m  ][2x] =  cos   A[2m  ][2x+1] = sin
 //   A[2m+1][2x] = -sin   A[2m+1][2x+1] = cos      (theta = 2 pi k / M)
 // A is generated per K tile from the exact integer phase index
-//   k = sum_i n_i (x_i) (M / L_i) mod M,  M = lcm(Lx, Ly, Lz),
+//   k = sum_i n_i x_i (M / L_i) mod M,  M = lcm(Lx, Ly, Lz),
 // and the FP64 table cos/sin(2 pi k / M); the source offset is applied later
 // in FP64 as a per-momentum constant phase (the finalize kernel).
 //
 //  k_project_f64   v_mfma_f64_16x16x4_f64, FP64 operands and accumulation.
-//  k_project_bf16  v_mfma_f32_16x16x32_bf16 on split operands a = a0+a1+a2,
-//                  b = b0+b1+b2 (bf16 pieces); 3 products (a0b0+a0b1+a1b0) or
-//                  6 (+a0b2+a1b1+a2b0); FP32 MFMA accumulation flushed into FP64
-//                  registers every FLUSH k-steps; FP64 partials per split.
+//  (the split-bf16 projection of the mixed path is in plg_project_bf16.hip)
 #include "plg_internal.h"
 
 namespace plg {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
-typedef float f4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
 constexpr int PJ_THREADS = 256;
 constexpr int PJ_NT = 128;  // columns per workgroup (4 waves x 32)
@@ -55,9 +51,8 @@ __device__ __forceinline__ void split_range(const ProjGeom &g, int s, int64_t &b
   e = b + per < g.V3 ? b + per : g.V3;
 }
 
-// ------------------------------------------------------------------ FP64
-constexpr int F64_KT = 16;                 // sites per K tile
-constexpr int F64_LD = 2 * F64_KT + 1;     // padded LDS row (doubles)
+constexpr int F64_KT = 16;              // sites per K tile
+constexpr int F64_LD = 2 * F64_KT + 1;  // padded LDS row (doubles)
 
 template <int MT>
 __global__ __launch_bounds__(PJ_THREADS) void k_project_f64(ProjGeom g, const cx<double> *__restrict__ c,
@@ -155,174 +150,8 @@ int launch_project_f64(const ProjGeom &g, const cx<double> *c, cx<double> *part,
   return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;
 }
 
-// ------------------------------------------------------------------ split bf16
-constexpr int BF_KT = 16;               // sites per K tile = one 16x16x32 k-step
-constexpr int BF_LD = 2 * BF_KT + 8;    // padded LDS row (bf16 elements)
-
-__device__ __forceinline__ uint16_t bf16_rn(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf16_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-__device__ __forceinline__ void split3(float v, uint16_t &p0, uint16_t &p1, uint16_t &p2) {
-  p0 = bf16_rn(v);
-  const float r1 = v - bf16_f(p0);
-  p1 = bf16_rn(r1);
-  const float r2 = r1 - bf16_f(p1);
-  p2 = bf16_rn(r2);
-}
-
-template <int MT, int TERMS, int FLUSH>
-__global__ __launch_bounds__(PJ_THREADS) void k_project_bf16(ProjGeom g, const cx<float> *__restrict__ c,
-                                                             const uint16_t *__restrict__ tab,
-                                                             double *__restrict__ part) {
-  constexpr int MB = MT / 16;
-  constexpr int NP = TERMS == 3 ? 2 : 3;  // pieces needed
-  __shared__ __align__(16) uint16_t sB[NP][PJ_NT][BF_LD];
-  __shared__ __align__(16) uint16_t sA[NP][MT][BF_LD];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t n0 = (int64_t)blockIdx.x * PJ_NT;
-  const int mr0 = blockIdx.y * MT;
-  const int split = blockIdx.z;
-  int64_t xb, xe;
-  split_range(g, split, xb, xe);
-  const int Mreal = 2 * g.nmom;
-
-  f4 acc[MB][2];
-  double acc64[MB][2][4];
-#pragma unroll
-  for (int i = 0; i < MB; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++) {
-      acc[i][j] = f4{0, 0, 0, 0};
-#pragma unroll
-      for (int v = 0; v < 4; v++) acc64[i][j][v] = 0;
-    }
-
-  int step = 0;
-  for (int64_t xt = xb; xt < xe; xt += BF_KT) {
-    {
-      const int s = tid & 15;
-      const int64_t x = xt + s;
-#pragma unroll
-      for (int i = 0; i < PJ_NT / 16; i++) {
-        const int col = (tid >> 4) + 16 * i;
-        const int64_t n = n0 + col;
-        cx<float> v = cmake<float>(0, 0);
-        if (x < xe && n < g.ncols) v = c[n * g.V3 + x];
-        uint16_t r0, r1, r2, i0, i1, i2;
-        split3(v.re, r0, r1, r2);
-        split3(v.im, i0, i1, i2);
-        sB[0][col][2 * s] = r0;
-        sB[0][col][2 * s + 1] = i0;
-        sB[1][col][2 * s] = r1;
-        sB[1][col][2 * s + 1] = i1;
-        if (NP == 3) {
-          sB[NP - 1][col][2 * s] = r2;
-          sB[NP - 1][col][2 * s + 1] = i2;
-        }
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < (MT * BF_KT) / PJ_THREADS; i++) {
-      const int e = tid + PJ_THREADS * i;
-      const int mr = e / BF_KT, s = e % BF_KT;
-      const int m = (mr0 + mr) >> 1, ri = (mr0 + mr) & 1;
-      const int64_t x = xt + s;
-      // tab: [k][6] = cos pieces 0..2, sin pieces 0..2
-      uint16_t a0[3] = {0, 0, 0}, a1[3] = {0, 0, 0};
-      if (m < g.nmom && x < xe) {
-        const int k = phase_index(g.mom_scaled, m, site_xyz(x, g), g.M);
-        const uint16_t *t = tab + 6 * k;
-#pragma unroll
-        for (int p = 0; p < 3; p++) {
-          const uint16_t cs = t[p], sn = t[3 + p];
-          a0[p] = ri ? (uint16_t)(sn ^ 0x8000u) : cs;
-          a1[p] = ri ? cs : sn;
-        }
-      }
-#pragma unroll
-      for (int p = 0; p < NP; p++) {
-        sA[p][mr][2 * s] = a0[p];
-        sA[p][mr][2 * s + 1] = a1[p];
-      }
-    }
-    __syncthreads();
-    {
-      // 16x16x32: lane l holds A[l%16][8*(l/16) .. +7], B[8*(l/16) .. +7][l%16]
-      const int kk = 8 * (lane >> 4);
-      bf8 bf[NP][2];
-#pragma unroll
-      for (int p = 0; p < NP; p++)
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-          bf[p][j] = *reinterpret_cast<const bf8 *>(&sB[p][wave * 32 + j * 16 + (lane & 15)][kk]);
-#pragma unroll
-      for (int i = 0; i < MB; i++) {
-        bf8 af[NP];
-#pragma unroll
-        for (int p = 0; p < NP; p++) af[p] = *reinterpret_cast<const bf8 *>(&sA[p][i * 16 + (lane & 15)][kk]);
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-          f4 a = acc[i][j];
-          if (TERMS == 6) {
-            a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[2], bf[0][j], a, 0, 0, 0);
-            a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1], bf[1][j], a, 0, 0, 0);
-            a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0], bf[2][j], a, 0, 0, 0);
-          }
-          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1], bf[0][j], a, 0, 0, 0);
-          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0], bf[1][j], a, 0, 0, 0);
-          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0], bf[0][j], a, 0, 0, 0);
-          acc[i][j] = a;
-        }
-      }
-    }
-    if (++step == FLUSH) {
-      step = 0;
-#pragma unroll
-      for (int i = 0; i < MB; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-#pragma unroll
-          for (int v = 0; v < 4; v++) acc64[i][j][v] += (double)acc[i][j][v];
-          acc[i][j] = f4{0, 0, 0, 0};
-        }
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int i = 0; i < MB; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int mr = mr0 + i * 16 + 4 * (lane >> 4) + v;
-        const int64_t n = n0 + wave * 32 + j * 16 + (lane & 15);
-        if (mr < Mreal && n < g.ncols)
-          part[((int64_t)split * g.ncols + n) * Mreal + mr] = acc64[i][j][v] + (double)acc[i][j][v];
-      }
-}
-
-int launch_project_bf16(const ProjGeom &g, const cx<float> *c, cx<double> *part, int terms,
-                        const uint16_t *tab, int mtile, hipStream_t st) {
-  const int Mreal = 2 * g.nmom;
-  dim3 block(PJ_THREADS);
-  dim3 grid((unsigned)((g.ncols + PJ_NT - 1) / PJ_NT), (unsigned)((Mreal + mtile - 1) / mtile),
-            (unsigned)g.nsplit);
-  double *pd = reinterpret_cast<double *>(part);
-  if (mtile == 32) {
-    if (terms == 6) k_project_bf16<32, 6, 8><<<grid, block, 0, st>>>(g, c, tab, pd);
-    else k_project_bf16<32, 3, 8><<<grid, block, 0, st>>>(g, c, tab, pd);
-  } else {
-    if (terms == 6) k_project_bf16<64, 6, 8><<<grid, block, 0, st>>>(g, c, tab, pd);
-    else k_project_bf16<64, 3, 8><<<grid, block, 0, st>>>(g, c, tab, pd);
-  }
-  return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;
-}
-
 // ------------------------------------------------------------------ finalize
-// out[base + ch_off[k] + m*m_stride] = exp(+i 2pi n_m . x0 / L) * sum_s part[s][n][m],
+// out[base + ch_off[k] + m*m_stride] = scale * exp(+i 2pi n_m . x0 / L) * sum_s part[s][n][m],
 // n = crow*nch_c + ch_base + csel[k].  With the GEMM's exp(-i 2pi n.x/L) this
 // is exp(-i 2pi n.(x - x0)/L) (PLEGMA_kernel_utils.cuh:833-838, sign -1).
 template <typename TO>

@@ -1,0 +1,284 @@
+// plg_project_bf16.hip -- mixed-precision momentum projection on MFMA.
+//
+// Same GEMM as the FP64 projection (interleaved re/im real form of
+// sum_x exp(-i 2pi n.x/L) c[n][x]), with split-bf16 operands on
+// v_mfma_f32_16x16x32_bf16:
+//   a = a0 + a1 + a2, b = b0 + b1 + b2 (bf16 pieces, ~2^-24 relative)
+//   3 products  a0b0 + a0b1 + a1b0               (split = 3)
+//   6 products  + a0b2 + a1b1 + a2b0             (split = 6)
+// smallest products first into an FP32 MFMA accumulator that is added into
+// FP64 registers every FLUSH k-steps; FP64 partials per K split.
+//
+// Work decomposition: a workgroup owns 32 columns (two 16-column blocks) and
+// 4*MBW row blocks of 16 real rows (2 momenta x re/im each) for one K range.
+// Wave w owns row blocks w, w+4, ...  Each site-correlator element is read
+// and split into pieces once per row group (once in total when one group
+// covers all momenta).  The A operand is never stored: every lane builds its
+// fragments from the packed phase table, whose 32-bit words already hold the
+// (re, im) bf16 pair of a piece:
+//   W[k][p]   = cos_p | sin_p << 16       (row re:  cos,  sin)
+//   W[k][3+p] = -sin_p | cos_p << 16      (row im: -sin,  cos)
+// for theta = 2 pi k / M, k = n.x (M/L) mod M.
+#include "plg_internal.h"
+
+namespace plg {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+
+constexpr int PB_THREADS = 256;
+constexpr int PB_NC = 32;            // columns per workgroup
+constexpr int PB_KT = 16;            // sites per k-step (32 real K)
+constexpr int PB_LDB = 2 * PB_KT + 8;  // padded bf16 row of the B stage
+constexpr int PB_MMAX = 512;         // phase-table entries held in LDS
+constexpr int PB_FLUSH = 8;
+
+__device__ __forceinline__ uint16_t bf16_rn_(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_f_(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// k mod M for 0 <= k < 2^24 with a float reciprocal (exact quotient +-1, fixed up)
+__device__ __forceinline__ int mod_m(int k, int M, float invM) {
+  int q = (int)((float)k * invM);
+  int r = k - q * M;
+  r += r < 0 ? M : 0;
+  r -= r >= M ? M : 0;
+  return r;
+}
+
+template <int MBW, int TERMS>
+__global__ __launch_bounds__(PB_THREADS) void k_project_bf16(ProjGeom g, const cx<float> *__restrict__ c,
+                                                             const uint32_t *__restrict__ wtab,
+                                                             double *__restrict__ part) {
+  constexpr int NP = TERMS == 3 ? 2 : 3;
+  __shared__ uint32_t sW[PB_MMAX * 6];
+  __shared__ __align__(16) uint16_t sB[2][NP][PB_NC][PB_LDB];
+  __shared__ int sXYZ[2][PB_KT];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t n0 = (int64_t)blockIdx.x * PB_NC;
+  const int split = blockIdx.y;
+  const int mgroup = blockIdx.z;
+  const int Mreal = 2 * g.nmom;
+  const int nblk = (Mreal + 15) >> 4;
+  const int M = g.M;
+  const float invM = 1.0f / (float)M;
+  int64_t xb, xe;
+  {
+    const int64_t per = (g.V3 + g.nsplit - 1) / g.nsplit;
+    xb = (int64_t)split * per;
+    xe = xb + per < g.V3 ? xb + per : g.V3;
+  }
+  for (int i = tid; i < M * 6; i += PB_THREADS) sW[i] = wtab[i];
+
+  // per row block: packed momentum (n_x, n_y, n_z scaled, 10 bits each),
+  // bit 30 = imaginary row, bit 31 = row exists
+  int rowinfo[MBW];
+#pragma unroll
+  for (int i = 0; i < MBW; i++) {
+    const int mb = mgroup * 4 * MBW + wave + 4 * i;
+    const int mr = mb * 16 + (lane & 15);
+    int info = 0;
+    if (mb < nblk && mr < Mreal) {
+      const int m = mr >> 1;
+      info = g.mom_scaled[3 * m] | (g.mom_scaled[3 * m + 1] << 10) | (g.mom_scaled[3 * m + 2] << 20) |
+             ((mr & 1) << 30) | (int)(1u << 31);
+    }
+    rowinfo[i] = info;
+  }
+
+  f4 acc[MBW][2];
+  double acc64[MBW][2][4];
+#pragma unroll
+  for (int i = 0; i < MBW; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      acc[i][j] = f4{0, 0, 0, 0};
+#pragma unroll
+      for (int v = 0; v < 4; v++) acc64[i][j][v] = 0;
+    }
+
+  // staging of one k-step, in two halves so that the global loads of step
+  // t+1 are in flight during the MFMAs of step t:
+  //   load():  thread -> column tid/8, sites 2*(tid%8) + {0,1} into registers
+  //   store(): split into bf16 pieces, write the LDS stage, site coordinates
+  const int scol = tid >> 3, s0 = 2 * (tid & 7);
+  float v[4];
+  auto load = [&](int64_t xt) {
+    const int64_t n = n0 + scol;
+#pragma unroll
+    for (int e = 0; e < 4; e++) v[e] = 0;
+    if (n < g.ncols) {
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        const int64_t x = xt + s0 + q;
+        if (x < xe) {
+          const cx<float> z = c[n * g.V3 + x];
+          v[2 * q] = z.re;
+          v[2 * q + 1] = z.im;
+        }
+      }
+    }
+  };
+  auto store = [&](int buf, int64_t xt) {
+    const int col = scol;
+    uint16_t pc[3][4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      pc[0][e] = bf16_rn_(v[e]);
+      const float r1 = v[e] - bf16_f_(pc[0][e]);
+      pc[1][e] = bf16_rn_(r1);
+      pc[2][e] = bf16_rn_(r1 - bf16_f_(pc[1][e]));
+    }
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+      uint2 w;
+      w.x = (uint32_t)pc[p][0] | ((uint32_t)pc[p][1] << 16);
+      w.y = (uint32_t)pc[p][2] | ((uint32_t)pc[p][3] << 16);
+      *reinterpret_cast<uint2 *>(&sB[buf][p][col][2 * s0]) = w;
+    }
+    if (tid < PB_KT) {
+      const int64_t x = xt + tid;
+      int packed = -1;  // site outside the range
+      if (x < xe) {
+        const int cx_ = (int)(x % g.L[0]);
+        const int64_t q = x / g.L[0];
+        packed = cx_ | ((int)(q % g.L[1]) << 10) | ((int)(q / g.L[1]) << 20);
+      }
+      sXYZ[buf][tid] = packed;
+    }
+  };
+
+  int buf = 0, step = 0;
+  load(xb);
+  store(0, xb);
+  __syncthreads();
+  for (int64_t xt = xb; xt < xe; xt += PB_KT) {
+    const bool more = xt + PB_KT < xe;
+    if (more) load(xt + PB_KT);
+    // B fragments: lane l holds K = 8*(l/16) .. +7 of column l%16
+    const int kk = 8 * (lane >> 4);
+    bf8 bf[NP][2];
+#pragma unroll
+    for (int p = 0; p < NP; p++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+        bf[p][j] = *reinterpret_cast<const bf8 *>(&sB[buf][p][j * 16 + (lane & 15)][kk]);
+    int xyz[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) xyz[q] = sXYZ[buf][4 * (lane >> 4) + q];
+#pragma unroll
+    for (int i = 0; i < MBW; i++) {
+      // skip whole padding blocks only: the test is uniform across the wave
+      if (mgroup * 4 * MBW + wave + 4 * i >= nblk) continue;
+      const int info = rowinfo[i];
+      const int mx = info & 1023, my = (info >> 10) & 1023, mz = (info >> 20) & 1023;
+      const int hi = (info >> 30) & 1;
+      u4 w[NP];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int s = xyz[q];
+        uint32_t wp[3] = {0, 0, 0};
+        if (info < 0 && s >= 0) {
+          const int k = mod_m(mx * (s & 1023) + my * ((s >> 10) & 1023) + mz * (s >> 20), M, invM);
+          const uint32_t *t = &sW[k * 6 + 3 * hi];
+#pragma unroll
+          for (int p = 0; p < NP; p++) wp[p] = t[p];
+        }
+#pragma unroll
+        for (int p = 0; p < NP; p++) w[p][q] = wp[p];
+      }
+      bf8 af[NP];
+#pragma unroll
+      for (int p = 0; p < NP; p++) af[p] = __builtin_bit_cast(bf8, w[p]);
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        f4 a = acc[i][j];
+        if (TERMS == 6) {
+          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[2], bf[0][j], a, 0, 0, 0);
+          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1], bf[1][j], a, 0, 0, 0);
+          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0], bf[2][j], a, 0, 0, 0);
+        }
+        a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1], bf[0][j], a, 0, 0, 0);
+        a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0], bf[1][j], a, 0, 0, 0);
+        a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0], bf[0][j], a, 0, 0, 0);
+        acc[i][j] = a;
+      }
+    }
+    if (more) store(buf ^ 1, xt + PB_KT);
+    if (++step == PB_FLUSH) {
+      step = 0;
+#pragma unroll
+      for (int i = 0; i < MBW; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) acc64[i][j][v] += (double)acc[i][j][v];
+          acc[i][j] = f4{0, 0, 0, 0};
+        }
+    }
+    __syncthreads();
+    buf ^= 1;
+  }
+  // D of 16x16x32 (f32): lane l, value v -> row 4*(l/16) + v, column l%16
+#pragma unroll
+  for (int i = 0; i < MBW; i++) {
+    const int mb = mgroup * 4 * MBW + wave + 4 * i;
+    if (mb >= nblk) continue;
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int mr = mb * 16 + 4 * (lane >> 4) + v;
+        const int64_t n = n0 + j * 16 + (lane & 15);
+        if (mr < Mreal && n < g.ncols)
+          part[((int64_t)split * g.ncols + n) * Mreal + mr] = acc64[i][j][v] + (double)acc[i][j][v];
+      }
+  }
+}
+
+// row blocks per wave for a momentum count: fewest padded blocks, then fewest groups
+int pick_mbw(int nmom) {
+  const int nblk = (2 * nmom + 15) / 16;
+  int best = 1, bestw = 1 << 30;
+  for (int mbw : {1, 2, 3, 5, 9}) {
+    const int per = 4 * mbw;
+    const int waste = ((nblk + per - 1) / per) * per - nblk;
+    if (waste <= bestw) {  // ties go to the larger tile (fewer row groups)
+      best = mbw;
+      bestw = waste;
+    }
+  }
+  return best;
+}
+
+int launch_project_bf16(const ProjGeom &g, const cx<float> *c, cx<double> *part, int terms, const uint32_t *wtab,
+                        int mbw, hipStream_t st) {
+  if (g.M > PB_MMAX) return PLG_ERR_UNSUPPORTED;
+  const int nblk = (2 * g.nmom + 15) / 16;
+  const int groups = (nblk + 4 * mbw - 1) / (4 * mbw);
+  dim3 block(PB_THREADS), grid((unsigned)((g.ncols + PB_NC - 1) / PB_NC), (unsigned)g.nsplit, (unsigned)groups);
+  double *pd = reinterpret_cast<double *>(part);
+#define PLG_PB(MB)                                                                 \
+  case MB:                                                                         \
+    if (terms == 6) k_project_bf16<MB, 6><<<grid, block, 0, st>>>(g, c, wtab, pd); \
+    else k_project_bf16<MB, 3><<<grid, block, 0, st>>>(g, c, wtab, pd);            \
+    break;
+  switch (mbw) {
+    PLG_PB(1)
+    PLG_PB(2)
+    PLG_PB(3)
+    PLG_PB(5)
+    PLG_PB(9)
+  default:
+    return PLG_ERR_INVALID;
+  }
+#undef PLG_PB
+  return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;
+}
+
+}  // namespace plg
