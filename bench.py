@@ -100,6 +100,8 @@ def main():
     ap.add_argument("--split", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="first run a plain 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -112,6 +114,14 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs a HIP device"
     dev = torch.device("cuda:0")
+    if args.calib_copy:
+        # 1 GiB read + 1 GiB written by one copy kernel: the reference point
+        # for the FETCH_SIZE / WRITE_SIZE counters of the other kernels
+        a = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     capi.init(L, 0)
     V3 = L[0] * L[1] * L[2]
     rng = np.random.default_rng(SEED)
