@@ -44,10 +44,11 @@ BF16_PEAK_TFS = 2500.0
 FP64_PEAK_TFS = 78.6
 FP32_VALU_PEAK_TFS = 157.3
 
-# implemented arithmetic per site of the fused mixed contraction kernel
-# (16 lanes per site): meson Gram rows 4 x 16 x 9 CMAC; per baryon block
-# 16 x (36 + 9 + 144 + 36) CMAC; 4 blocks (proton + 3 Delta++); 8 flop / CMAC
-CONTRACT_FLOPS_PER_SITE = 8 * (4 * 16 * 9 + 4 * 16 * 225)
+# implemented arithmetic per site of the fused mixed contraction kernel, in
+# complex MACs per lane (16 lanes per site), 8 flop per CMAC:
+#   meson Gram rows 36; two diquark-class groups (proton + Delta_3,
+#   Delta_1 + Delta_2): Q 2 x 36, D 4 x 9, exchange T 2 x 4 x 36, X 4 x 36
+CONTRACT_FLOPS_PER_SITE = 8 * 16 * (36 + 2 * 36 + 4 * 9 + 2 * 4 * 36 + 4 * 36)
 
 
 def log(msg):

@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libplegma_amd.so")
-SOURCES = ["plg_api.hip", "plg_contract.hip", "plg_project.hip", "plg_project_bf16.hip",
+SOURCES = ["plg_api.hip", "plg_contract.hip", "plg_baryon.hip", "plg_project.hip", "plg_project_bf16.hip",
            "plg_synth.hip"]
 ARCH = "gfx950"
 

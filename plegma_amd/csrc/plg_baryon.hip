@@ -1,0 +1,377 @@
+// plg_baryon.hip -- baryon two-point contractions (and the fused meson +
+// baryon kernel of the mixed path) on gfx950.
+//
+// Replaces PLEGMA_baryons_NtoN.cu:6-46 (proton, Cg5 x Cg5) and
+// PLEGMA_baryons_deltas.cu:61-110 (Delta iso-3/2, Cg_k x g4 (Cg_k)^dag g4)
+// with factorised forms.  For a diquark matrix A (Abar = g4 A^dag g4) and
+// propagators P (quarks outside the diquark's partner) and Qm:
+//   St^{bb'}_{nu nu'} = A_{nu lam} Qm^{bb'}_{lam lam'} Abar_{lam' nu'}
+//   Q^{cc'}  = eps_abc eps_a'b'c' sum_{nu nu'} P^{aa'}_{nu nu'} St^{bb'}_{nu nu'}
+//   D_{mu mu'} = sum_{cc'} Q^{cc'} P^{cc'}_{mu mu'}
+//   X_{mu mu'} = eps eps' sum_{nu nu'} P^{ca'}_{mu nu'} St^{bb'}_{nu nu'} P^{ac'}_{nu mu'}
+//   proton  (Qm = other flavour)      = D - X
+//   Delta_k (uuu, Cg_k symmetric)     = 2 D - 4 X
+// The reference bodies cost 27648 complex MACs per proton dataset and about
+// 99000 per Delta component; this form costs 2304 per block (all four blocks
+// of one dataset: 9216), with 16 lanes of a wave per lattice site:
+//
+//   step 1, lane (nu, nu'):  Q partials, DPP row sum over the 16 lanes, D
+//   step 2, lane (nu', mu'): W^{c a'}_{nu' mu'} = sum_nu eps eps' St P   -> LDS
+//   step 3, lane (mu, mu'):  X = sum_{nu' c a'} P^{c a'}_{mu nu'} W_{nu' mu'}
+//
+// The four diquarks Cg5, Cg1, Cg2, Cg3 are signed permutations of two xor
+// classes; blocks of one class that share Qm (proton and Delta_3, Delta_1 and
+// Delta_2 on one propagator) differ only by phases, so their step-2 colour
+// contraction T(nu) is computed once and accumulated with both blocks' row
+// phases (compile-time); lane phases are applied once per block.
+#include "plg_internal.h"
+
+namespace plg {
+
+constexpr int BT_SITES = 16;  // sites per workgroup tile
+constexpr int BT_THREADS = 256;
+
+// LDS of one workgroup.  w holds the exchange intermediates of a block pair,
+// the meson Gram rows before the blocks and the output stage after them.
+// s2 is last so the one-propagator variant is allocated without it.
+template <typename T> struct BaryonShared {
+  cx<T> s1[144][BT_SITES];
+  cx<T> w[BT_SITES][2][16][9];
+  cx<T> s2[144][BT_SITES];
+};
+
+// per meson channel op (GAMMAS order): xor class and phases of X = g5 G
+struct MesonX {
+  int cls[16];
+  int ph[16][4];
+};
+constexpr MesonX make_mesonx() {
+  MesonX m{};
+  for (int op = 0; op < 16; op++) {
+    SPerm X = sp_mul(G_5, gamma_op(op));
+    m.cls[op] = sp_class(X);
+    for (int i = 0; i < 4; i++) m.ph[op][i] = X.ph[i];
+  }
+  return m;
+}
+__constant__ MesonX c_mesonx = make_mesonx();
+
+// ---- DPP row reductions (16-lane rows = the lanes of one site)
+template <int CTRL> __device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL> __device__ __forceinline__ double dpp(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <typename T> __device__ __forceinline__ T rowsum16(T v) {
+  v += dpp<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp<0x141>(v);  // row_half_mirror: the other quad of the half row
+  v += dpp<0x140>(v);  // row_mirror: the other half row
+  return v;
+}
+
+// runtime phase i^ph (lane-dependent)
+template <typename T> __device__ __forceinline__ cx<T> cmul_iph_rt(cx<T> a, int ph) {
+  ph &= 3;
+  const T re = (ph & 1) ? -a.im : a.re;
+  const T im = (ph & 1) ? a.re : a.im;
+  return (ph & 2) ? cmake<T>(-re, -im) : cmake<T>(re, im);
+}
+
+// eps eps' contraction of two colour blocks x[a][a'] y[b][b'] into out[c][c'] (+=)
+template <typename T> __device__ __forceinline__ void eps2(const cx<T> *x, const cx<T> *y, cx<T> *out) {
+  static_for<0, 6>([&](auto e1) {
+    static_for<0, 6>([&](auto e2) {
+      constexpr int a = EPS[e1.value][0], b = EPS[e1.value][1], c = EPS[e1.value][2];
+      constexpr int ap = EPS[e2.value][0], bp = EPS[e2.value][1], cp = EPS[e2.value][2];
+      const cx<T> p = x[a * 3 + ap] * y[b * 3 + bp];
+      if constexpr (SGN_EPS[e1.value] * SGN_EPS[e2.value] > 0) out[c * 3 + cp] = out[c * 3 + cp] + p;
+      else out[c * 3 + cp] = out[c * 3 + cp] - p;
+    });
+  });
+}
+// exchange form: out[c][a'] += eps eps' st[b][b'] p[a][c']
+template <typename T> __device__ __forceinline__ void eps2x(const cx<T> *st, const cx<T> *p, cx<T> *out) {
+  static_for<0, 6>([&](auto e1) {
+    static_for<0, 6>([&](auto e2) {
+      constexpr int a = EPS[e1.value][0], b = EPS[e1.value][1], c = EPS[e1.value][2];
+      constexpr int ap = EPS[e2.value][0], bp = EPS[e2.value][1], cp = EPS[e2.value][2];
+      const cx<T> q = st[b * 3 + bp] * p[a * 3 + cp];
+      if constexpr (SGN_EPS[e1.value] * SGN_EPS[e2.value] > 0) out[c * 3 + ap] = out[c * 3 + ap] + q;
+      else out[c * 3 + ap] = out[c * 3 + ap] - q;
+    });
+  });
+}
+
+// D and X of diquark W0 and, if W1 >= 0, of W1 (same spin class, same Qm).
+// pa: this lane's P[hi][lo] colour block (lane = (nu, nu') = (mu, mu')).
+template <typename T, int W0, int W1>
+__device__ __forceinline__ void baryon_group(const cx<T> (*P)[BT_SITES], const cx<T> (*Qm)[BT_SITES],
+                                             cx<T> (*Wsh)[16][9], int s, int lane, const cx<T> *pa,
+                                             cx<T> *D, cx<T> *X) {
+  constexpr int NB = W1 >= 0 ? 2 : 1;
+  constexpr SPerm A0 = diquark(W0), Ab0 = diquark_bar(W0);
+  constexpr SPerm A1 = diquark(W1 >= 0 ? W1 : W0), Ab1 = diquark_bar(W1 >= 0 ? W1 : W0);
+  constexpr int c = sp_class(A0);
+  static_assert(sp_class(A1) == c && sp_class(Ab0) == c && sp_class(Ab1) == c, "one spin class per group");
+  const int hi = lane >> 2, lo = lane & 3;
+
+  // ---- step 1: lane (nu, nu') -> Q partial, row sum, D on lane (mu, mu')
+  {
+    cx<T> q[9], Qr[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      q[i] = Qm[pidx(hi ^ c, lo ^ c, 0, 0) + i][s];
+      Qr[i] = cmake<T>(0, 0);
+    }
+    eps2(pa, q, Qr);
+    static_for<0, NB>([&](auto k) {
+      constexpr SPerm A = k.value == 0 ? A0 : A1;
+      constexpr SPerm Ab = k.value == 0 ? Ab0 : Ab1;
+      int ph = 0;
+      // lane phase A[nu][nu^c] * Abar[nu'^c][nu']
+#pragma unroll
+      for (int i = 0; i < 4; i++) ph += (hi == i ? A.ph[i] : 0) + (lo == i ? Ab.ph[i ^ c] : 0);
+      cx<T> d = cmake<T>(0, 0);
+#pragma unroll
+      for (int i = 0; i < 9; i++) {
+        cx<T> v = cmul_iph_rt(Qr[i], ph);
+        v.re = rowsum16(v.re);
+        v.im = rowsum16(v.im);
+        cfma(d, v, pa[i]);
+      }
+      D[k.value] = d;
+    });
+  }
+  // ---- step 2: lane (nu', mu') -> W = Abar-phase * sum_nu A-phase(nu) T(nu)
+  {
+    cx<T> W[NB][9];
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+#pragma unroll
+      for (int i = 0; i < 9; i++) W[b][i] = cmake<T>(0, 0);
+    static_for<0, 4>([&](auto nu) {
+      cx<T> sq[9], pp[9], Tn[9];
+#pragma unroll
+      for (int i = 0; i < 9; i++) {
+        sq[i] = Qm[pidx(nu.value ^ c, hi ^ c, 0, 0) + i][s];
+        pp[i] = P[pidx(nu.value, lo, 0, 0) + i][s];
+        Tn[i] = cmake<T>(0, 0);
+      }
+      eps2x(sq, pp, Tn);
+#pragma unroll
+      for (int i = 0; i < 9; i++) {
+        W[0][i] = W[0][i] + cmul_iph(Tn[i], A0.ph[nu.value]);
+        if constexpr (NB == 2) W[1][i] = W[1][i] + cmul_iph(Tn[i], A1.ph[nu.value]);
+      }
+    });
+    static_for<0, NB>([&](auto k) {
+      constexpr SPerm Ab = k.value == 0 ? Ab0 : Ab1;
+      int ph = 0;
+#pragma unroll
+      for (int i = 0; i < 4; i++) ph += (hi == i ? Ab.ph[i ^ c] : 0);
+#pragma unroll
+      for (int i = 0; i < 9; i++) Wsh[k.value][lane][i] = cmul_iph_rt(W[k.value][i], ph);
+    });
+  }
+  __syncthreads();
+  // ---- step 3: lane (mu, mu') -> X = sum_{nu'} P[mu][nu'] . W[nu'][mu']
+  {
+    cx<T> x[NB];
+#pragma unroll
+    for (int b = 0; b < NB; b++) x[b] = cmake<T>(0, 0);
+#pragma unroll
+    for (int nup = 0; nup < 4; nup++)
+#pragma unroll
+      for (int i = 0; i < 9; i++) {
+        const cx<T> p = P[pidx(hi, nup, 0, 0) + i][s];
+        cfma(x[0], p, Wsh[0][nup * 4 + lo][i]);
+        if constexpr (NB == 2) cfma(x[1], p, Wsh[1][nup * 4 + lo][i]);
+      }
+#pragma unroll
+    for (int b = 0; b < NB; b++) X[b] = x[b];
+  }
+  __syncthreads();
+}
+
+// Output channels per row (from ch0): [16 mesons of S1 if MES], then for each
+// dataset d (2 if TWO) and each selected group g in the reference order
+// (nucl_nucl, deltapp_deltamm_11, _22, _33): 16 components mu*4 + mu'.
+template <typename T, bool TWO, bool MES, int GMASK>
+__global__ __launch_bounds__(BT_THREADS) void k_baryon(const PropDev<T> *__restrict__ props,
+                                                       const RowDev *__restrict__ rows,
+                                                       cx<T> *__restrict__ out, int64_t V3, int T_lat,
+                                                       int nch_total, int ch0, int ntiles) {
+  constexpr bool H0 = GMASK & 1, H1 = GMASK & (1 << 7), H2 = GMASK & (1 << 8), H3 = GMASK & (1 << 9);
+  constexpr int NG = (int)H0 + (int)H1 + (int)H2 + (int)H3;
+  constexpr int R0 = 0, R1 = (int)H0, R2 = (int)H0 + (int)H1, R3 = (int)H0 + (int)H1 + (int)H2;
+  constexpr int ND = TWO ? 2 : 1;
+  constexpr int MOFF = MES ? 1 : 0;
+  constexpr int NOUT = MOFF + ND * NG;
+  static_assert(NOUT <= 9, "output stage holds 9 values per lane");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  BaryonShared<T> &sh = *reinterpret_cast<BaryonShared<T> *>(smem_raw);
+
+  const RowDev row = rows[blockIdx.y];
+  const PropDev<T> P1 = props[row.pa];
+  const PropDev<T> P2 = props[TWO ? row.pb : row.pa];
+  const int tid = threadIdx.x;
+  const int s = tid >> 4, lane = tid & 15;
+  const int hi = lane >> 2, lo = lane & 3;
+  const int64_t plane1 = (int64_t)((row.t - P1.t_first + T_lat) % T_lat) * V3;
+  const int64_t plane2 = (int64_t)((row.t - P2.t_first + T_lat) % T_lat) * V3;
+
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t x0 = (int64_t)tile * BT_SITES;
+    {
+      const int ls = tid & 15;
+      const int64_t x = x0 + ls;
+      const bool ok = x < V3;
+      const cx<T> *b1 = P1.data + plane1 + (ok ? x : 0);
+      const cx<T> *b2 = P2.data + plane2 + (ok ? x : 0);
+      for (int i = tid >> 4; i < 144; i += BT_THREADS / BT_SITES) {
+        sh.s1[i][ls] = ok ? b1[(int64_t)i * P1.cs] : cmake<T>(0, 0);
+        if (TWO) sh.s2[i][ls] = ok ? b2[(int64_t)i * P2.cs] : cmake<T>(0, 0);
+      }
+    }
+    __syncthreads();
+    const cx<T>(*S1)[BT_SITES] = sh.s1;
+    const cx<T>(*S2)[BT_SITES] = TWO ? sh.s2 : sh.s1;
+
+    cx<T> res[NOUT];
+    cx<T> pa1[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) pa1[i] = S1[pidx(hi, lo, 0, 0) + i][s];
+    // ---- mesons: lane p = (alpha, beta) forms G[p][p ^ (d,d)], lane op sums
+    if constexpr (MES) {
+      cx<T>(*g)[16] = reinterpret_cast<cx<T>(*)[16]>(&sh.w[s][0][0][0]);
+      static_for<0, 4>([&](auto d) {
+        const int q = lane ^ ((d.value << 2) | d.value);
+        cx<T> gv = cmake<T>(0, 0);
+#pragma unroll
+        for (int cc = 0; cc < 9; cc++) cfma_conj(gv, pa1[cc], d.value == 0 ? pa1[cc] : S1[q * 9 + cc][s]);
+        g[d.value][lane] = gv;
+      });
+      __syncthreads();
+      const int cX = c_mesonx.cls[lane];
+      cx<T> acc = cmake<T>(0, 0);
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+          acc = acc + cmul_iph_rt(g[cX][a * 4 + b], c_mesonx.ph[lane][a ^ cX] + c_mesonx.ph[lane][b]);
+      res[0] = acc;
+      __syncthreads();
+    }
+    // ---- baryon blocks per dataset
+    static_for<0, ND>([&](auto dd) {
+      constexpr int d = dd.value;
+      const cx<T>(*P)[BT_SITES] = d == 0 ? S1 : S2;
+      const cx<T>(*Qo)[BT_SITES] = d == 0 ? S2 : S1;  // the proton's other flavour
+      cx<T> pa[9];
+#pragma unroll
+      for (int i = 0; i < 9; i++) pa[i] = d == 0 ? pa1[i] : P[pidx(hi, lo, 0, 0) + i][s];
+      constexpr int base = MOFF + d * NG;
+      cx<T> D[2], X[2];
+      // class of Cg5 and Cg3: proton with Delta_3 when both use this propagator
+      if constexpr (H0 && H3 && !TWO) {
+        baryon_group<T, 0, 3>(P, P, sh.w[s], s, lane, pa, D, X);
+        res[base + R0] = D[0] - X[0];
+        res[base + R3] = cmake<T>(2, 0) * D[1] - cmake<T>(4, 0) * X[1];
+      } else {
+        if constexpr (H0) {
+          baryon_group<T, 0, -1>(P, Qo, sh.w[s], s, lane, pa, D, X);
+          res[base + R0] = D[0] - X[0];
+        }
+        if constexpr (H3) {
+          baryon_group<T, 3, -1>(P, P, sh.w[s], s, lane, pa, D, X);
+          res[base + R3] = cmake<T>(2, 0) * D[0] - cmake<T>(4, 0) * X[0];
+        }
+      }
+      // class of Cg1 and Cg2
+      if constexpr (H1 && H2) {
+        baryon_group<T, 1, 2>(P, P, sh.w[s], s, lane, pa, D, X);
+        res[base + R1] = cmake<T>(2, 0) * D[0] - cmake<T>(4, 0) * X[0];
+        res[base + R2] = cmake<T>(2, 0) * D[1] - cmake<T>(4, 0) * X[1];
+      } else {
+        if constexpr (H1) {
+          baryon_group<T, 1, -1>(P, P, sh.w[s], s, lane, pa, D, X);
+          res[base + R1] = cmake<T>(2, 0) * D[0] - cmake<T>(4, 0) * X[0];
+        }
+        if constexpr (H2) {
+          baryon_group<T, 2, -1>(P, P, sh.w[s], s, lane, pa, D, X);
+          res[base + R2] = cmake<T>(2, 0) * D[0] - cmake<T>(4, 0) * X[0];
+        }
+      }
+    });
+    // ---- stage [k][lane] per site in w, then coalesced stores along x
+    cx<T> *stage = &sh.w[0][0][0][0];
+    constexpr int SST = 2 * 16 * 9;  // complex per site in w
+#pragma unroll
+    for (int k = 0; k < NOUT; k++) stage[s * SST + lane * 9 + k] = res[k];
+    __syncthreads();
+    {
+      const int ls = tid & 15;
+      const int64_t x = x0 + ls;
+      if (x < V3) {
+        cx<T> *o = out + ((int64_t)blockIdx.y * nch_total + ch0) * V3 + x;
+        for (int ch = tid >> 4; ch < NOUT * 16; ch += BT_THREADS / BT_SITES)
+          o[(int64_t)ch * V3] = stage[ls * SST + (ch & 15) * 9 + (ch >> 4)];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, bool TWO, bool MES, int GM>
+static int launch_baryon_t(const PropDev<T> *props, const RowDev *rows, int nrows, cx<T> *out, int64_t V3,
+                           int T_lat, int nch_total, int ch0, int tiles_per_block, hipStream_t st) {
+  const int ntiles = (int)((V3 + BT_SITES - 1) / BT_SITES);
+  const int gx = (ntiles + tiles_per_block - 1) / tiles_per_block;
+  dim3 grid((unsigned)gx, (unsigned)nrows), block(BT_THREADS);
+  size_t shm = sizeof(BaryonShared<T>);
+  if (!TWO) shm -= sizeof(cx<T>) * 144 * BT_SITES;  // s2 is the last member
+  static bool attr_set = false;
+  if (!attr_set) {
+    if (hipFuncSetAttribute((const void *)k_baryon<T, TWO, MES, GM>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)shm) != hipSuccess)
+      return PLG_ERR_HIP;
+    attr_set = true;
+  }
+  k_baryon<T, TWO, MES, GM><<<grid, block, shm, st>>>(props, rows, out, V3, T_lat, nch_total, ch0, ntiles);
+  return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;
+}
+
+template <typename T>
+int launch_baryon(bool two, bool mes, int gmask, const PropDev<T> *props, const RowDev *rows, int nrows,
+                  cx<T> *out, int64_t V3, int T_lat, int nch_total, int ch0, int tiles_per_block,
+                  hipStream_t st) {
+#define PLG_B(TW, ME, GMV)                                                                       \
+  if (two == TW && mes == ME && gmask == GMV)                                                    \
+    return launch_baryon_t<T, TW, ME, GMV>(props, rows, nrows, out, V3, T_lat, nch_total, ch0,  \
+                                           tiles_per_block, st);
+  constexpr int NUC = 1, DEL = (1 << 7) | (1 << 8) | (1 << 9), ALL = NUC | DEL;
+  PLG_B(true, false, NUC)
+  PLG_B(true, false, DEL)
+  PLG_B(true, false, ALL)
+  PLG_B(false, false, NUC)
+  PLG_B(false, false, DEL)
+  PLG_B(false, false, ALL)
+  PLG_B(false, true, 0)
+  PLG_B(false, true, NUC)
+  PLG_B(false, true, DEL)
+  PLG_B(false, true, ALL)
+#undef PLG_B
+  return PLG_ERR_INVALID;
+}
+
+template int launch_baryon<double>(bool, bool, int, const PropDev<double> *, const RowDev *, int, cx<double> *,
+                                   int64_t, int, int, int, int, hipStream_t);
+template int launch_baryon<float>(bool, bool, int, const PropDev<float> *, const RowDev *, int, cx<float> *,
+                                  int64_t, int, int, int, int, hipStream_t);
+
+}  // namespace plg
