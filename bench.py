@@ -153,8 +153,9 @@ def main():
         jobs.append((x0, w, s, off))
         sizes.append(n)
         off += n
+    log("planning + autotuning the mixed batch")
     mixed = capi.Batch(kind, mask, v32, jobs, moms, capi.MODE_MIXED, split=args.split,
-                       flags=capi.BATCH_GRAPH)
+                       flags=capi.BATCH_GRAPH | capi.BATCH_TUNE)
     nout, nsites, nrows = mixed.info()
     out_mx = torch.zeros((nout, 2), dtype=torch.float64, device=dev)
     stream = torch.cuda.Stream()
@@ -198,7 +199,8 @@ def main():
 
     # FP64 plain path: one plan per source, launched one after another
     if not args.no_fp64:
-        plain = [capi.Batch(kind, mask, [v64[s]], [(x0, w, 0, 0)], moms, capi.MODE_F64)
+        plain = [capi.Batch(kind, mask, [v64[s]], [(x0, w, 0, 0)], moms, capi.MODE_F64,
+                            flags=capi.BATCH_TUNE)
                  for s, (x0, w) in enumerate(srcs)]
         outs64 = [torch.zeros((n, 2), dtype=torch.float64, device=dev) for n in sizes]
 

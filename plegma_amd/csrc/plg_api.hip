@@ -457,11 +457,12 @@ static void choose_tiles(Plan &P) {
     P.mtile = Mreal <= 96 ? 32 : 64;
     tiles = ((ncols + 127) / 128) * ((Mreal + P.mtile - 1) / P.mtile);
   } else {
-    P.mbw = pick_mbw(std::max(P.nmom, 1));
+    P.mbw = pick_proj(std::max(P.nmom, 1));
     const int nblk = (Mreal + 15) / 16;
-    tiles = ((ncols + 31) / 32) * ((nblk + 4 * P.mbw - 1) / (4 * P.mbw));
+    const int rb = (P.mbw - 100) / 10, nbw = P.mbw % 10;
+    tiles = ((ncols + 64 * nbw - 1) / (64 * nbw)) * ((nblk + rb - 1) / rb);
   }
-  int64_t ns = (2048 + tiles - 1) / tiles;
+  int64_t ns = (1024 + tiles - 1) / tiles;
   ns = std::max<int64_t>(1, std::min<int64_t>(ns, g_ctx.V3 / 512));
   P.nsplit = (int)std::max<int64_t>(1, ns);
   const int64_t btiles = (g_ctx.V3 + 15) / 16;
@@ -571,6 +572,145 @@ static int enqueue(Plan &P, void *c, void *part, void *out, int out_prec, hipStr
                        P.d_choff, P.m_stride, out, out_prec, st);
   if (log) log->mark("finalize", st);
   return rc ? fail(rc, "finalize launch failed") : PLG_OK;
+}
+
+// ------------------------------------------------------------------ autotuner
+// Times the candidate tiles of a plan's projection (tile code / row tile and
+// K splits) and its contraction (tiles per workgroup) on the plan's own
+// buffers, keeps the fastest, and remembers it in a text cache keyed by
+// device architecture, lattice, momenta, columns and precision.
+static std::string tune_cache_path() {
+  const char *e = getenv("PLG_TUNE_CACHE");
+  if (e && *e) return e;
+  const char *h = getenv("HOME");
+  return std::string(h && *h ? h : "/tmp") + "/.plegma_amd_tunecache";
+}
+
+static std::string tune_key(const Plan &P) {
+  hipDeviceProp_t prop;
+  hipGetDeviceProperties(&prop, g_ctx.device);
+  std::ostringstream k;
+  k << prop.gcnArchName << " L=" << g_ctx.L[0] << "x" << g_ctx.L[1] << "x" << g_ctx.L[2] << "x" << g_ctx.L[3]
+    << " nmom=" << P.nmom << " cols=" << P.rows.size() * P.nch_c << " rows=" << P.rows.size() << " mode=" << P.mode
+    << " terms=" << P.terms << " kind=" << P.kind << " mask=" << P.mask;
+  for (auto &l : P.launches) k << " L" << l.what << "." << l.set << "." << l.two << l.mes << "." << l.gmask;
+  std::string s = k.str();
+  for (char &ch : s)
+    if (ch == '\n') ch = ' ';
+  return s;
+}
+
+static bool tune_lookup(const std::string &key, int &a, int &b, int &c) {
+  std::ifstream f(tune_cache_path());
+  std::string line;
+  bool found = false;
+  while (std::getline(f, line)) {
+    const size_t bar = line.find(" | ");
+    if (bar == std::string::npos || line.substr(0, bar) != key) continue;
+    std::istringstream v(line.substr(bar + 3));
+    if (v >> a >> b >> c) found = true;  // last entry wins
+  }
+  return found;
+}
+
+static void tune_store(const std::string &key, int a, int b, int c, double ms) {
+  std::ofstream f(tune_cache_path(), std::ios::app);
+  if (f) f << key << " | " << a << " " << b << " " << c << " " << ms << "\n";
+}
+
+template <typename F> static double time_ms(F &&launch, hipStream_t st, int reps = 2) {
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  launch();  // warm
+  float best = 1e30f;
+  for (int r = 0; r < reps; r++) {
+    hipEventRecord(e0, st);
+    launch();
+    hipEventRecord(e1, st);
+    hipEventSynchronize(e1);
+    float t = 0;
+    hipEventElapsedTime(&t, e0, e1);
+    best = std::min(best, t);
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return best;
+}
+
+static int autotune(Plan &P, hipStream_t st) {
+  const std::string key = tune_key(P);
+  int a, b, c;
+  if (tune_lookup(key, a, b, c)) {
+    if (P.T == PLG_F64) P.mtile = a;
+    else P.mbw = a;
+    P.nsplit = b;
+    P.tpb = c;
+    return PLG_OK;
+  }
+  const int nrows = (int)P.rows.size();
+  // contraction: tiles per workgroup
+  std::vector<int> tpbs = {1, 2, 4, 8, 16, 32, 64};
+  double best_c = 1e30;
+  int best_tpb = P.tpb;
+  for (int tpb : tpbs) {
+    if (tpb > P.tpb * 8 || tpb * 4 < P.tpb) continue;
+    P.tpb = tpb;
+    const double ms = time_ms(
+        [&] {
+          for (const Launch &l : P.launches) {
+            if (l.what != Launch::BARYON) continue;
+            if (P.T == PLG_F64)
+              launch_baryon<double>(l.two, l.mes, l.gmask, (const PropDev<double> *)P.d_props, P.d_rows, nrows,
+                                    (cx<double> *)P.c, g_ctx.V3, g_ctx.L[3], P.nch_c, l.ch0, P.tpb, st);
+            else
+              launch_baryon<float>(l.two, l.mes, l.gmask, (const PropDev<float> *)P.d_props, P.d_rows, nrows,
+                                   (cx<float> *)P.c, g_ctx.V3, g_ctx.L[3], P.nch_c, l.ch0, P.tpb, st);
+          }
+        },
+        st);
+    if (ms < best_c) {
+      best_c = ms;
+      best_tpb = tpb;
+    }
+  }
+  P.tpb = best_tpb;
+  // projection: tile code and K splits
+  ProjGeom g;
+  g.V3 = g_ctx.V3;
+  for (int i = 0; i < 3; i++) g.L[i] = g_ctx.L[i];
+  g.M = g_ctx.M;
+  g.nmom = P.nmom;
+  g.ncols = nrows * P.nch_c;
+  g.mom_scaled = P.d_ms;
+  g.cos_tab = g_ctx.d_cos;
+  g.sin_tab = g_ctx.d_sin;
+  std::vector<int> codes = P.T == PLG_F64 ? std::vector<int>{32, 64} : proj_candidates(P.nmom);
+  const size_t per_split = part_bytes(P) / std::max(P.nsplit, 1);
+  double best_p = 1e30;
+  int best_code = P.T == PLG_F64 ? P.mtile : P.mbw, best_ns = P.nsplit;
+  for (int code : codes)
+    for (int ns : {1, 2, 4, 8, 16, 32}) {
+      if ((int64_t)ns * 256 > g_ctx.V3 || per_split * ns > P.part_bytes) continue;
+      g.nsplit = ns;
+      const double ms = time_ms(
+          [&] {
+            if (P.T == PLG_F64) launch_project_f64(g, (const cx<double> *)P.c, (cx<double> *)P.part, code, st);
+            else launch_project_bf16(g, (const cx<float> *)P.c, (cx<double> *)P.part, P.terms, g_ctx.d_wtab, code, st);
+          },
+          st);
+      if (ms < best_p) {
+        best_p = ms;
+        best_code = code;
+        best_ns = ns;
+      }
+    }
+  if (P.T == PLG_F64) P.mtile = best_code;
+  else P.mbw = best_code;
+  P.nsplit = best_ns;
+  if (hipGetLastError() != hipSuccess) return fail(PLG_ERR_HIP, "autotune launch failed");
+  tune_store(key, best_code, best_ns, best_tpb, best_c + best_p);
+  return PLG_OK;
 }
 
 }  // namespace plg
@@ -746,6 +886,14 @@ int plg_batch_create(int kind, int group_mask, const plg_prop *props, int nprops
   if (!rc) {
     P.own_ws = true;
     rc = ensure(&P.c, &P.c_bytes, c_bytes(P));
+    if (!rc && (flags & PLG_BATCH_TUNE)) {
+      // room for the largest K-split candidate
+      const int ns0 = P.nsplit;
+      P.nsplit = (int)std::max<int64_t>(ns0, std::min<int64_t>(32, g_ctx.V3 / 256));
+      rc = ensure(&P.part, &P.part_bytes, part_bytes(P));
+      P.nsplit = ns0;
+      if (!rc) rc = autotune(P, nullptr);
+    }
     if (!rc) rc = ensure(&P.part, &P.part_bytes, part_bytes(P));
   }
   if (!rc && (flags & PLG_BATCH_GRAPH)) {

@@ -5,6 +5,7 @@
 #include "plg_common.h"
 
 #include <string>
+#include <vector>
 
 namespace plg {
 
@@ -79,6 +80,11 @@ int launch_project_f64(const ProjGeom &g, const cx<double> *c, cx<double> *part,
 int launch_project_bf16(const ProjGeom &g, const cx<float> *c, cx<double> *part, int split_terms,
                         const uint32_t *wtab, int mbw, hipStream_t st);
 int pick_mbw(int nmom);
+// projection tile codes: < 100 per-wave A (MBW row blocks per wave);
+// 100 + 10*RB + NBW: A tile shared by the workgroup (RB row blocks, 4*NBW
+// column blocks)
+int pick_proj(int nmom);
+std::vector<int> proj_candidates(int nmom);
 // sum over splits, source phase, scatter into the H layout
 // nch_c: channels per computed row; nout: output channels per out row
 int launch_finalize(const ProjGeom &g, const cx<double> *part, const OutRowDev *orows, int norows,

@@ -241,6 +241,194 @@ __global__ __launch_bounds__(PB_THREADS) void k_project_bf16(ProjGeom g, const c
   }
 }
 
+// ------------------------------------------------------------------ shared-A variant
+// A workgroup owns RB row blocks (one row group) and 4*NBW column blocks; the
+// phase tile of each k-step (RB*16 rows x 16 sites, packed operand words per
+// piece) is built once in LDS by all 256 threads and read by the four waves,
+// which split the columns.  Two barriers per k-step; the site-correlator
+// loads of step t+1 are issued before the MFMAs of step t.
+constexpr int PV_AST = 20;  // words per A-tile row (16 sites, padded for alignment)
+
+template <int RB, int NBW, int TERMS>
+__global__ __launch_bounds__(PB_THREADS, 2) void k_project_bf16_sa(ProjGeom g, const cx<float> *__restrict__ c,
+                                                                   const uint32_t *__restrict__ wtab,
+                                                                   double *__restrict__ part) {
+  constexpr int NP = TERMS == 3 ? 2 : 3;
+  constexpr int NC = 4 * NBW * 16;            // columns per workgroup
+  constexpr int RM = RB * 8;                  // momenta per row group
+  constexpr int MPT = (RM + 15) / 16;         // momentum slots per thread in the tile build
+  constexpr int CPT = NC * PB_KT / PB_THREADS;  // complex B values staged per thread
+  __shared__ uint32_t sWt[PB_MMAX * 6];
+  __shared__ __align__(16) uint32_t sA[NP][RB * 16][PV_AST];
+  __shared__ __align__(16) uint16_t sB[NP][NC][PB_LDB];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t n0 = (int64_t)blockIdx.x * NC;
+  const int split = blockIdx.y;
+  const int row0 = blockIdx.z * RB * 16;
+  const int Mreal = 2 * g.nmom;
+  const int M = g.M;
+  const float invM = 1.0f / (float)M;
+  int64_t xb, xe;
+  {
+    const int64_t per = (g.V3 + g.nsplit - 1) / g.nsplit;
+    xb = (int64_t)split * per;
+    xe = xb + per < g.V3 ? xb + per : g.V3;
+  }
+  for (int i = tid; i < M * 6; i += PB_THREADS) sWt[i] = wtab[i];
+
+  // tile build assignment: site tid%16, momenta (tid/16) + 16 j of this group
+  const int ts = tid & 15;
+  int mpk[MPT];
+#pragma unroll
+  for (int j = 0; j < MPT; j++) {
+    const int mi = (tid >> 4) + 16 * j;
+    const int m = row0 / 2 + mi;
+    mpk[j] = (mi < RM && m < g.nmom)
+                 ? (g.mom_scaled[3 * m] | (g.mom_scaled[3 * m + 1] << 10) | (g.mom_scaled[3 * m + 2] << 20))
+                 : -1;
+  }
+  // staging assignment: column tid / (16/CPT) ... CPT consecutive sites of one column
+  constexpr int TPC = PB_KT / CPT;            // threads per column
+  const int bcol = tid / TPC, bs0 = (tid % TPC) * CPT;
+
+  f4 acc[RB][NBW];
+  double acc64[RB][NBW][4];
+#pragma unroll
+  for (int i = 0; i < RB; i++)
+#pragma unroll
+    for (int j = 0; j < NBW; j++) {
+      acc[i][j] = f4{0, 0, 0, 0};
+#pragma unroll
+      for (int v = 0; v < 4; v++) acc64[i][j][v] = 0;
+    }
+
+  cx<float> bv[CPT];
+  auto load = [&](int64_t xt) {
+    const int64_t n = n0 + bcol;
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+      const int64_t x = xt + bs0 + q;
+      bv[q] = (n < g.ncols && x < xe) ? c[n * g.V3 + x] : cmake<float>(0, 0);
+    }
+  };
+
+  int step = 0;
+  load(xb);
+  __syncthreads();  // sWt ready
+  for (int64_t xt = xb; xt < xe; xt += PB_KT) {
+    // ---- stage B pieces of this step
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+      const float e2[2] = {bv[q].re, bv[q].im};
+      uint32_t w[3];
+      uint16_t h[3][2];
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        h[0][e] = bf16_rn_(e2[e]);
+        const float r1 = e2[e] - bf16_f_(h[0][e]);
+        h[1][e] = bf16_rn_(r1);
+        h[2][e] = bf16_rn_(r1 - bf16_f_(h[1][e]));
+      }
+#pragma unroll
+      for (int p = 0; p < NP; p++) {
+        w[p] = (uint32_t)h[p][0] | ((uint32_t)h[p][1] << 16);
+        *reinterpret_cast<uint32_t *>(&sB[p][bcol][2 * (bs0 + q)]) = w[p];
+      }
+    }
+    // ---- build the phase tile of this step
+    {
+      const int64_t x = xt + ts;
+      const bool sv = x < xe;
+      int cx_ = 0, cy_ = 0, cz_ = 0;
+      if (sv) {
+        cx_ = (int)(x % g.L[0]);
+        const int64_t q = x / g.L[0];
+        cy_ = (int)(q % g.L[1]);
+        cz_ = (int)(q / g.L[1]);
+      }
+#pragma unroll
+      for (int j = 0; j < MPT; j++) {
+        const int mi = (tid >> 4) + 16 * j;
+        if (mi < RM) {
+          const int pk = mpk[j];
+          uint32_t wr[3] = {0, 0, 0}, wi[3] = {0, 0, 0};
+          if (pk >= 0 && sv) {
+            const int k = mod_m((pk & 1023) * cx_ + ((pk >> 10) & 1023) * cy_ + (pk >> 20) * cz_, M, invM);
+#pragma unroll
+            for (int p = 0; p < NP; p++) {
+              wr[p] = sWt[6 * k + p];
+              wi[p] = sWt[6 * k + 3 + p];
+            }
+          }
+#pragma unroll
+          for (int p = 0; p < NP; p++) {
+            sA[p][2 * mi][ts] = wr[p];
+            sA[p][2 * mi + 1][ts] = wi[p];
+          }
+        }
+      }
+    }
+    const bool more = xt + PB_KT < xe;
+    if (more) load(xt + PB_KT);
+    __syncthreads();
+    // ---- MFMAs
+    {
+      const int kk = 8 * (lane >> 4);
+      bf8 bf[NP][NBW];
+#pragma unroll
+      for (int p = 0; p < NP; p++)
+#pragma unroll
+        for (int j = 0; j < NBW; j++)
+          bf[p][j] = *reinterpret_cast<const bf8 *>(&sB[p][(wave * NBW + j) * 16 + (lane & 15)][kk]);
+#pragma unroll
+      for (int i = 0; i < RB; i++) {
+        if (row0 + i * 16 >= Mreal) continue;  // uniform: a padding block
+        bf8 af[NP];
+#pragma unroll
+        for (int p = 0; p < NP; p++)
+          af[p] = __builtin_bit_cast(bf8, *reinterpret_cast<const u4 *>(&sA[p][i * 16 + (lane & 15)][4 * (lane >> 4)]));
+#pragma unroll
+        for (int j = 0; j < NBW; j++) {
+          f4 a = acc[i][j];
+          if (TERMS == 6) {
+            a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[2], bf[0][j], a, 0, 0, 0);
+            a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1], bf[1][j], a, 0, 0, 0);
+            a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0], bf[2][j], a, 0, 0, 0);
+          }
+          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1], bf[0][j], a, 0, 0, 0);
+          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0], bf[1][j], a, 0, 0, 0);
+          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0], bf[0][j], a, 0, 0, 0);
+          acc[i][j] = a;
+        }
+      }
+    }
+    if (++step == PB_FLUSH) {
+      step = 0;
+#pragma unroll
+      for (int i = 0; i < RB; i++)
+#pragma unroll
+        for (int j = 0; j < NBW; j++) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) acc64[i][j][v] += (double)acc[i][j][v];
+          acc[i][j] = f4{0, 0, 0, 0};
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < RB; i++)
+#pragma unroll
+    for (int j = 0; j < NBW; j++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int mr = row0 + i * 16 + 4 * (lane >> 4) + v;
+        const int64_t n = n0 + (wave * NBW + j) * 16 + (lane & 15);
+        if (mr < Mreal && n < g.ncols)
+          part[((int64_t)split * g.ncols + n) * Mreal + mr] = acc64[i][j][v] + (double)acc[i][j][v];
+      }
+}
+
 // row blocks per wave for a momentum count: fewest padded blocks, then fewest groups
 int pick_mbw(int nmom) {
   const int nblk = (2 * nmom + 15) / 16;
@@ -268,6 +456,26 @@ int launch_project_bf16(const ProjGeom &g, const cx<float> *c, cx<double> *part,
     if (terms == 6) k_project_bf16<MB, 6><<<grid, block, 0, st>>>(g, c, wtab, pd); \
     else k_project_bf16<MB, 3><<<grid, block, 0, st>>>(g, c, wtab, pd);            \
     break;
+  if (mbw >= 100) {
+    // shared-A variant: code = 100 + 10*RB + NBW
+    const int rb = (mbw - 100) / 10, nbw = mbw % 10;
+    const int nc = 64 * nbw;
+    dim3 g3((unsigned)((g.ncols + nc - 1) / nc), (unsigned)g.nsplit, (unsigned)((nblk + rb - 1) / rb));
+#define PLG_SA(RB, NBW)                                                                             \
+  if (rb == RB && nbw == NBW) {                                                                     \
+    if (terms == 6) k_project_bf16_sa<RB, NBW, 6><<<g3, block, 0, st>>>(g, c, wtab, pd);             \
+    else k_project_bf16_sa<RB, NBW, 3><<<g3, block, 0, st>>>(g, c, wtab, pd);                        \
+    return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;                                  \
+  }
+    PLG_SA(11, 1)
+    PLG_SA(9, 1)
+    PLG_SA(5, 2)
+    PLG_SA(3, 2)
+    PLG_SA(2, 2)
+    PLG_SA(1, 2)
+#undef PLG_SA
+    return PLG_ERR_INVALID;
+  }
   switch (mbw) {
     PLG_PB(1)
     PLG_PB(2)
@@ -279,6 +487,34 @@ int launch_project_bf16(const ProjGeom &g, const cx<float> *c, cx<double> *part,
   }
 #undef PLG_PB
   return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;
+}
+
+// tile candidates of the autotuner for a momentum count (projection codes as
+// accepted by launch_project_bf16)
+std::vector<int> proj_candidates(int nmom) {
+  const int nblk = (2 * nmom + 15) / 16;
+  std::vector<int> v;
+  for (int code : {211, 191, 152, 132, 122, 112}) {
+    const int rb = (code - 100) / 10;
+    if (rb <= 2 * nblk) v.push_back(code);
+  }
+  v.push_back(pick_mbw(nmom));
+  return v;
+}
+
+// default: the shared-A tile whose row groups waste the fewest padded blocks
+int pick_proj(int nmom) {
+  const int nblk = (2 * nmom + 15) / 16;
+  int best = 112, bestw = 1 << 30;
+  for (int code : {112, 122, 132, 152, 191, 211}) {
+    const int rb = (code - 100) / 10;
+    const int waste = ((nblk + rb - 1) / rb) * rb - nblk;
+    if (waste <= bestw) {
+      best = code;
+      bestw = waste;
+    }
+  }
+  return best;
 }
 
 }  // namespace plg

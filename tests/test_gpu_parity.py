@@ -240,6 +240,36 @@ def test_batch_graph_replays(props):
     graph.close()
 
 
+def test_autotuned_batch(props, tmp_path, monkeypatch):
+    """the autotuner picks tiles and caches them; results stay within the
+    mixed path's tolerance of the untuned plan, and a second plan reads the cache"""
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("PLG_TUNE_CACHE", str(cache))
+    (t1, v1, h1), (t2, v2, h2) = props
+    torch = G.torch_cuda()
+    t32, v32 = G.cast_f32(t1, v1)
+    mask = capi.G_NUCL | capi.G_DELTAPP_ALL
+    moms = capi.momlist(Q1)
+    jobs = [([0, 0, 0, 0], 16, 0, 0)]
+    plain = capi.Batch(capi.TWOPT, mask, [v32], jobs, moms, capi.MODE_MIXED)
+    tuned = capi.Batch(capi.TWOPT, mask, [v32], jobs, moms, capi.MODE_MIXED,
+                       flags=capi.BATCH_TUNE)
+    n, _, _ = plain.info()
+    a = torch.zeros((n, 2), dtype=torch.float64, device="cuda")
+    b = torch.zeros((n, 2), dtype=torch.float64, device="cuda")
+    plain.run(a)
+    tuned.run(b)
+    ha, hb = G.host(a), G.host(b)
+    assert np.abs(ha - hb).max() <= 1e-6 * np.abs(ha).max()
+    lines = cache.read_text().splitlines()
+    assert len(lines) == 1 and " | " in lines[0]
+    again = capi.Batch(capi.TWOPT, mask, [v32], jobs, moms, capi.MODE_MIXED,
+                       flags=capi.BATCH_TUNE)
+    assert len(cache.read_text().splitlines()) == 1   # served from the cache
+    for x in (plain, tuned, again):
+        x.close()
+
+
 def test_error_codes(props):
     (t1, v1, h1), _ = props
     torch = G.torch_cuda()
