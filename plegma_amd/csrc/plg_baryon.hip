@@ -74,6 +74,15 @@ template <typename T> __device__ __forceinline__ T rowsum16(T v) {
   return v;
 }
 
+// The 16 lanes of a site are one DPP row of one wavefront, so data a site's
+// lanes exchange through LDS needs ordering at wavefront scope only (LDS
+// operations of a wavefront complete in issue order).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // runtime phase i^ph (lane-dependent)
 template <typename T> __device__ __forceinline__ cx<T> cmul_iph_rt(cx<T> a, int ph) {
   ph &= 3;
@@ -178,7 +187,7 @@ __device__ __forceinline__ void baryon_group(const cx<T> (*P)[BT_SITES], const c
       for (int i = 0; i < 9; i++) Wsh[k.value][lane][i] = cmul_iph_rt(W[k.value][i], ph);
     });
   }
-  __syncthreads();
+  wave_lds_sync();
   // ---- step 3: lane (mu, mu') -> X = sum_{nu'} P[mu][nu'] . W[nu'][mu']
   {
     cx<T> x[NB];
@@ -195,7 +204,7 @@ __device__ __forceinline__ void baryon_group(const cx<T> (*P)[BT_SITES], const c
 #pragma unroll
     for (int b = 0; b < NB; b++) X[b] = x[b];
   }
-  __syncthreads();
+  wave_lds_sync();  // W is overwritten by the next group of this wavefront
 }
 
 // Output channels per row (from ch0): [16 mesons of S1 if MES], then for each
@@ -225,20 +234,33 @@ __global__ __launch_bounds__(BT_THREADS) void k_baryon(const PropDev<T> *__restr
   const int64_t plane1 = (int64_t)((row.t - P1.t_first + T_lat) % T_lat) * V3;
   const int64_t plane2 = (int64_t)((row.t - P2.t_first + T_lat) % T_lat) * V3;
 
+  // tile staging: thread -> site tid%16, components tid/16 + 16 j (j < 9).
+  // The components of tile t+1 are fetched into registers while tile t is
+  // contracted (coalesced: 16 consecutive sites per component).
+  constexpr int NF = 144 / (BT_THREADS / BT_SITES);
+  cx<T> pf1[NF], pf2[TWO ? NF : 1];
+  auto fetch = [&](int tl) {
+    const int64_t x = (int64_t)tl * BT_SITES + (tid & 15);
+    const bool ok = x < V3;
+    const cx<T> *b1 = P1.data + plane1 + (ok ? x : 0);
+    const cx<T> *b2 = P2.data + plane2 + (ok ? x : 0);
+#pragma unroll
+    for (int j = 0; j < NF; j++) {
+      const int64_t i = (tid >> 4) + 16 * j;
+      pf1[j] = ok ? b1[i * P1.cs] : cmake<T>(0, 0);
+      if constexpr (TWO) pf2[j] = ok ? b2[i * P2.cs] : cmake<T>(0, 0);
+    }
+  };
+  if (blockIdx.x < ntiles) fetch(blockIdx.x);
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t x0 = (int64_t)tile * BT_SITES;
-    {
-      const int ls = tid & 15;
-      const int64_t x = x0 + ls;
-      const bool ok = x < V3;
-      const cx<T> *b1 = P1.data + plane1 + (ok ? x : 0);
-      const cx<T> *b2 = P2.data + plane2 + (ok ? x : 0);
-      for (int i = tid >> 4; i < 144; i += BT_THREADS / BT_SITES) {
-        sh.s1[i][ls] = ok ? b1[(int64_t)i * P1.cs] : cmake<T>(0, 0);
-        if (TWO) sh.s2[i][ls] = ok ? b2[(int64_t)i * P2.cs] : cmake<T>(0, 0);
-      }
+#pragma unroll
+    for (int j = 0; j < NF; j++) {
+      sh.s1[(tid >> 4) + 16 * j][tid & 15] = pf1[j];
+      if constexpr (TWO) sh.s2[(tid >> 4) + 16 * j][tid & 15] = pf2[j];
     }
     __syncthreads();
+    if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
     const cx<T>(*S1)[BT_SITES] = sh.s1;
     const cx<T>(*S2)[BT_SITES] = TWO ? sh.s2 : sh.s1;
 
@@ -256,7 +278,7 @@ __global__ __launch_bounds__(BT_THREADS) void k_baryon(const PropDev<T> *__restr
         for (int cc = 0; cc < 9; cc++) cfma_conj(gv, pa1[cc], d.value == 0 ? pa1[cc] : S1[q * 9 + cc][s]);
         g[d.value][lane] = gv;
       });
-      __syncthreads();
+      wave_lds_sync();
       const int cX = c_mesonx.cls[lane];
       cx<T> acc = cmake<T>(0, 0);
 #pragma unroll
@@ -265,7 +287,7 @@ __global__ __launch_bounds__(BT_THREADS) void k_baryon(const PropDev<T> *__restr
         for (int b = 0; b < 4; b++)
           acc = acc + cmul_iph_rt(g[cX][a * 4 + b], c_mesonx.ph[lane][a ^ cX] + c_mesonx.ph[lane][b]);
       res[0] = acc;
-      __syncthreads();
+      wave_lds_sync();
     }
     // ---- baryon blocks per dataset
     static_for<0, ND>([&](auto dd) {
