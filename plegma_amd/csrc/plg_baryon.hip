@@ -30,14 +30,21 @@ namespace plg {
 
 constexpr int BT_SITES = 16;  // sites per workgroup tile
 constexpr int BT_THREADS = 256;
+// LDS strides chosen for the access patterns of a wavefront (4 sites x 16
+// lanes, 8-byte complex = 2 banks): a component row of 17 sites makes the 16
+// lanes of a site that read components 9 l + i fall on 16 distinct bank
+// pairs; a per-site exchange stride of 292 (= 8 banks mod 32) keeps the 4
+// sites x 4 columns of the step-3 reads on distinct banks.
+constexpr int BT_SP = BT_SITES + 1;
+constexpr int BT_WS = 2 * 16 * 9 + 4;
 
 // LDS of one workgroup.  w holds the exchange intermediates of a block pair,
 // the meson Gram rows before the blocks and the output stage after them.
 // s2 is last so the one-propagator variant is allocated without it.
 template <typename T> struct BaryonShared {
-  cx<T> s1[144][BT_SITES];
-  cx<T> w[BT_SITES][2][16][9];
-  cx<T> s2[144][BT_SITES];
+  cx<T> s1[144][BT_SP];
+  cx<T> w[BT_SITES][BT_WS];
+  cx<T> s2[144][BT_SP];
 };
 
 // per meson channel op (GAMMAS order): xor class and phases of X = g5 G
@@ -91,27 +98,27 @@ template <typename T> __device__ __forceinline__ cx<T> cmul_iph_rt(cx<T> a, int 
   return (ph & 2) ? cmake<T>(-re, -im) : cmake<T>(re, im);
 }
 
-// eps eps' contraction of two colour blocks x[a][a'] y[b][b'] into out[c][c'] (+=)
+// eps eps' contraction of two colour blocks x[a][a'] y[b][b'] into out[c][c'] (+=),
+// every term one complex FMA
 template <typename T> __device__ __forceinline__ void eps2(const cx<T> *x, const cx<T> *y, cx<T> *out) {
   static_for<0, 6>([&](auto e1) {
     static_for<0, 6>([&](auto e2) {
       constexpr int a = EPS[e1.value][0], b = EPS[e1.value][1], c = EPS[e1.value][2];
       constexpr int ap = EPS[e2.value][0], bp = EPS[e2.value][1], cp = EPS[e2.value][2];
-      const cx<T> p = x[a * 3 + ap] * y[b * 3 + bp];
-      if constexpr (SGN_EPS[e1.value] * SGN_EPS[e2.value] > 0) out[c * 3 + cp] = out[c * 3 + cp] + p;
-      else out[c * 3 + cp] = out[c * 3 + cp] - p;
+      constexpr int ph = SGN_EPS[e1.value] * SGN_EPS[e2.value] > 0 ? 0 : 2;
+      cfma_iph<ph>(out[c * 3 + cp], x[a * 3 + ap], y[b * 3 + bp]);
     });
   });
 }
-// exchange form: out[c][a'] += eps eps' st[b][b'] p[a][c']
-template <typename T> __device__ __forceinline__ void eps2x(const cx<T> *st, const cx<T> *p, cx<T> *out) {
+// exchange form: out[c][a'] += i^PH eps eps' st[b][b'] p[a][c']
+template <int PH, typename T>
+__device__ __forceinline__ void eps2x(const cx<T> *st, const cx<T> *p, cx<T> *out) {
   static_for<0, 6>([&](auto e1) {
     static_for<0, 6>([&](auto e2) {
       constexpr int a = EPS[e1.value][0], b = EPS[e1.value][1], c = EPS[e1.value][2];
       constexpr int ap = EPS[e2.value][0], bp = EPS[e2.value][1], cp = EPS[e2.value][2];
-      const cx<T> q = st[b * 3 + bp] * p[a * 3 + cp];
-      if constexpr (SGN_EPS[e1.value] * SGN_EPS[e2.value] > 0) out[c * 3 + ap] = out[c * 3 + ap] + q;
-      else out[c * 3 + ap] = out[c * 3 + ap] - q;
+      constexpr int ph = PH + (SGN_EPS[e1.value] * SGN_EPS[e2.value] > 0 ? 0 : 2);
+      cfma_iph<ph>(out[c * 3 + ap], st[b * 3 + bp], p[a * 3 + cp]);
     });
   });
 }
@@ -119,7 +126,7 @@ template <typename T> __device__ __forceinline__ void eps2x(const cx<T> *st, con
 // D and X of diquark W0 and, if W1 >= 0, of W1 (same spin class, same Qm).
 // pa: this lane's P[hi][lo] colour block (lane = (nu, nu') = (mu, mu')).
 template <typename T, int W0, int W1>
-__device__ __forceinline__ void baryon_group(const cx<T> (*P)[BT_SITES], const cx<T> (*Qm)[BT_SITES],
+__device__ __forceinline__ void baryon_group(const cx<T> (*P)[BT_SP], const cx<T> (*Qm)[BT_SP],
                                              cx<T> (*Wsh)[16][9], int s, int lane, const cx<T> *pa,
                                              cx<T> *D, cx<T> *X) {
   constexpr int NB = W1 >= 0 ? 2 : 1;
@@ -157,27 +164,36 @@ __device__ __forceinline__ void baryon_group(const cx<T> (*P)[BT_SITES], const c
     });
   }
   // ---- step 2: lane (nu', mu') -> W = Abar-phase * sum_nu A-phase(nu) T(nu)
+  // The second block's row phases are A1.ph = A0.ph + RA or + RA + 2 (mod 4),
+  // so U_0 and U_1 (the A0-phased T(nu) of the two kinds of rows) are
+  // accumulated by FMA and W_0 = U_0 + U_1, W_1 = i^RA (U_0 - U_1).
   {
+    constexpr int RA = (A1.ph[0] - A0.ph[0]) & 3;
     cx<T> W[NB][9];
 #pragma unroll
     for (int b = 0; b < NB; b++)
 #pragma unroll
       for (int i = 0; i < 9; i++) W[b][i] = cmake<T>(0, 0);
     static_for<0, 4>([&](auto nu) {
-      cx<T> sq[9], pp[9], Tn[9];
+      constexpr int r = (A1.ph[nu.value] - A0.ph[nu.value] - RA) & 3;
+      static_assert(r == 0 || r == 2, "row phases of a class pair differ by +-i^RA");
+      constexpr int acc = (NB == 2 && r == 2) ? 1 : 0;
+      cx<T> sq[9], pp[9];
 #pragma unroll
       for (int i = 0; i < 9; i++) {
         sq[i] = Qm[pidx(nu.value ^ c, hi ^ c, 0, 0) + i][s];
         pp[i] = P[pidx(nu.value, lo, 0, 0) + i][s];
-        Tn[i] = cmake<T>(0, 0);
       }
-      eps2x(sq, pp, Tn);
+      eps2x<A0.ph[nu.value]>(sq, pp, W[acc]);
+    });
+    if constexpr (NB == 2) {
 #pragma unroll
       for (int i = 0; i < 9; i++) {
-        W[0][i] = W[0][i] + cmul_iph(Tn[i], A0.ph[nu.value]);
-        if constexpr (NB == 2) W[1][i] = W[1][i] + cmul_iph(Tn[i], A1.ph[nu.value]);
+        const cx<T> u0 = W[0][i], u1 = W[1][i];
+        W[0][i] = u0 + u1;
+        W[1][i] = cmul_iph(u0 - u1, RA);
       }
-    });
+    }
     static_for<0, NB>([&](auto k) {
       constexpr SPerm Ab = k.value == 0 ? Ab0 : Ab1;
       int ph = 0;
@@ -210,8 +226,11 @@ __device__ __forceinline__ void baryon_group(const cx<T> (*P)[BT_SITES], const c
 // Output channels per row (from ch0): [16 mesons of S1 if MES], then for each
 // dataset d (2 if TWO) and each selected group g in the reference order
 // (nucl_nucl, deltapp_deltamm_11, _22, _33): 16 components mu*4 + mu'.
+// FP32 tiles fit two workgroups per CU in LDS, so their register budget is
+// capped at two waves per SIMD.
 template <typename T, bool TWO, bool MES, int GMASK>
-__global__ __launch_bounds__(BT_THREADS) void k_baryon(const PropDev<T> *__restrict__ props,
+__global__ __launch_bounds__(BT_THREADS) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1)))
+void k_baryon(const PropDev<T> *__restrict__ props,
                                                        const RowDev *__restrict__ rows,
                                                        cx<T> *__restrict__ out, int64_t V3, int T_lat,
                                                        int nch_total, int ch0, int ntiles) {
@@ -261,8 +280,8 @@ __global__ __launch_bounds__(BT_THREADS) void k_baryon(const PropDev<T> *__restr
     }
     __syncthreads();
     if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
-    const cx<T>(*S1)[BT_SITES] = sh.s1;
-    const cx<T>(*S2)[BT_SITES] = TWO ? sh.s2 : sh.s1;
+    const cx<T>(*S1)[BT_SP] = sh.s1;
+    const cx<T>(*S2)[BT_SP] = TWO ? sh.s2 : sh.s1;
 
     cx<T> res[NOUT];
     cx<T> pa1[9];
@@ -270,7 +289,7 @@ __global__ __launch_bounds__(BT_THREADS) void k_baryon(const PropDev<T> *__restr
     for (int i = 0; i < 9; i++) pa1[i] = S1[pidx(hi, lo, 0, 0) + i][s];
     // ---- mesons: lane p = (alpha, beta) forms G[p][p ^ (d,d)], lane op sums
     if constexpr (MES) {
-      cx<T>(*g)[16] = reinterpret_cast<cx<T>(*)[16]>(&sh.w[s][0][0][0]);
+      cx<T>(*g)[16] = reinterpret_cast<cx<T>(*)[16]>(&sh.w[s][0]);
       static_for<0, 4>([&](auto d) {
         const int q = lane ^ ((d.value << 2) | d.value);
         cx<T> gv = cmake<T>(0, 0);
@@ -292,8 +311,8 @@ __global__ __launch_bounds__(BT_THREADS) void k_baryon(const PropDev<T> *__restr
     // ---- baryon blocks per dataset
     static_for<0, ND>([&](auto dd) {
       constexpr int d = dd.value;
-      const cx<T>(*P)[BT_SITES] = d == 0 ? S1 : S2;
-      const cx<T>(*Qo)[BT_SITES] = d == 0 ? S2 : S1;  // the proton's other flavour
+      const cx<T>(*P)[BT_SP] = d == 0 ? S1 : S2;
+      const cx<T>(*Qo)[BT_SP] = d == 0 ? S2 : S1;  // the proton's other flavour
       cx<T> pa[9];
 #pragma unroll
       for (int i = 0; i < 9; i++) pa[i] = d == 0 ? pa1[i] : P[pidx(hi, lo, 0, 0) + i][s];
@@ -301,38 +320,38 @@ __global__ __launch_bounds__(BT_THREADS) void k_baryon(const PropDev<T> *__restr
       cx<T> D[2], X[2];
       // class of Cg5 and Cg3: proton with Delta_3 when both use this propagator
       if constexpr (H0 && H3 && !TWO) {
-        baryon_group<T, 0, 3>(P, P, sh.w[s], s, lane, pa, D, X);
+        baryon_group<T, 0, 3>(P, P, reinterpret_cast<cx<T>(*)[16][9]>(&sh.w[s][0]), s, lane, pa, D, X);
         res[base + R0] = D[0] - X[0];
         res[base + R3] = cmake<T>(2, 0) * D[1] - cmake<T>(4, 0) * X[1];
       } else {
         if constexpr (H0) {
-          baryon_group<T, 0, -1>(P, Qo, sh.w[s], s, lane, pa, D, X);
+          baryon_group<T, 0, -1>(P, Qo, reinterpret_cast<cx<T>(*)[16][9]>(&sh.w[s][0]), s, lane, pa, D, X);
           res[base + R0] = D[0] - X[0];
         }
         if constexpr (H3) {
-          baryon_group<T, 3, -1>(P, P, sh.w[s], s, lane, pa, D, X);
+          baryon_group<T, 3, -1>(P, P, reinterpret_cast<cx<T>(*)[16][9]>(&sh.w[s][0]), s, lane, pa, D, X);
           res[base + R3] = cmake<T>(2, 0) * D[0] - cmake<T>(4, 0) * X[0];
         }
       }
       // class of Cg1 and Cg2
       if constexpr (H1 && H2) {
-        baryon_group<T, 1, 2>(P, P, sh.w[s], s, lane, pa, D, X);
+        baryon_group<T, 1, 2>(P, P, reinterpret_cast<cx<T>(*)[16][9]>(&sh.w[s][0]), s, lane, pa, D, X);
         res[base + R1] = cmake<T>(2, 0) * D[0] - cmake<T>(4, 0) * X[0];
         res[base + R2] = cmake<T>(2, 0) * D[1] - cmake<T>(4, 0) * X[1];
       } else {
         if constexpr (H1) {
-          baryon_group<T, 1, -1>(P, P, sh.w[s], s, lane, pa, D, X);
+          baryon_group<T, 1, -1>(P, P, reinterpret_cast<cx<T>(*)[16][9]>(&sh.w[s][0]), s, lane, pa, D, X);
           res[base + R1] = cmake<T>(2, 0) * D[0] - cmake<T>(4, 0) * X[0];
         }
         if constexpr (H2) {
-          baryon_group<T, 2, -1>(P, P, sh.w[s], s, lane, pa, D, X);
+          baryon_group<T, 2, -1>(P, P, reinterpret_cast<cx<T>(*)[16][9]>(&sh.w[s][0]), s, lane, pa, D, X);
           res[base + R2] = cmake<T>(2, 0) * D[0] - cmake<T>(4, 0) * X[0];
         }
       }
     });
     // ---- stage [k][lane] per site in w, then coalesced stores along x
-    cx<T> *stage = &sh.w[0][0][0][0];
-    constexpr int SST = 2 * 16 * 9;  // complex per site in w
+    cx<T> *stage = &sh.w[0][0];
+    constexpr int SST = BT_WS;  // complex per site in w
 #pragma unroll
     for (int k = 0; k < NOUT; k++) stage[s * SST + lane * 9 + k] = res[k];
     __syncthreads();
@@ -356,7 +375,7 @@ static int launch_baryon_t(const PropDev<T> *props, const RowDev *rows, int nrow
   const int gx = (ntiles + tiles_per_block - 1) / tiles_per_block;
   dim3 grid((unsigned)gx, (unsigned)nrows), block(BT_THREADS);
   size_t shm = sizeof(BaryonShared<T>);
-  if (!TWO) shm -= sizeof(cx<T>) * 144 * BT_SITES;  // s2 is the last member
+  if (!TWO) shm -= sizeof(cx<T>) * 144 * BT_SP;  // s2 is the last member
   static bool attr_set = false;
   if (!attr_set) {
     if (hipFuncSetAttribute((const void *)k_baryon<T, TWO, MES, GM>, hipFuncAttributeMaxDynamicSharedMemorySize,

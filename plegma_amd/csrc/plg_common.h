@@ -113,6 +113,31 @@ template <typename T> __host__ __device__ inline void cfma_conj(cx<T> &a, cx<T> 
   a.im = fma(b.im, c.re, a.im);
   a.im = fma(-b.re, c.im, a.im);
 }
+// a += i^PH * b * c as four FMAs (the phase folds into operand signs)
+template <int PH, typename T> __host__ __device__ inline void cfma_iph(cx<T> &a, cx<T> b, cx<T> c) {
+  constexpr int p = PH & 3;
+  if constexpr (p == 0) {
+    a.re = fma(b.re, c.re, a.re);
+    a.re = fma(-b.im, c.im, a.re);
+    a.im = fma(b.re, c.im, a.im);
+    a.im = fma(b.im, c.re, a.im);
+  } else if constexpr (p == 1) {  // i (x + iy) = -y + ix
+    a.re = fma(-b.re, c.im, a.re);
+    a.re = fma(-b.im, c.re, a.re);
+    a.im = fma(b.re, c.re, a.im);
+    a.im = fma(-b.im, c.im, a.im);
+  } else if constexpr (p == 2) {
+    a.re = fma(-b.re, c.re, a.re);
+    a.re = fma(b.im, c.im, a.re);
+    a.im = fma(-b.re, c.im, a.im);
+    a.im = fma(-b.im, c.re, a.im);
+  } else {  // -i (x + iy) = y - ix
+    a.re = fma(b.re, c.im, a.re);
+    a.re = fma(b.im, c.re, a.re);
+    a.im = fma(-b.re, c.re, a.im);
+    a.im = fma(b.im, c.im, a.im);
+  }
+}
 // multiply by i^ph (compile-time ph folds to swaps/negations)
 template <typename T> __host__ __device__ inline cx<T> cmul_iph(cx<T> a, int ph) {
   switch (ph & 3) {
