@@ -45,7 +45,11 @@ enum plg_kind {
   PLG_MESONS_NEW = 2,  /* contractMesonsNew: 10 ch, S1 x conj(S2), 1 dataset  */
   PLG_MESONS_ALL = 3,  /* contractMesonsAll: 34 ch, 1 dataset                 */
   PLG_BARYONS = 4,     /* contractBaryons: groups by mask, 16 ch, 2 datasets  */
-  PLG_MESON_MESON = 5, /* four-propagator meson-meson, 20 ch, 1 dataset       */
+  PLG_MESON_MESON = 5, /* S1 = light, S2 = heavy: 16 mesons Tr[G S1 g5 G S2^dag
+                          g5] (GAMMAS order), then (Tr A_G)^2 - Tr[A_G A_G]
+                          for G = g5, g1, g2, g3 (A_G the 12x12 product);
+                          20 ch, 1 dataset.  Counterpart of contractTetraquarks
+                          (lib/kernels/PLEGMA_heavy_light_tetraquarks.cu:163-212) */
   PLG_TWOPT = 6        /* one propagator: group 0 = 16 mesons, then baryon
                           groups by mask, 16 ch each, 1 dataset               */
 };

@@ -145,15 +145,14 @@ static int kind_shape(int kind, int mask, int &ng, int &nd, int &nch) {
   case PLG_TWOPT:
     if (mask != 0 && !baryon_mask_ok(mask)) return fail(PLG_ERR_UNSUPPORTED, "twopt groups: nucl_nucl and deltapp_deltamm_kk only");
     ng = 1 + popcount_groups(mask); nd = 1; nch = 16; return PLG_OK;
-  case PLG_MESON_MESON:
-    return fail(PLG_ERR_UNSUPPORTED, "meson-meson kind not built into this library version");
+  case PLG_MESON_MESON: ng = 1; nd = 1; nch = 20; return PLG_OK;
   }
   return fail(PLG_ERR_INVALID, "unknown kind " + std::to_string(kind));
 }
 
 // ------------------------------------------------------------------ plans
 struct Launch {
-  enum { MESON, BARYON } what;
+  enum { MESON, BARYON, MESMES } what;
   int set;          // meson set
   bool two, mes;    // baryon
   int gmask;
@@ -309,6 +308,10 @@ static int build_plan(Plan &P, int kind, int mask, const std::vector<plg_prop> &
     P.nch_c = 2 * gcomp * 16;
     break;
   }
+  case PLG_MESON_MESON:
+    P.nch_c = 20;
+    P.launches.push_back({Launch::MESMES, 0, true, false, 0, 0});
+    break;
   case PLG_TWOPT: {
     int gm = 0;
     if (mask) baryon_groups(mask, gm);
@@ -401,6 +404,24 @@ static int build_plan(Plan &P, int kind, int mask, const std::vector<plg_prop> &
           }
       }
       break;
+    case PLG_MESON_MESON:
+      // two out rows per time slice: the 16 mesons (one factor of each
+      // propagator) and the 4 meson-meson channels (two of each)
+      for (int j = 0; j < lt; j++) {
+        const int t = row_time(g_ctx.L, J.src, w, j);
+        P.rows.push_back({t, J.pa, J.pb, (int)jb});
+        for (int part = 0; part < 2; part++) {
+          OutRowDev o{};
+          o.crow = crow0 + j;
+          o.ch_base = part == 0 ? 0 : 16;
+          o.nch_row = part == 0 ? 16 : 4;
+          o.sx = J.src[0]; o.sy = J.src[1]; o.sz = J.src[2];
+          o.base = J.out_off + (int64_t)j * inner * nch + o.ch_base;
+          set_scale(o, pa, pb, t, part == 0 ? 1 : 2, part == 0 ? 1 : 2);
+          P.orows.push_back(o);
+        }
+      }
+      break;
     case PLG_TWOPT:
       for (int j = 0; j < lt; j++) {
         P.rows.push_back({row_time(g_ctx.L, J.src, w, j), J.pa, J.pa, (int)jb});
@@ -424,7 +445,7 @@ static int build_plan(Plan &P, int kind, int mask, const std::vector<plg_prop> &
     if (!two_props_baryons) P.nch_c = gcomp * 16;
   }
   // every out row reads nout channels contiguous from ch_base
-  const int nout = (kind == PLG_BARYONS || kind == PLG_TWOPT) ? 16 : nch;
+  const int nout = (kind == PLG_BARYONS || kind == PLG_TWOPT || kind == PLG_MESON_MESON) ? 16 : nch;
   for (int k = 0; k < nout; k++) {
     P.csel.push_back(k);
     P.ch_off.push_back(space == PLG_MOMENTUM_SPACE ? k : k);
@@ -521,7 +542,25 @@ struct StageLog {
 static std::string launch_name(const Plan &P, const Launch &l) {
   std::string t = P.T == PLG_F64 ? "f64" : "f32";
   if (l.what == Launch::MESON) return "contract_mesons_" + t;
+  if (l.what == Launch::MESMES) return "contract_mesmes_" + t;
   return std::string(l.mes ? "contract_twopt_" : "contract_baryons_") + t;
+}
+
+template <typename T> static int launch_contract_t(const Plan &P, const Launch &l, void *c, hipStream_t st) {
+  auto *pd = (const PropDev<T> *)P.d_props;
+  const int nrows = (int)P.rows.size();
+  switch (l.what) {
+  case Launch::MESON:
+    return launch_meson<T>(l.set, pd, P.d_rows, nrows, (cx<T> *)c, g_ctx.V3, g_ctx.L[3], P.nch_c, l.ch0, st);
+  case Launch::MESMES:
+    return launch_mesmes<T>(pd, P.d_rows, nrows, (cx<T> *)c, g_ctx.V3, g_ctx.L[3], P.nch_c, l.ch0, P.tpb, st);
+  default:
+    return launch_baryon<T>(l.two, l.mes, l.gmask, pd, P.d_rows, nrows, (cx<T> *)c, g_ctx.V3, g_ctx.L[3],
+                            P.nch_c, l.ch0, P.tpb, st);
+  }
+}
+static int launch_contract(const Plan &P, const Launch &l, void *c, hipStream_t st) {
+  return P.T == PLG_F64 ? launch_contract_t<double>(P, l, c, st) : launch_contract_t<float>(P, l, c, st);
 }
 
 static int enqueue(Plan &P, void *c, void *part, void *out, int out_prec, hipStream_t st,
@@ -530,21 +569,7 @@ static int enqueue(Plan &P, void *c, void *part, void *out, int out_prec, hipStr
   int rc;
   if (log) log->mark("start", st);
   for (const Launch &l : P.launches) {
-    if (P.T == PLG_F64) {
-      auto *pd = (const PropDev<double> *)P.d_props;
-      if (l.what == Launch::MESON)
-        rc = launch_meson<double>(l.set, pd, P.d_rows, nrows, (cx<double> *)c, g_ctx.V3, g_ctx.L[3], P.nch_c, l.ch0, st);
-      else
-        rc = launch_baryon<double>(l.two, l.mes, l.gmask, pd, P.d_rows, nrows, (cx<double> *)c, g_ctx.V3,
-                                   g_ctx.L[3], P.nch_c, l.ch0, P.tpb, st);
-    } else {
-      auto *pd = (const PropDev<float> *)P.d_props;
-      if (l.what == Launch::MESON)
-        rc = launch_meson<float>(l.set, pd, P.d_rows, nrows, (cx<float> *)c, g_ctx.V3, g_ctx.L[3], P.nch_c, l.ch0, st);
-      else
-        rc = launch_baryon<float>(l.two, l.mes, l.gmask, pd, P.d_rows, nrows, (cx<float> *)c, g_ctx.V3,
-                                  g_ctx.L[3], P.nch_c, l.ch0, P.tpb, st);
-    }
+    rc = launch_contract(P, l, c, st);
     if (rc) return fail(rc, "contraction launch failed: " + std::string(hipGetErrorString(hipGetLastError())));
     if (log) log->mark(launch_name(P, l), st);
   }
@@ -658,15 +683,8 @@ static int autotune(Plan &P, hipStream_t st) {
     P.tpb = tpb;
     const double ms = time_ms(
         [&] {
-          for (const Launch &l : P.launches) {
-            if (l.what != Launch::BARYON) continue;
-            if (P.T == PLG_F64)
-              launch_baryon<double>(l.two, l.mes, l.gmask, (const PropDev<double> *)P.d_props, P.d_rows, nrows,
-                                    (cx<double> *)P.c, g_ctx.V3, g_ctx.L[3], P.nch_c, l.ch0, P.tpb, st);
-            else
-              launch_baryon<float>(l.two, l.mes, l.gmask, (const PropDev<float> *)P.d_props, P.d_rows, nrows,
-                                   (cx<float> *)P.c, g_ctx.V3, g_ctx.L[3], P.nch_c, l.ch0, P.tpb, st);
-          }
+          for (const Launch &l : P.launches)
+            if (l.what != Launch::MESON) launch_contract(P, l, P.c, st);
         },
         st);
     if (ms < best_c) {

@@ -138,6 +138,32 @@ template <int PH, typename T> __host__ __device__ inline void cfma_iph(cx<T> &a,
     a.im = fma(b.im, c.im, a.im);
   }
 }
+// a += i^PH * b * conj(c) as four FMAs
+template <int PH, typename T> __host__ __device__ inline void cfma_conj_iph(cx<T> &a, cx<T> b, cx<T> c) {
+  // b conj(c) = (br cr + bi ci) + i (bi cr - br ci)
+  constexpr int p = PH & 3;
+  if constexpr (p == 0) {
+    a.re = fma(b.re, c.re, a.re);
+    a.re = fma(b.im, c.im, a.re);
+    a.im = fma(b.im, c.re, a.im);
+    a.im = fma(-b.re, c.im, a.im);
+  } else if constexpr (p == 1) {
+    a.re = fma(b.re, c.im, a.re);
+    a.re = fma(-b.im, c.re, a.re);
+    a.im = fma(b.re, c.re, a.im);
+    a.im = fma(b.im, c.im, a.im);
+  } else if constexpr (p == 2) {
+    a.re = fma(-b.re, c.re, a.re);
+    a.re = fma(-b.im, c.im, a.re);
+    a.im = fma(-b.im, c.re, a.im);
+    a.im = fma(b.re, c.im, a.im);
+  } else {
+    a.re = fma(b.im, c.re, a.re);
+    a.re = fma(-b.re, c.im, a.re);
+    a.im = fma(-b.re, c.re, a.im);
+    a.im = fma(-b.im, c.im, a.im);
+  }
+}
 // multiply by i^ph (compile-time ph folds to swaps/negations)
 template <typename T> __host__ __device__ inline cx<T> cmul_iph(cx<T> a, int ph) {
   switch (ph & 3) {

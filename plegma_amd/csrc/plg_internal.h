@@ -38,7 +38,7 @@ struct OutRowDev {
   int crow;      // row of the site correlator / partial sums
   int ch_base;
   int sx, sy, sz;
-  int pad;
+  int nch_row;   // channels of this row (0: the launch-wide count)
   int64_t base;  // H offset of (m = 0, k = 0)
   // result scale 2^(ka*ea[pla] + kb*eb[plb]) undoes the propagators' plane
   // exponents (ea/eb NULL: no scaling)
@@ -55,6 +55,9 @@ __device__ __forceinline__ double row_scale(const OutRowDev &R) {
 template <typename T>
 int launch_meson(int set, const PropDev<T> *props, const RowDev *rows, int nrows, cx<T> *out,
                  int64_t V3, int T_lat, int nch_total, int ch0, hipStream_t st);
+template <typename T>
+int launch_mesmes(const PropDev<T> *props, const RowDev *rows, int nrows, cx<T> *out, int64_t V3, int T_lat,
+                  int nch_total, int ch0, int tiles_per_block, hipStream_t st);
 template <typename T>
 int launch_baryon(bool two, bool mes, int gmask, const PropDev<T> *props, const RowDev *rows,
                   int nrows, cx<T> *out, int64_t V3, int T_lat, int nch_total, int ch0,

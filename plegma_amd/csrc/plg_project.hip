@@ -159,9 +159,10 @@ __global__ void k_finalize(ProjGeom g, const cx<double> *__restrict__ part, cons
                            int nch_c, int nout, const int *__restrict__ csel, const int64_t *__restrict__ ch_off,
                            int64_t m_stride, cx<TO> *__restrict__ out) {
   const OutRowDev R = orows[blockIdx.y];
+  const int nr = R.nch_row > 0 ? R.nch_row : nout;
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)g.nmom * nout) return;
-  const int m = (int)(e / nout), k = (int)(e % nout);
+  if (e >= (int64_t)g.nmom * nr) return;
+  const int m = (int)(e / nr), k = (int)(e % nr);
   const int64_t n = (int64_t)R.crow * nch_c + R.ch_base + csel[k];
   double re = 0, im = 0;
   for (int s = 0; s < g.nsplit; s++) {
@@ -200,10 +201,11 @@ __global__ void k_scatter_pos(const cx<TI> *__restrict__ c, int64_t V3, const Ou
                               int nch_c, int nout, const int *__restrict__ csel,
                               const int64_t *__restrict__ ch_off, int64_t x_stride, cx<TO> *__restrict__ out) {
   const OutRowDev R = orows[blockIdx.y];
+  const int nr = R.nch_row > 0 ? R.nch_row : nout;
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= V3 * nout) return;
-  const int64_t x = e / nout;
-  const int k = (int)(e % nout);
+  if (e >= V3 * nr) return;
+  const int64_t x = e / nr;
+  const int k = (int)(e % nr);
   const cx<TI> v = c[((int64_t)R.crow * nch_c + R.ch_base + csel[k]) * V3 + x];
   const double sc = row_scale(R);
   cx<TO> o;

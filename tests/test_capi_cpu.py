@@ -43,7 +43,8 @@ def test_negative_q2_is_an_error():
 @pytest.mark.parametrize("kind,okind,mask", [
     (capi.MESONS, O.MESONS, 0), (capi.MESONS16, O.MESONS16, 0),
     (capi.MESONS_NEW, O.MESONS_NEW, 0), (capi.MESONS_ALL, O.MESONS_ALL, 0),
-    (capi.BARYONS, O.BARYONS, capi.G_NUCL | capi.G_DELTAPP_ALL)])
+    (capi.BARYONS, O.BARYONS, capi.G_NUCL | capi.G_DELTAPP_ALL),
+    (capi.MESON_MESON, O.MESON_MESON, 0)])
 def test_kind_shapes(kind, okind, mask):
     assert capi.kind_shape(kind, mask) == O.kind_shape(okind, mask)
 

@@ -302,3 +302,51 @@ def test_window_view(props):
     Hf = G.run_correlator(capi.MESONS16, 0, v1, v1, src, totalT, Q1)
     Hw = G.run_correlator(capi.MESONS16, 0, vw, vw, src, totalT, Q1)
     np.testing.assert_allclose(Hw, Hf, rtol=0, atol=1e-13 * np.abs(Hf).max())
+
+
+# ---- meson-meson (four propagators: light S1, heavy S2)
+@pytest.mark.parametrize("same", [False, True])
+def test_meson_meson_f64(props, same):
+    (t1, v1, h1), (t2, v2, h2) = props
+    if same:
+        v2, h2 = v1, h1
+    src, totalT = [3, 1, 6, 10], 12
+    print(f"[gpu] meson-meson f64 same={same}", flush=True)
+    H = G.shaped(G.run_correlator(capi.MESON_MESON, 0, v1, v2, src, totalT, Q1),
+                 capi.MESON_MESON, 0, src, totalT, Q1)
+    rows = [0, 5, 11]
+    R = G.oracle_H(O.MESON_MESON, L1, src, totalT, Q1, h1, h2, rows=rows)
+    assert G.rel_err(H[:, :, rows], R[:, :, rows]) < TOL64
+
+
+def test_meson_meson_position_space(props):
+    """position space: per-site values of both channel blocks (16 mesons at
+    one scale, 4 meson-meson channels at another) land in their slots"""
+    (t1, v1, h1), (t2, v2, h2) = props
+    src, totalT = [0, 0, 0, 14], 4
+    H = G.run_correlator(capi.MESON_MESON, 0, v1, v2, src, totalT, 0, space=capi.POSITION_SPACE)
+    V3 = L1[0] * L1[1] * L1[2]
+    H = H.reshape(totalT, V3, 20)
+    for j in range(totalT):
+        t = O.row_time(L1, src, totalT, j)
+        ref = O.site_correlator(O.MESON_MESON, L1, t, h1, h2)
+        assert G.rel_err(H[j], ref, axis=0) < 1e-12
+
+
+def test_meson_meson_mixed_two_flavours():
+    """light (lambda 0.5) and heavy (lambda 1.2) propagators in FP32 with
+    per-plane exponents over a wrapped window: mixed path vs FP64 oracle"""
+    torch = G.torch_cuda()
+    src, totalT = [2, 3, 1, 12], 16
+    tl, vl = G.make_prop(L1, 7, 11, src[3], 0.5)
+    th, vh = G.make_prop(L1, 7, 12, src[3], 1.2)
+    hl, hh = G.host(tl), G.host(th)
+    kl, vl32 = G.cast_f32(tl, vl, scaled=True)
+    kh, vh32 = G.cast_f32(th, vh, scaled=True)
+    H = G.shaped(G.run_correlator(capi.MESON_MESON, 0, vl32, vh32, src, totalT, Q1,
+                                  mode=capi.MODE_MIXED), capi.MESON_MESON, 0, src, totalT, Q1)
+    rows = [0, 3, 8, 15]
+    R = G.oracle_H(O.MESON_MESON, L1, src, totalT, Q1, hl, hh, rows=rows)
+    err = G.rel_err(H[:, :, rows], R[:, :, rows])
+    print(f"[gpu] meson-meson mixed rel err {err:.2e}", flush=True)
+    assert err < 1e-6
