@@ -5,6 +5,13 @@ from a seed, each with its own time window of 8..32 slices; per source a
 seeded propagator stored over its window; 16 meson channels + proton +
 Delta++ (Cg1, Cg2, Cg3) = 80 channels; all momenta with |n|^2 <= 16 (257).
 
+Workload "largest" (--workload largest): lattice 64^3 x 128, one source at
+the origin, two flavour propagators (light lambda 0.5, heavy lambda 1.2) in
+FP32 with per-time-slice exponents (2 x 38.65 GB), 16 heavy-light mesons +
+4 meson-meson channels, |n|^2 <= 9 (123 momenta), all 128 time slices; the
+FP64 plain path and the error check run on 16 of the time slices
+(t = 120..127, 0..7) from FP64 copies of those slices.
+
 A step is one pass over all sources:
   mixed (the metric): FP32 contraction + split-bf16 (bf16x6) MFMA projection,
         one batched plan replayed as a hipGraph;
@@ -18,7 +25,7 @@ against the FP64 path per (channel, momentum, time slice), the dominant
 kernel against its roofline (HIP events on the launch stream), and the CPU
 oracle (reference per-site bodies, FP64, OpenMP) on a bounded sample.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload largest]
 """
 from __future__ import annotations
 
@@ -49,6 +56,13 @@ FP32_VALU_PEAK_TFS = 157.3
 #   meson Gram rows 36; two diquark-class groups (proton + Delta_3,
 #   Delta_1 + Delta_2): Q 2 x 36, D 4 x 9, exchange T 2 x 4 x 36, X 4 x 36
 CONTRACT_FLOPS_PER_SITE = 8 * 16 * (36 + 2 * 36 + 4 * 9 + 2 * 4 * 36 + 4 * 36)
+# meson-meson kernel, complex MACs per lane: meson Gram rows 36, two operator
+# pairs x (4 x 27 colour products + 2 x 9 crossed-term products)
+MESMES_FLOPS_PER_SITE = 8 * 16 * (36 + 2 * (4 * 27 + 2 * 9))
+
+L4 = [64, 64, 64, 128]
+Q2MAX4 = 9
+LAMBDA_HEAVY = 1.2
 
 
 def log(msg):
@@ -92,37 +106,81 @@ def cpu_baseline(budget_s=12.0, max_rows=8):
                       f"bodies + per-site DFT, {t_used:.1f} s"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--fp64-steps", type=int, default=None)
-    ap.add_argument("--split", type=int, default=6)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--calib-copy", action="store_true",
-                    help="first run a plain 1 GiB device copy (counter calibration)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-              f"(sources and time slices shard without communication; see the design notes)",
-              file=sys.stderr)
-        sys.exit(2)
+def cpu_baseline_largest(budget_s=12.0):
+    """Oracle meson-meson (FP64 12x12 products per site + per-site DFT,
+    OpenMP) on one time slice of the 64^3 x 128 workload"""
+    from oracle import oracle as O
+    nth = min(os.cpu_count() or 1, 16)
+    V3 = L4[0] * L4[1] * L4[2]
+    t = 3
+    PL = O.synth_prop(L4, SEED, 0, 0, LAMBDA, t_first=t, nt=1, nthreads=nth)
+    PH = O.synth_prop(L4, SEED, 1, 0, LAMBDA_HEAVY, t_first=t, nt=1, nthreads=nth)
+    view = (V3, t, 1)
+    t0 = time.perf_counter()
+    O.correlator(O.MESON_MESON, L4, [0, 0, 0, 0], L4[3], Q2MAX4, PL, PH, rows=[t], nthreads=nth,
+                 view1=view, view2=view)
+    used = time.perf_counter() - t0
+    return {"value": V3 / used, "unit": "sites/s", "cores": nth, "kind": "port",
+            "sample": f"time slice t={t} of the 64^3x128 workload ({V3} sites), 16 mesons + "
+                      f"4 meson-meson channels, 123 momenta, FP64 12x12 products + per-site "
+                      f"DFT, {used:.1f} s"}
 
-    import torch
-    from plegma_amd import capi
 
-    assert torch.cuda.is_available(), "bench needs a HIP device"
-    dev = torch.device("cuda:0")
-    if args.calib_copy:
-        # 1 GiB read + 1 GiB written by one copy kernel: the reference point
-        # for the FETCH_SIZE / WRITE_SIZE counters of the other kernels
-        a = torch.ones(1 << 28, dtype=torch.float32, device=dev)
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
+def timed(torch, stream, fn, steps):
+    """mean milliseconds per call of fn over `steps` calls (HIP events on the
+    launch stream) and the host wall time"""
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps, time.perf_counter() - t0
+
+
+def stage_ms(batch, out, stream, reps=3):
+    """median per-stage milliseconds of eager passes of a batch"""
+    runs = [batch.profile(out, stream) for _ in range(reps)]
+    return {name: float(np.median([dict(r)[name] for r in runs])) for name, _ in runs[0]}
+
+
+def roofline(st_ms, nsites, nmom, nch, split, contract_flops, prop_bytes):
+    """the dominant stage against its roofline: the contraction against HBM
+    (compulsory propagator read + site-correlator write) with its VALU rate
+    beside it, the projection against BF16 MFMA (useful flops)"""
+    dom = max(st_ms, key=st_ms.get)
+    t = st_ms[dom] / 1e3
+    if dom.startswith("contract"):
+        byts = nsites * (prop_bytes + nch * 8)
+        achieved = byts / t / 1e9
+        r = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+             "algorithmic": f"{prop_bytes} B FP32 propagator read + {nch * 8} B site "
+                            f"correlator write per site",
+             "valu": {"achieved": nsites * contract_flops / t / 1e12, "peak": FP32_VALU_PEAK_TFS,
+                      "unit": "TFLOP/s", "flops_per_site": contract_flops}}
+        r["valu"]["frac"] = r["valu"]["achieved"] / FP32_VALU_PEAK_TFS
+    else:
+        useful = nsites * 8 * nmom * nch
+        achieved = useful / t / 1e12
+        r = {"kernel": dom, "bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFS,
+             "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFS, "traffic": None,
+             "algorithmic": f"8 flop x {nmom} momenta x {nch} channels per site (useful)",
+             "executed_frac": achieved * split / BF16_PEAK_TFS}
+    return r
+
+
+def rel_err_block(A, B):
+    """[group, momentum] max over time slices and channels of
+    |A - B| / max_momenta |B|; A, B: [group, row, momentum, channel]"""
+    den = np.abs(B).max(axis=2, keepdims=True)
+    den = np.where(den == 0, 1.0, den)
+    return (np.abs(A - B) / den).max(axis=(1, 3))
+
+
+def run_batching(args, torch, capi, dev, stream):
     capi.init(L, 0)
     V3 = L[0] * L[1] * L[2]
     rng = np.random.default_rng(SEED)
@@ -158,33 +216,13 @@ def main():
                        flags=capi.BATCH_GRAPH | capi.BATCH_TUNE)
     nout, nsites, nrows = mixed.info()
     out_mx = torch.zeros((nout, 2), dtype=torch.float64, device=dev)
-    stream = torch.cuda.Stream()
-
-    def timed(fn, steps):
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        t0 = time.perf_counter()
-        e0.record(stream)
-        for _ in range(steps):
-            fn()
-        e1.record(stream)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        return e0.elapsed_time(e1) / steps, wall
 
     log(f"mixed: {nrows} rows, {nsites} sites; warmup {args.warmup}")
     for _ in range(args.warmup):
         mixed.run(out_mx, stream)
-    ms_mx, wall_mx = timed(lambda: mixed.run(out_mx, stream), args.steps)
+    ms_mx, _ = timed(torch, stream, lambda: mixed.run(out_mx, stream), args.steps)
     log(f"mixed: {ms_mx:.3f} ms/step")
-
-    # per-stage durations of the same plan (eager pass, HIP events on `stream`)
-    stages = []
-    for _ in range(3):
-        stages.append(mixed.profile(out_mx, stream))
-    st_ms = {}
-    for name, _ in stages[0]:
-        st_ms[name] = float(np.median([dict(s)[name] for s in stages]))
+    st_ms = stage_ms(mixed, out_mx, stream)
 
     res = {"metric": METRIC, "value": nsites / (ms_mx / 1e3), "unit": "sites/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_mx,
@@ -210,11 +248,9 @@ def main():
 
         run_plain()
         k64 = args.fp64_steps or max(1, args.steps // 2)
-        ms_64, _ = timed(run_plain, k64)
+        ms_64, _ = timed(torch, stream, run_plain, k64)
         log(f"fp64 plain: {ms_64:.3f} ms/step")
         st64 = {name: ms for name, ms in plain[0].profile(outs64[0], stream)}
-        # error of the mixed path against the FP64 path, per (group, time slice,
-        # channel) normalised by the largest |C| over momenta
         torch.cuda.synchronize()
         hm = out_mx.cpu().numpy()
         errs = []
@@ -223,8 +259,7 @@ def main():
             b = outs64[s].cpu().numpy()
             A = (a[:, 0] + 1j * a[:, 1]).reshape(5, w, len(moms), 16)
             B = (b[:, 0] + 1j * b[:, 1]).reshape(5, w, len(moms), 16)
-            den = np.abs(B).max(axis=2, keepdims=True)
-            errs.append((np.abs(A - B) / den).max(axis=(1, 3)))   # [group, momentum]
+            errs.append(rel_err_block(A, B))
         E = np.max(np.stack(errs), axis=0)
         res["fp64_plain"] = {"value": nsites / (ms_64 / 1e3), "unit": "sites/s",
                              "ms_per_step": ms_64, "steps": k64,
@@ -237,36 +272,143 @@ def main():
                                                 "per group and momentum"}
         del plain
 
-    # roofline of the dominant kernel
-    dom = max(st_ms, key=st_ms.get)
-    t = st_ms[dom] / 1e3
-    nch = 80
-    if dom.startswith("contract"):
-        byts = nsites * (144 * 8 + nch * 8)
-        achieved = byts / t / 1e9
-        res["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": achieved,
-                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                           "traffic": None,
-                           "algorithmic": "1152 B FP32 propagator read + 640 B site correlator "
-                                          "write per site",
-                           "valu": {"achieved": nsites * CONTRACT_FLOPS_PER_SITE / t / 1e12,
-                                    "peak": FP32_VALU_PEAK_TFS, "unit": "TFLOP/s",
-                                    "flops_per_site": CONTRACT_FLOPS_PER_SITE}}
-        res["roofline"]["valu"]["frac"] = res["roofline"]["valu"]["achieved"] / FP32_VALU_PEAK_TFS
-    else:
-        useful = nsites * 8 * len(moms) * nch
-        achieved = useful / t / 1e12
-        res["roofline"] = {"kernel": dom, "bound": "mfma", "achieved": achieved,
-                           "peak": BF16_PEAK_TFS, "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFS,
-                           "traffic": None,
-                           "algorithmic": "8 flop x 257 momenta x 80 channels per site (useful)",
-                           "executed_frac": achieved * args.split / BF16_PEAK_TFS}
+    res["roofline"] = roofline(st_ms, nsites, len(moms), 80, args.split, CONTRACT_FLOPS_PER_SITE,
+                               144 * 8)
     res["stages_ms"] = st_ms
-
     if not args.no_cpu:
         log("cpu baseline")
         res["cpu_baseline"] = cpu_baseline()
     mixed.close()
+    return res
+
+
+def run_largest(args, torch, capi, dev, stream):
+    capi.init(L4, 0)
+    V3 = L4[0] * L4[1] * L4[2]
+    T = L4[3]
+    src = [0, 0, 0, 0]
+    moms = capi.momlist(Q2MAX4)
+    kind = capi.MESON_MESON
+    log("synthesising two FP32 flavour propagators (2 x 38.65 GB)")
+    keep, v32 = [], []
+    for fl, lam in enumerate((LAMBDA, LAMBDA_HEAVY)):
+        t32 = torch.empty((144, T * V3, 2), dtype=torch.float32, device=dev)
+        e32 = torch.empty(T, dtype=torch.int32, device=dev)
+        v = capi.plg_prop(t32.data_ptr(), capi.F32, 0, T, 0, T * V3, e32.data_ptr())
+        capi.synth_prop(v, SEED, fl, 0, lam)
+        keep += [t32, e32]
+        v32.append(v)
+    torch.cuda.synchronize()
+    nout = capi.correlator_size(kind, 0, src, T, len(moms))
+    log("planning + autotuning the mixed plan")
+    mixed = capi.Batch(kind, 0, v32, [(src, T, 0, 0, 1)], moms, capi.MODE_MIXED, split=args.split,
+                       flags=capi.BATCH_GRAPH | capi.BATCH_TUNE)
+    _, nsites, nrows = mixed.info()
+    out_mx = torch.zeros((nout, 2), dtype=torch.float64, device=dev)
+    log(f"mixed: {nrows} rows, {nsites} sites; warmup {args.warmup}")
+    for _ in range(args.warmup):
+        mixed.run(out_mx, stream)
+    ms_mx, _ = timed(torch, stream, lambda: mixed.run(out_mx, stream), args.steps)
+    log(f"mixed: {ms_mx:.3f} ms/step")
+    st_ms = stage_ms(mixed, out_mx, stream)
+    res = {"metric": METRIC, "value": nsites / (ms_mx / 1e3), "unit": "sites/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_mx,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": f"f32 contraction + bf16x{args.split} MFMA projection (fp32 acc, fp64 sums)",
+           "data": "synthetic",
+           "config": {"workload": "largest: lattice 64^3x128, one source, light + heavy FP32 "
+                                  "propagators, 16 heavy-light mesons + 4 meson-meson, "
+                                  "|n|^2<=9 (123 momenta)",
+                      "lattice": L4, "sources": 1, "sites": nsites, "channels": 20,
+                      "nmom": len(moms), "parallelism": "single GPU"}}
+    mixed.close()
+
+    if not args.no_fp64:
+        # FP64 copies of 16 time slices t = 120..127, 0..7 (a view that wraps),
+        # contracted as the window of a source at t0 = 120 (rows j -> t by the
+        # window rule: t = 0..7 first, then 120..127)
+        nt, tf = 16, 120
+        v64, k64 = [], []
+        for fl, lam in enumerate((LAMBDA, LAMBDA_HEAVY)):
+            t64 = torch.empty((144, nt * V3, 2), dtype=torch.float64, device=dev)
+            v = capi.plg_prop(t64.data_ptr(), capi.F64, tf, nt, 0, nt * V3)
+            capi.synth_prop(v, SEED, fl, 0, lam)
+            k64.append(t64)
+            v64.append(v)
+        src64 = [0, 0, 0, tf]
+        n64 = capi.correlator_size(kind, 0, src64, nt, len(moms))
+        plain = capi.Batch(kind, 0, v64, [(src64, nt, 0, 0, 1)], moms, capi.MODE_F64,
+                           flags=capi.BATCH_TUNE)
+        out64 = torch.zeros((n64, 2), dtype=torch.float64, device=dev)
+        plain.run(out64, stream)
+        ms_64, _ = timed(torch, stream, lambda: plain.run(out64, stream), max(1, args.steps // 2))
+        st64 = stage_ms(plain, out64, stream, reps=1)
+        torch.cuda.synchronize()
+        a = out_mx.cpu().numpy()
+        b = out64.cpu().numpy()
+        A = (a[:, 0] + 1j * a[:, 1]).reshape(1, T, len(moms), 20)
+        B = (b[:, 0] + 1j * b[:, 1]).reshape(1, nt, len(moms), 20)
+        rows = [capi.row_time(src64, nt, j) for j in range(nt)]
+        E = np.concatenate([rel_err_block(A[:, rows, :, :16], B[..., :16]),
+                            rel_err_block(A[:, rows, :, 16:], B[..., 16:])], 0)
+        sites64 = nt * V3
+        res["fp64_plain"] = {"value": sites64 / (ms_64 / 1e3), "unit": "sites/s",
+                             "ms_per_step": ms_64, "sites": sites64, "stages_ms": st64,
+                             "sample": "time slices 120..127, 0..7"}
+        res["speedup_vs_fp64"] = res["value"] / res["fp64_plain"]["value"]
+        res["rel_err_vs_fp64"] = {"max": float(E.max()), "median": float(np.median(E)),
+                                  "per_group_max": {"mesons": float(E[0].max()),
+                                                    "meson_meson": float(E[1].max())},
+                                  "definition": "max over the 16 compared time slices and "
+                                                "channels of |C_mixed - C_fp64| / "
+                                                "max_momenta |C_fp64|, per channel block and "
+                                                "momentum"}
+        plain.close()
+        del k64
+    res["roofline"] = roofline(st_ms, nsites, len(moms), 20, args.split, MESMES_FLOPS_PER_SITE,
+                               2 * 144 * 8)
+    res["stages_ms"] = st_ms
+    if not args.no_cpu:
+        log("cpu baseline")
+        res["cpu_baseline"] = cpu_baseline_largest()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--split", type=int, default=6)
+    ap.add_argument("--workload", choices=["batching", "largest"], default="batching")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="first run a plain 1 GiB device copy (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              f"(sources and time slices shard without communication; see the design notes)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from plegma_amd import capi
+
+    assert torch.cuda.is_available(), "bench needs a HIP device"
+    dev = torch.device("cuda:0")
+    if args.calib_copy:
+        # 1 GiB read + 1 GiB written by one copy kernel: the reference point
+        # for the FETCH_SIZE / WRITE_SIZE counters of the other kernels
+        a = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+    stream = torch.cuda.Stream()
+    run = run_largest if args.workload == "largest" else run_batching
+    res = run(args, torch, capi, dev, stream)
     capi.finalize()
     print(json.dumps(res), flush=True)
 

@@ -125,7 +125,10 @@ int plg_correlator_size(int kind, int group_mask, const int source[4], int total
 typedef struct {
   int source[4];
   int totalT;
-  int prop;            /* index into the props array */
+  int prop;            /* index into the props array: S1 */
+  int prop2;           /* S2 for kinds that read two propagators (mesons S2
+                          dataset, contractMesonsNew/All, baryons, meson-meson
+                          heavy flavour); -1: the same as prop */
   int64_t out_offset;  /* complex elements into the output buffer */
 } plg_job;
 typedef struct plg_batch_s *plg_batch;

@@ -46,7 +46,7 @@ class plg_prop(C.Structure):
 
 class plg_job(C.Structure):
     _fields_ = [("source", C.c_int * 4), ("totalT", C.c_int), ("prop", C.c_int),
-                ("out_offset", C.c_int64)]
+                ("prop2", C.c_int), ("out_offset", C.c_int64)]
 
 
 _lib = None
@@ -162,15 +162,18 @@ def correlator_size(kind, mask, source, totalT, nmom, space=MOMENTUM_SPACE):
 
 
 class Batch:
-    """plg_batch_* wrapper: many (source, window, propagator) jobs in one plan."""
+    """plg_batch_* wrapper: many jobs in one plan.  A job is
+    (source, totalT, prop, out_offset[, prop2]) with indices into props."""
 
     def __init__(self, kind, mask, props, jobs, moms, mode, split=SPLIT_BF16X6, flags=0):
         P = (plg_prop * len(props))(*props)
         J = (plg_job * len(jobs))()
-        for i, (src, totalT, prop, off) in enumerate(jobs):
+        for i, job in enumerate(jobs):
+            src, totalT, prop, off = job[:4]
             J[i].source = (C.c_int * 4)(*src)
             J[i].totalT = totalT
             J[i].prop = prop
+            J[i].prop2 = job[4] if len(job) > 4 else -1
             J[i].out_offset = off
         flat = [int(v) for m in moms for v in m]
         h = C.c_void_p()

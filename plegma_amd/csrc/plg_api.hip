@@ -889,10 +889,12 @@ int plg_batch_create(int kind, int group_mask, const plg_prop *props, int nprops
   for (int i = 0; i < njobs; i++) {
     const plg_job &j = jobs[i];
     if (j.prop < 0 || j.prop >= nprops) return fail(PLG_ERR_INVALID, "job propagator index out of range");
+    if (j.prop2 < -1 || j.prop2 >= nprops) return fail(PLG_ERR_INVALID, "job prop2 index out of range");
     int lt;
     int rc = plg_window(j.source, j.totalT, nullptr, nullptr, &lt);
     if (rc) return rc;
-    jv.push_back({{j.source[0], j.source[1], j.source[2], j.source[3]}, j.totalT, j.prop, j.prop, j.out_offset});
+    jv.push_back({{j.source[0], j.source[1], j.source[2], j.source[3]}, j.totalT, j.prop,
+                  j.prop2 < 0 ? j.prop : j.prop2, j.out_offset});
   }
   plg_batch b = new plg_batch_s();
   Plan &P = b->plan;
