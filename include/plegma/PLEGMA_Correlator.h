@@ -1,0 +1,268 @@
+// PLEGMA_Correlator.h -- the two-point correlator with the interface of
+// include/PLEGMA_Correlator.h:18-331 for one process on one GPU: window
+// accessors, dataset/group bookkeeping, H_elem in the reference's layout,
+// contract* overwriting the buffer, the ASCII writer.  Every contraction runs
+// in libplegma_amd.so through plg_correlator: Float = double is the FP64
+// path (PLG_MODE_F64), Float = float the mixed path (FP32 contraction,
+// split-bf16 MFMA projection, FP64 sums).
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <memory>
+#include <string>
+
+#include "PLEGMA_FT.h"
+#include "PLEGMA_Propagator.h"
+
+namespace plegma {
+
+template <typename Float> class PLEGMA_Correlator {
+protected:
+  const site source;
+  const int totalT;
+  std::vector<int> shape;
+  std::vector<std::string> datasets;
+  std::vector<std::string> groups;
+  std::string description;
+  const CORR_SPACE corr_space;
+  std::shared_ptr<Float> corr_pos_space;  // host buffer of position-space results
+  size_t pos_size = 0;
+  std::shared_ptr<PLEGMA_FT<Float>> corr_mom_space;
+  void *d_out = nullptr;  // device result buffer, reused across contractions
+  size_t d_bytes = 0;
+
+  int L(int d) const { return Lattice::get().L[d]; }
+
+  // lib/PLEGMA_Correlator.cu:25-41: (re)allocate and zero the host buffer
+  void initialize() {
+    if (corr_space == MOMENTUM_SPACE) {
+      corr_mom_space->checkAllocation((int)getSiteSize());
+    } else {
+      pos_size = getTotalSize() * 2;
+      corr_pos_space.reset(new Float[pos_size](), std::default_delete<Float[]>());
+    }
+  }
+
+  void run(int kind, int mask, const PLEGMA_Propagator<Float> &p1, const PLEGMA_Propagator<Float> &p2) {
+    if (p1.getAllocation() == NONE || p2.getAllocation() == NONE) PLEGMA_error("propagator is not allocated");
+    int64_t n = 0;
+    std::vector<int> moms = corr_space == MOMENTUM_SPACE ? corr_mom_space->intMomenta() : std::vector<int>{};
+    const int nmom = (int)moms.size() / 3;
+    const int space = corr_space == MOMENTUM_SPACE ? PLG_MOMENTUM_SPACE : PLG_POSITION_SPACE;
+    plg_check(plg_correlator_size(kind, mask, source.data(), totalT, nmom, space, &n), "plg_correlator_size");
+    if ((size_t)n != getTotalSize())  // PLEGMA_mesons.cuh:140-141
+      PLEGMA_error("Site size of the correlator does not match the contraction (%lld vs %zu)", (long long)n,
+                   getTotalSize());
+    const size_t bytes = (size_t)n * 2 * sizeof(Float);
+    if (bytes > d_bytes) {
+      if (d_out) (void)hipFree(d_out);
+      hip_check(hipMalloc(&d_out, bytes), "correlator result buffer");
+      d_bytes = bytes;
+    }
+    plg_prop v1 = p1.view(), v2 = p2.view();
+    const int mode = sizeof(Float) == 8 ? PLG_MODE_F64 : PLG_MODE_MIXED;
+    plg_check(plg_correlator(kind, mask, &v1, &v2, source.data(), totalT, moms.empty() ? nullptr : moms.data(), nmom,
+                             mode, space, plg_precision_of<Float>(), d_out, nullptr),
+              "plg_correlator");
+    hip_check(hipMemcpy(H_elem(), d_out, bytes, hipMemcpyDeviceToHost), "result copy");
+  }
+
+public:
+  // include/PLEGMA_Correlator.h:41-79 with one process (procPosition = 0)
+  bool hasSource(int dir) const { return dir >= 0 && dir < 4 && source[dir] >= 0 && source[dir] < L(dir); }
+  bool hasSource() const {
+    bool r = true;
+    for (int d = 0; r && d < 4; d++) r &= hasSource(d);
+    return r;
+  }
+  int startT() const {
+    int s = 0, e = 0, lt = 0;
+    plg_check(plg_window(source.data(), totalT, &s, &e, &lt), "plg_window");
+    return s;
+  }
+  int endT() const {
+    int s = 0, e = 0, lt = 0;
+    plg_check(plg_window(source.data(), totalT, &s, &e, &lt), "plg_window");
+    return e;
+  }
+  int localT() const {
+    int s = 0, e = 0, lt = 0;
+    plg_check(plg_window(source.data(), totalT, &s, &e, &lt), "plg_window");
+    return lt;
+  }
+
+  // include/PLEGMA_Correlator.h:81-85
+  PLEGMA_Correlator(CORR_SPACE corr_space, site source, int Q2_max = 0, int totalT = Lattice::get().L[3])
+      : source(source), totalT(totalT), corr_space(corr_space) {
+    if (!Lattice::get().init) PLEGMA_error("PLEGMA_amd_init must be called before creating correlators");
+    if (corr_space == MOMENTUM_SPACE) corr_mom_space.reset(new PLEGMA_FT<Float>(Q2_max, 3, false, localT()));
+  }
+  PLEGMA_Correlator(const PLEGMA_Correlator &) = delete;
+  PLEGMA_Correlator &operator=(const PLEGMA_Correlator &) = delete;
+  ~PLEGMA_Correlator() {
+    if (d_out) (void)hipFree(d_out);
+  }
+
+  CORR_SPACE getCorrSpace() const { return corr_space; }
+  Float *H_elem() const { return corr_space == MOMENTUM_SPACE ? corr_mom_space->H_elem() : corr_pos_space.get(); }
+  site getSource() { return source; }
+  int getTotalT() { return totalT; }
+  size_t nDatasets() const { return std::max(datasets.size(), (size_t)1); }
+  size_t nGroups() const { return std::max(groups.size(), (size_t)1); }
+  // include/PLEGMA_Correlator.h:113-127
+  size_t getSiteSize() const {
+    size_t s = nDatasets() * nGroups();
+    for (int n : shape) s *= n;
+    return s;
+  }
+  size_t getVolSize() const {
+    if (corr_space == MOMENTUM_SPACE) return (size_t)corr_mom_space->Nmoms() * corr_mom_space->DimT();
+    return (size_t)Lattice::V3() * localT();
+  }
+  size_t getTotalSize() const { return getSiteSize() * getVolSize(); }
+  void setQ2max(int Q2_max) { corr_mom_space.reset(new PLEGMA_FT<Float>(Q2_max, 3, false, localT())); }
+  void setFixMomVec(std::vector<int> &v) { corr_mom_space.reset(new PLEGMA_FT<Float>(v, 3, false, localT())); }
+  void setFixMomList(std::vector<std::vector<int>> &l) {
+    corr_mom_space.reset(new PLEGMA_FT<Float>(l, 3, false, localT()));
+  }
+  std::vector<std::vector<int>> getMomList() {
+    std::vector<std::vector<int>> out;
+    if (corr_space != MOMENTUM_SPACE) return out;
+    for (auto &m : corr_mom_space->MomList()) {
+      out.push_back({});
+      for (auto x : m) out.back().push_back((int)std::lround(x));
+    }
+    return out;
+  }
+  std::vector<std::string> getDatasets() const { return datasets; }
+  std::vector<std::string> getGroups() const { return groups; }
+  void setDatasets(std::vector<std::string> d) {
+    if (datasets.size() == d.size() || (datasets.empty() && d.size() == 1)) datasets = d;
+    else PLEGMA_error("Given vector size do not match. This would change the correlator size.");
+  }
+  void setDatasets(std::string s) { setDatasets(std::vector<std::string>{s}); }
+  void setGroups(std::vector<std::string> d) {
+    if (groups.size() == d.size() || (groups.empty() && d.size() == 1)) groups = d;
+    else PLEGMA_error("Given vector size do not match. This would change the correlator size.");
+  }
+  void setGroups(std::string s) { setGroups(std::vector<std::string>{s}); }
+  std::string getDescription() const { return description; }
+
+  // ---- contractions (lib/PLEGMA_Correlator.cu:43-135)
+  void contractMesons(PLEGMA_Propagator<Float> &prop1, PLEGMA_Propagator<Float> &prop2) {
+    shape = {10};
+    datasets = {"twop_meson_1", "twop_meson_2"};
+    groups = {"mesons"};
+    description = "pseudoscalar, scalar, g5g1, g5g2, g5g3, g5g4, g1, g2, g3, g4";
+    initialize();
+    run(PLG_MESONS, 0, prop1, prop2);
+  }
+  void contractMesonsNew(PLEGMA_Propagator<Float> &prop1, PLEGMA_Propagator<Float> &prop2) {
+    shape = {10};
+    datasets = {"twop_meson"};
+    groups = {"mesons"};
+    description = "pseudoscalar, scalar, g5g1, g5g2, g5g3, g5g4, g1, g2, g3, g4";
+    initialize();
+    run(PLG_MESONS_NEW, 0, prop1, prop2);
+  }
+  void contractMesonsAll(PLEGMA_Propagator<Float> &prop1, PLEGMA_Propagator<Float> &prop2) {
+    shape = {34};
+    datasets = {"twop_meson"};
+    groups = {"mesons"};
+    description = "pseudoscalar, scalar";
+    for (int p = 0; p < 2; p++) {
+      const std::string va = p ? "" : "g5";
+      for (int i = 1; i < 5; i++)
+        for (int j = 1; j < 5; j++)
+          description += ", (" + va + "g" + std::to_string(i) + "," + va + "g" + std::to_string(j) + ")";
+    }
+    initialize();
+    run(PLG_MESONS_ALL, 0, prop1, prop2);
+  }
+  // groups: nucl_nucl; with group_mask also deltapp_deltamm_{11,22,33}
+  // (PLG_G_* bits; the reference's PLEGMA_LIGHT_BARYONS build adds them)
+  void contractBaryons(PLEGMA_Propagator<Float> &prop1, PLEGMA_Propagator<Float> &prop2,
+                       int group_mask = PLG_G_NUCL) {
+    shape = {16};
+    datasets = {"twop_baryon_1", "twop_baryon_2"};
+    groups.clear();
+    const char *names[10] = {"baryons/nucl_nucl", "baryons/nucl_nucl2", "baryons/nucl2_nucl",
+                             "baryons/nucl2_nucl2", "baryons/deltap_deltaz_11", "baryons/deltap_deltaz_22",
+                             "baryons/deltap_deltaz_33", "baryons/deltapp_deltamm_11",
+                             "baryons/deltapp_deltamm_22", "baryons/deltapp_deltamm_33"};
+    for (int b = 0; b < 10; b++)
+      if (group_mask & (1 << b)) groups.push_back(names[b]);
+    description = "1,g1,g2,g3,g4,g5,g5g1,g5g2,g5g3,g5g4,s12,s13,s23,s41,s42,s43";
+    initialize();
+    run(PLG_BARYONS, group_mask, prop1, prop2);
+  }
+  // ---- kinds without a single reference function
+  // 16 mesons + proton/Delta++ groups on one propagator (the fused kernel of
+  // the mixed path); datasets: one
+  void contractTwoPoint(PLEGMA_Propagator<Float> &prop, int group_mask = PLG_G_NUCL | PLG_G_DELTAPP_ALL) {
+    shape = {16};
+    datasets = {"twop"};
+    groups = {"mesons16"};
+    const char *names[4] = {"baryons/nucl_nucl", "baryons/deltapp_deltamm_11", "baryons/deltapp_deltamm_22",
+                            "baryons/deltapp_deltamm_33"};
+    const int bits[4] = {0, 7, 8, 9};
+    for (int k = 0; k < 4; k++)
+      if (group_mask & (1 << bits[k])) groups.push_back(names[k]);
+    description = "mesons: GAMMAS order; baryons: mu*4 + mu'";
+    initialize();
+    run(PLG_TWOPT, group_mask, prop, prop);
+  }
+  // heavy-light mesons and the four-propagator meson-meson block, the
+  // counterpart of contractTetraquarks (lib/kernels/PLEGMA_heavy_light_tetraquarks.cu:163-212)
+  void contractMesonMeson(PLEGMA_Propagator<Float> &propLight, PLEGMA_Propagator<Float> &propHeavy) {
+    shape = {20};
+    datasets = {"meson_meson"};
+    groups = {"heavy_light"};
+    description = "Tr[G SL g5 G SH^dag g5] for the 16 GAMMAS, then (Tr A)^2 - Tr[A A] for G = g5, g1, g2, g3";
+    initialize();
+    run(PLG_MESON_MESON, 0, propLight, propHeavy);
+  }
+
+  // ---- output (lib/PLEGMA_Correlator.cu:721-806, momentum space)
+  void writeASCII(std::string filename) const {
+    if (corr_space != MOMENTUM_SPACE) PLEGMA_error("WriteASCII do not support writing in position space.");
+    const int T = L(3), Nm = corr_mom_space->Nmoms(), lt = localT();
+    const size_t ss = getSiteSize(), sr = ss / (nDatasets() * nGroups());
+    char tail[64];
+    std::snprintf(tail, sizeof(tail), "_sx%02dsy%02dsz%02dst%02d.dat", source[0], source[1], source[2], source[3]);
+    FILE *f = std::fopen((filename + tail).c_str(), "w");
+    if (!f) PLEGMA_error("Error opening file for writing");
+    const auto moms = corr_mom_space->MomList();
+    const Float *H = H_elem();
+    for (int it = 0; it < T; it++) {
+      const int t = (it + source[3]) % T;  // lattice time of line it
+      int j = -1;                         // buffer row holding t
+      for (int r = 0; r < lt && j < 0; r++) {
+        int tr = 0;
+        plg_check(plg_row_time(source.data(), totalT, r, &tr), "plg_row_time");
+        if (tr == t) j = r;
+      }
+      if (j < 0) continue;
+      for (int m = 0; m < Nm; m++) {
+        std::fprintf(f, "%d  %+d  %+d  %+d ", it, (int)std::round(moms[m][0]), (int)std::round(moms[m][1]),
+                     (int)std::round(moms[m][2]));
+        for (size_t id = 0; id < nDatasets(); id++)
+          for (size_t ig = 0; ig < nGroups(); ig++)
+            for (size_t is = 0; is < sr; is++) {
+              const size_t k = (((ig * nDatasets() + id) * lt + j) * Nm + m) * sr + is;
+              std::fprintf(f, "%+e %+eI ", (double)H[2 * k], (double)H[2 * k + 1]);
+            }
+        std::fprintf(f, "\n");
+      }
+    }
+    std::fclose(f);
+  }
+  void writeHDF5(std::string) const { PLEGMA_error("HDF5 output is not part of this library"); }
+  void writeFile(std::string filename, FILE_FORMAT format = DEFAULT_FORMAT) const {
+    if (format == ASCII_FORMAT || format == DEFAULT_FORMAT) writeASCII(filename);
+    else writeHDF5(filename);
+  }
+};
+
+}  // namespace plegma

@@ -162,6 +162,29 @@ int plg_synth_prop_perturbed(const plg_prop *dst, uint64_t seed, uint32_t base_s
  * the copy stores entry * 2^-e */
 int plg_cast_prop(const plg_prop *dst, const plg_prop *src, void *stream);
 
+/* ---- learned part (no counterpart in the reference): a multilayer
+ * perceptron x -> [Linear -> SiLU] x (nlayers - 1) -> Linear -> y with BF16
+ * weights/activations, FP32 accumulation and FP32 master weights, trained
+ * with Adam on mean squared error.  dims[0] inputs ... dims[nlayers] outputs.
+ * Parameter vector (FP32): for each layer W [out][in] row-major, then b [out].
+ * x, t, y are device pointers to row-major [batch][features] FP32 arrays. */
+typedef struct plg_mlp_s *plg_mlp;
+int plg_mlp_create(const int *dims, int nlayers, int max_batch, plg_mlp *out);
+int plg_mlp_nparams(plg_mlp net, int64_t *n);
+/* host copies of the FP32 parameters; set also resets the optimiser state */
+int plg_mlp_set_params(plg_mlp net, const float *host);
+int plg_mlp_get_params(plg_mlp net, float *host);
+/* inference: y = net(x) */
+int plg_mlp_forward(plg_mlp net, const float *x, int batch, float *y, void *stream);
+/* loss = mean (net(x) - t)^2 and its gradient w.r.t. the parameters
+ * (grad_host: nparams floats, may be NULL) */
+int plg_mlp_gradient(plg_mlp net, const float *x, const float *t, int batch, double *loss,
+                     float *grad_host, void *stream);
+/* `steps` full-batch Adam steps (beta 0.9/0.999, eps 1e-8); loss after the last */
+int plg_mlp_train(plg_mlp net, const float *x, const float *t, int batch, int steps, float lr,
+                  double *loss, void *stream);
+int plg_mlp_destroy(plg_mlp net);
+
 #ifdef __cplusplus
 }
 #endif

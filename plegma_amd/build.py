@@ -1,7 +1,9 @@
 """Build recipe of libplegma_amd.so (hipcc, gfx950, in-tree).
 
 Compiles plegma_amd/csrc/*.hip into plegma_amd/libplegma_amd.so.  Objects go
-to plegma_amd/_build/.  Called by __graft_entry__.build(); runnable directly:
+to plegma_amd/_build/, as does class_driver, the C++ program that drives the
+class layer of include/plegma/ (its tests run it).  Called by
+__graft_entry__.build(); runnable directly:
     python -m plegma_amd.build
 """
 from __future__ import annotations
@@ -17,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libplegma_amd.so")
 SOURCES = ["plg_api.hip", "plg_contract.hip", "plg_baryon.hip", "plg_project.hip", "plg_project_bf16.hip",
-           "plg_synth.hip", "plg_mesmes.hip"]
+           "plg_synth.hip", "plg_mesmes.hip", "plg_mlp.hip"]
 ARCH = "gfx950"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -49,9 +51,26 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_driver()
     if verbose:
         print("built", LIB)
     return LIB
+
+
+DRIVER = os.path.join(BUILD, "class_driver")
+
+
+def build_driver() -> str:
+    """host C++ (g++) over the class headers, linked to the library in-tree"""
+    src = os.path.join(ROOT, "tests", "cpp", "class_driver.cpp")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), "-I/opt/rocm/include",
+           src, "-o", DRIVER + ".tmp", "-L" + HERE, "-lplegma_amd", "-L/opt/rocm/lib", "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"class driver build failed:\n{r.stderr}")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -29,7 +29,9 @@ STATUS = {0: "PLG_OK", -1: "PLG_ERR_INVALID", -2: "PLG_ERR_NOT_INIT", -3: "PLG_E
 EXPORTS = ["plg_init", "plg_finalize", "plg_last_error", "plg_version", "plg_momlist",
            "plg_window", "plg_row_time", "plg_kind_shape", "plg_correlator",
            "plg_correlator_size", "plg_batch_create", "plg_batch_run", "plg_batch_info",
-           "plg_batch_destroy", "plg_batch_profile", "plg_synth_prop", "plg_synth_prop_perturbed", "plg_cast_prop"]
+           "plg_batch_destroy", "plg_batch_profile", "plg_synth_prop", "plg_synth_prop_perturbed", "plg_cast_prop",
+           "plg_mlp_create", "plg_mlp_nparams", "plg_mlp_set_params", "plg_mlp_get_params",
+           "plg_mlp_forward", "plg_mlp_gradient", "plg_mlp_train", "plg_mlp_destroy"]
 
 
 class PlgError(RuntimeError):
@@ -85,6 +87,17 @@ def lib():
                                                C.c_uint32, C.c_double, C.c_int, C.c_double,
                                                C.c_void_p]
         L.plg_cast_prop.argtypes = [C.POINTER(plg_prop), C.POINTER(plg_prop), C.c_void_p]
+        fp = C.POINTER(C.c_float)
+        L.plg_mlp_create.argtypes = [ip, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        L.plg_mlp_nparams.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        L.plg_mlp_set_params.argtypes = [C.c_void_p, fp]
+        L.plg_mlp_get_params.argtypes = [C.c_void_p, fp]
+        L.plg_mlp_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        L.plg_mlp_gradient.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                       C.POINTER(C.c_double), fp, C.c_void_p]
+        L.plg_mlp_train.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_float,
+                                    C.POINTER(C.c_double), C.c_void_p]
+        L.plg_mlp_destroy.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -223,3 +236,53 @@ def synth_prop_perturbed(view, seed, base_stream, pert_stream, eps, t0, lam, str
 
 def cast_prop(dst, src, stream=None):
     check(lib().plg_cast_prop(C.byref(dst), C.byref(src), _stream(stream)))
+
+
+class Mlp:
+    """plg_mlp_* wrapper.  Inputs/targets/outputs are device tensors (torch,
+    float32, contiguous [batch][features]); parameters travel as numpy float32."""
+
+    def __init__(self, dims, max_batch):
+        self.dims = [int(d) for d in dims]
+        h = C.c_void_p()
+        check(lib().plg_mlp_create(ivec(self.dims), len(self.dims) - 1, max_batch, C.byref(h)))
+        self.h = h
+        n = C.c_int64()
+        check(lib().plg_mlp_nparams(self.h, C.byref(n)))
+        self.nparams = n.value
+
+    def set_params(self, p):
+        import numpy as np
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparams
+        check(lib().plg_mlp_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def get_params(self):
+        import numpy as np
+        p = np.empty(self.nparams, dtype=np.float32)
+        check(lib().plg_mlp_get_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))))
+        return p
+
+    def forward(self, x, y, stream=None):
+        check(lib().plg_mlp_forward(self.h, C.c_void_p(x.data_ptr()), x.shape[0],
+                                    C.c_void_p(y.data_ptr()), _stream(stream)))
+
+    def gradient(self, x, t, stream=None):
+        import numpy as np
+        g = np.empty(self.nparams, dtype=np.float32)
+        loss = C.c_double()
+        check(lib().plg_mlp_gradient(self.h, C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()),
+                                     x.shape[0], C.byref(loss),
+                                     g.ctypes.data_as(C.POINTER(C.c_float)), _stream(stream)))
+        return loss.value, g
+
+    def train(self, x, t, steps, lr, stream=None):
+        loss = C.c_double()
+        check(lib().plg_mlp_train(self.h, C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()),
+                                  x.shape[0], steps, lr, C.byref(loss), _stream(stream)))
+        return loss.value
+
+    def close(self):
+        if self.h:
+            lib().plg_mlp_destroy(self.h)
+            self.h = None

@@ -113,6 +113,14 @@ template <typename T> __host__ __device__ inline void cfma_conj(cx<T> &a, cx<T> 
   a.im = fma(b.im, c.re, a.im);
   a.im = fma(-b.re, c.im, a.im);
 }
+// bf16 bit pattern of a finite float, rounded to nearest even, and back
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_val(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
 // a += i^PH * b * c as four FMAs (the phase folds into operand signs)
 template <int PH, typename T> __host__ __device__ inline void cfma_iph(cx<T> &a, cx<T> b, cx<T> c) {
   constexpr int p = PH & 3;

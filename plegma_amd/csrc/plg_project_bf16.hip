@@ -34,12 +34,6 @@ constexpr int PB_LDB = 2 * PB_KT + 8;  // padded bf16 row of the B stage
 constexpr int PB_MMAX = 512;         // phase-table entries held in LDS
 constexpr int PB_FLUSH = 8;
 
-__device__ __forceinline__ uint16_t bf16_rn_(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf16_f_(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // k mod M for 0 <= k < 2^24 with a float reciprocal (exact quotient +-1, fixed up)
 __device__ __forceinline__ int mod_m(int k, int M, float invM) {
@@ -129,10 +123,10 @@ __global__ __launch_bounds__(PB_THREADS) void k_project_bf16(ProjGeom g, const c
     uint16_t pc[3][4];
 #pragma unroll
     for (int e = 0; e < 4; e++) {
-      pc[0][e] = bf16_rn_(v[e]);
-      const float r1 = v[e] - bf16_f_(pc[0][e]);
-      pc[1][e] = bf16_rn_(r1);
-      pc[2][e] = bf16_rn_(r1 - bf16_f_(pc[1][e]));
+      pc[0][e] = bf16_rne(v[e]);
+      const float r1 = v[e] - bf16_val(pc[0][e]);
+      pc[1][e] = bf16_rne(r1);
+      pc[2][e] = bf16_rne(r1 - bf16_val(pc[1][e]));
     }
 #pragma unroll
     for (int p = 0; p < NP; p++) {
@@ -325,10 +319,10 @@ __global__ __launch_bounds__(PB_THREADS, 2) void k_project_bf16_sa(ProjGeom g, c
       uint16_t h[3][2];
 #pragma unroll
       for (int e = 0; e < 2; e++) {
-        h[0][e] = bf16_rn_(e2[e]);
-        const float r1 = e2[e] - bf16_f_(h[0][e]);
-        h[1][e] = bf16_rn_(r1);
-        h[2][e] = bf16_rn_(r1 - bf16_f_(h[1][e]));
+        h[0][e] = bf16_rne(e2[e]);
+        const float r1 = e2[e] - bf16_val(h[0][e]);
+        h[1][e] = bf16_rne(r1);
+        h[2][e] = bf16_rne(r1 - bf16_val(h[1][e]));
       }
 #pragma unroll
       for (int p = 0; p < NP; p++) {

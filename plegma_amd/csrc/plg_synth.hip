@@ -8,6 +8,8 @@
 // (plegma/Calc_2pt.cpp:85-114), which this path does not compute.
 #include "plg_internal.h"
 
+#include <algorithm>
+
 namespace plg {
 
 __device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
@@ -97,22 +99,34 @@ __global__ void k_fill_int(int *p, int n, int v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
-__global__ void k_plane_exp(int *__restrict__ pexp, int dtf, const cx<double> *__restrict__ src, int64_t scs,
-                            int stf, int64_t V3, int T_lat) {
-  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int pl = blockIdx.y, comp = blockIdx.z;
+// largest binary exponent of the entries of each plane: a grid of (blocks,
+// planes); every block strides over all 144 components of its plane and
+// reduces in registers, across the wavefront and through LDS, so each block
+// issues one atomic
+__global__ __launch_bounds__(256) void k_plane_exp(int *__restrict__ pexp, int dtf,
+                                                   const cx<double> *__restrict__ src, int64_t scs, int stf,
+                                                   int64_t V3, int T_lat) {
+  __shared__ int red[4];
+  const int pl = blockIdx.y;
+  const int t = (dtf + pl) % T_lat;
+  int spl = t - stf;
+  spl += spl < 0 ? T_lat : 0;
   int e = INT_MIN;
-  if (x < V3) {
-    const int t = (dtf + pl) % T_lat;
-    int spl = t - stf;
-    spl += spl < 0 ? T_lat : 0;
-    const cx<double> v = src[(int64_t)comp * scs + (int64_t)spl * V3 + x];
-    const double a = fmax(fabs(v.re), fabs(v.im));
-    if (a > 0) e = ilogb(a);
+  for (int comp = 0; comp < 144; comp++) {
+    const cx<double> *b = src + (int64_t)comp * scs + (int64_t)spl * V3;
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < V3; x += (int64_t)gridDim.x * blockDim.x) {
+      const cx<double> v = b[x];
+      const double a = fmax(fabs(v.re), fabs(v.im));
+      if (a > 0) e = max(e, ilogb(a));
+    }
   }
-  // wave reduction, then one atomic per wave
   for (int off = 32; off > 0; off >>= 1) e = max(e, __shfl_xor(e, off));
-  if ((threadIdx.x & 63) == 0 && e != INT_MIN) atomicMax(&pexp[pl], e);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    e = max(max(red[0], red[1]), max(red[2], red[3]));
+    if (e != INT_MIN) atomicMax(&pexp[pl], e);
+  }
 }
 
 // dst (F32) <- src (F64), round to nearest, over dst's planes
@@ -144,8 +158,9 @@ int launch_cast(const plg_prop &dst, const plg_prop &src, int64_t V3, int T_lat,
   int *pe = const_cast<int *>(dst.plane_exp);
   if (pe) {
     k_fill_int<<<(dst.nt + 255) / 256, 256, 0, st>>>(pe, dst.nt, INT_MIN);
-    k_plane_exp<<<grid, block, 0, st>>>(pe, dst.t_first, (const cx<double> *)src.data, src.comp_stride,
-                                        src.t_first, V3, T_lat);
+    dim3 pgrid((unsigned)std::min<int64_t>(64, (V3 + 255) / 256), (unsigned)dst.nt);
+    k_plane_exp<<<pgrid, block, 0, st>>>(pe, dst.t_first, (const cx<double> *)src.data, src.comp_stride,
+                                         src.t_first, V3, T_lat);
   }
   k_cast<<<grid, block, 0, st>>>((cx<float> *)dst.data, dst.comp_stride, dst.t_first, (const cx<double> *)src.data,
                                  src.comp_stride, src.t_first, V3, T_lat, pe);

@@ -1,0 +1,71 @@
+// class_driver.cpp -- drives the C++ class layer (include/plegma/) the way a
+// PLEGMA driver does (plegma/Calc_2pt.cpp:121-132, 160-266): one lattice,
+// seeded propagators, a correlator per contraction, H_elem and the ASCII file.
+// Writes each H_elem as raw little-endian Float pairs to <outdir>/<name>.bin
+// for the GPU tests to compare with the CPU oracle.
+//
+//   class_driver <outdir> <f64|f32>
+#include <cstdio>
+#include <string>
+
+#include "plegma/PLEGMA_Correlator.h"
+
+using namespace plegma;
+
+template <typename Float> static void dump(const PLEGMA_Correlator<Float> &c, const std::string &path) {
+  FILE *f = std::fopen(path.c_str(), "wb");
+  if (!f) PLEGMA_error("cannot write %s", path.c_str());
+  std::fwrite(c.H_elem(), sizeof(Float), 2 * c.getTotalSize(), f);
+  std::fclose(f);
+  std::printf("%s %zu\n", path.c_str(), c.getTotalSize());
+}
+
+template <typename Float> static int run(const std::string &out) {
+  const std::string tag = sizeof(Float) == 8 ? "f64" : "f32";
+  PLEGMA_Propagator<Float> u, d;
+  if (sizeof(Float) == 8) {
+    u.synthesize(1234, 1, 0, 0.5);
+    d.synthesize(1234, 2, 0, 0.5);
+  } else {
+    PLEGMA_Propagator<double> u64, d64;
+    u64.synthesize(1234, 1, 0, 0.5);
+    d64.synthesize(1234, 2, 0, 0.5);
+    u.castFrom(u64);
+    d.castFrom(d64);
+  }
+  site src(1, 2, 3, 5);
+  {
+    PLEGMA_Correlator<Float> c(MOMENTUM_SPACE, src, 4, 6);
+    c.contractMesons(u, d);
+    dump(c, out + "/mesons_" + tag + ".bin");
+    c.writeASCII(out + "/mesons_" + tag);
+  }
+  {
+    PLEGMA_Correlator<Float> c(MOMENTUM_SPACE, src, 4, 6);
+    c.contractBaryons(u, d, PLG_G_NUCL | PLG_G_DELTAPP_ALL);
+    dump(c, out + "/baryons_" + tag + ".bin");
+  }
+  {
+    PLEGMA_Correlator<Float> c(MOMENTUM_SPACE, src, 1);  // full time extent
+    c.contractMesonsAll(u, d);
+    dump(c, out + "/mesons_all_" + tag + ".bin");
+    c.writeASCII(out + "/mesons_all_" + tag);
+  }
+  {
+    PLEGMA_Correlator<Float> c(MOMENTUM_SPACE, src, 4, 6);
+    c.contractMesonMeson(u, d);
+    dump(c, out + "/mesmes_" + tag + ".bin");
+  }
+  return 0;
+}
+
+int main(int argc, char **argv) {
+  if (argc != 3) {
+    std::fprintf(stderr, "usage: %s <outdir> <f64|f32>\n", argv[0]);
+    return 2;
+  }
+  PLEGMA_amd_init({4, 4, 4, 8}, 0);
+  const int rc = std::string(argv[2]) == "f32" ? run<float>(argv[1]) : run<double>(argv[1]);
+  PLEGMA_amd_finalize();
+  return rc;
+}

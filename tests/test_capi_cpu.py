@@ -59,3 +59,15 @@ def test_calls_before_init_fail_cleanly():
     with pytest.raises(capi.PlgError) as e:
         capi.window([0, 0, 0, 0], 4)
     assert e.value.code == -2
+
+
+def test_class_headers_compile_and_driver_is_built():
+    """the C++ class layer is plain host C++17 over the C ABI"""
+    import subprocess
+    from plegma_amd import build
+    src = os.path.join(ROOT, "tests", "cpp", "class_driver.cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror",
+                        "-I" + os.path.join(ROOT, "include"), "-I/opt/rocm/include", src],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert os.access(build.DRIVER, os.X_OK)

@@ -1,0 +1,72 @@
+"""The C++ class layer (include/plegma/: PLEGMA_Propagator, PLEGMA_FT,
+PLEGMA_Correlator over the C ABI), driven by tests/cpp/class_driver.cpp as a
+PLEGMA driver would use it, against the CPU oracle on the same seeded
+propagators: H_elem layout and values, and the ASCII file."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from plegma_amd import build
+
+import gpu_util as G
+
+pytestmark = pytest.mark.gpu
+
+L = [4, 4, 4, 8]
+SRC = [1, 2, 3, 5]
+
+
+@pytest.fixture(scope="module")
+def outputs(tmp_path_factory):
+    G.torch_cuda()
+    d = tmp_path_factory.mktemp("classes")
+    for prec in ("f64", "f32"):
+        print(f"[gpu] class driver {prec}", flush=True)
+        r = subprocess.run([build.DRIVER, str(d), prec], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+    u = O.synth_prop(L, 1234, 1, 0, 0.5)
+    dn = O.synth_prop(L, 1234, 2, 0, 0.5)
+    return d, u, dn
+
+
+def _load(d, name, prec):
+    a = np.fromfile(os.path.join(d, f"{name}_{prec}.bin"), dtype=np.float64 if prec == "f64" else np.float32)
+    a = a.astype(np.float64)
+    return a[0::2] + 1j * a[1::2]
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-6)])
+@pytest.mark.parametrize("name,kind,q2,totalT,mask", [
+    ("mesons", O.MESONS, 4, 6, O.G_NUCL),
+    ("baryons", O.BARYONS, 4, 6, O.G_NUCL | sum(O.G_DELTA32)),
+    ("mesons_all", O.MESONS_ALL, 1, 8, O.G_NUCL),
+    ("mesmes", O.MESON_MESON, 4, 6, O.G_NUCL)])
+def test_class_results(outputs, prec, tol, name, kind, q2, totalT, mask):
+    d, u, dn = outputs
+    got = _load(d, name, prec)
+    ref = O.correlator(kind, L, SRC, totalT, q2, u, dn, mask=mask)
+    assert got.size == ref.size
+    got = got.reshape(ref.shape)
+    assert G.rel_err(got, ref) < tol
+
+
+def test_ascii_file(outputs):
+    """lines 'it nx ny nz (re imI) x site size' ordered by time from the source
+    (lib/PLEGMA_Correlator.cu:786-797); totalT = T so every time is present"""
+    d, u, dn = outputs
+    path = os.path.join(d, "mesons_all_f64_sx01sy02sz03st05.dat")
+    lines = open(path).read().splitlines()
+    moms = O.momlist(1)
+    assert len(lines) == 8 * len(moms)
+    ref = O.correlator(O.MESONS_ALL, L, SRC, 8, 1, u, dn)          # [g][d][j][m][c]
+    for ln in (lines[0], lines[len(moms) + 3], lines[-1]):
+        tok = ln.split()
+        it, n = int(tok[0]), [int(x) for x in tok[1:4]]
+        m = [tuple(v) for v in moms.tolist()].index(tuple(n))
+        t = (it + SRC[3]) % 8
+        j = [O.row_time(L, SRC, 8, r) for r in range(8)].index(t)
+        vals = [complex(float(a), float(b[:-1])) for a, b in zip(tok[4::2], tok[5::2])]
+        np.testing.assert_allclose(vals, ref[0, 0, j, m], rtol=1e-5, atol=1e-6 * np.abs(ref).max())

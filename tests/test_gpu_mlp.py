@@ -1,0 +1,193 @@
+"""GPU checks of the learned part: the BF16 forward and backward kernels
+against the CPU network oracle (same rounding points, FP64 sums), training on
+the device, the bias-corrected estimator's gain over the labelled sources
+alone, and the two-point pipeline that feeds it (perturbed propagators)."""
+import numpy as np
+import pytest
+
+from oracle import mlp as M
+from oracle import oracle as O
+from plegma_amd import capi, learned
+
+import gpu_util as G
+
+pytestmark = pytest.mark.gpu
+
+# The device sums in FP32 in another order than the oracle's FP64 sums; where a
+# hidden activation lies within that difference of a bf16 rounding midpoint,
+# the two round it to neighbouring bf16 values (one unit, 2^-8 relative), and
+# every output of that sample moves by about 1e-3.  Such ties are rare: at most
+# one sample in 32 is allowed to carry one; every other sample agrees to 1e-5.
+ROW_TOL, TIE_TOL, MAX_TIE_ROWS = 1e-5, 1e-2, 1
+
+
+def _dev(a):
+    torch = G.torch_cuda()
+    return torch.tensor(np.ascontiguousarray(a, dtype=np.float32), device="cuda")
+
+
+def _net(dims, seed, max_batch=64):
+    net = capi.Mlp(dims, max_batch)
+    p = learned.init_params(dims, seed)
+    net.set_params(p)
+    return net, p.astype(np.float64)
+
+
+@pytest.mark.parametrize("dims,B", [([40, 64, 48, 24], 16), ([1088, 256, 256, 704], 32)])
+def test_forward_matches_oracle(dims, B):
+    torch = G.torch_cuda()
+    print(f"[gpu] mlp forward dims={dims} B={B}", flush=True)
+    net, p = _net(dims, 11, max_batch=B)
+    rng = np.random.default_rng(12)
+    x = rng.normal(size=(B, dims[0]))
+    y = torch.empty((B, dims[-1]), dtype=torch.float32, device="cuda")
+    net.forward(_dev(x), y)
+    got = y.cpu().numpy().astype(np.float64)
+    ref, _ = M.forward(p, dims, x.astype(np.float32).astype(np.float64), emulate=True)
+    row_err = np.abs(got - ref).max(axis=1) / np.abs(ref).max()
+    print(f"[gpu] forward rel err per sample: median {np.median(row_err):.2e}, "
+          f"max {row_err.max():.2e}", flush=True)
+    assert np.sum(row_err > ROW_TOL) <= MAX_TIE_ROWS
+    assert row_err.max() < TIE_TOL
+    net.close()
+
+
+@pytest.mark.parametrize("dims,B", [([40, 64, 48, 24], 16), ([1088, 256, 256, 704], 32)])
+def test_gradient_matches_oracle(dims, B):
+    """device loss and gradient vs the oracle with the device's bf16 rounding
+    points (differences: FP32 vs FP64 sums), and vs the all-FP64 network
+    (differences: bf16 storage, a few per cent)"""
+    print(f"[gpu] mlp gradient dims={dims} B={B}", flush=True)
+    net, p = _net(dims, 13, max_batch=B)
+    rng = np.random.default_rng(14)
+    x = rng.normal(size=(B, dims[0])).astype(np.float32).astype(np.float64)
+    t = rng.normal(size=(B, dims[-1])).astype(np.float32).astype(np.float64)
+    loss, g = net.gradient(_dev(x), _dev(t))
+    lref, gref = M.loss_grad(p, dims, x, t, emulate=True)
+    l64, g64 = M.loss_grad(p, dims, x, t, emulate=False)
+    e_emu = np.linalg.norm(g - gref) / np.linalg.norm(gref)
+    e_64 = np.linalg.norm(g - g64) / np.linalg.norm(g64)
+    print(f"[gpu] grad rel err vs emulated oracle {e_emu:.2e}, vs fp64 {e_64:.2e}", flush=True)
+    # Every dZ operand is rounded to bf16 too, so the backward pass has ~40k
+    # rounding points at the full size and a few of them fall on ties (see
+    # ROW_TOL), each moving one element by 2^-8.  The device must sit much
+    # closer to the emulated oracle than to the all-FP64 network.
+    assert loss == pytest.approx(lref, rel=1e-4)
+    assert e_emu < min(1e-3, 0.2 * e_64)
+    assert e_64 < 5e-2
+    # per layer as well (a wrong layer cannot hide behind a large one)
+    o = 0
+    for i, j in zip(dims[:-1], dims[1:]):
+        for n in (i * j, j):
+            seg = slice(o, o + n)
+            assert np.linalg.norm(g[seg] - gref[seg]) <= 2e-3 * np.linalg.norm(gref[seg]) + 1e-12
+            o += n
+    net.close()
+
+
+def test_training_fits_a_smooth_map():
+    torch = G.torch_cuda()
+    rng = np.random.default_rng(21)
+    dims = [8, 64, 64, 4]
+    A = rng.normal(size=(8, 4)) / np.sqrt(8)
+    x = rng.normal(size=(64, 8))
+    t = np.tanh(x @ A) + 0.1 * (x @ A) ** 2
+    net, _ = _net(dims, 22)
+    l0, _ = net.gradient(_dev(x), _dev(t))
+    l1 = net.train(_dev(x), _dev(t), 1500, 3e-3)
+    print(f"[gpu] training loss {l0:.3e} -> {l1:.3e}", flush=True)
+    assert l1 < l0 / 50
+    # the trained parameters moved and their bf16 copies are in use
+    p1 = net.get_params()
+    y = torch.empty((64, 4), dtype=torch.float32, device="cuda")
+    net.forward(_dev(x), y)
+    ref, _ = M.forward(p1.astype(np.float64), dims, x.astype(np.float32).astype(np.float64))
+    row_err = np.abs(y.cpu().numpy() - ref).max(axis=1) / np.abs(ref).max()
+    assert np.sum(row_err > ROW_TOL) <= 2 and row_err.max() < TIE_TOL
+    net.close()
+
+
+def test_bias_corrected_estimator_gains():
+    """With 64 labelled sources of 256 (32 train, 32 held out) and inputs that
+    carry information about the outputs, the bias-corrected mean over all
+    sources has a smaller error than the mean of the labelled sources alone
+    (the method without the network); over repeated draws.  (With 32 training
+    samples the map must be learnable from 32 points: 4 inputs here; with 12
+    the network does not generalise and the gain drops to 1.)"""
+    rng = np.random.default_rng(31)
+    d, nout = 4, 3
+    A = rng.normal(size=(d, nout)) / np.sqrt(d)
+    err_ml, err_lab = [], []
+    for trial in range(6):
+        x = rng.normal(size=(256, d))
+        y = x @ A + 0.2 * np.tanh(x @ A) ** 2 + 0.05 * rng.normal(size=(256, nout))
+        truth = y.mean(axis=0)
+        net = learned.CorrelatorNet(d, nout, hidden=(32, 32), seed=trial, max_batch=256)
+        net.fit(x[:32], y[:32], steps=400, lr=3e-3)
+        est = learned.bias_corrected(net, x[32:64], y[32:64], x[64:])
+        net.close()
+        ml = (y[:64].sum(axis=0) + est.pred.sum(axis=0)) / 256
+        err_ml.append(np.sum((ml - truth) ** 2))
+        err_lab.append(np.sum((y[:64].mean(axis=0) - truth) ** 2))
+    gain = np.mean(err_lab) / np.mean(err_ml)
+    print(f"[gpu] estimator mse: labelled only {np.mean(err_lab):.3e}, "
+          f"bias-corrected {np.mean(err_ml):.3e}, gain {gain:.1f}x", flush=True)
+    assert gain > 3
+
+
+def test_perturbed_synth_matches_oracle():
+    torch = G.torch_cuda()
+    L = [4, 4, 4, 8]
+    capi.init(L, 0)
+    try:
+        V3 = 64
+        t = torch.empty((144, 8 * V3, 2), dtype=torch.float64, device="cuda")
+        v = capi.plg_prop(t.data_ptr(), capi.F64, 0, 8, 0, 8 * V3)
+        capi.synth_prop_perturbed(v, 77, 1000, 5, 0.05, 3, 0.5)
+        got = G.host(t)
+        ref = O.synth_prop(L, 77, 1000, 3, 0.5) + 0.05 * O.synth_prop(L, 77, 5, 3, 0.5)
+        np.testing.assert_allclose(got, ref, rtol=1e-13, atol=1e-15)
+    finally:
+        capi.finalize()
+
+
+def test_learned_pipeline_small_lattice():
+    """the learned workload end to end on 4^3 x 8 with 64 sources: perturbed
+    propagators -> plain FP64 two-point (mesons + nucleon, |n|^2 <= 16) ->
+    features / targets -> device training -> bias-corrected predictions,
+    whose mean over the predicted sources agrees with the exact one within
+    the held-out error estimate"""
+    torch = G.torch_cuda()
+    L = [4, 4, 4, 8]
+    capi.init(L, 0)
+    try:
+        V3, T = 64, 8
+        moms = capi.momlist(16)
+        rng = np.random.default_rng(41)
+        t = torch.empty((144, T * V3, 2), dtype=torch.float64, device="cuda")
+        v = capi.plg_prop(t.data_ptr(), capi.F64, 0, T, 0, T * V3)
+        xs, ys = [], []
+        n = capi.correlator_size(capi.TWOPT, capi.G_NUCL, [0, 0, 0, 0], T, len(moms))
+        out = torch.empty((n, 2), dtype=torch.float64, device="cuda")
+        for s in range(64):
+            src = [int(rng.integers(0, 4)) for _ in range(3)] + [int(rng.integers(0, T))]
+            capi.synth_prop_perturbed(v, 5, 1000, s, 0.05, src[3], 0.5)
+            capi.correlator(capi.TWOPT, capi.G_NUCL, v, v, src, T, moms, capi.MODE_F64,
+                            capi.MOMENTUM_SPACE, capi.F64, out)
+            h = G.host(out).reshape(n, 2)
+            x, y = learned.features_targets(h[:, 0] + 1j * h[:, 1], moms, T)
+            xs.append(x)
+            ys.append(y)
+        X, Y = np.array(xs), np.array(ys)
+        assert X.shape == (64, T * 17) and Y.shape == (64, T * 11)
+        net = learned.CorrelatorNet(X.shape[1], Y.shape[1], hidden=(64, 64), seed=1, max_batch=64)
+        net.fit(X[:16], Y[:16], steps=300, lr=1e-3)
+        est = learned.bias_corrected(net, X[16:32], Y[16:32], X[32:])
+        net.close()
+        exact = Y[32:].mean(axis=0)
+        se = est.resid_rms / np.sqrt(16) + Y[32:].std(axis=0) / np.sqrt(32)
+        assert np.all(np.abs(est.mean - exact) <= 5 * se + 1e-30)
+        fb = learned.with_fallback(est, lambda: Y[32:], tol=0.0)
+        assert fb.used_fallback and np.array_equal(fb.pred, Y[32:])
+    finally:
+        capi.finalize()
