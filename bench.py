@@ -148,17 +148,29 @@ def stage_ms(batch, out, stream, reps=3):
     return {name: float(np.median([dict(r)[name] for r in runs])) for name, _ in runs[0]}
 
 
-def roofline(st_ms, nsites, nmom, nch, split, contract_flops, prop_bytes):
+def measured_traffic(workload, stage):
+    """HBM bytes per launch of a stage from the committed counter summary
+    (profiles/traffic.json), or None"""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            e = json.load(f).get(workload, {}).get(stage)
+        return (e["bytes"], e["kernel"]) if e else None
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def roofline(st_ms, nsites, nmom, nch, split, contract_flops, prop_bytes, workload):
     """the dominant stage against its roofline: the contraction against HBM
     (compulsory propagator read + site-correlator write) with its VALU rate
     beside it, the projection against BF16 MFMA (useful flops)"""
     dom = max(st_ms, key=st_ms.get)
     t = st_ms[dom] / 1e3
+    tr = measured_traffic(workload, dom)
     if dom.startswith("contract"):
         byts = nsites * (prop_bytes + nch * 8)
         achieved = byts / t / 1e9
         r = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": tr[0] if tr else None,
              "algorithmic": f"{prop_bytes} B FP32 propagator read + {nch * 8} B site "
                             f"correlator write per site",
              "valu": {"achieved": nsites * contract_flops / t / 1e12, "peak": FP32_VALU_PEAK_TFS,
@@ -168,9 +180,11 @@ def roofline(st_ms, nsites, nmom, nch, split, contract_flops, prop_bytes):
         useful = nsites * 8 * nmom * nch
         achieved = useful / t / 1e12
         r = {"kernel": dom, "bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFS,
-             "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFS, "traffic": None,
+             "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFS, "traffic": tr[0] if tr else None,
              "algorithmic": f"8 flop x {nmom} momenta x {nch} channels per site (useful)",
              "executed_frac": achieved * split / BF16_PEAK_TFS}
+    if tr:
+        r["traffic_source"] = f"rocprofv3 FETCH_SIZE/WRITE_SIZE of {tr[1]} (profiles/traffic.json)"
     return r
 
 
@@ -275,7 +289,7 @@ def run_batching(args, torch, capi, dev, stream):
         del plain
 
     res["roofline"] = roofline(st_ms, nsites, len(moms), 80, args.split, CONTRACT_FLOPS_PER_SITE,
-                               144 * 8)
+                               144 * 8, "batching")
     res["stages_ms"] = st_ms
     if not args.no_cpu:
         log("cpu baseline")
@@ -368,7 +382,7 @@ def run_largest(args, torch, capi, dev, stream):
         plain.close()
         del k64
     res["roofline"] = roofline(st_ms, nsites, len(moms), 20, args.split, MESMES_FLOPS_PER_SITE,
-                               2 * 144 * 8)
+                               2 * 144 * 8, "largest")
     res["stages_ms"] = st_ms
     if not args.no_cpu:
         log("cpu baseline")
