@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -479,9 +480,9 @@ static void choose_tiles(Plan &P) {
     tiles = ((ncols + 127) / 128) * ((Mreal + P.mtile - 1) / P.mtile);
   } else {
     P.mbw = pick_proj(std::max(P.nmom, 1));
-    const int nblk = (Mreal + 15) / 16;
-    const int rb = (P.mbw - 100) / 10, nbw = P.mbw % 10;
-    tiles = ((ncols + 64 * nbw - 1) / (64 * nbw)) * ((nblk + rb - 1) / rb);
+    // PLG_PROJ_CODE pins the projection tile (tile studies and tests)
+    if (const char *e = std::getenv("PLG_PROJ_CODE")) P.mbw = std::atoi(e);
+    tiles = proj_workgroups(P.mbw, ncols, std::max(P.nmom, 1));
   }
   int64_t ns = (1024 + tiles - 1) / tiles;
   ns = std::max<int64_t>(1, std::min<int64_t>(ns, g_ctx.V3 / 512));

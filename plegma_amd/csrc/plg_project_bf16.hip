@@ -423,6 +423,197 @@ __global__ __launch_bounds__(PB_THREADS, 2) void k_project_bf16_sa(ProjGeom g, c
       }
 }
 
+
+// ------------------------------------------------------------------ register-A variant
+// The four waves own disjoint row blocks (RBW each) and share one B tile of
+// NCB column blocks.  Every lane builds its A fragments in registers: for each
+// of its 4 sites and RBW rows it computes the phase index k = n.x (M/L) mod M
+// and reads the three pieces of its row's (re, im) operand pair with one
+// 16-byte LDS read (phase table entries of 8 words: re-row pieces, pad,
+// im-row pieces, pad).  The A operand never passes through LDS as a tile, and
+// the B tile is double-buffered, so a k-step has one barrier.
+constexpr int PR_TW = 8;
+
+template <int RBW, int NCB, int TERMS>
+__global__ __launch_bounds__(PB_THREADS) void k_project_bf16_rw(ProjGeom g, const cx<float> *__restrict__ c,
+                                                                const uint32_t *__restrict__ wtab,
+                                                                double *__restrict__ part) {
+  constexpr int NP = TERMS == 3 ? 2 : 3;
+  constexpr int NC = NCB * 16;                    // columns per workgroup
+  constexpr int CPT = NC * PB_KT / PB_THREADS;    // complex B values staged per thread
+  constexpr int TPC = PB_KT / CPT;                // threads per column
+  static_assert(CPT >= 1 && PB_KT % CPT == 0, "staging shape");
+  __shared__ __align__(16) uint32_t sW[PB_MMAX * PR_TW];
+  __shared__ __align__(16) uint16_t sB[2][NP][NC][PB_LDB];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t n0 = (int64_t)blockIdx.x * NC;
+  const int split = blockIdx.y;
+  const int rbw0 = (blockIdx.z * 4 + wave) * RBW;  // first row block of this wave
+  const int Mreal = 2 * g.nmom;
+  const int M = g.M;
+  const float invM = 1.0f / (float)M;
+  int64_t xb, xe;
+  {
+    const int64_t per = (g.V3 + g.nsplit - 1) / g.nsplit;
+    xb = (int64_t)split * per;
+    xe = xb + per < g.V3 ? xb + per : g.V3;
+  }
+  for (int i = tid; i < M; i += PB_THREADS) {
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+      sW[i * PR_TW + p] = wtab[6 * i + p];
+      sW[i * PR_TW + 4 + p] = wtab[6 * i + 3 + p];
+    }
+    sW[i * PR_TW + 3] = 0;
+    sW[i * PR_TW + 7] = 0;
+  }
+  // this lane's A rows: row block rbw0 + i, row (lane & 15) of it
+  int mpk[RBW], roff[RBW];
+#pragma unroll
+  for (int i = 0; i < RBW; i++) {
+    const int r = (rbw0 + i) * 16 + (lane & 15);
+    const int m = r >> 1;
+    roff[i] = (r & 1) * 4;
+    mpk[i] = (r < Mreal) ? (g.mom_scaled[3 * m] | (g.mom_scaled[3 * m + 1] << 10) | (g.mom_scaled[3 * m + 2] << 20))
+                         : -1;
+  }
+  const int bcol = tid / TPC, bs0 = (tid % TPC) * CPT;
+
+  f4 acc[RBW][NCB];
+  double acc64[RBW][NCB][4];
+#pragma unroll
+  for (int i = 0; i < RBW; i++)
+#pragma unroll
+    for (int j = 0; j < NCB; j++) {
+      acc[i][j] = f4{0, 0, 0, 0};
+#pragma unroll
+      for (int v = 0; v < 4; v++) acc64[i][j][v] = 0;
+    }
+
+  cx<float> bv[CPT];
+  auto load = [&](int64_t xt) {
+    const int64_t n = n0 + bcol;
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+      const int64_t x = xt + bs0 + q;
+      bv[q] = (n < g.ncols && x < xe) ? c[n * g.V3 + x] : cmake<float>(0, 0);
+    }
+  };
+  auto stage = [&](int b) {
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+      const float e2[2] = {bv[q].re, bv[q].im};
+      uint16_t h[3][2];
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        h[0][e] = bf16_rne(e2[e]);
+        const float r1 = e2[e] - bf16_val(h[0][e]);
+        h[1][e] = bf16_rne(r1);
+        h[2][e] = bf16_rne(r1 - bf16_val(h[1][e]));
+      }
+#pragma unroll
+      for (int p = 0; p < NP; p++)
+        *reinterpret_cast<uint32_t *>(&sB[b][p][bcol][2 * (bs0 + q)]) =
+            (uint32_t)h[p][0] | ((uint32_t)h[p][1] << 16);
+    }
+  };
+
+  int step = 0, buf = 0;
+  if (xb < xe) {
+    load(xb);
+    stage(0);
+  }
+  __syncthreads();  // phase table and the first B tile
+  for (int64_t xt = xb; xt < xe; xt += PB_KT) {
+    const bool more = xt + PB_KT < xe;
+    if (more) load(xt + PB_KT);
+    // ---- A fragments: sites xs + j (j < 4) of this lane, RBW rows
+    u4 af[RBW][NP];
+    {
+      const int64_t xs = xt + 4 * (lane >> 4);
+      int cx_ = (int)(xs % g.L[0]);
+      const int64_t q = xs / g.L[0];
+      int cy_ = (int)(q % g.L[1]);
+      int cz_ = (int)(q / g.L[1]);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const bool sv = xs + j < xe;
+#pragma unroll
+        for (int i = 0; i < RBW; i++) {
+          const int pk = mpk[i];
+          u4 w = {0, 0, 0, 0};
+          if (pk >= 0 && sv) {
+            const int k = mod_m((pk & 1023) * cx_ + ((pk >> 10) & 1023) * cy_ + (pk >> 20) * cz_, M, invM);
+            w = *reinterpret_cast<const u4 *>(&sW[k * PR_TW + roff[i]]);
+          }
+#pragma unroll
+          for (int p = 0; p < NP; p++) af[i][p][j] = w[p];
+        }
+        if (++cx_ == g.L[0]) {
+          cx_ = 0;
+          if (++cy_ == g.L[1]) {
+            cy_ = 0;
+            ++cz_;
+          }
+        }
+      }
+    }
+    // ---- MFMAs over the column blocks of the shared B tile
+    const int kk = 8 * (lane >> 4);
+#pragma unroll
+    for (int cb = 0; cb < NCB; cb++) {
+      bf8 bf[NP];
+#pragma unroll
+      for (int p = 0; p < NP; p++)
+        bf[p] = *reinterpret_cast<const bf8 *>(&sB[buf][p][cb * 16 + (lane & 15)][kk]);
+#pragma unroll
+      for (int i = 0; i < RBW; i++) {
+        if ((rbw0 + i) * 16 >= Mreal) continue;  // uniform: a padding block
+        f4 a = acc[i][cb];
+        const bf8 a0 = __builtin_bit_cast(bf8, af[i][0]), a1 = __builtin_bit_cast(bf8, af[i][1]);
+        if constexpr (TERMS == 6) {
+          const bf8 a2 = __builtin_bit_cast(bf8, af[i][2]);
+          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, bf[0], a, 0, 0, 0);
+          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bf[1], a, 0, 0, 0);
+          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bf[2], a, 0, 0, 0);
+        }
+        a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bf[0], a, 0, 0, 0);
+        a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bf[1], a, 0, 0, 0);
+        a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bf[0], a, 0, 0, 0);
+        acc[i][cb] = a;
+      }
+    }
+    if (++step == PB_FLUSH) {
+      step = 0;
+#pragma unroll
+      for (int i = 0; i < RBW; i++)
+#pragma unroll
+        for (int j = 0; j < NCB; j++) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) acc64[i][j][v] += (double)acc[i][j][v];
+          acc[i][j] = f4{0, 0, 0, 0};
+        }
+    }
+    // ---- the next B tile into the other buffer (read by no wave this step)
+    if (more) stage(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+  // D of 16x16x32 (f32): lane l, value v -> row 4*(l/16) + v, column l%16
+#pragma unroll
+  for (int i = 0; i < RBW; i++)
+#pragma unroll
+    for (int j = 0; j < NCB; j++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int mr = (rbw0 + i) * 16 + 4 * (lane >> 4) + v;
+        const int64_t n = n0 + j * 16 + (lane & 15);
+        if (mr < Mreal && n < g.ncols)
+          part[((int64_t)split * g.ncols + n) * Mreal + mr] = acc64[i][j][v] + (double)acc[i][j][v];
+      }
+}
+
 // row blocks per wave for a momentum count: fewest padded blocks, then fewest groups
 int pick_mbw(int nmom) {
   const int nblk = (2 * nmom + 15) / 16;
@@ -450,6 +641,26 @@ int launch_project_bf16(const ProjGeom &g, const cx<float> *c, cx<double> *part,
     if (terms == 6) k_project_bf16<MB, 6><<<grid, block, 0, st>>>(g, c, wtab, pd); \
     else k_project_bf16<MB, 3><<<grid, block, 0, st>>>(g, c, wtab, pd);            \
     break;
+  if (mbw >= 300) {
+    // register-A variant: code = 300 + 10*RBW + NCB
+    const int rbw = (mbw - 300) / 10, ncb = mbw % 10;
+    dim3 g3((unsigned)((g.ncols + 16 * ncb - 1) / (16 * ncb)), (unsigned)g.nsplit,
+            (unsigned)((nblk + 4 * rbw - 1) / (4 * rbw)));
+#define PLG_RW(RBW, NCB)                                                                            \
+  if (rbw == RBW && ncb == NCB) {                                                                   \
+    if (terms == 6) k_project_bf16_rw<RBW, NCB, 6><<<g3, block, 0, st>>>(g, c, wtab, pd);            \
+    else k_project_bf16_rw<RBW, NCB, 3><<<g3, block, 0, st>>>(g, c, wtab, pd);                       \
+    return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;                                  \
+  }
+    PLG_RW(1, 4)
+    PLG_RW(2, 4)
+    PLG_RW(3, 4)
+    PLG_RW(2, 2)
+    PLG_RW(3, 2)
+    PLG_RW(4, 2)
+#undef PLG_RW
+    return PLG_ERR_INVALID;
+  }
   if (mbw >= 100) {
     // shared-A variant: code = 100 + 10*RB + NBW
     const int rb = (mbw - 100) / 10, nbw = mbw % 10;
@@ -492,8 +703,26 @@ std::vector<int> proj_candidates(int nmom) {
     const int rb = (code - 100) / 10;
     if (rb <= 2 * nblk) v.push_back(code);
   }
+  for (int code : {314, 324, 334, 322, 332, 342}) {
+    const int rbw = (code - 300) / 10;
+    if (rbw <= nblk) v.push_back(code);
+  }
   v.push_back(pick_mbw(nmom));
   return v;
+}
+
+// workgroups of one K split for a projection code
+int64_t proj_workgroups(int code, int64_t ncols, int nmom) {
+  const int nblk = (2 * nmom + 15) / 16;
+  if (code >= 300) {
+    const int rbw = (code - 300) / 10, ncb = code % 10;
+    return ((ncols + 16 * ncb - 1) / (16 * ncb)) * ((nblk + 4 * rbw - 1) / (4 * rbw));
+  }
+  if (code >= 100) {
+    const int rb = (code - 100) / 10, nbw = code % 10;
+    return ((ncols + 64 * nbw - 1) / (64 * nbw)) * ((nblk + rb - 1) / rb);
+  }
+  return ((ncols + PB_NC - 1) / PB_NC) * ((nblk + 4 * code - 1) / (4 * code));
 }
 
 // default: the shared-A tile whose row groups waste the fewest padded blocks

@@ -350,3 +350,22 @@ def test_meson_meson_mixed_two_flavours():
     err = G.rel_err(H[:, :, rows], R[:, :, rows])
     print(f"[gpu] meson-meson mixed rel err {err:.2e}", flush=True)
     assert err < 1e-6
+
+
+# every projection tile the autotuner may pick, pinned one at a time
+PROJ_CODES = [211, 191, 152, 132, 122, 112, 314, 324, 334, 322, 332, 342, 1, 2, 3, 5, 9]
+
+
+@pytest.mark.parametrize("code", PROJ_CODES)
+def test_projection_tiles(props, code, monkeypatch):
+    (t1, v1, h1), _ = props
+    monkeypatch.setenv("PLG_PROJ_CODE", str(code))
+    keep, v32 = G.cast_f32(t1, v1, scaled=True)
+    src, totalT = [5, 3, 1, 7], 16
+    mask = capi.G_NUCL | capi.G_DELTAPP_ALL
+    print(f"[gpu] projection tile {code}", flush=True)
+    H = G.shaped(G.run_correlator(capi.TWOPT, mask, v32, v32, src, totalT, Q1, mode=capi.MODE_MIXED),
+                 capi.TWOPT, mask, src, totalT, Q1)
+    rows = [0, 9]
+    R = _twopt_oracle(h1, src, totalT, Q1, rows)
+    assert G.rel_err(H[:, :, rows], R[:, :, rows]) < 1e-6
