@@ -97,6 +97,29 @@ public:
   void scale(Float a) {
     for (size_t i = 0; i < sizeN; i++) h_elem.get()[i] *= a;
   }
+  // lib/PLEGMA_FT.cu:144-162: H_elem[t][dof][mom] = sum_x f(x, t) exp(sign i 2pi n.x/L)
+  // (FT_NAIVE and FT_GEMV are the same sum; the buffer is re-allocated and
+  // overwritten, as checkAllocation does in the reference).  Field: any type
+  // with D_elem(), Field_length(), Total_length() in the PLEGMA_Field layout.
+  template <class Field> void apply(const Field &f, FT_TYPE type = FT_GEMV, int sign = -1) {
+    if (type == FT_FFT) PLEGMA_error("Not implemented yet");
+    if (sign != +1 && sign != -1) PLEGMA_error("Sign should be either +1 or -1");
+    const bool is3d = f.Total_length() != (size_t)Lattice::V();
+    if (is3d) dimT = 1;
+    else if (dimT != Lattice::get().L[3]) PLEGMA_error("a 4D field is transformed over all T time slices");
+    checkAllocation(f.Field_length());
+    const size_t bytes = sizeN * sizeof(Float);
+    void *d = nullptr;
+    hip_check(hipMalloc(&d, bytes), "FT result buffer");
+    std::vector<int> moms = intMomenta();
+    const int prec = plg_precision_of<Float>();
+    const int rc = plg_ft_apply(f.D_elem(), prec, f.Field_length(), dimT, moms.data(), Nmoms(), sign, 0, d, prec,
+                                nullptr);
+    if (rc == PLG_OK) hip_check(hipMemcpy(h_elem.get(), d, bytes, hipMemcpyDeviceToHost), "FT result copy");
+    (void)hipFree(d);
+    plg_check(rc, "plg_ft_apply");
+  }
+
   // lib/PLEGMA_FT.cu:164-182 (the buffer read as [t][dof][mom])
   void mulConstMomentumPhases(std::vector<int> src, int sign) {
     if (dims == 3 && src.size() != 3) PLEGMA_error("src size is incompatible with the dimensionality of the FT");

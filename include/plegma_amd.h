@@ -120,6 +120,18 @@ int plg_correlator(int kind, int group_mask, const plg_prop *p1, const plg_prop 
 int plg_correlator_size(int kind, int group_mask, const int source[4], int totalT, int nmom,
                         int space, int64_t *elems);
 
+/* Standalone momentum projection of a field, replacing PLEGMA_FT::apply with
+ * FT_GEMV (lib/PLEGMA_FT.cu:144-162, lib/kernels/PLEGMA_FT.cuh:74-100):
+ *   out[t][i][m] (+)= sum_x f_i(x, t) exp(sign i 2pi n_m . x / L)
+ * with x the lattice coordinates (no source shift: PLEGMA_FT's
+ * mulConstMomentumPhases applies that).  field: device, PLEGMA_Field layout,
+ * dof components of nt*V3 complex (nt = T for a 4D field, 1 for a 3D field);
+ * precision PLG_F64 (FP64 MFMA) or PLG_F32 (split-bf16 MFMA, FP64 sums).
+ * moms: host nmom*3 ints.  out: device [nt][dof][nmom] complex of
+ * out_precision, overwritten, or added to when accumulate != 0 (IsAccum). */
+int plg_ft_apply(const void *field, int precision, int dof, int nt, const int *moms, int nmom,
+                 int sign, int accumulate, void *out, int out_precision, void *stream);
+
 /* ---- batched path (no single reference counterpart: the drivers loop over
  * sources calling contractMesons/contractBaryons, e.g. plegma/mesons_2pt.cpp) */
 typedef struct {

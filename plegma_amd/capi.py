@@ -31,7 +31,7 @@ EXPORTS = ["plg_init", "plg_finalize", "plg_last_error", "plg_version", "plg_mom
            "plg_correlator_size", "plg_batch_create", "plg_batch_run", "plg_batch_info",
            "plg_batch_destroy", "plg_batch_profile", "plg_synth_prop", "plg_synth_prop_perturbed", "plg_cast_prop",
            "plg_mlp_create", "plg_mlp_nparams", "plg_mlp_set_params", "plg_mlp_get_params",
-           "plg_mlp_forward", "plg_mlp_gradient", "plg_mlp_train", "plg_mlp_destroy"]
+           "plg_mlp_forward", "plg_mlp_gradient", "plg_mlp_train", "plg_mlp_destroy", "plg_ft_apply"]
 
 
 class PlgError(RuntimeError):
@@ -87,6 +87,8 @@ def lib():
                                                C.c_uint32, C.c_double, C.c_int, C.c_double,
                                                C.c_void_p]
         L.plg_cast_prop.argtypes = [C.POINTER(plg_prop), C.POINTER(plg_prop), C.c_void_p]
+        L.plg_ft_apply.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, ip, C.c_int, C.c_int, C.c_int,
+                                   C.c_void_p, C.c_int, C.c_void_p]
         fp = C.POINTER(C.c_float)
         L.plg_mlp_create.argtypes = [ip, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         L.plg_mlp_nparams.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
@@ -166,6 +168,12 @@ def correlator(kind, mask, p1, p2, source, totalT, moms, mode, space, out_prec, 
     check(lib().plg_correlator(kind, mask, C.byref(p1), C.byref(p2) if p2 is not None else None,
                                ivec(source), totalT, ivec(flat) if flat else None, len(moms), mode,
                                space, out_prec, C.c_void_p(out.data_ptr()), _stream(stream)))
+
+
+def ft_apply(field, precision, dof, nt, moms, sign, out, out_prec=F64, accumulate=False, stream=None):
+    flat = [int(v) for m in moms for v in m]
+    check(lib().plg_ft_apply(C.c_void_p(field.data_ptr()), precision, dof, nt, ivec(flat), len(moms), sign,
+                             int(accumulate), C.c_void_p(out.data_ptr()), out_prec, _stream(stream)))
 
 
 def correlator_size(kind, mask, source, totalT, nmom, space=MOMENTUM_SPACE):

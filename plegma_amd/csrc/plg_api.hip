@@ -881,6 +881,65 @@ int plg_correlator(int kind, int group_mask, const plg_prop *p1, const plg_prop 
   return PLG_OK;
 }
 
+int plg_ft_apply(const void *field, int precision, int dof, int nt, const int *moms, int nmom, int sign,
+                 int accumulate, void *out, int out_precision, void *stream) {
+  if (!g_ctx.init) return fail(PLG_ERR_NOT_INIT, "plg_init not called");
+  if (!field || !out || !moms || nmom <= 0 || dof <= 0) return fail(PLG_ERR_INVALID, "NULL or empty argument");
+  if (precision != PLG_F64 && precision != PLG_F32) return fail(PLG_ERR_INVALID, "bad field precision");
+  if (out_precision != PLG_F64 && out_precision != PLG_F32) return fail(PLG_ERR_INVALID, "bad out precision");
+  if (nt != 1 && nt != g_ctx.L[3]) return fail(PLG_ERR_INVALID, "nt must be 1 (3D field) or T (4D field)");
+  if (sign != 1 && sign != -1) return fail(PLG_ERR_INVALID, "sign must be +1 or -1");
+  hipStream_t st = (hipStream_t)stream;
+  // exp(sign i 2pi n.x/L): the GEMM applies exp(-i ...), so sign = +1 negates the momenta
+  std::vector<int> ms;
+  for (int m = 0; m < nmom; m++)
+    for (int i = 0; i < 3; i++) {
+      long v = (long)(sign > 0 ? -moms[3 * m + i] : moms[3 * m + i]) * (g_ctx.M / g_ctx.L[i]);
+      v %= g_ctx.M;
+      if (v < 0) v += g_ctx.M;
+      ms.push_back((int)v);
+    }
+  ProjGeom g;
+  g.V3 = g_ctx.V3;
+  for (int i = 0; i < 3; i++) g.L[i] = g_ctx.L[i];
+  g.M = g_ctx.M;
+  g.nmom = nmom;
+  g.ncols = dof * nt;
+  g.cos_tab = g_ctx.d_cos;
+  g.sin_tab = g_ctx.d_sin;
+  const int Mreal = 2 * nmom;
+  int code;
+  int64_t tiles;
+  if (precision == PLG_F64) {
+    code = Mreal <= 96 ? 32 : 64;
+    tiles = ((g.ncols + 127) / 128) * ((Mreal + code - 1) / code);
+  } else {
+    code = pick_proj(nmom);
+    tiles = proj_workgroups(code, g.ncols, nmom);
+  }
+  g.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>((1024 + tiles - 1) / tiles, g_ctx.V3 / 512));
+  int *d_ms = nullptr;
+  int rc = upload(&d_ms, ms, st);
+  if (rc) return rc;
+  g.mom_scaled = d_ms;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    rc = ensure(&g_ctx.ws_part, &g_ctx.ws_part_bytes, (size_t)g.nsplit * g.ncols * Mreal * sizeof(double));
+  }
+  if (!rc) {
+    if (precision == PLG_F64)
+      rc = launch_project_f64(g, (const cx<double> *)field, (cx<double> *)g_ctx.ws_part, code, st);
+    else
+      rc = launch_project_bf16(g, (const cx<float> *)field, (cx<double> *)g_ctx.ws_part, PLG_SPLIT_BF16X6,
+                               g_ctx.d_wtab, code, st);
+    if (!rc) rc = launch_ft_out(g, (const cx<double> *)g_ctx.ws_part, dof, nt, accumulate, out, out_precision, st);
+    if (rc) rc = fail(rc, "FT launch failed");
+  }
+  if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(PLG_ERR_HIP, "FT synchronisation failed");
+  hipFree(d_ms);
+  return rc;
+}
+
 int plg_batch_create(int kind, int group_mask, const plg_prop *props, int nprops, const plg_job *jobs, int njobs,
                      const int *moms, int nmom, int mode, int split, int flags, plg_batch *out) {
   if (!g_ctx.init) return fail(PLG_ERR_NOT_INIT, "plg_init not called");

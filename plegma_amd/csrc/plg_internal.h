@@ -81,6 +81,8 @@ int launch_project_f64(const ProjGeom &g, const cx<double> *c, cx<double> *part,
 // wtab: M x 6 packed words (cos_p | sin_p << 16, -sin_p | cos_p << 16) for
 // the three bf16 pieces p; mbw: 16-row blocks per wave (1, 2, 3, 5, 9)
 int64_t proj_workgroups(int code, int64_t ncols, int nmom);
+int launch_ft_out(const ProjGeom &g, const cx<double> *part, int dof, int nt, int accumulate, void *out,
+                  int out_prec, hipStream_t st);
 int launch_project_bf16(const ProjGeom &g, const cx<float> *c, cx<double> *part, int split_terms,
                         const uint32_t *wtab, int mbw, hipStream_t st);
 int pick_mbw(int nmom);

@@ -195,6 +195,44 @@ int launch_finalize(const ProjGeom &g, const cx<double> *part, const OutRowDev *
   return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;
 }
 
+// ------------------------------------------------------------------ standalone FT output
+// column col = i * nt + t of the GEMM (field component i, time slice t) ->
+// out[t][i][m] (the buffer layout of lib/kernels/PLEGMA_FT.cuh:95-97), added
+// to the previous contents when accumulate is set
+template <typename TO>
+__global__ void k_ft_out(ProjGeom g, const cx<double> *__restrict__ part, int dof, int nt, int accumulate,
+                         cx<TO> *__restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)g.ncols * g.nmom) return;
+  const int col = (int)(e / g.nmom), m = (int)(e % g.nmom);
+  double re = 0, im = 0;
+  for (int s = 0; s < g.nsplit; s++) {
+    const cx<double> v = part[((int64_t)s * g.ncols + col) * g.nmom + m];
+    re += v.re;
+    im += v.im;
+  }
+  const int i = col / nt, t = col % nt;
+  cx<TO> &o = out[((int64_t)t * dof + i) * g.nmom + m];
+  if (accumulate) {
+    o.re = (TO)((double)o.re + re);
+    o.im = (TO)((double)o.im + im);
+  } else {
+    o.re = (TO)re;
+    o.im = (TO)im;
+  }
+}
+
+int launch_ft_out(const ProjGeom &g, const cx<double> *part, int dof, int nt, int accumulate, void *out,
+                  int out_prec, hipStream_t st) {
+  const int64_t ne = (int64_t)g.ncols * g.nmom;
+  dim3 block(256), grid((unsigned)((ne + 255) / 256));
+  if (out_prec == PLG_F64)
+    k_ft_out<double><<<grid, block, 0, st>>>(g, part, dof, nt, accumulate, (cx<double> *)out);
+  else
+    k_ft_out<float><<<grid, block, 0, st>>>(g, part, dof, nt, accumulate, (cx<float> *)out);
+  return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;
+}
+
 // ------------------------------------------------------------------ position space
 template <typename TI, typename TO>
 __global__ void k_scatter_pos(const cx<TI> *__restrict__ c, int64_t V3, const OutRowDev *__restrict__ orows,

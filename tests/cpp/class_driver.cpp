@@ -56,6 +56,15 @@ template <typename Float> static int run(const std::string &out) {
     c.contractMesonMeson(u, d);
     dump(c, out + "/mesmes_" + tag + ".bin");
   }
+  {
+    // a propagator as a 144-component field through PLEGMA_FT::apply
+    PLEGMA_FT<Float> ft(2);
+    ft.apply(u, FT_GEMV, -1);
+    const std::string path = out + "/ft_" + tag + ".bin";
+    FILE *f = std::fopen(path.c_str(), "wb");
+    std::fwrite(ft.H_elem(), sizeof(Float), (size_t)2 * ft.Nmoms() * ft.DimT() * ft.Dof(), f);
+    std::fclose(f);
+  }
   return 0;
 }
 

@@ -70,3 +70,25 @@ def test_ascii_file(outputs):
         j = [O.row_time(L, SRC, 8, r) for r in range(8)].index(t)
         vals = [complex(float(a), float(b[:-1])) for a, b in zip(tok[4::2], tok[5::2])]
         np.testing.assert_allclose(vals, ref[0, 0, j, m], rtol=1e-5, atol=1e-6 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-6)])
+def test_ft_apply_class(outputs, prec, tol):
+    """PLEGMA_FT<Float>::apply on a propagator: H_elem[t][144][mom].  The FP32
+    propagator of the driver stores each time slice times 2^-e with
+    e = floor(log2 max |entry|) of the slice (plg_cast_prop); a raw field
+    transform sees those stored values."""
+    import closed_forms as cf
+    d, u, dn = outputs
+    got = _load(d, "ft", prec)
+    moms = O.momlist(2)
+    V3 = 64
+    got = got.reshape(8, 144, len(moms))
+    U = (u[0::2] + 1j * u[1::2]).reshape(144, 8, V3)
+    if prec == "f32":
+        amax = np.maximum(np.abs(U.real), np.abs(U.imag)).max(axis=(0, 2))    # per time slice
+        e = np.floor(np.log2(amax)).astype(int)
+        U = (U * np.ldexp(1.0, -e)[None, :, None]).astype(np.complex64).astype(np.complex128)
+    for t in (0, 5):
+        ref = cf.dft(U[:, t, :].T, L, [0, 0, 0], moms)       # [mom][144]
+        assert G.rel_err(got[t].T, ref, axis=0) < tol

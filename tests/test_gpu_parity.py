@@ -369,3 +369,26 @@ def test_projection_tiles(props, code, monkeypatch):
     rows = [0, 9]
     R = _twopt_oracle(h1, src, totalT, Q1, rows)
     assert G.rel_err(H[:, :, rows], R[:, :, rows]) < 1e-6
+
+
+# ---- standalone projection of a field (PLEGMA_FT::apply)
+@pytest.mark.parametrize("prec,tol", [(capi.F64, 1e-12), (capi.F32, 1e-6)])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_ft_apply(prec, tol, sign):
+    import closed_forms as cf
+    torch = G.torch_cuda()
+    rng = np.random.default_rng(17)
+    dof, T, V3 = 3, L1[3], L1[0] * L1[1] * L1[2]
+    f = rng.normal(size=(dof, T, V3)) + 1j * rng.normal(size=(dof, T, V3))
+    dt = torch.float64 if prec == capi.F64 else torch.float32
+    fd = torch.tensor(np.stack([f.real, f.imag], -1), dtype=dt, device="cuda")
+    moms = capi.momlist(Q1)
+    out = torch.full((T, dof, len(moms), 2), float("nan"), dtype=torch.float64, device="cuda")
+    capi.ft_apply(fd, prec, dof, T, moms, sign, out)
+    h = G.host(out).reshape(T, dof, len(moms), 2)
+    got = h[..., 0] + 1j * h[..., 1]
+    fq = f if prec == capi.F64 else f.astype(np.complex64).astype(np.complex128)
+    mm = np.array(moms) * (-1 if sign > 0 else 1)           # exp(+i n.x) = exp(-i (-n).x)
+    for t in (0, 7, T - 1):
+        ref = cf.dft(fq[:, t, :].T, L1, [0, 0, 0], mm)      # [mom][dof]
+        assert G.rel_err(got[t].T, ref, axis=0) < tol
