@@ -44,6 +44,22 @@ __device__ __forceinline__ int mod_m(int k, int M, float invM) {
   return r;
 }
 
+// (re, im) -> three packed bf16 pairs p0 + p1 + p2, each rounded to nearest
+// even by v_cvt_pk_bf16_f32 (one instruction per pair)
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef __bf16 b2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void split3(cx<float> v, uint32_t *w) {
+  const f2v x = {v.re, v.im};
+  const b2v h0 = __builtin_convertvector(x, b2v);
+  const f2v r1 = x - __builtin_convertvector(h0, f2v);
+  const b2v h1 = __builtin_convertvector(r1, b2v);
+  const f2v r2 = r1 - __builtin_convertvector(h1, f2v);
+  const b2v h2 = __builtin_convertvector(r2, b2v);
+  w[0] = __builtin_bit_cast(uint32_t, h0);
+  w[1] = __builtin_bit_cast(uint32_t, h1);
+  w[2] = __builtin_bit_cast(uint32_t, h2);
+}
+
 template <int MBW, int TERMS>
 __global__ __launch_bounds__(PB_THREADS) void k_project_bf16(ProjGeom g, const cx<float> *__restrict__ c,
                                                              const uint32_t *__restrict__ wtab,
@@ -120,19 +136,14 @@ __global__ __launch_bounds__(PB_THREADS) void k_project_bf16(ProjGeom g, const c
   };
   auto store = [&](int buf, int64_t xt) {
     const int col = scol;
-    uint16_t pc[3][4];
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-      pc[0][e] = bf16_rne(v[e]);
-      const float r1 = v[e] - bf16_val(pc[0][e]);
-      pc[1][e] = bf16_rne(r1);
-      pc[2][e] = bf16_rne(r1 - bf16_val(pc[1][e]));
-    }
+    uint32_t w0[3], w1[3];
+    split3(cmake<float>(v[0], v[1]), w0);
+    split3(cmake<float>(v[2], v[3]), w1);
 #pragma unroll
     for (int p = 0; p < NP; p++) {
       uint2 w;
-      w.x = (uint32_t)pc[p][0] | ((uint32_t)pc[p][1] << 16);
-      w.y = (uint32_t)pc[p][2] | ((uint32_t)pc[p][3] << 16);
+      w.x = w0[p];
+      w.y = w1[p];
       *reinterpret_cast<uint2 *>(&sB[buf][p][col][2 * s0]) = w;
     }
     if (tid < PB_KT) {
@@ -252,8 +263,9 @@ __global__ __launch_bounds__(PB_THREADS, 2) void k_project_bf16_sa(ProjGeom g, c
   constexpr int RM = RB * 8;                  // momenta per row group
   constexpr int MPT = (RM + 15) / 16;         // momentum slots per thread in the tile build
   constexpr int CPT = NC * PB_KT / PB_THREADS;  // complex B values staged per thread
-  __shared__ uint32_t sWt[PB_MMAX * 6];
-  __shared__ __align__(16) uint32_t sA[NP][RB * 16][PV_AST];
+  // phase table, 8 words per k: (re-row, im-row) word pairs of pieces 0..2
+  __shared__ __align__(16) uint32_t sWt[PB_MMAX * 8];
+  __shared__ __align__(16) uint32_t sA[NP][MPT * 32][PV_AST];
   __shared__ __align__(16) uint16_t sB[NP][NC][PB_LDB];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -269,7 +281,15 @@ __global__ __launch_bounds__(PB_THREADS, 2) void k_project_bf16_sa(ProjGeom g, c
     xb = (int64_t)split * per;
     xe = xb + per < g.V3 ? xb + per : g.V3;
   }
-  for (int i = tid; i < M * 6; i += PB_THREADS) sWt[i] = wtab[i];
+  for (int i = tid; i < M; i += PB_THREADS) {
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+      sWt[8 * i + 2 * p] = wtab[6 * i + p];
+      sWt[8 * i + 2 * p + 1] = wtab[6 * i + 3 + p];
+    }
+    sWt[8 * i + 6] = 0;
+    sWt[8 * i + 7] = 0;
+  }
 
   // tile build assignment: site tid%16, momenta (tid/16) + 16 j of this group
   const int ts = tid & 15;
@@ -309,57 +329,49 @@ __global__ __launch_bounds__(PB_THREADS, 2) void k_project_bf16_sa(ProjGeom g, c
 
   int step = 0;
   load(xb);
+  // lattice coordinates of this thread's site (xb + ts), advanced by PB_KT per step
+  int cx_, cy_, cz_;
+  {
+    const int x = (int)(xb + ts);
+    cx_ = x % g.L[0];
+    const int q = x / g.L[0];
+    cy_ = q % g.L[1];
+    cz_ = q / g.L[1];
+  }
   __syncthreads();  // sWt ready
   for (int64_t xt = xb; xt < xe; xt += PB_KT) {
     // ---- stage B pieces of this step
 #pragma unroll
     for (int q = 0; q < CPT; q++) {
-      const float e2[2] = {bv[q].re, bv[q].im};
       uint32_t w[3];
-      uint16_t h[3][2];
+      split3(bv[q], w);
 #pragma unroll
-      for (int e = 0; e < 2; e++) {
-        h[0][e] = bf16_rne(e2[e]);
-        const float r1 = e2[e] - bf16_val(h[0][e]);
-        h[1][e] = bf16_rne(r1);
-        h[2][e] = bf16_rne(r1 - bf16_val(h[1][e]));
-      }
-#pragma unroll
-      for (int p = 0; p < NP; p++) {
-        w[p] = (uint32_t)h[p][0] | ((uint32_t)h[p][1] << 16);
-        *reinterpret_cast<uint32_t *>(&sB[p][bcol][2 * (bs0 + q)]) = w[p];
-      }
+      for (int p = 0; p < NP; p++) *reinterpret_cast<uint32_t *>(&sB[p][bcol][2 * (bs0 + q)]) = w[p];
     }
-    // ---- build the phase tile of this step
+    // ---- build the phase tile of this step (rows beyond the group are
+    // padding rows of sA that no MFMA reads)
     {
-      const int64_t x = xt + ts;
-      const bool sv = x < xe;
-      int cx_ = 0, cy_ = 0, cz_ = 0;
-      if (sv) {
-        cx_ = (int)(x % g.L[0]);
-        const int64_t q = x / g.L[0];
-        cy_ = (int)(q % g.L[1]);
-        cz_ = (int)(q / g.L[1]);
-      }
+      const bool sv = xt + ts < xe;
 #pragma unroll
       for (int j = 0; j < MPT; j++) {
         const int mi = (tid >> 4) + 16 * j;
-        if (mi < RM) {
-          const int pk = mpk[j];
-          uint32_t wr[3] = {0, 0, 0}, wi[3] = {0, 0, 0};
-          if (pk >= 0 && sv) {
-            const int k = mod_m((pk & 1023) * cx_ + ((pk >> 10) & 1023) * cy_ + (pk >> 20) * cz_, M, invM);
+        const int pk = mpk[j];
+        const bool ok = pk >= 0 && sv;
+        const int k = ok ? mod_m((pk & 1023) * cx_ + ((pk >> 10) & 1023) * cy_ + (pk >> 20) * cz_, M, invM) : 0;
+        const uint2 *e = reinterpret_cast<const uint2 *>(&sWt[8 * k]);
 #pragma unroll
-            for (int p = 0; p < NP; p++) {
-              wr[p] = sWt[6 * k + p];
-              wi[p] = sWt[6 * k + 3 + p];
-            }
-          }
-#pragma unroll
-          for (int p = 0; p < NP; p++) {
-            sA[p][2 * mi][ts] = wr[p];
-            sA[p][2 * mi + 1][ts] = wi[p];
-          }
+        for (int p = 0; p < NP; p++) {
+          const uint2 w = e[p];
+          sA[p][2 * mi][ts] = ok ? w.x : 0u;
+          sA[p][2 * mi + 1][ts] = ok ? w.y : 0u;
+        }
+      }
+      cx_ += PB_KT;
+      while (cx_ >= g.L[0]) {
+        cx_ -= g.L[0];
+        if (++cy_ == g.L[1]) {
+          cy_ = 0;
+          ++cz_;
         }
       }
     }
@@ -423,6 +435,206 @@ __global__ __launch_bounds__(PB_THREADS, 2) void k_project_bf16_sa(ProjGeom g, c
       }
 }
 
+
+// ------------------------------------------------------------------ pipelined shared-A variant
+// 512 threads (8 waves) share the phase tile of RB row blocks and a B tile of
+// 128 columns.  Waves form a 2 x 4 grid: each owns RB/2 row blocks and two
+// column blocks, so every A fragment read from LDS feeds 2 column blocks and
+// every B fragment RB/2 row blocks.  Phase and site-correlator tiles are
+// double-buffered: in step t a wave issues the MFMAs of tile t and then builds
+// its share of tile t+1 while they run; one barrier per step.  The phase table
+// (M entries) is dynamic LDS.
+constexpr int PP_THREADS = 512;
+
+template <int RB, int TERMS>
+__global__ __launch_bounds__(PP_THREADS) void k_project_bf16_pp(ProjGeom g, const cx<float> *__restrict__ c,
+                                                                const uint32_t *__restrict__ wtab,
+                                                                double *__restrict__ part) {
+  static_assert(RB % 2 == 0, "two row halves");
+  constexpr int NP = TERMS == 3 ? 2 : 3;
+  constexpr int RH = RB / 2;                      // row blocks per wave
+  constexpr int NC = 128;                         // columns per workgroup
+  constexpr int RM = RB * 8;                      // momenta per row group
+  constexpr int MPT = (RM + 31) / 32;             // momentum slots per thread in the tile build
+  constexpr int CPT = NC * PB_KT / PP_THREADS;    // complex B values staged per thread
+  constexpr int TPC = PB_KT / CPT;
+  __shared__ __align__(16) uint32_t sA[2][NP][RB * 16][PV_AST];
+  __shared__ __align__(16) uint16_t sB[2][NP][NC][PB_LDB];
+  extern __shared__ __align__(16) uint32_t sWt[];  // M * 8 words
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 2, wc = wave & 3;        // row half, column pair of this wave
+  const int64_t n0 = (int64_t)blockIdx.x * NC;
+  const int split = blockIdx.y;
+  const int row0 = blockIdx.z * RB * 16;
+  const int Mreal = 2 * g.nmom;
+  const int M = g.M;
+  const float invM = 1.0f / (float)M;
+  int64_t xb, xe;
+  {
+    const int64_t per = (g.V3 + g.nsplit - 1) / g.nsplit;
+    xb = (int64_t)split * per;
+    xe = xb + per < g.V3 ? xb + per : g.V3;
+  }
+  for (int i = tid; i < M; i += PP_THREADS) {
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+      sWt[8 * i + 2 * p] = wtab[6 * i + p];
+      sWt[8 * i + 2 * p + 1] = wtab[6 * i + 3 + p];
+    }
+    sWt[8 * i + 6] = 0;
+    sWt[8 * i + 7] = 0;
+  }
+  // tile build: site tid % 16, momenta (tid / 16) + 32 j of this group
+  const int ts = tid & 15;
+  int mpk[MPT];
+#pragma unroll
+  for (int j = 0; j < MPT; j++) {
+    const int mi = (tid >> 4) + 32 * j;
+    const int m = row0 / 2 + mi;
+    mpk[j] = (mi < RM && m < g.nmom)
+                 ? (g.mom_scaled[3 * m] | (g.mom_scaled[3 * m + 1] << 10) | (g.mom_scaled[3 * m + 2] << 20))
+                 : -1;
+  }
+  const int bcol = tid / TPC, bs0 = (tid % TPC) * CPT;
+
+  f4 acc[RH][2];
+  double acc64[RH][2][4];
+#pragma unroll
+  for (int i = 0; i < RH; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      acc[i][j] = f4{0, 0, 0, 0};
+#pragma unroll
+      for (int v = 0; v < 4; v++) acc64[i][j][v] = 0;
+    }
+
+  cx<float> bv[CPT];
+  auto load = [&](int64_t xt) {
+    const int64_t n = n0 + bcol;
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+      const int64_t x = xt + bs0 + q;
+      bv[q] = (n < g.ncols && x < xe) ? c[n * g.V3 + x] : cmake<float>(0, 0);
+    }
+  };
+  int cx_, cy_, cz_;
+  {
+    const int x = (int)(xb + ts);
+    cx_ = x % g.L[0];
+    const int q = x / g.L[0];
+    cy_ = q % g.L[1];
+    cz_ = q / g.L[1];
+  }
+  // tile of the step starting at xt (coordinates: this thread's site of it)
+  auto build = [&](int b, int64_t xt) {
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+      uint32_t w[3];
+      split3(bv[q], w);
+#pragma unroll
+      for (int p = 0; p < NP; p++) *reinterpret_cast<uint32_t *>(&sB[b][p][bcol][2 * (bs0 + q)]) = w[p];
+    }
+    const bool sv = xt + ts < xe;
+#pragma unroll
+    for (int j = 0; j < MPT; j++) {
+      const int mi = (tid >> 4) + 32 * j;
+      if (32 * (j + 1) <= RM || mi < RM) {  // only the last slot can fall outside the group
+        const int pk = mpk[j];
+        const bool ok = pk >= 0 && sv;
+        const int k = ok ? mod_m((pk & 1023) * cx_ + ((pk >> 10) & 1023) * cy_ + (pk >> 20) * cz_, M, invM) : 0;
+        const uint2 *e = reinterpret_cast<const uint2 *>(&sWt[8 * k]);
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+          const uint2 w = e[p];
+          sA[b][p][2 * mi][ts] = ok ? w.x : 0u;
+          sA[b][p][2 * mi + 1][ts] = ok ? w.y : 0u;
+        }
+      }
+    }
+    cx_ += PB_KT;
+    while (cx_ >= g.L[0]) {
+      cx_ -= g.L[0];
+      if (++cy_ == g.L[1]) {
+        cy_ = 0;
+        ++cz_;
+      }
+    }
+  };
+
+  int step = 0, buf = 0;
+  __syncthreads();  // phase table
+  if (xb < xe) {
+    load(xb);
+    build(0, xb);
+    if (xb + PB_KT < xe) load(xb + PB_KT);
+  }
+  __syncthreads();
+  const int kk = 8 * (lane >> 4);
+  for (int64_t xt = xb; xt < xe; xt += PB_KT) {
+    // ---- MFMAs of tile t: rows wr*RH + i, columns (2 wc + j) * 16
+    {
+      bf8 bf[2][NP];
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int p = 0; p < NP; p++)
+          bf[j][p] = *reinterpret_cast<const bf8 *>(&sB[buf][p][(2 * wc + j) * 16 + (lane & 15)][kk]);
+      bf8 af[RH][NP];
+#pragma unroll
+      for (int i = 0; i < RH; i++)
+#pragma unroll
+        for (int p = 0; p < NP; p++)
+          af[i][p] = __builtin_bit_cast(
+              bf8, *reinterpret_cast<const u4 *>(&sA[buf][p][(wr * RH + i) * 16 + (lane & 15)][4 * (lane >> 4)]));
+      // smallest products first, term-major over the tiles
+      auto term = [&](int pa, int pb) {
+#pragma unroll
+        for (int i = 0; i < RH; i++)
+#pragma unroll
+          for (int j = 0; j < 2; j++)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][pa], bf[j][pb], acc[i][j], 0, 0, 0);
+      };
+      if constexpr (TERMS == 6) {
+        term(2, 0);
+        term(1, 1);
+        term(0, 2);
+      }
+      term(1, 0);
+      term(0, 1);
+      term(0, 0);
+    }
+    // ---- tile t+1 into the other buffers while the MFMAs run
+    if (xt + PB_KT < xe) {
+      build(buf ^ 1, xt + PB_KT);
+      if (xt + 2 * PB_KT < xe) load(xt + 2 * PB_KT);
+    }
+    if (++step == PB_FLUSH) {
+      step = 0;
+#pragma unroll
+      for (int i = 0; i < RH; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) acc64[i][j][v] += (double)acc[i][j][v];
+          acc[i][j] = f4{0, 0, 0, 0};
+        }
+    }
+    __syncthreads();
+    buf ^= 1;
+  }
+#pragma unroll
+  for (int i = 0; i < RH; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int mr = row0 + (wr * RH + i) * 16 + 4 * (lane >> 4) + v;
+        const int64_t n = n0 + (2 * wc + j) * 16 + (lane & 15);
+        if (mr < Mreal && n < g.ncols)
+          part[((int64_t)split * g.ncols + n) * Mreal + mr] = acc64[i][j][v] + (double)acc[i][j][v];
+      }
+}
 
 // ------------------------------------------------------------------ register-A variant
 // The four waves own disjoint row blocks (RBW each) and share one B tile of
@@ -503,19 +715,10 @@ __global__ __launch_bounds__(PB_THREADS) void k_project_bf16_rw(ProjGeom g, cons
   auto stage = [&](int b) {
 #pragma unroll
     for (int q = 0; q < CPT; q++) {
-      const float e2[2] = {bv[q].re, bv[q].im};
-      uint16_t h[3][2];
+      uint32_t w[3];
+      split3(bv[q], w);
 #pragma unroll
-      for (int e = 0; e < 2; e++) {
-        h[0][e] = bf16_rne(e2[e]);
-        const float r1 = e2[e] - bf16_val(h[0][e]);
-        h[1][e] = bf16_rne(r1);
-        h[2][e] = bf16_rne(r1 - bf16_val(h[1][e]));
-      }
-#pragma unroll
-      for (int p = 0; p < NP; p++)
-        *reinterpret_cast<uint32_t *>(&sB[b][p][bcol][2 * (bs0 + q)]) =
-            (uint32_t)h[p][0] | ((uint32_t)h[p][1] << 16);
+      for (int p = 0; p < NP; p++) *reinterpret_cast<uint32_t *>(&sB[b][p][bcol][2 * (bs0 + q)]) = w[p];
     }
   };
 
@@ -641,6 +844,26 @@ int launch_project_bf16(const ProjGeom &g, const cx<float> *c, cx<double> *part,
     if (terms == 6) k_project_bf16<MB, 6><<<grid, block, 0, st>>>(g, c, wtab, pd); \
     else k_project_bf16<MB, 3><<<grid, block, 0, st>>>(g, c, wtab, pd);            \
     break;
+  if (mbw >= 400) {
+    // pipelined shared-A variant: code = 400 + RB
+    const int rb = mbw - 400;
+    dim3 g3((unsigned)((g.ncols + 127) / 128), (unsigned)g.nsplit, (unsigned)((nblk + rb - 1) / rb));
+    dim3 b512(PP_THREADS);
+    const size_t dyn = (size_t)g.M * 8 * sizeof(uint32_t);
+#define PLG_PP(RB)                                                                                  \
+  if (rb == RB) {                                                                                   \
+    if (terms == 6) k_project_bf16_pp<RB, 6><<<g3, b512, dyn, st>>>(g, c, wtab, pd);                 \
+    else k_project_bf16_pp<RB, 3><<<g3, b512, dyn, st>>>(g, c, wtab, pd);                            \
+    return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;                                  \
+  }
+    PLG_PP(10)
+    PLG_PP(8)
+    PLG_PP(6)
+    PLG_PP(4)
+    PLG_PP(2)
+#undef PLG_PP
+    return PLG_ERR_INVALID;
+  }
   if (mbw >= 300) {
     // register-A variant: code = 300 + 10*RBW + NCB
     const int rbw = (mbw - 300) / 10, ncb = mbw % 10;
@@ -703,9 +926,9 @@ std::vector<int> proj_candidates(int nmom) {
     const int rb = (code - 100) / 10;
     if (rb <= 2 * nblk) v.push_back(code);
   }
-  for (int code : {314, 324, 334, 322, 332, 342}) {
-    const int rbw = (code - 300) / 10;
-    if (rbw <= nblk) v.push_back(code);
+  for (int code : {410, 408, 406, 404, 402}) {
+    const int rb = code - 400;
+    if (rb <= 2 * nblk) v.push_back(code);
   }
   v.push_back(pick_mbw(nmom));
   return v;
@@ -714,6 +937,10 @@ std::vector<int> proj_candidates(int nmom) {
 // workgroups of one K split for a projection code
 int64_t proj_workgroups(int code, int64_t ncols, int nmom) {
   const int nblk = (2 * nmom + 15) / 16;
+  if (code >= 400) {
+    const int rb = code - 400;
+    return ((ncols + 127) / 128) * ((nblk + rb - 1) / rb);
+  }
   if (code >= 300) {
     const int rbw = (code - 300) / 10, ncb = code % 10;
     return ((ncols + 16 * ncb - 1) / (16 * ncb)) * ((nblk + 4 * rbw - 1) / (4 * rbw));
