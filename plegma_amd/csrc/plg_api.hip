@@ -520,7 +520,7 @@ static int upload_plan(Plan &P, hipStream_t st) {
 }
 
 static size_t c_bytes(const Plan &P) {
-  return (size_t)P.rows.size() * P.nch_c * g_ctx.V3 * (P.T == PLG_F64 ? 16 : 8);
+  return (size_t)P.rows.size() * P.nch_c * ctile_sites(g_ctx.V3) * (P.T == PLG_F64 ? 16 : 8);
 }
 static size_t part_bytes(const Plan &P) {
   return (size_t)P.nsplit * P.rows.size() * P.nch_c * std::max(P.nmom, 1) * 16;
@@ -576,7 +576,8 @@ static int enqueue(Plan &P, void *c, void *part, void *out, int out_prec, hipStr
   }
   const int nout = (int)P.csel.size();
   if (P.space == PLG_POSITION_SPACE) {
-    rc = launch_scatter_pos(c, P.T, g_ctx.V3, P.d_orows, (int)P.orows.size(), P.nch_c, nout, P.d_csel, P.d_choff,
+    rc = launch_scatter_pos(c, P.T, g_ctx.V3, (int64_t)P.rows.size() * P.nch_c, P.d_orows, (int)P.orows.size(),
+                            P.nch_c, nout, P.d_csel, P.d_choff,
                             P.m_stride, out, out_prec, st);
     return rc ? fail(rc, "position-space scatter failed") : PLG_OK;
   }
@@ -588,6 +589,7 @@ static int enqueue(Plan &P, void *c, void *part, void *out, int out_prec, hipStr
   g.ncols = nrows * P.nch_c;
   g.nsplit = P.nsplit;
   g.mom_scaled = P.d_ms;
+  g.ctile = 1;
   g.cos_tab = g_ctx.d_cos;
   g.sin_tab = g_ctx.d_sin;
   if (P.T == PLG_F64) rc = launch_project_f64(g, (const cx<double> *)c, (cx<double> *)part, P.mtile, st);
@@ -702,6 +704,7 @@ static int autotune(Plan &P, hipStream_t st) {
   g.nmom = P.nmom;
   g.ncols = nrows * P.nch_c;
   g.mom_scaled = P.d_ms;
+  g.ctile = 1;
   g.cos_tab = g_ctx.d_cos;
   g.sin_tab = g_ctx.d_sin;
   std::vector<int> codes = P.T == PLG_F64 ? std::vector<int>{32, 64} : proj_candidates(P.nmom);
@@ -712,12 +715,19 @@ static int autotune(Plan &P, hipStream_t st) {
     for (int ns : {1, 2, 4, 8, 16, 32}) {
       if ((int64_t)ns * 256 > g_ctx.V3 || per_split * ns > P.part_bytes) continue;
       g.nsplit = ns;
+      int lrc = PLG_OK;
       const double ms = time_ms(
           [&] {
-            if (P.T == PLG_F64) launch_project_f64(g, (const cx<double> *)P.c, (cx<double> *)P.part, code, st);
-            else launch_project_bf16(g, (const cx<float> *)P.c, (cx<double> *)P.part, P.terms, g_ctx.d_wtab, code, st);
+            if (P.T == PLG_F64) lrc = launch_project_f64(g, (const cx<double> *)P.c, (cx<double> *)P.part, code, st);
+            else
+              lrc = launch_project_bf16(g, (const cx<float> *)P.c, (cx<double> *)P.part, P.terms, g_ctx.d_wtab,
+                                        code, st);
           },
           st);
+      if (lrc != PLG_OK) {  // a tile this geometry does not provide (LDS size)
+        (void)hipGetLastError();
+        continue;
+      }
       if (ms < best_p) {
         best_p = ms;
         best_code = code;
@@ -922,6 +932,7 @@ int plg_ft_apply(const void *field, int precision, int dof, int nt, const int *m
   int rc = upload(&d_ms, ms, st);
   if (rc) return rc;
   g.mom_scaled = d_ms;
+  g.ctile = 0;
   {
     std::lock_guard<std::mutex> lk(g_mu);
     rc = ensure(&g_ctx.ws_part, &g_ctx.ws_part_bytes, (size_t)g.nsplit * g.ncols * Mreal * sizeof(double));

@@ -300,9 +300,9 @@ void k_baryon(const PropDev<T> *__restrict__ props,
       const int ls = tid & 15;
       const int64_t x = x0 + ls;
       if (x < V3) {
-        cx<T> *o = out + ((int64_t)blockIdx.y * nch_total + ch0) * V3 + x;
+        const int64_t ncols = (int64_t)gridDim.y * nch_total, col0 = (int64_t)blockIdx.y * nch_total + ch0;
         for (int ch = tid >> 4; ch < NOUT * 16; ch += BT_THREADS / BT_SITES)
-          o[(int64_t)ch * V3] = stage[ls * SST + (ch & 15) * 9 + (ch >> 4)];
+          out[ctile_idx(col0 + ch, x, ncols)] = stage[ls * SST + (ch & 15) * 9 + (ch >> 4)];
       }
     }
     __syncthreads();

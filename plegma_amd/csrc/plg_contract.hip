@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) void k_meson_herm(const PropDev<T> *__restrict
       });
     });
   }
-  cx<T> *o = out + ((int64_t)r * nch_total + ch0) * V3 + x;
+  const int64_t ncols = (int64_t)gridDim.y * nch_total, col0 = (int64_t)r * nch_total + ch0;
   static_for<0, TableOf<SET>::v.nch>([&](auto ch) {
     cx<T> acc = cmake<T>(0, 0);
     static_for<0, 16>([&](auto i) {
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(256) void k_meson_herm(const PropDev<T> *__restrict
       else v = cconj(gx[d - 1][herm_rank(d, tm.q)]);
       acc = acc + cmul_iph(v, tm.ph);
     });
-    o[(int64_t)decltype(ch)::value * V3] = acc;
+    out[ctile_idx(col0 + decltype(ch)::value, x, ncols)] = acc;
   });
 }
 
@@ -192,9 +192,9 @@ __global__ __launch_bounds__(256) void k_meson_pair(const PropDev<T> *__restrict
       });
     });
   }
-  cx<T> *o = out + ((int64_t)r * nch_total + ch0) * V3 + x;
+  const int64_t ncols = (int64_t)gridDim.y * nch_total, col0 = (int64_t)r * nch_total + ch0;
 #pragma unroll
-  for (int ch = C0; ch < C1; ch++) o[(int64_t)ch * V3] = acc[ch - C0];
+  for (int ch = C0; ch < C1; ch++) out[ctile_idx(col0 + ch, x, ncols)] = acc[ch - C0];
 }
 
 // ------------------------------------------------------------------ launchers

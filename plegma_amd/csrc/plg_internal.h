@@ -75,7 +75,22 @@ struct ProjGeom {
   const int *mom_scaled;   // device, nmom*3: n_k * (M / L_k) mod M
   const double *cos_tab;   // device, M entries cos(2 pi k / M)
   const double *sin_tab;
+  // layout of the K x N operand: 1 = site correlator, tile-major (below);
+  // 0 = a field, column-major (column n, site x at n*V3 + x)
+  int ctile;
 };
+// The site correlator is stored tile-major: the 16 sites of a site tile are
+// contiguous per column, and the columns of a tile are contiguous, so that a
+// projection step (16 sites x many columns) reads one contiguous block and a
+// contraction workgroup writes one.  Element (column n, site x):
+__host__ __device__ __forceinline__ int64_t ctile_idx(int64_t n, int64_t x, int64_t ncols) {
+  return ((x >> 4) * ncols + n) * 16 + (x & 15);
+}
+__host__ __device__ __forceinline__ int64_t cidx(const ProjGeom &g, int64_t n, int64_t x) {
+  return g.ctile ? ctile_idx(n, x, g.ncols) : n * g.V3 + x;
+}
+// sites of a site-correlator column rounded up to whole tiles
+__host__ __device__ __forceinline__ int64_t ctile_sites(int64_t V3) { return (V3 + 15) / 16 * 16; }
 int launch_project_f64(const ProjGeom &g, const cx<double> *c, cx<double> *part, int mtile,
                        hipStream_t st);
 // wtab: M x 6 packed words (cos_p | sin_p << 16, -sin_p | cos_p << 16) for
@@ -97,7 +112,7 @@ int launch_finalize(const ProjGeom &g, const cx<double> *part, const OutRowDev *
                     int nch_c, int nout, const int *csel, const int64_t *ch_off, int64_t m_stride,
                     void *out, int out_prec, hipStream_t st);
 // position space: c[crow][ch][x] -> out[base + ch_off[k] + x*x_stride] (H position layout)
-int launch_scatter_pos(const void *c, int c_prec, int64_t V3, const OutRowDev *orows, int norows,
+int launch_scatter_pos(const void *c, int c_prec, int64_t V3, int64_t ncols, const OutRowDev *orows, int norows,
                        int nch_c, int nout, const int *csel, const int64_t *ch_off, int64_t x_stride,
                        void *out, int out_prec, hipStream_t st);
 

@@ -84,7 +84,7 @@ __global__ __launch_bounds__(PJ_THREADS) void k_project_f64(ProjGeom g, const cx
         const int col = (tid >> 4) + 16 * i;
         const int64_t n = n0 + col;
         cx<double> v = cmake<double>(0, 0);
-        if (x < xe && n < g.ncols) v = c[n * g.V3 + x];
+        if (x < xe && n < g.ncols) v = c[cidx(g, n, x)];
         sB[col][2 * s] = v.re;
         sB[col][2 * s + 1] = v.im;
       }
@@ -235,8 +235,8 @@ int launch_ft_out(const ProjGeom &g, const cx<double> *part, int dof, int nt, in
 
 // ------------------------------------------------------------------ position space
 template <typename TI, typename TO>
-__global__ void k_scatter_pos(const cx<TI> *__restrict__ c, int64_t V3, const OutRowDev *__restrict__ orows,
-                              int nch_c, int nout, const int *__restrict__ csel,
+__global__ void k_scatter_pos(const cx<TI> *__restrict__ c, int64_t V3, int64_t ncols,
+                              const OutRowDev *__restrict__ orows, int nch_c, int nout, const int *__restrict__ csel,
                               const int64_t *__restrict__ ch_off, int64_t x_stride, cx<TO> *__restrict__ out) {
   const OutRowDev R = orows[blockIdx.y];
   const int nr = R.nch_row > 0 ? R.nch_row : nout;
@@ -244,7 +244,7 @@ __global__ void k_scatter_pos(const cx<TI> *__restrict__ c, int64_t V3, const Ou
   if (e >= V3 * nr) return;
   const int64_t x = e / nr;
   const int k = (int)(e % nr);
-  const cx<TI> v = c[((int64_t)R.crow * nch_c + R.ch_base + csel[k]) * V3 + x];
+  const cx<TI> v = c[ctile_idx((int64_t)R.crow * nch_c + R.ch_base + csel[k], x, ncols)];
   const double sc = row_scale(R);
   cx<TO> o;
   o.re = (TO)(v.re * sc);
@@ -253,23 +253,26 @@ __global__ void k_scatter_pos(const cx<TI> *__restrict__ c, int64_t V3, const Ou
 }
 
 template <typename TI, typename TO>
-static void scatter_t(const void *c, int64_t V3, const OutRowDev *orows, int norows, int nch_c, int nout,
-                      const int *csel, const int64_t *ch_off, int64_t x_stride, void *out, hipStream_t st) {
+static void scatter_t(const void *c, int64_t V3, int64_t ncols, const OutRowDev *orows, int norows, int nch_c,
+                      int nout, const int *csel, const int64_t *ch_off, int64_t x_stride, void *out,
+                      hipStream_t st) {
   const int64_t ne = V3 * nout;
   dim3 block(256), grid((unsigned)((ne + 255) / 256), (unsigned)norows);
-  k_scatter_pos<TI, TO><<<grid, block, 0, st>>>((const cx<TI> *)c, V3, orows, nch_c, nout, csel, ch_off,
+  k_scatter_pos<TI, TO><<<grid, block, 0, st>>>((const cx<TI> *)c, V3, ncols, orows, nch_c, nout, csel, ch_off,
                                                 x_stride, (cx<TO> *)out);
 }
 
-int launch_scatter_pos(const void *c, int c_prec, int64_t V3, const OutRowDev *orows, int norows,
+int launch_scatter_pos(const void *c, int c_prec, int64_t V3, int64_t ncols, const OutRowDev *orows, int norows,
                        int nch_c, int nout, const int *csel, const int64_t *ch_off, int64_t x_stride,
                        void *out, int out_prec, hipStream_t st) {
   if (c_prec == PLG_F64) {
-    if (out_prec == PLG_F64) scatter_t<double, double>(c, V3, orows, norows, nch_c, nout, csel, ch_off, x_stride, out, st);
-    else scatter_t<double, float>(c, V3, orows, norows, nch_c, nout, csel, ch_off, x_stride, out, st);
+    if (out_prec == PLG_F64)
+      scatter_t<double, double>(c, V3, ncols, orows, norows, nch_c, nout, csel, ch_off, x_stride, out, st);
+    else scatter_t<double, float>(c, V3, ncols, orows, norows, nch_c, nout, csel, ch_off, x_stride, out, st);
   } else {
-    if (out_prec == PLG_F64) scatter_t<float, double>(c, V3, orows, norows, nch_c, nout, csel, ch_off, x_stride, out, st);
-    else scatter_t<float, float>(c, V3, orows, norows, nch_c, nout, csel, ch_off, x_stride, out, st);
+    if (out_prec == PLG_F64)
+      scatter_t<float, double>(c, V3, ncols, orows, norows, nch_c, nout, csel, ch_off, x_stride, out, st);
+    else scatter_t<float, float>(c, V3, ncols, orows, norows, nch_c, nout, csel, ch_off, x_stride, out, st);
   }
   return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;
 }

@@ -127,7 +127,7 @@ __global__ __launch_bounds__(PB_THREADS) void k_project_bf16(ProjGeom g, const c
       for (int q = 0; q < 2; q++) {
         const int64_t x = xt + s0 + q;
         if (x < xe) {
-          const cx<float> z = c[n * g.V3 + x];
+          const cx<float> z = c[cidx(g, n, x)];
           v[2 * q] = z.re;
           v[2 * q + 1] = z.im;
         }
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(PB_THREADS, 2) void k_project_bf16_sa(ProjGeom g, c
 #pragma unroll
     for (int q = 0; q < CPT; q++) {
       const int64_t x = xt + bs0 + q;
-      bv[q] = (n < g.ncols && x < xe) ? c[n * g.V3 + x] : cmake<float>(0, 0);
+      bv[q] = (n < g.ncols && x < xe) ? c[cidx(g, n, x)] : cmake<float>(0, 0);
     }
   };
 
@@ -436,6 +436,201 @@ __global__ __launch_bounds__(PB_THREADS, 2) void k_project_bf16_sa(ProjGeom g, c
 }
 
 
+// ------------------------------------------------------------------ 32x32 variant
+// v_mfma_f32_32x32x16_bf16 sustains close to the BF16 peak with only a couple
+// of independent accumulators, where 16x16x32 needs many more than the
+// FP64-shadowed accumulators leave registers for.  256 threads (4 waves, one
+// per SIMD, up to 512 registers each); a workgroup owns RT row tiles of 32
+// real rows and 128 columns, each wave one 32-column tile and all RT row
+// tiles.  Operand layouts (lane l, t < 8):
+//   A: A[l % 32][8 (l / 32) + t]     B: B[8 (l / 32) + t][l % 32]
+//   D (16 values v): row 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32
+// A k-step covers 16 sites = two MFMA K-slices of 8 sites.  Phase and
+// site-correlator tiles are double-buffered; one barrier per step.
+constexpr int M32_THREADS = 256;
+constexpr int M32_NC = 128;
+
+template <int RT, int TERMS>
+__global__ __launch_bounds__(M32_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_project_bf16_m32(ProjGeom g, const cx<float> *__restrict__ c, const uint32_t *__restrict__ wtab,
+                        double *__restrict__ part) {
+  typedef float f16v __attribute__((ext_vector_type(16)));
+  constexpr int NP = TERMS == 3 ? 2 : 3;
+  constexpr int NC = M32_NC;
+  constexpr int RM = RT * 16;                      // momenta per row group
+  constexpr int MPT = RM * PB_KT / M32_THREADS;    // phase entries per thread per step (= RT)
+  constexpr int CPT = NC * PB_KT / M32_THREADS;    // complex B values staged per thread (8)
+  constexpr int TPC = PB_KT / CPT;
+  __shared__ __align__(16) uint32_t sA[2][NP][RT * 32][PV_AST];
+  __shared__ __align__(16) uint16_t sB[2][NP][NC][PB_LDB];
+  extern __shared__ __align__(16) uint32_t sWt[];  // M * 8 words
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t n0 = (int64_t)blockIdx.x * NC;
+  const int split = blockIdx.y;
+  const int row0 = blockIdx.z * RT * 32;
+  const int Mreal = 2 * g.nmom;
+  const int M = g.M;
+  const float invM = 1.0f / (float)M;
+  int64_t xb, xe;
+  {
+    const int64_t per = (g.V3 + g.nsplit - 1) / g.nsplit;
+    xb = (int64_t)split * per;
+    xe = xb + per < g.V3 ? xb + per : g.V3;
+  }
+  for (int i = tid; i < M; i += M32_THREADS) {
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+      sWt[8 * i + 2 * p] = wtab[6 * i + p];
+      sWt[8 * i + 2 * p + 1] = wtab[6 * i + 3 + p];
+    }
+    sWt[8 * i + 6] = 0;
+    sWt[8 * i + 7] = 0;
+  }
+  // tile build: site tid % 16, momenta (tid / 16) + 16 j of this group
+  const int ts = tid & 15;
+  int mpk[MPT];
+#pragma unroll
+  for (int j = 0; j < MPT; j++) {
+    const int mi = (tid >> 4) + 16 * j;
+    const int m = row0 / 2 + mi;
+    mpk[j] = (m < g.nmom)
+                 ? (g.mom_scaled[3 * m] | (g.mom_scaled[3 * m + 1] << 10) | (g.mom_scaled[3 * m + 2] << 20))
+                 : -1;
+  }
+  const int bcol = tid / TPC, bs0 = (tid % TPC) * CPT;
+
+  f16v acc[RT];
+  double acc64[RT][16];
+#pragma unroll
+  for (int i = 0; i < RT; i++)
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+      acc[i][v] = 0;
+      acc64[i][v] = 0;
+    }
+
+  cx<float> bv[CPT];
+  auto load = [&](int64_t xt) {
+    const int64_t n = n0 + bcol;
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+      const int64_t x = xt + bs0 + q;
+      bv[q] = (n < g.ncols && x < xe) ? c[cidx(g, n, x)] : cmake<float>(0, 0);
+    }
+  };
+  int cx_, cy_, cz_;
+  {
+    const int x = (int)(xb + ts);
+    cx_ = x % g.L[0];
+    const int q = x / g.L[0];
+    cy_ = q % g.L[1];
+    cz_ = q / g.L[1];
+  }
+  auto build = [&](int b, int64_t xt) {
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+      uint32_t w[3];
+      split3(bv[q], w);
+#pragma unroll
+      for (int p = 0; p < NP; p++) *reinterpret_cast<uint32_t *>(&sB[b][p][bcol][2 * (bs0 + q)]) = w[p];
+    }
+    const bool sv = xt + ts < xe;
+#pragma unroll
+    for (int j = 0; j < MPT; j++) {
+      const int mi = (tid >> 4) + 16 * j;
+      const int pk = mpk[j];
+      const bool ok = pk >= 0 && sv;
+      const int k = ok ? mod_m((pk & 1023) * cx_ + ((pk >> 10) & 1023) * cy_ + (pk >> 20) * cz_, M, invM) : 0;
+      const uint2 *e = reinterpret_cast<const uint2 *>(&sWt[8 * k]);
+#pragma unroll
+      for (int p = 0; p < NP; p++) {
+        const uint2 w = e[p];
+        sA[b][p][2 * mi][ts] = ok ? w.x : 0u;
+        sA[b][p][2 * mi + 1][ts] = ok ? w.y : 0u;
+      }
+    }
+    cx_ += PB_KT;
+    while (cx_ >= g.L[0]) {
+      cx_ -= g.L[0];
+      if (++cy_ == g.L[1]) {
+        cy_ = 0;
+        ++cz_;
+      }
+    }
+  };
+
+  int step = 0, buf = 0;
+  __syncthreads();  // phase table
+  if (xb < xe) {
+    load(xb);
+    build(0, xb);
+    if (xb + PB_KT < xe) load(xb + PB_KT);
+  }
+  __syncthreads();
+  const int h = lane >> 5, r32 = lane & 31;
+  for (int64_t xt = xb; xt < xe; xt += PB_KT) {
+    // ---- MFMAs of tile t: two K slices of 8 sites
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+      bf8 bf[NP];
+#pragma unroll
+      for (int p = 0; p < NP; p++)
+        bf[p] = *reinterpret_cast<const bf8 *>(&sB[buf][p][wave * 32 + r32][16 * s + 8 * h]);
+      bf8 af[RT][NP];
+#pragma unroll
+      for (int i = 0; i < RT; i++)
+#pragma unroll
+        for (int p = 0; p < NP; p++)
+          af[i][p] = __builtin_bit_cast(bf8, *reinterpret_cast<const u4 *>(&sA[buf][p][i * 32 + r32][8 * s + 4 * h]));
+      auto term = [&](int pa, int pb) {
+#pragma unroll
+        for (int i = 0; i < RT; i++)
+          acc[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i][pa], bf[pb], acc[i], 0, 0, 0);
+      };
+      if constexpr (TERMS == 6) {
+        term(2, 0);
+        term(1, 1);
+        term(0, 2);
+      }
+      term(1, 0);
+      term(0, 1);
+      term(0, 0);
+    }
+    // ---- tile t+1 into the other buffers while the MFMAs run
+    if (xt + PB_KT < xe) {
+      build(buf ^ 1, xt + PB_KT);
+      if (xt + 2 * PB_KT < xe) load(xt + 2 * PB_KT);
+    }
+    if (++step == PB_FLUSH) {
+      step = 0;
+#pragma unroll
+      for (int i = 0; i < RT; i++)
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+          acc64[i][v] += (double)acc[i][v];
+          acc[i][v] = 0;
+        }
+    }
+    __syncthreads();
+    buf ^= 1;
+  }
+#pragma unroll
+  for (int i = 0; i < RT; i++)
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+      const int mr = row0 + i * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+      const int64_t n = n0 + wave * 32 + r32;
+      if (mr < Mreal && n < g.ncols)
+        part[((int64_t)split * g.ncols + n) * Mreal + mr] = acc64[i][v] + (double)acc[i][v];
+    }
+}
+
+// LDS bytes of a 32x32 workgroup (static tiles + phase table)
+static size_t m32_lds(int rt, int np, int M) {
+  return (size_t)2 * np * rt * 32 * PV_AST * 4 + (size_t)2 * np * M32_NC * PB_LDB * 2 + (size_t)M * 32;
+}
+
 // ------------------------------------------------------------------ pipelined shared-A variant
 // 512 threads (8 waves) share the phase tile of RB row blocks and a B tile of
 // 128 columns.  Waves form a 2 x 4 grid: each owns RB/2 row blocks and two
@@ -515,7 +710,7 @@ __global__ __launch_bounds__(PP_THREADS) void k_project_bf16_pp(ProjGeom g, cons
 #pragma unroll
     for (int q = 0; q < CPT; q++) {
       const int64_t x = xt + bs0 + q;
-      bv[q] = (n < g.ncols && x < xe) ? c[n * g.V3 + x] : cmake<float>(0, 0);
+      bv[q] = (n < g.ncols && x < xe) ? c[cidx(g, n, x)] : cmake<float>(0, 0);
     }
   };
   int cx_, cy_, cz_;
@@ -636,187 +831,6 @@ __global__ __launch_bounds__(PP_THREADS) void k_project_bf16_pp(ProjGeom g, cons
       }
 }
 
-// ------------------------------------------------------------------ register-A variant
-// The four waves own disjoint row blocks (RBW each) and share one B tile of
-// NCB column blocks.  Every lane builds its A fragments in registers: for each
-// of its 4 sites and RBW rows it computes the phase index k = n.x (M/L) mod M
-// and reads the three pieces of its row's (re, im) operand pair with one
-// 16-byte LDS read (phase table entries of 8 words: re-row pieces, pad,
-// im-row pieces, pad).  The A operand never passes through LDS as a tile, and
-// the B tile is double-buffered, so a k-step has one barrier.
-constexpr int PR_TW = 8;
-
-template <int RBW, int NCB, int TERMS>
-__global__ __launch_bounds__(PB_THREADS) void k_project_bf16_rw(ProjGeom g, const cx<float> *__restrict__ c,
-                                                                const uint32_t *__restrict__ wtab,
-                                                                double *__restrict__ part) {
-  constexpr int NP = TERMS == 3 ? 2 : 3;
-  constexpr int NC = NCB * 16;                    // columns per workgroup
-  constexpr int CPT = NC * PB_KT / PB_THREADS;    // complex B values staged per thread
-  constexpr int TPC = PB_KT / CPT;                // threads per column
-  static_assert(CPT >= 1 && PB_KT % CPT == 0, "staging shape");
-  __shared__ __align__(16) uint32_t sW[PB_MMAX * PR_TW];
-  __shared__ __align__(16) uint16_t sB[2][NP][NC][PB_LDB];
-
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t n0 = (int64_t)blockIdx.x * NC;
-  const int split = blockIdx.y;
-  const int rbw0 = (blockIdx.z * 4 + wave) * RBW;  // first row block of this wave
-  const int Mreal = 2 * g.nmom;
-  const int M = g.M;
-  const float invM = 1.0f / (float)M;
-  int64_t xb, xe;
-  {
-    const int64_t per = (g.V3 + g.nsplit - 1) / g.nsplit;
-    xb = (int64_t)split * per;
-    xe = xb + per < g.V3 ? xb + per : g.V3;
-  }
-  for (int i = tid; i < M; i += PB_THREADS) {
-#pragma unroll
-    for (int p = 0; p < 3; p++) {
-      sW[i * PR_TW + p] = wtab[6 * i + p];
-      sW[i * PR_TW + 4 + p] = wtab[6 * i + 3 + p];
-    }
-    sW[i * PR_TW + 3] = 0;
-    sW[i * PR_TW + 7] = 0;
-  }
-  // this lane's A rows: row block rbw0 + i, row (lane & 15) of it
-  int mpk[RBW], roff[RBW];
-#pragma unroll
-  for (int i = 0; i < RBW; i++) {
-    const int r = (rbw0 + i) * 16 + (lane & 15);
-    const int m = r >> 1;
-    roff[i] = (r & 1) * 4;
-    mpk[i] = (r < Mreal) ? (g.mom_scaled[3 * m] | (g.mom_scaled[3 * m + 1] << 10) | (g.mom_scaled[3 * m + 2] << 20))
-                         : -1;
-  }
-  const int bcol = tid / TPC, bs0 = (tid % TPC) * CPT;
-
-  f4 acc[RBW][NCB];
-  double acc64[RBW][NCB][4];
-#pragma unroll
-  for (int i = 0; i < RBW; i++)
-#pragma unroll
-    for (int j = 0; j < NCB; j++) {
-      acc[i][j] = f4{0, 0, 0, 0};
-#pragma unroll
-      for (int v = 0; v < 4; v++) acc64[i][j][v] = 0;
-    }
-
-  cx<float> bv[CPT];
-  auto load = [&](int64_t xt) {
-    const int64_t n = n0 + bcol;
-#pragma unroll
-    for (int q = 0; q < CPT; q++) {
-      const int64_t x = xt + bs0 + q;
-      bv[q] = (n < g.ncols && x < xe) ? c[n * g.V3 + x] : cmake<float>(0, 0);
-    }
-  };
-  auto stage = [&](int b) {
-#pragma unroll
-    for (int q = 0; q < CPT; q++) {
-      uint32_t w[3];
-      split3(bv[q], w);
-#pragma unroll
-      for (int p = 0; p < NP; p++) *reinterpret_cast<uint32_t *>(&sB[b][p][bcol][2 * (bs0 + q)]) = w[p];
-    }
-  };
-
-  int step = 0, buf = 0;
-  if (xb < xe) {
-    load(xb);
-    stage(0);
-  }
-  __syncthreads();  // phase table and the first B tile
-  for (int64_t xt = xb; xt < xe; xt += PB_KT) {
-    const bool more = xt + PB_KT < xe;
-    if (more) load(xt + PB_KT);
-    // ---- A fragments: sites xs + j (j < 4) of this lane, RBW rows
-    u4 af[RBW][NP];
-    {
-      const int64_t xs = xt + 4 * (lane >> 4);
-      int cx_ = (int)(xs % g.L[0]);
-      const int64_t q = xs / g.L[0];
-      int cy_ = (int)(q % g.L[1]);
-      int cz_ = (int)(q / g.L[1]);
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const bool sv = xs + j < xe;
-#pragma unroll
-        for (int i = 0; i < RBW; i++) {
-          const int pk = mpk[i];
-          u4 w = {0, 0, 0, 0};
-          if (pk >= 0 && sv) {
-            const int k = mod_m((pk & 1023) * cx_ + ((pk >> 10) & 1023) * cy_ + (pk >> 20) * cz_, M, invM);
-            w = *reinterpret_cast<const u4 *>(&sW[k * PR_TW + roff[i]]);
-          }
-#pragma unroll
-          for (int p = 0; p < NP; p++) af[i][p][j] = w[p];
-        }
-        if (++cx_ == g.L[0]) {
-          cx_ = 0;
-          if (++cy_ == g.L[1]) {
-            cy_ = 0;
-            ++cz_;
-          }
-        }
-      }
-    }
-    // ---- MFMAs over the column blocks of the shared B tile
-    const int kk = 8 * (lane >> 4);
-#pragma unroll
-    for (int cb = 0; cb < NCB; cb++) {
-      bf8 bf[NP];
-#pragma unroll
-      for (int p = 0; p < NP; p++)
-        bf[p] = *reinterpret_cast<const bf8 *>(&sB[buf][p][cb * 16 + (lane & 15)][kk]);
-#pragma unroll
-      for (int i = 0; i < RBW; i++) {
-        if ((rbw0 + i) * 16 >= Mreal) continue;  // uniform: a padding block
-        f4 a = acc[i][cb];
-        const bf8 a0 = __builtin_bit_cast(bf8, af[i][0]), a1 = __builtin_bit_cast(bf8, af[i][1]);
-        if constexpr (TERMS == 6) {
-          const bf8 a2 = __builtin_bit_cast(bf8, af[i][2]);
-          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, bf[0], a, 0, 0, 0);
-          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bf[1], a, 0, 0, 0);
-          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bf[2], a, 0, 0, 0);
-        }
-        a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bf[0], a, 0, 0, 0);
-        a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bf[1], a, 0, 0, 0);
-        a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bf[0], a, 0, 0, 0);
-        acc[i][cb] = a;
-      }
-    }
-    if (++step == PB_FLUSH) {
-      step = 0;
-#pragma unroll
-      for (int i = 0; i < RBW; i++)
-#pragma unroll
-        for (int j = 0; j < NCB; j++) {
-#pragma unroll
-          for (int v = 0; v < 4; v++) acc64[i][j][v] += (double)acc[i][j][v];
-          acc[i][j] = f4{0, 0, 0, 0};
-        }
-    }
-    // ---- the next B tile into the other buffer (read by no wave this step)
-    if (more) stage(buf ^ 1);
-    __syncthreads();
-    buf ^= 1;
-  }
-  // D of 16x16x32 (f32): lane l, value v -> row 4*(l/16) + v, column l%16
-#pragma unroll
-  for (int i = 0; i < RBW; i++)
-#pragma unroll
-    for (int j = 0; j < NCB; j++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int mr = (rbw0 + i) * 16 + 4 * (lane >> 4) + v;
-        const int64_t n = n0 + j * 16 + (lane & 15);
-        if (mr < Mreal && n < g.ncols)
-          part[((int64_t)split * g.ncols + n) * Mreal + mr] = acc64[i][j][v] + (double)acc[i][j][v];
-      }
-}
-
 // row blocks per wave for a momentum count: fewest padded blocks, then fewest groups
 int pick_mbw(int nmom) {
   const int nblk = (2 * nmom + 15) / 16;
@@ -844,6 +858,28 @@ int launch_project_bf16(const ProjGeom &g, const cx<float> *c, cx<double> *part,
     if (terms == 6) k_project_bf16<MB, 6><<<grid, block, 0, st>>>(g, c, wtab, pd); \
     else k_project_bf16<MB, 3><<<grid, block, 0, st>>>(g, c, wtab, pd);            \
     break;
+  if (mbw >= 500) {
+    // 32x32 variant: code = 500 + RT
+    const int rt = mbw - 500;
+    if (m32_lds(rt, terms == 3 ? 2 : 3, g.M) > 160 * 1024) return PLG_ERR_UNSUPPORTED;
+    const int nt32 = (2 * g.nmom + 31) / 32;
+    dim3 g3((unsigned)((g.ncols + M32_NC - 1) / M32_NC), (unsigned)g.nsplit, (unsigned)((nt32 + rt - 1) / rt));
+    const size_t dyn = (size_t)g.M * 8 * sizeof(uint32_t);
+#define PLG_M32(RT)                                                                                 \
+  if (rt == RT) {                                                                                   \
+    if (terms == 6) k_project_bf16_m32<RT, 6><<<g3, block, dyn, st>>>(g, c, wtab, pd);               \
+    else k_project_bf16_m32<RT, 3><<<g3, block, dyn, st>>>(g, c, wtab, pd);                          \
+    return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;                                  \
+  }
+    PLG_M32(6)
+    PLG_M32(5)
+    PLG_M32(4)
+    PLG_M32(3)
+    PLG_M32(2)
+    PLG_M32(1)
+#undef PLG_M32
+    return PLG_ERR_INVALID;
+  }
   if (mbw >= 400) {
     // pipelined shared-A variant: code = 400 + RB
     const int rb = mbw - 400;
@@ -862,26 +898,6 @@ int launch_project_bf16(const ProjGeom &g, const cx<float> *c, cx<double> *part,
     PLG_PP(4)
     PLG_PP(2)
 #undef PLG_PP
-    return PLG_ERR_INVALID;
-  }
-  if (mbw >= 300) {
-    // register-A variant: code = 300 + 10*RBW + NCB
-    const int rbw = (mbw - 300) / 10, ncb = mbw % 10;
-    dim3 g3((unsigned)((g.ncols + 16 * ncb - 1) / (16 * ncb)), (unsigned)g.nsplit,
-            (unsigned)((nblk + 4 * rbw - 1) / (4 * rbw)));
-#define PLG_RW(RBW, NCB)                                                                            \
-  if (rbw == RBW && ncb == NCB) {                                                                   \
-    if (terms == 6) k_project_bf16_rw<RBW, NCB, 6><<<g3, block, 0, st>>>(g, c, wtab, pd);            \
-    else k_project_bf16_rw<RBW, NCB, 3><<<g3, block, 0, st>>>(g, c, wtab, pd);                       \
-    return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;                                  \
-  }
-    PLG_RW(1, 4)
-    PLG_RW(2, 4)
-    PLG_RW(3, 4)
-    PLG_RW(2, 2)
-    PLG_RW(3, 2)
-    PLG_RW(4, 2)
-#undef PLG_RW
     return PLG_ERR_INVALID;
   }
   if (mbw >= 100) {
@@ -930,6 +946,9 @@ std::vector<int> proj_candidates(int nmom) {
     const int rb = code - 400;
     if (rb <= 2 * nblk) v.push_back(code);
   }
+  const int nt32 = (2 * nmom + 31) / 32;
+  for (int code : {506, 505, 504, 503, 502, 501})
+    if (code - 500 <= nt32) v.push_back(code);
   v.push_back(pick_mbw(nmom));
   return v;
 }
@@ -937,13 +956,13 @@ std::vector<int> proj_candidates(int nmom) {
 // workgroups of one K split for a projection code
 int64_t proj_workgroups(int code, int64_t ncols, int nmom) {
   const int nblk = (2 * nmom + 15) / 16;
+  if (code >= 500) {
+    const int rt = code - 500, nt32 = (2 * nmom + 31) / 32;
+    return ((ncols + M32_NC - 1) / M32_NC) * ((nt32 + rt - 1) / rt);
+  }
   if (code >= 400) {
     const int rb = code - 400;
     return ((ncols + 127) / 128) * ((nblk + rb - 1) / rb);
-  }
-  if (code >= 300) {
-    const int rbw = (code - 300) / 10, ncb = code % 10;
-    return ((ncols + 16 * ncb - 1) / (16 * ncb)) * ((nblk + 4 * rbw - 1) / (4 * rbw));
   }
   if (code >= 100) {
     const int rb = (code - 100) / 10, nbw = code % 10;
