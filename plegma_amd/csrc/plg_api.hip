@@ -173,6 +173,9 @@ struct Plan {
   int nch_c = 0;    // computed channels per row
   int nmom = 0;
   std::vector<int> moms;
+  std::vector<int> ms;                 // scaled momenta n_k (M / L_k) mod M
+  int npairs = 0;                      // (n, -n) pairs of the mixed projection
+  std::vector<int> pair_ms, mom_pair;
   std::vector<RowDev> rows;
   std::vector<OutRowDev> orows;
   std::vector<int> csel;
@@ -190,7 +193,7 @@ struct Plan {
   OutRowDev *d_orows = nullptr;
   int *d_csel = nullptr;
   int64_t *d_choff = nullptr;
-  int *d_ms = nullptr;
+  int *d_ms = nullptr, *d_pair_ms = nullptr, *d_mom_pair = nullptr;
   // batch-owned workspace (single calls use the context workspace)
   void *c = nullptr, *part = nullptr;
   size_t c_bytes = 0, part_bytes = 0;
@@ -205,7 +208,8 @@ struct Plan {
     if (exec) hipGraphExecDestroy(exec);
     if (graph) hipGraphDestroy(graph);
     if (cap_stream) hipStreamDestroy(cap_stream);
-    for (void *p : {(void *)d_props, (void *)d_rows, (void *)d_orows, (void *)d_csel, (void *)d_choff, (void *)d_ms})
+    for (void *p : {(void *)d_props, (void *)d_rows, (void *)d_orows, (void *)d_csel, (void *)d_choff, (void *)d_ms,
+                    (void *)d_pair_ms, (void *)d_mom_pair})
       if (p) hipFree(p);
     if (own_ws) {
       if (c) hipFree(c);
@@ -232,6 +236,19 @@ static int check_prop(const plg_prop &p, int want_prec, int t) {
   return PLG_OK;
 }
 
+// n_k (M / L_k) mod M for a momentum list; sign -1 negates the momenta
+static std::vector<int> scaled_moms(const int *moms, int nmom, int sign) {
+  std::vector<int> ms;
+  for (int m = 0; m < nmom; m++)
+    for (int i = 0; i < 3; i++) {
+      long v = (long)sign * moms[3 * m + i] * (g_ctx.M / g_ctx.L[i]);
+      v %= g_ctx.M;
+      if (v < 0) v += g_ctx.M;
+      ms.push_back((int)v);
+    }
+  return ms;
+}
+
 // build the host side of a plan for a list of jobs
 static int build_plan(Plan &P, int kind, int mask, const std::vector<plg_prop> &props,
                       const std::vector<JobIn> &jobs, const int *moms, int nmom, int mode, int terms,
@@ -249,6 +266,8 @@ static int build_plan(Plan &P, int kind, int mask, const std::vector<plg_prop> &
   P.props = props;
   P.nmom = space == PLG_MOMENTUM_SPACE ? nmom : 0;
   P.moms.assign(moms, moms + 3 * P.nmom);
+  P.ms = scaled_moms(moms, P.nmom, 1);
+  if (P.T == PLG_F32 && P.nmom > 0) P.npairs = make_pairs(P.ms, g_ctx.M, P.pair_ms, P.mom_pair);
   const int64_t V3 = g_ctx.V3;
   const int64_t inner = space == PLG_MOMENTUM_SPACE ? (int64_t)nmom : V3;  // per-row extent
 
@@ -479,10 +498,10 @@ static void choose_tiles(Plan &P) {
     P.mtile = Mreal <= 96 ? 32 : 64;
     tiles = ((ncols + 127) / 128) * ((Mreal + P.mtile - 1) / P.mtile);
   } else {
-    P.mbw = pick_proj(std::max(P.nmom, 1));
+    P.mbw = pick_proj(std::max(P.npairs, 1));
     // PLG_PROJ_CODE pins the projection tile (tile studies and tests)
     if (const char *e = std::getenv("PLG_PROJ_CODE")) P.mbw = std::atoi(e);
-    tiles = proj_workgroups(P.mbw, ncols, std::max(P.nmom, 1));
+    tiles = proj_workgroups(P.mbw, ncols, std::max(P.npairs, 1));
   }
   int64_t ns = (1024 + tiles - 1) / tiles;
   ns = std::max<int64_t>(1, std::min<int64_t>(ns, g_ctx.V3 / 512));
@@ -503,19 +522,13 @@ static int upload_plan(Plan &P, hipStream_t st) {
     fill_propdev(P.props, pd);
     if ((rc = upload((PropDev<float> **)&P.d_props, pd, st))) return rc;
   }
-  std::vector<int> ms;
-  for (int m = 0; m < P.nmom; m++)
-    for (int i = 0; i < 3; i++) {
-      long v = (long)P.moms[3 * m + i] * (g_ctx.M / g_ctx.L[i]);
-      v %= g_ctx.M;
-      if (v < 0) v += g_ctx.M;
-      ms.push_back((int)v);
-    }
   if ((rc = upload(&P.d_rows, P.rows, st))) return rc;
   if ((rc = upload(&P.d_orows, P.orows, st))) return rc;
   if ((rc = upload(&P.d_csel, P.csel, st))) return rc;
   if ((rc = upload(&P.d_choff, P.ch_off, st))) return rc;
-  if ((rc = upload(&P.d_ms, ms, st))) return rc;
+  if ((rc = upload(&P.d_ms, P.ms, st))) return rc;
+  if ((rc = upload(&P.d_pair_ms, P.pair_ms, st))) return rc;
+  if ((rc = upload(&P.d_mom_pair, P.mom_pair, st))) return rc;
   return PLG_OK;
 }
 
@@ -523,7 +536,26 @@ static size_t c_bytes(const Plan &P) {
   return (size_t)P.rows.size() * P.nch_c * ctile_sites(g_ctx.V3) * (P.T == PLG_F64 ? 16 : 8);
 }
 static size_t part_bytes(const Plan &P) {
-  return (size_t)P.nsplit * P.rows.size() * P.nch_c * std::max(P.nmom, 1) * 16;
+  return proj_part_bytes(P.nsplit, (int64_t)P.rows.size() * P.nch_c, std::max(P.nmom, 1), P.npairs);
+}
+
+// projection geometry of a plan
+static ProjGeom plan_geom(const Plan &P) {
+  ProjGeom g;
+  g.V3 = g_ctx.V3;
+  for (int i = 0; i < 3; i++) g.L[i] = g_ctx.L[i];
+  g.M = g_ctx.M;
+  g.nmom = P.nmom;
+  g.ncols = (int)P.rows.size() * P.nch_c;
+  g.nsplit = P.nsplit;
+  g.mom_scaled = P.d_ms;
+  g.ctile = 1;
+  g.cos_tab = g_ctx.d_cos;
+  g.sin_tab = g_ctx.d_sin;
+  g.npairs = P.npairs;
+  g.pair_ms = P.d_pair_ms;
+  g.mom_pair = P.d_mom_pair;
+  return g;
 }
 
 // enqueue the whole plan on a stream (capturable: no allocation, no memset)
@@ -581,22 +613,12 @@ static int enqueue(Plan &P, void *c, void *part, void *out, int out_prec, hipStr
                             P.m_stride, out, out_prec, st);
     return rc ? fail(rc, "position-space scatter failed") : PLG_OK;
   }
-  ProjGeom g;
-  g.V3 = g_ctx.V3;
-  for (int i = 0; i < 3; i++) g.L[i] = g_ctx.L[i];
-  g.M = g_ctx.M;
-  g.nmom = P.nmom;
-  g.ncols = nrows * P.nch_c;
-  g.nsplit = P.nsplit;
-  g.mom_scaled = P.d_ms;
-  g.ctile = 1;
-  g.cos_tab = g_ctx.d_cos;
-  g.sin_tab = g_ctx.d_sin;
+  const ProjGeom g = plan_geom(P);
   if (P.T == PLG_F64) rc = launch_project_f64(g, (const cx<double> *)c, (cx<double> *)part, P.mtile, st);
-  else rc = launch_project_bf16(g, (const cx<float> *)c, (cx<double> *)part, P.terms, g_ctx.d_wtab, P.mbw, st);
+  else rc = launch_project_bf16(g, (const cx<float> *)c, (double *)part, P.terms, g_ctx.d_wtab, P.mbw, st);
   if (rc) return fail(rc, "projection launch failed");
   if (log) log->mark(P.T == PLG_F64 ? "project_f64" : (P.terms == 6 ? "project_bf16x6" : "project_bf16x3"), st);
-  rc = launch_finalize(g, (const cx<double> *)part, P.d_orows, (int)P.orows.size(), P.nch_c, nout, P.d_csel,
+  rc = launch_finalize(g, (const double *)part, P.d_orows, (int)P.orows.size(), P.nch_c, nout, P.d_csel,
                        P.d_choff, P.m_stride, out, out_prec, st);
   if (log) log->mark("finalize", st);
   return rc ? fail(rc, "finalize launch failed") : PLG_OK;
@@ -620,7 +642,8 @@ static std::string tune_key(const Plan &P) {
   std::ostringstream k;
   k << prop.gcnArchName << " L=" << g_ctx.L[0] << "x" << g_ctx.L[1] << "x" << g_ctx.L[2] << "x" << g_ctx.L[3]
     << " nmom=" << P.nmom << " cols=" << P.rows.size() * P.nch_c << " rows=" << P.rows.size() << " mode=" << P.mode
-    << " terms=" << P.terms << " kind=" << P.kind << " mask=" << P.mask;
+    << " npairs=" << P.npairs << " terms=" << P.terms << " kind=" << P.kind << " mask=" << P.mask
+    << " proj=pair32";
   for (auto &l : P.launches) k << " L" << l.what << "." << l.set << "." << l.two << l.mes << "." << l.gmask;
   std::string s = k.str();
   for (char &ch : s)
@@ -697,17 +720,8 @@ static int autotune(Plan &P, hipStream_t st) {
   }
   P.tpb = best_tpb;
   // projection: tile code and K splits
-  ProjGeom g;
-  g.V3 = g_ctx.V3;
-  for (int i = 0; i < 3; i++) g.L[i] = g_ctx.L[i];
-  g.M = g_ctx.M;
-  g.nmom = P.nmom;
-  g.ncols = nrows * P.nch_c;
-  g.mom_scaled = P.d_ms;
-  g.ctile = 1;
-  g.cos_tab = g_ctx.d_cos;
-  g.sin_tab = g_ctx.d_sin;
-  std::vector<int> codes = P.T == PLG_F64 ? std::vector<int>{32, 64} : proj_candidates(P.nmom);
+  ProjGeom g = plan_geom(P);
+  std::vector<int> codes = P.T == PLG_F64 ? std::vector<int>{32, 64} : proj_candidates(P.npairs);
   const size_t per_split = part_bytes(P) / std::max(P.nsplit, 1);
   double best_p = 1e30;
   int best_code = P.T == PLG_F64 ? P.mtile : P.mbw, best_ns = P.nsplit;
@@ -720,7 +734,7 @@ static int autotune(Plan &P, hipStream_t st) {
           [&] {
             if (P.T == PLG_F64) lrc = launch_project_f64(g, (const cx<double> *)P.c, (cx<double> *)P.part, code, st);
             else
-              lrc = launch_project_bf16(g, (const cx<float> *)P.c, (cx<double> *)P.part, P.terms, g_ctx.d_wtab,
+              lrc = launch_project_bf16(g, (const cx<float> *)P.c, (double *)P.part, P.terms, g_ctx.d_wtab,
                                         code, st);
           },
           st);
@@ -901,14 +915,9 @@ int plg_ft_apply(const void *field, int precision, int dof, int nt, const int *m
   if (sign != 1 && sign != -1) return fail(PLG_ERR_INVALID, "sign must be +1 or -1");
   hipStream_t st = (hipStream_t)stream;
   // exp(sign i 2pi n.x/L): the GEMM applies exp(-i ...), so sign = +1 negates the momenta
-  std::vector<int> ms;
-  for (int m = 0; m < nmom; m++)
-    for (int i = 0; i < 3; i++) {
-      long v = (long)(sign > 0 ? -moms[3 * m + i] : moms[3 * m + i]) * (g_ctx.M / g_ctx.L[i]);
-      v %= g_ctx.M;
-      if (v < 0) v += g_ctx.M;
-      ms.push_back((int)v);
-    }
+  const std::vector<int> ms = scaled_moms(moms, nmom, -sign);
+  std::vector<int> pair_ms, mom_pair;
+  const int npairs = precision == PLG_F32 ? make_pairs(ms, g_ctx.M, pair_ms, mom_pair) : 0;
   ProjGeom g;
   g.V3 = g_ctx.V3;
   for (int i = 0; i < 3; i++) g.L[i] = g_ctx.L[i];
@@ -924,30 +933,35 @@ int plg_ft_apply(const void *field, int precision, int dof, int nt, const int *m
     code = Mreal <= 96 ? 32 : 64;
     tiles = ((g.ncols + 127) / 128) * ((Mreal + code - 1) / code);
   } else {
-    code = pick_proj(nmom);
-    tiles = proj_workgroups(code, g.ncols, nmom);
+    code = pick_proj(npairs);
+    tiles = proj_workgroups(code, g.ncols, npairs);
   }
   g.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>((1024 + tiles - 1) / tiles, g_ctx.V3 / 512));
-  int *d_ms = nullptr;
+  int *d_ms = nullptr, *d_pair_ms = nullptr, *d_mom_pair = nullptr;
   int rc = upload(&d_ms, ms, st);
-  if (rc) return rc;
+  if (!rc) rc = upload(&d_pair_ms, pair_ms, st);
+  if (!rc) rc = upload(&d_mom_pair, mom_pair, st);
   g.mom_scaled = d_ms;
+  g.npairs = npairs;
+  g.pair_ms = d_pair_ms;
+  g.mom_pair = d_mom_pair;
   g.ctile = 0;
-  {
+  if (!rc) {
     std::lock_guard<std::mutex> lk(g_mu);
-    rc = ensure(&g_ctx.ws_part, &g_ctx.ws_part_bytes, (size_t)g.nsplit * g.ncols * Mreal * sizeof(double));
+    rc = ensure(&g_ctx.ws_part, &g_ctx.ws_part_bytes, proj_part_bytes(g.nsplit, g.ncols, nmom, npairs));
   }
   if (!rc) {
     if (precision == PLG_F64)
       rc = launch_project_f64(g, (const cx<double> *)field, (cx<double> *)g_ctx.ws_part, code, st);
     else
-      rc = launch_project_bf16(g, (const cx<float> *)field, (cx<double> *)g_ctx.ws_part, PLG_SPLIT_BF16X6,
+      rc = launch_project_bf16(g, (const cx<float> *)field, (double *)g_ctx.ws_part, PLG_SPLIT_BF16X6,
                                g_ctx.d_wtab, code, st);
-    if (!rc) rc = launch_ft_out(g, (const cx<double> *)g_ctx.ws_part, dof, nt, accumulate, out, out_precision, st);
+    if (!rc) rc = launch_ft_out(g, (const double *)g_ctx.ws_part, dof, nt, accumulate, out, out_precision, st);
     if (rc) rc = fail(rc, "FT launch failed");
   }
   if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(PLG_ERR_HIP, "FT synchronisation failed");
-  hipFree(d_ms);
+  for (void *p : {(void *)d_ms, (void *)d_pair_ms, (void *)d_mom_pair})
+    if (p) hipFree(p);
   return rc;
 }
 

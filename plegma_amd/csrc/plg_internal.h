@@ -4,6 +4,7 @@
 #include "plegma_amd.h"
 #include "plg_common.h"
 
+#include <array>
 #include <string>
 #include <vector>
 
@@ -63,8 +64,11 @@ int launch_baryon(bool two, bool mes, int gmask, const PropDev<T> *props, const 
                   int nrows, cx<T> *out, int64_t V3, int T_lat, int nch_total, int ch0,
                   int tiles_per_block, hipStream_t st);
 
-// projection: partial sums part[split][n][m] (complex double) of
-// sum_x exp(-i 2pi n.x / L) c[n][x] over the split's site range
+// projection: partial sums over each K split's site range of
+// sum_x exp(-i 2pi n.x / L) c[n][x], in one of two layouts:
+//   npairs == 0 (FP64 path): part[split][n][m], complex double;
+//   npairs > 0 (mixed path): part[split][2n + {re, im}][2j + {cos, sin}],
+//     real double, j = pair of momenta (n, -n) (see part_value)
 struct ProjGeom {
   int64_t V3;
   int L[3];
@@ -75,6 +79,9 @@ struct ProjGeom {
   const int *mom_scaled;   // device, nmom*3: n_k * (M / L_k) mod M
   const double *cos_tab;   // device, M entries cos(2 pi k / M)
   const double *sin_tab;
+  int npairs = 0;              // momentum pairs of the paired layout
+  const int *pair_ms = nullptr;   // device, npairs*3: scaled representative momenta
+  const int *mom_pair = nullptr;  // device, nmom: 2j (+1 for the -n member)
   // layout of the K x N operand: 1 = site correlator, tile-major (below);
   // 0 = a field, column-major (column n, site x at n*V3 + x)
   int ctile;
@@ -93,22 +100,58 @@ __host__ __device__ __forceinline__ int64_t cidx(const ProjGeom &g, int64_t n, i
 __host__ __device__ __forceinline__ int64_t ctile_sites(int64_t V3) { return (V3 + 15) / 16 * 16; }
 int launch_project_f64(const ProjGeom &g, const cx<double> *c, cx<double> *part, int mtile,
                        hipStream_t st);
-// wtab: M x 6 packed words (cos_p | sin_p << 16, -sin_p | cos_p << 16) for
-// the three bf16 pieces p; mbw: 16-row blocks per wave (1, 2, 3, 5, 9)
-int64_t proj_workgroups(int code, int64_t ncols, int nmom);
-int launch_ft_out(const ProjGeom &g, const cx<double> *part, int dof, int nt, int accumulate, void *out,
+// mixed projection in the paired layout.  wtab: M x 6 packed words, the
+// first three (cos_p | sin_p << 16) for the three bf16 pieces p of
+// cos/sin(2 pi k / M); code: tile code 100 + 10 RB + NBW (RB row blocks of
+// 16 rows, 64 NBW real columns per workgroup)
+int launch_project_bf16(const ProjGeom &g, const cx<float> *c, double *part, int split_terms,
+                        const uint32_t *wtab, int code, hipStream_t st);
+int64_t proj_workgroups(int code, int64_t ncols, int npairs);
+int pick_proj(int npairs);
+std::vector<int> proj_candidates(int npairs);
+// pairs (n, -n) of scaled momenta; returns the number of pairs
+int make_pairs(const std::vector<int> &ms, int M, std::vector<int> &pair_ms, std::vector<int> &mom_pair);
+// the projection sum_s part[s] of column n, momentum m, in either layout
+__device__ __forceinline__ cx<double> part_value(const ProjGeom &g, const double *part, int64_t n, int m) {
+  double re = 0, im = 0;
+  if (g.npairs > 0) {
+    const int pm = g.mom_pair[m];
+    const int R = 2 * g.npairs, rc = pm & ~1;
+    const int64_t nreal = 2 * (int64_t)g.ncols;
+    double cr = 0, ci = 0, sr = 0, si = 0;
+    for (int s = 0; s < g.nsplit; s++) {
+      const double *p = part + ((int64_t)s * nreal + 2 * n) * R + rc;
+      cr += p[0];
+      sr += p[1];
+      ci += p[R];
+      si += p[R + 1];
+    }
+    // P(n) = (Cr + Si) + i (Ci - Sr), P(-n) = (Cr - Si) + i (Ci + Sr)
+    if (pm & 1) {
+      re = cr - si;
+      im = ci + sr;
+    } else {
+      re = cr + si;
+      im = ci - sr;
+    }
+  } else {
+    for (int s = 0; s < g.nsplit; s++) {
+      const double *p = part + 2 * (((int64_t)s * g.ncols + n) * g.nmom + m);
+      re += p[0];
+      im += p[1];
+    }
+  }
+  return cmake<double>(re, im);
+}
+// bytes of the partial sums of a projection
+inline size_t proj_part_bytes(int nsplit, int64_t ncols, int nmom, int npairs) {
+  return npairs > 0 ? (size_t)nsplit * ncols * npairs * 32 : (size_t)nsplit * ncols * nmom * 16;
+}
+int launch_ft_out(const ProjGeom &g, const double *part, int dof, int nt, int accumulate, void *out,
                   int out_prec, hipStream_t st);
-int launch_project_bf16(const ProjGeom &g, const cx<float> *c, cx<double> *part, int split_terms,
-                        const uint32_t *wtab, int mbw, hipStream_t st);
-int pick_mbw(int nmom);
-// projection tile codes: < 100 per-wave A (MBW row blocks per wave);
-// 100 + 10*RB + NBW: A tile shared by the workgroup (RB row blocks, 4*NBW
-// column blocks)
-int pick_proj(int nmom);
-std::vector<int> proj_candidates(int nmom);
 // sum over splits, source phase, scatter into the H layout
 // nch_c: channels per computed row; nout: output channels per out row
-int launch_finalize(const ProjGeom &g, const cx<double> *part, const OutRowDev *orows, int norows,
+int launch_finalize(const ProjGeom &g, const double *part, const OutRowDev *orows, int norows,
                     int nch_c, int nout, const int *csel, const int64_t *ch_off, int64_t m_stride,
                     void *out, int out_prec, hipStream_t st);
 // position space: c[crow][ch][x] -> out[base + ch_off[k] + x*x_stride] (H position layout)

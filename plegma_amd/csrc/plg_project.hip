@@ -155,7 +155,7 @@ int launch_project_f64(const ProjGeom &g, const cx<double> *c, cx<double> *part,
 // n = crow*nch_c + ch_base + csel[k].  With the GEMM's exp(-i 2pi n.x/L) this
 // is exp(-i 2pi n.(x - x0)/L) (PLEGMA_kernel_utils.cuh:833-838, sign -1).
 template <typename TO>
-__global__ void k_finalize(ProjGeom g, const cx<double> *__restrict__ part, const OutRowDev *__restrict__ orows,
+__global__ void k_finalize(ProjGeom g, const double *__restrict__ part, const OutRowDev *__restrict__ orows,
                            int nch_c, int nout, const int *__restrict__ csel, const int64_t *__restrict__ ch_off,
                            int64_t m_stride, cx<TO> *__restrict__ out) {
   const OutRowDev R = orows[blockIdx.y];
@@ -164,12 +164,8 @@ __global__ void k_finalize(ProjGeom g, const cx<double> *__restrict__ part, cons
   if (e >= (int64_t)g.nmom * nr) return;
   const int m = (int)(e / nr), k = (int)(e % nr);
   const int64_t n = (int64_t)R.crow * nch_c + R.ch_base + csel[k];
-  double re = 0, im = 0;
-  for (int s = 0; s < g.nsplit; s++) {
-    const cx<double> v = part[((int64_t)s * g.ncols + n) * g.nmom + m];
-    re += v.re;
-    im += v.im;
-  }
+  const cx<double> v = part_value(g, part, n, m);
+  const double re = v.re, im = v.im;
   int ph = g.mom_scaled[3 * m] * R.sx + g.mom_scaled[3 * m + 1] * R.sy + g.mom_scaled[3 * m + 2] * R.sz;
   ph %= g.M;
   if (ph < 0) ph += g.M;
@@ -181,7 +177,7 @@ __global__ void k_finalize(ProjGeom g, const cx<double> *__restrict__ part, cons
   out[R.base + ch_off[k] + (int64_t)m * m_stride] = o;
 }
 
-int launch_finalize(const ProjGeom &g, const cx<double> *part, const OutRowDev *orows, int norows,
+int launch_finalize(const ProjGeom &g, const double *part, const OutRowDev *orows, int norows,
                     int nch_c, int nout, const int *csel, const int64_t *ch_off, int64_t m_stride,
                     void *out, int out_prec, hipStream_t st) {
   const int64_t ne = (int64_t)g.nmom * nout;
@@ -200,17 +196,13 @@ int launch_finalize(const ProjGeom &g, const cx<double> *part, const OutRowDev *
 // out[t][i][m] (the buffer layout of lib/kernels/PLEGMA_FT.cuh:95-97), added
 // to the previous contents when accumulate is set
 template <typename TO>
-__global__ void k_ft_out(ProjGeom g, const cx<double> *__restrict__ part, int dof, int nt, int accumulate,
+__global__ void k_ft_out(ProjGeom g, const double *__restrict__ part, int dof, int nt, int accumulate,
                          cx<TO> *__restrict__ out) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (int64_t)g.ncols * g.nmom) return;
   const int col = (int)(e / g.nmom), m = (int)(e % g.nmom);
-  double re = 0, im = 0;
-  for (int s = 0; s < g.nsplit; s++) {
-    const cx<double> v = part[((int64_t)s * g.ncols + col) * g.nmom + m];
-    re += v.re;
-    im += v.im;
-  }
+  const cx<double> v = part_value(g, part, col, m);
+  const double re = v.re, im = v.im;
   const int i = col / nt, t = col % nt;
   cx<TO> &o = out[((int64_t)t * dof + i) * g.nmom + m];
   if (accumulate) {
@@ -222,7 +214,7 @@ __global__ void k_ft_out(ProjGeom g, const cx<double> *__restrict__ part, int do
   }
 }
 
-int launch_ft_out(const ProjGeom &g, const cx<double> *part, int dof, int nt, int accumulate, void *out,
+int launch_ft_out(const ProjGeom &g, const double *part, int dof, int nt, int accumulate, void *out,
                   int out_prec, hipStream_t st) {
   const int64_t ne = (int64_t)g.ncols * g.nmom;
   dim3 block(256), grid((unsigned)((ne + 255) / 256));
