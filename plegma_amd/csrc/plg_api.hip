@@ -194,6 +194,7 @@ struct Plan {
   int *d_csel = nullptr;
   int64_t *d_choff = nullptr;
   int *d_ms = nullptr, *d_pair_ms = nullptr, *d_mom_pair = nullptr;
+  uint16_t *d_pidx = nullptr;  // phase indices of the mixed projection
   // batch-owned workspace (single calls use the context workspace)
   void *c = nullptr, *part = nullptr;
   size_t c_bytes = 0, part_bytes = 0;
@@ -209,7 +210,7 @@ struct Plan {
     if (graph) hipGraphDestroy(graph);
     if (cap_stream) hipStreamDestroy(cap_stream);
     for (void *p : {(void *)d_props, (void *)d_rows, (void *)d_orows, (void *)d_csel, (void *)d_choff, (void *)d_ms,
-                    (void *)d_pair_ms, (void *)d_mom_pair})
+                    (void *)d_pair_ms, (void *)d_mom_pair, (void *)d_pidx})
       if (p) hipFree(p);
     if (own_ws) {
       if (c) hipFree(c);
@@ -529,6 +530,18 @@ static int upload_plan(Plan &P, hipStream_t st) {
   if ((rc = upload(&P.d_ms, P.ms, st))) return rc;
   if ((rc = upload(&P.d_pair_ms, P.pair_ms, st))) return rc;
   if ((rc = upload(&P.d_mom_pair, P.mom_pair, st))) return rc;
+  if (P.npairs > 0) {
+    if (P.d_pidx) hipFree(P.d_pidx);
+    P.d_pidx = nullptr;
+    HIPCHK(hipMalloc((void **)&P.d_pidx, phase_index_bytes(g_ctx.V3, P.npairs)));
+    ProjGeom g;
+    g.V3 = g_ctx.V3;
+    for (int i = 0; i < 3; i++) g.L[i] = g_ctx.L[i];
+    g.M = g_ctx.M;
+    g.npairs = P.npairs;
+    g.pair_ms = P.d_pair_ms;
+    if ((rc = launch_phase_index(g, P.d_pidx, st))) return fail(rc, "phase index launch failed");
+  }
   return PLG_OK;
 }
 
@@ -555,6 +568,7 @@ static ProjGeom plan_geom(const Plan &P) {
   g.npairs = P.npairs;
   g.pair_ms = P.d_pair_ms;
   g.mom_pair = P.d_mom_pair;
+  g.pidx = P.d_pidx;
   return g;
 }
 
@@ -643,7 +657,7 @@ static std::string tune_key(const Plan &P) {
   k << prop.gcnArchName << " L=" << g_ctx.L[0] << "x" << g_ctx.L[1] << "x" << g_ctx.L[2] << "x" << g_ctx.L[3]
     << " nmom=" << P.nmom << " cols=" << P.rows.size() * P.nch_c << " rows=" << P.rows.size() << " mode=" << P.mode
     << " npairs=" << P.npairs << " terms=" << P.terms << " kind=" << P.kind << " mask=" << P.mask
-    << " proj=pair32";
+    << " proj=pair32t";
   for (auto &l : P.launches) k << " L" << l.what << "." << l.set << "." << l.two << l.mes << "." << l.gmask;
   std::string s = k.str();
   for (char &ch : s)
@@ -734,8 +748,7 @@ static int autotune(Plan &P, hipStream_t st) {
           [&] {
             if (P.T == PLG_F64) lrc = launch_project_f64(g, (const cx<double> *)P.c, (cx<double> *)P.part, code, st);
             else
-              lrc = launch_project_bf16(g, (const cx<float> *)P.c, (double *)P.part, P.terms, g_ctx.d_wtab,
-                                        code, st);
+              lrc = launch_project_bf16(g, (const cx<float> *)P.c, (double *)P.part, P.terms, g_ctx.d_wtab, code, st);
           },
           st);
       if (lrc != PLG_OK) {  // a tile this geometry does not provide (LDS size)
@@ -938,6 +951,7 @@ int plg_ft_apply(const void *field, int precision, int dof, int nt, const int *m
   }
   g.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>((1024 + tiles - 1) / tiles, g_ctx.V3 / 512));
   int *d_ms = nullptr, *d_pair_ms = nullptr, *d_mom_pair = nullptr;
+  uint16_t *d_pidx = nullptr;
   int rc = upload(&d_ms, ms, st);
   if (!rc) rc = upload(&d_pair_ms, pair_ms, st);
   if (!rc) rc = upload(&d_mom_pair, mom_pair, st);
@@ -946,21 +960,35 @@ int plg_ft_apply(const void *field, int precision, int dof, int nt, const int *m
   g.pair_ms = d_pair_ms;
   g.mom_pair = d_mom_pair;
   g.ctile = 0;
+  if (!rc && npairs > 0) {
+    if (hipMalloc((void **)&d_pidx, phase_index_bytes(g_ctx.V3, npairs)) != hipSuccess)
+      rc = fail(PLG_ERR_HIP, "phase index allocation failed");
+    else if ((rc = launch_phase_index(g, d_pidx, st)))
+      rc = fail(rc, "phase index launch failed");
+    g.pidx = d_pidx;
+  }
   if (!rc) {
     std::lock_guard<std::mutex> lk(g_mu);
     rc = ensure(&g_ctx.ws_part, &g_ctx.ws_part_bytes, proj_part_bytes(g.nsplit, g.ncols, nmom, npairs));
+  }
+  if (!rc && precision == PLG_F32) {
+    // the mixed projection reads the tile-major layout of the site correlator
+    std::lock_guard<std::mutex> lk(g_mu);
+    rc = ensure(&g_ctx.ws_c, &g_ctx.ws_c_bytes, (size_t)g.ncols * ctile_sites(g_ctx.V3) * sizeof(cx<float>));
+    if (!rc) rc = launch_to_tiles((const cx<float> *)field, g.ncols, g_ctx.V3, (cx<float> *)g_ctx.ws_c, st);
+    g.ctile = 1;
   }
   if (!rc) {
     if (precision == PLG_F64)
       rc = launch_project_f64(g, (const cx<double> *)field, (cx<double> *)g_ctx.ws_part, code, st);
     else
-      rc = launch_project_bf16(g, (const cx<float> *)field, (double *)g_ctx.ws_part, PLG_SPLIT_BF16X6,
+      rc = launch_project_bf16(g, (const cx<float> *)g_ctx.ws_c, (double *)g_ctx.ws_part, PLG_SPLIT_BF16X6,
                                g_ctx.d_wtab, code, st);
     if (!rc) rc = launch_ft_out(g, (const double *)g_ctx.ws_part, dof, nt, accumulate, out, out_precision, st);
     if (rc) rc = fail(rc, "FT launch failed");
   }
   if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(PLG_ERR_HIP, "FT synchronisation failed");
-  for (void *p : {(void *)d_ms, (void *)d_pair_ms, (void *)d_mom_pair})
+  for (void *p : {(void *)d_ms, (void *)d_pair_ms, (void *)d_mom_pair, (void *)d_pidx})
     if (p) hipFree(p);
   return rc;
 }

@@ -82,8 +82,10 @@ struct ProjGeom {
   int npairs = 0;              // momentum pairs of the paired layout
   const int *pair_ms = nullptr;   // device, npairs*3: scaled representative momenta
   const int *mom_pair = nullptr;  // device, nmom: 2j (+1 for the -n member)
+  const uint16_t *pidx = nullptr;  // device, phase indices n_j.x (M/L) mod M of the pairs
   // layout of the K x N operand: 1 = site correlator, tile-major (below);
-  // 0 = a field, column-major (column n, site x at n*V3 + x)
+  // 0 = a field, column-major (column n, site x at n*V3 + x).  The mixed
+  // projection reads the tile-major layout only.
   int ctile;
 };
 // The site correlator is stored tile-major: the 16 sites of a site tile are
@@ -100,12 +102,17 @@ __host__ __device__ __forceinline__ int64_t cidx(const ProjGeom &g, int64_t n, i
 __host__ __device__ __forceinline__ int64_t ctile_sites(int64_t V3) { return (V3 + 15) / 16 * 16; }
 int launch_project_f64(const ProjGeom &g, const cx<double> *c, cx<double> *part, int mtile,
                        hipStream_t st);
-// mixed projection in the paired layout.  wtab: M x 6 packed words, the
-// first three (cos_p | sin_p << 16) for the three bf16 pieces p of
-// cos/sin(2 pi k / M); code: tile code 100 + 10 RB + NBW (RB row blocks of
-// 16 rows, 64 NBW real columns per workgroup)
+// mixed projection in the paired layout, on the phase indices g.pidx and
+// wtab (M x 6 packed words, the first three cos_p | sin_p << 16 for the bf16
+// pieces p of cos/sin(2 pi k / M)); code: tile code 1000 + 100 WR + 10 WC
+// + GR (see plg_project_bf16.hip)
 int launch_project_bf16(const ProjGeom &g, const cx<float> *c, double *part, int split_terms,
                         const uint32_t *wtab, int code, hipStream_t st);
+// phase indices n_j . x (M / L) mod M of g's pairs
+size_t phase_index_bytes(int64_t V3, int npairs);
+int launch_phase_index(const ProjGeom &g, uint16_t *tab, hipStream_t st);
+// column-major FP32 field (column n at n * V3) -> tile-major layout (ctile_idx)
+int launch_to_tiles(const cx<float> *f, int64_t ncols, int64_t V3, cx<float> *t, hipStream_t st);
 int64_t proj_workgroups(int code, int64_t ncols, int npairs);
 int pick_proj(int npairs);
 std::vector<int> proj_candidates(int npairs);

@@ -19,11 +19,20 @@
 // small products first into an FP32 accumulator that is added into FP64
 // registers every FLUSH steps (128 sites); FP64 partials per K split.
 //
+// The phase matrix does not depend on the column (sources and time slices
+// take their source phase in the finalize step).  Its integer phase indices
+// k = n_j . x (M / L) mod M are built once per momentum list
+// (k_phase_index), step-major,
+//   pidx[(kstep * PP + j) * 32 + s],  PP = pairs rounded to 8,
+// and the workgroups expand them through an LDS table of the bf16 pieces of
+// cos/sin(2 pi k / M): 2 bytes read per phase instead of 12.
 // Workgroup: 256 threads, RB row blocks (16 rows) x 64*NBW real columns over
-// one K range.  The phase tile of a step (RB*16 rows x 32 sites, NP pieces)
-// is built once in LDS from the phase table and read by the four waves,
-// which split the columns (wave w: real column blocks w*NBW + j).
+// one K range.  Per step it stages the phase tile (RB*16 rows x 32 sites, NP
+// pieces) and the split column tile in LDS; the four waves split the columns
+// (wave w: real column blocks w*NBW + j).  Global loads of step s + 1 are in
+// flight during the MFMAs of step s.
 #include <map>
+#include <type_traits>
 
 #include "plg_internal.h"
 
@@ -35,17 +44,65 @@ typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 
 constexpr int PQ_THREADS = 256;
 constexpr int PQ_KT = 32;              // sites per step (the MFMA K)
-constexpr int PQ_LD = PQ_KT + 8;       // bf16 per LDS tile row (80 B, 16-byte aligned)
-constexpr int PQ_MMAX = 512;           // phase-table entries (10-bit packed momenta)
-constexpr int PQ_FLUSH = 4;
 
-// k mod M for 0 <= k < 2^24 with a float reciprocal (exact quotient +-1, fixed up)
-__device__ __forceinline__ int mod_m(int k, int M, float invM) {
-  int q = (int)((float)k * invM);
-  int r = k - q * M;
-  r += r < 0 ? M : 0;
-  r -= r >= M ? M : 0;
-  return r;
+// LDS tiles hold 64-byte rows (32 bf16 sites) without padding; the four
+// 16-byte quarters q of row r sit at quarter q ^ ((r >> 1) & 3).  With that
+// swizzle the MFMA fragment access (lane l: row l % 16, quarter l / 16) and
+// the tile stores, which use the same lane pattern, are free of LDS bank
+// conflicts on gfx950 (measured: the plain and row-padded layouts take 2-3x
+// the cycles).
+__device__ __forceinline__ int lds_q(int r, int q) { return r * PQ_KT + 8 * (q ^ ((r >> 1) & 3)); }
+constexpr int PQ_FLUSH = 4;
+constexpr int PQ_INFLIGHT = 3;         // register sets of steps loaded ahead by a producer
+
+constexpr int PQ_MMAX = 512;           // phase-table entries in LDS
+
+// pair rows of the phase index table (the phase rows 2j, 2j + 1 rounded to 16)
+__host__ __device__ __forceinline__ int64_t pidx_pairs(int npairs) { return (npairs + 7) / 8 * 8; }
+
+size_t phase_index_bytes(int64_t V3, int npairs) {
+  return (size_t)((V3 + PQ_KT - 1) / PQ_KT) * pidx_pairs(npairs) * PQ_KT * sizeof(uint16_t);
+}
+
+// one thread per (step, pair, site of the step): k = n_j . x (M / L) mod M,
+// exact in integers; 0 for padding pairs and sites past the volume
+__global__ void k_phase_index(ProjGeom g, uint16_t *__restrict__ tab) {
+  const int64_t PP = pidx_pairs(g.npairs);
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nk = (g.V3 + PQ_KT - 1) / PQ_KT;
+  if (e >= nk * PP * PQ_KT) return;
+  const int s = (int)(e % PQ_KT);
+  const int j = (int)((e / PQ_KT) % PP);
+  const int64_t x = (e / (PQ_KT * PP)) * PQ_KT + s;
+  int64_t k = 0;
+  if (x < g.V3 && j < g.npairs) {
+    const int64_t ux = x % g.L[0], uy = (x / g.L[0]) % g.L[1], uz = x / ((int64_t)g.L[0] * g.L[1]);
+    k = (g.pair_ms[3 * j] * ux + g.pair_ms[3 * j + 1] * uy + g.pair_ms[3 * j + 2] * uz) % g.M;
+  }
+  tab[e] = (uint16_t)k;
+}
+
+int launch_phase_index(const ProjGeom &g, uint16_t *tab, hipStream_t st) {
+  if (g.npairs <= 0 || !g.pair_ms) return PLG_ERR_INVALID;
+  const int64_t n = (g.V3 + PQ_KT - 1) / PQ_KT * pidx_pairs(g.npairs) * PQ_KT;
+  k_phase_index<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st>>>(g, tab);
+  return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;
+}
+
+// column-major field f[n][x] (n < ncols, x < V3) -> tile-major site
+// correlator layout (ctile_idx), zero at the padding sites of the last tile
+__global__ void k_to_tiles(const cx<float> *__restrict__ f, int64_t ncols, int64_t V3, cx<float> *__restrict__ t) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t S = ctile_sites(V3);
+  if (e >= ncols * S) return;
+  const int64_t n = e / S, x = e % S;
+  t[ctile_idx(n, x, ncols)] = x < V3 ? f[n * V3 + x] : cmake<float>(0, 0);
+}
+
+int launch_to_tiles(const cx<float> *f, int64_t ncols, int64_t V3, cx<float> *t, hipStream_t st) {
+  const int64_t n = ncols * ctile_sites(V3);
+  k_to_tiles<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st>>>(f, ncols, V3, t);
+  return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;
 }
 
 // two floats -> three packed bf16 pairs p0 + p1 + p2, each rounded to nearest
@@ -64,29 +121,51 @@ __device__ __forceinline__ void split3(float a, float b, uint32_t *w) {
   w[2] = __builtin_bit_cast(uint32_t, h2);
 }
 
-template <int RB, int NBW, int TERMS>
-__global__ __launch_bounds__(PQ_THREADS, 2) void k_project_pair(ProjGeom g, const cx<float> *__restrict__ c,
-                                                                const uint32_t *__restrict__ wtab,
-                                                                double *__restrict__ part) {
+// Warp-specialised workgroup of 512 threads: waves 0-3 run the MFMAs of
+// step k on one stage of the LDS tiles while waves 4-7 (the producers) split
+// step k + 1 into the other stage; one barrier per step.  The producers keep
+// the global loads of the next PQ_INFLIGHT steps in flight in a ring of
+// register sets (a load takes about a microsecond under load, several
+// steps).  A consumer and a producer wave share each SIMD, so the
+// producers' VALU work runs in the shadow of the consumers' MFMAs.
+template <int WR, int WC, int GR, int TERMS>
+__global__ __launch_bounds__(2 * PQ_THREADS, 1) void k_project_pair(ProjGeom g, const cx<float> *__restrict__ c,
+                                                                    const uint32_t *__restrict__ wtab, double *__restrict__ part,
+                                                                    int nrg, int ncg) {
   constexpr int NP = TERMS == 3 ? 2 : 3;
-  constexpr int NR = 4 * NBW * 16;        // real columns per workgroup
+  constexpr int GC = 4 / GR;              // consumer waves along the columns
+  constexpr int RB = WR * GR;             // row blocks per workgroup
+  constexpr int NR = WC * GC * 16;        // real columns per workgroup
   constexpr int NCX = NR / 2;             // complex columns per workgroup
-  constexpr int NPR = RB * 8;             // pairs per row group
-  constexpr int AIT = NPR * (PQ_KT / 2);  // phase items (pair, site pair) per step
+  // phase items (pair, 8 sites) per step, in groups of 16 pairs x 4 octets
+  constexpr int AIT = (RB * 8 + 15) / 16 * 64;
   constexpr int API = (AIT + PQ_THREADS - 1) / PQ_THREADS;
   constexpr int BIT = NCX * (PQ_KT / 8);  // column items (complex column, 8 sites) per step
   constexpr int BPI = (BIT + PQ_THREADS - 1) / PQ_THREADS;
-  __shared__ __align__(16) uint16_t sA[NP][RB * 16][PQ_LD];
-  __shared__ __align__(16) uint16_t sB[NP][NR][PQ_LD];
-  extern __shared__ __align__(16) uint32_t sW[];  // M x 4 words: cos_p | sin_p << 16 (p < 3), 0
+  constexpr int ASZ = RB * 16 * PQ_KT, BSZ = NR * PQ_KT;  // bf16 per piece of a stage
+  static_assert(NCX % 16 == 0, "column tile of whole 16-column groups");
+  // two stages of tiles (step k in stage k % 2), pieces p at p * ASZ / p * BSZ.
+  // B rows: real part of the tile's complex column n at row n, imaginary
+  // part at row NCX + n
+  __shared__ __align__(16) uint16_t sA[2][NP * ASZ];
+  __shared__ __align__(16) uint16_t sB[2][NP * BSZ];
+  __shared__ u4 junk[64];
+  __shared__ u4 sW[PQ_MMAX];  // entry k: cos_p | sin_p << 16 of cos/sin(2 pi k / M), p < 3
 
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t cx0 = (int64_t)blockIdx.x * NCX;  // first complex column
+  // Workgroups are dispatched round-robin over the 8 XCDs (blockIdx.x mod
+  // 8).  The row groups of one column group go to the same XCD at nearly the
+  // same time, so that their common column tile is read from HBM once and
+  // then from that XCD's L2.
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int rg = slot % nrg, cg = (slot / nrg) * 8 + xcd;
+  if (cg >= ncg) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool producer = wave >= 4;
+  const int64_t cx0 = (int64_t)cg * NCX;  // first complex column
   const int split = blockIdx.y;
-  const int row0 = blockIdx.z * RB * 16;
+  const int row0 = rg * RB * 16;
   const int R = 2 * g.npairs;
-  const int M = g.M;
-  const float invM = 1.0f / (float)M;
+  const int64_t PP = pidx_pairs(g.npairs);
   // K range of the split, whole steps of 32 sites
   int64_t xb, xe;
   {
@@ -94,173 +173,222 @@ __global__ __launch_bounds__(PQ_THREADS, 2) void k_project_pair(ProjGeom g, cons
     xb = (int64_t)split * per;
     xe = xb + per < g.V3 ? xb + per : g.V3;
   }
-  for (int i = tid; i < M; i += PQ_THREADS) {
-#pragma unroll
-    for (int p = 0; p < 3; p++) sW[4 * i + p] = wtab[6 * i + p];
-    sW[4 * i + 3] = 0;
-  }
-  // phase items of this thread: pair pj[q] of the row group, site pair
-  // tid & 15 (the same for every q); packed scaled momentum, -1 = padding row
-  int ppk[API], pj[API];
-#pragma unroll
-  for (int q = 0; q < API; q++) {
-    const int it = tid + PQ_THREADS * q;
-    pj[q] = it >> 4;
-    const int pair = row0 / 2 + pj[q];
-    ppk[q] = (it < AIT && pair < g.npairs)
-                 ? (g.pair_ms[3 * pair] | (g.pair_ms[3 * pair + 1] << 10) | (g.pair_ms[3 * pair + 2] << 20))
-                 : -1;
-  }
-  const int sp = tid & 15;
-  // coordinates of site xt + 2 sp, advanced by one step per iteration
-  int ux, uy, uz;
-  {
-    const int x = (int)(xb + 2 * sp);
-    ux = x % g.L[0];
-    const int qd = x / g.L[0];
-    uy = qd % g.L[1];
-    uz = qd / g.L[1];
-  }
+  const int nk = xe > xb ? (int)((xe - xb + PQ_KT - 1) / PQ_KT) : 0;
+  for (int i = threadIdx.x; i < g.M; i += 2 * PQ_THREADS) sW[i] = u4{wtab[6 * i], wtab[6 * i + 1], wtab[6 * i + 2], 0};
+  __syncthreads();
 
-  f4 acc[RB][NBW];
-  double acc64[RB][NBW][4];
+  if (producer) {
+    const int ptid = threadIdx.x - PQ_THREADS;
+    // phase item q: pair aj of the tile, sites 8 ao .. 8 ao + 7 (rows 2 aj
+    // cos, 2 aj + 1 sin); reads clamped into the index table (pairs past it
+    // feed outputs that are not stored), items past the tile store into junk
+    int aj[API], ao[API];
+    bool alive[API];
+    const uint16_t *asrc[API];
+    const int64_t astep = PP * PQ_KT;
 #pragma unroll
-  for (int i = 0; i < RB; i++)
-#pragma unroll
-    for (int j = 0; j < NBW; j++) {
-      acc[i][j] = f4{0, 0, 0, 0};
-#pragma unroll
-      for (int v = 0; v < 4; v++) acc64[i][j][v] = 0;
+    for (int q = 0; q < API; q++) {
+      const int it = ptid + PQ_THREADS * q;
+      aj[q] = (it & 15) + 16 * (it >> 6);
+      ao[q] = (it >> 4) & 3;
+      alive[q] = (API * PQ_THREADS == AIT || it < AIT) && aj[q] < RB * 8;
+      const int64_t pr = min((int64_t)(row0 / 2 + aj[q]), PP - 1);
+      asrc[q] = g.pidx + (xb / PQ_KT * PP + pr) * PQ_KT + 8 * ao[q];
     }
-
-  // site-correlator values of this thread's column items (8 sites each)
-  cx<float> bv[BPI][8];
-  auto load = [&](int64_t xt) {
+    // column item q: complex column bn of the tile, sites 8 bo .. 8 bo + 7;
+    // reads clamped to the last column, items past the tile store into junk
+    int boff[BPI], bo[BPI];
+    bool bin[BPI];
+    const cx<float> *bsrc[BPI];
+    const int64_t bstep = 2 * (int64_t)g.ncols * 16;  // two site tiles
 #pragma unroll
     for (int q = 0; q < BPI; q++) {
-      const int it = tid + PQ_THREADS * q;
-      const int64_t n = cx0 + (it >> 2);
-      const int64_t x0 = xt + 8 * (it & 3);
-      const bool live = it < BIT && n < g.ncols;
-      if (live && g.ctile && x0 + 8 <= xe) {
-        // 8 sites of one column inside a 16-site tile: 64 contiguous bytes
-        const f4 *src = reinterpret_cast<const f4 *>(c + ctile_idx(n, x0, g.ncols));
+      const int it = ptid + PQ_THREADS * q;
+      const int bn = (it & 15) + 16 * (it >> 6);
+      bo[q] = (it >> 4) & 3;
+      boff[q] = (BPI * PQ_THREADS == BIT || it < BIT) ? lds_q(bn, bo[q]) : -1;
+      bin[q] = cx0 + bn < g.ncols;
+      bsrc[q] = c + ctile_idx(min(cx0 + bn, (int64_t)g.ncols - 1), xb + 8 * bo[q], g.ncols);
+    }
+    constexpr int NQ = PQ_INFLIGHT;
+    cx<float> bvr[NQ][BPI][8];
+    u4 avr[NQ][API];  // 8 phase indices (uint16) per item
+    // full step k into register set SL: unconditional loads, so that the
+    // compiler counts the loads in flight exactly
+    auto load = [&](auto slot, int k) {
+      constexpr int SL = decltype(slot)::value;
+#pragma unroll
+      for (int q = 0; q < API; q++) avr[SL][q] = *reinterpret_cast<const u4 *>(asrc[q] + k * astep);
+#pragma unroll
+      for (int q = 0; q < BPI; q++) {
+        const f4 *src = reinterpret_cast<const f4 *>(bsrc[q] + k * bstep);
 #pragma unroll
         for (int s = 0; s < 4; s++) {
           const f4 v = src[s];
-          bv[q][2 * s] = cmake<float>(v[0], v[1]);
-          bv[q][2 * s + 1] = cmake<float>(v[2], v[3]);
+          bvr[SL][q][2 * s] = cmake<float>(v[0], v[1]);
+          bvr[SL][q][2 * s + 1] = cmake<float>(v[2], v[3]);
         }
-      } else {
+      }
+    };
+    // the partial last step k (sites past the split zero) into register set 0
+    auto load_tail = [&](int k) {
+      const int64_t xt = xb + (int64_t)k * PQ_KT;
+#pragma unroll
+      for (int q = 0; q < API; q++) avr[0][q] = *reinterpret_cast<const u4 *>(asrc[q] + k * astep);
+#pragma unroll
+      for (int q = 0; q < BPI; q++) {
+        const int64_t x0 = xt + 8 * bo[q];
 #pragma unroll
         for (int s = 0; s < 8; s++)
-          bv[q][s] = (live && x0 + s < xe) ? c[cidx(g, n, x0 + s)] : cmake<float>(0, 0);
+          bvr[0][q][s] = (bin[q] && x0 + s < xe) ? bsrc[q][k * bstep + s] : cmake<float>(0, 0);
       }
-    }
-  };
-
-  int step = 0;
-  load(xb);
-  __syncthreads();  // phase table
-  for (int64_t xt = xb; xt < xe; xt += PQ_KT) {
-    // ---- column tile: real columns 2n (Re) and 2n + 1 (Im), 8 sites per item
+    };
+    // register set SL -> tiles of one stage (column values split into bf16 pieces)
+    auto stage = [&](auto slot, uint16_t *tA, uint16_t *tB) {
+      constexpr int SL = decltype(slot)::value;
 #pragma unroll
-    for (int q = 0; q < BPI; q++) {
-      const int it = tid + PQ_THREADS * q;
-      if (BPI * PQ_THREADS == BIT || it < BIT) {
-        const int nl = it >> 2, o = it & 3;
-        u4 wr[3], wi[3];
+      for (int q = 0; q < BPI; q++) {
+        u4 wre[3], wim[3];
 #pragma unroll
         for (int s = 0; s < 4; s++) {
           uint32_t a[3], b[3];
-          split3(bv[q][2 * s].re, bv[q][2 * s + 1].re, a);
-          split3(bv[q][2 * s].im, bv[q][2 * s + 1].im, b);
+          split3(bvr[SL][q][2 * s].re, bvr[SL][q][2 * s + 1].re, a);
+          split3(bvr[SL][q][2 * s].im, bvr[SL][q][2 * s + 1].im, b);
 #pragma unroll
           for (int p = 0; p < 3; p++) {
-            wr[p][s] = a[p];
-            wi[p][s] = b[p];
+            wre[p][s] = a[p];
+            wim[p][s] = b[p];
           }
         }
 #pragma unroll
         for (int p = 0; p < NP; p++) {
-          *reinterpret_cast<u4 *>(&sB[p][2 * nl][8 * o]) = wr[p];
-          *reinterpret_cast<u4 *>(&sB[p][2 * nl + 1][8 * o]) = wi[p];
+          *(boff[q] >= 0 ? reinterpret_cast<u4 *>(tB + p * BSZ + boff[q]) : &junk[lane]) = wre[p];
+          *(boff[q] >= 0 ? reinterpret_cast<u4 *>(tB + p * BSZ + boff[q] + NCX * PQ_KT) : &junk[lane]) = wim[p];
         }
       }
-    }
-    // ---- phase tile: rows 2j (cos) and 2j + 1 (sin) of pair j at sites
-    // xt + 2 sp and xt + 2 sp + 1
-    {
-      int vx = ux + 1, vy = uy, vz = uz;
-      if (vx == g.L[0]) {
-        vx = 0;
-        if (++vy == g.L[1]) {
-          vy = 0;
-          ++vz;
-        }
-      }
-      const bool in0 = xt + 2 * sp < xe, in1 = xt + 2 * sp + 1 < xe;
+      // phase items: table entries of 8 sites -> cos row and sin row, pieces
+      // p, two sites per word (v_perm_b32 picks the 16-bit halves)
 #pragma unroll
       for (int q = 0; q < API; q++) {
-        const int it = tid + PQ_THREADS * q;
-        if (API * PQ_THREADS == AIT || it < AIT) {
-          const int pk = ppk[q];
-          const int mx = pk & 1023, my = (pk >> 10) & 1023, mz = (pk >> 20) & 1023;
-          const bool ok0 = pk >= 0 && in0, ok1 = pk >= 0 && in1;
-          const int k0 = ok0 ? mod_m(mx * ux + my * uy + mz * uz, M, invM) : 0;
-          const int k1 = ok1 ? mod_m(mx * vx + my * vy + mz * vz, M, invM) : 0;
-          const u4 e0 = *reinterpret_cast<const u4 *>(&sW[4 * k0]);
-          const u4 e1 = *reinterpret_cast<const u4 *>(&sW[4 * k1]);
+        u4 e[8];
 #pragma unroll
-          for (int p = 0; p < NP; p++) {
-            const uint32_t w0 = ok0 ? e0[p] : 0u, w1 = ok1 ? e1[p] : 0u;
-            *reinterpret_cast<uint32_t *>(&sA[p][2 * pj[q]][2 * sp]) = (w0 & 0xFFFFu) | (w1 << 16);
-            *reinterpret_cast<uint32_t *>(&sA[p][2 * pj[q] + 1][2 * sp]) = (w0 >> 16) | (w1 & 0xFFFF0000u);
+        for (int t = 0; t < 4; t++) {
+          e[2 * t] = sW[avr[SL][q][t] & 0xFFFFu];
+          e[2 * t + 1] = sW[avr[SL][q][t] >> 16];
+        }
+        const int oc = lds_q(2 * aj[q], ao[q]), os = lds_q(2 * aj[q] + 1, ao[q]);
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+          u4 wc, ws;
+#pragma unroll
+          for (int t = 0; t < 4; t++) {
+            wc[t] = __builtin_amdgcn_perm(e[2 * t + 1][p], e[2 * t][p], 0x05040100u);
+            ws[t] = __builtin_amdgcn_perm(e[2 * t + 1][p], e[2 * t][p], 0x07060302u);
           }
+          *(alive[q] ? reinterpret_cast<u4 *>(tA + p * ASZ + oc) : &junk[lane]) = wc;
+          *(alive[q] ? reinterpret_cast<u4 *>(tA + p * ASZ + os) : &junk[lane]) = ws;
         }
       }
-      ux += PQ_KT;
-      while (ux >= g.L[0]) {
-        ux -= g.L[0];
-        if (++uy == g.L[1]) {
-          uy = 0;
-          ++uz;
-        }
-      }
+    };
+    using S0 = std::integral_constant<int, 0>;
+    using S1 = std::integral_constant<int, 1>;
+    using S2 = std::integral_constant<int, 2>;
+    static_assert(NQ == 3, "three register sets");
+    // step k sits in register set k % 3.  Consumers compute step k while
+    // the producers stage step k + 1; barrier count: 1 + nk on both sides.
+    const int nf = (int)((xe > xb ? xe - xb : 0) / PQ_KT);  // full steps
+    const int last = nf > 0 ? nf - 1 : 0;
+    if (nf > 0) {
+      load(S0{}, 0);
+      load(S1{}, min(1, last));
+      load(S2{}, min(2, last));
+      stage(S0{}, sA[0], sB[0]);
+      load(S0{}, min(3, last));
+    } else if (nk > 0) {
+      load_tail(0);
+      stage(S0{}, sA[0], sB[0]);
     }
-    if (xt + PQ_KT < xe) load(xt + PQ_KT);
     __syncthreads();
-    // ---- MFMAs, one A piece at a time (RB*NBW independent accumulators
-    // between two products into the same one)
-    {
-      const int kk = 8 * (lane >> 4);
-      bf8 bf[NBW][NP];
-#pragma unroll
-      for (int j = 0; j < NBW; j++)
-#pragma unroll
-        for (int p = 0; p < NP; p++)
-          bf[j][p] = *reinterpret_cast<const bf8 *>(&sB[p][(wave * NBW + j) * 16 + (lane & 15)][kk]);
-#pragma unroll
-      for (int pa = NP - 1; pa >= 0; pa--) {
-        bf8 af[RB];
-#pragma unroll
-        for (int i = 0; i < RB; i++) af[i] = *reinterpret_cast<const bf8 *>(&sA[pa][i * 16 + (lane & 15)][kk]);
-        // B pieces that pair with A piece pa: pa + pb <= 2 (split 6), <= 1 (split 3)
-#pragma unroll
-        for (int pb = NP - 1 - pa; pb >= 0; pb--)
-#pragma unroll
-          for (int i = 0; i < RB; i++)
-#pragma unroll
-            for (int j = 0; j < NBW; j++)
-              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bf[j][pb], acc[i][j], 0, 0, 0);
+    // steady state: stage full steps k + 1 <= nf - 1 and refill their sets
+    int k = 0;
+    for (; k + 3 <= nf - 1; k += 3) {
+      stage(S1{}, sA[(k + 1) & 1], sB[(k + 1) & 1]);
+      load(S1{}, min(k + 4, last));
+      __syncthreads();
+      stage(S2{}, sA[(k + 2) & 1], sB[(k + 2) & 1]);
+      load(S2{}, min(k + 5, last));
+      __syncthreads();
+      stage(S0{}, sA[(k + 3) & 1], sB[(k + 3) & 1]);
+      load(S0{}, min(k + 6, last));
+      __syncthreads();
+    }
+    // remaining full steps (their loads are in flight already)
+    if (k + 1 <= nf - 1) {
+      stage(S1{}, sA[(k + 1) & 1], sB[(k + 1) & 1]);
+      __syncthreads();
+      ++k;
+      if (k + 1 <= nf - 1) {
+        stage(S2{}, sA[(k + 1) & 1], sB[(k + 1) & 1]);
+        __syncthreads();
+        ++k;
       }
     }
-    if (++step == PQ_FLUSH) {
-      step = 0;
+    // the partial step after the full ones, then the consumers' last step
+    if (nf > 0 && nk > nf) {
+      load_tail(nf);
+      stage(S0{}, sA[nf & 1], sB[nf & 1]);
+      __syncthreads();
+    }
+    if (nk > 0) __syncthreads();
+    return;
+  }
+
+  // ---- consumers
+  const int wr = wave / GC, wc = wave % GC;  // this wave's row and column group
+  f4 acc[WR][WC];
+  double acc64[WR][WC][4];
 #pragma unroll
-      for (int i = 0; i < RB; i++)
+  for (int i = 0; i < WR; i++)
 #pragma unroll
-        for (int j = 0; j < NBW; j++) {
+    for (int j = 0; j < WC; j++) {
+      acc[i][j] = f4{0, 0, 0, 0};
+#pragma unroll
+      for (int v = 0; v < 4; v++) acc64[i][j][v] = 0;
+    }
+  __syncthreads();
+  int nacc = 0;
+  for (int k = 0; k < nk; k++) {
+    const uint16_t *tA = sA[k & 1], *tB = sB[k & 1];
+    // MFMAs one A piece at a time (WR*WC independent accumulators between
+    // two products into the same one)
+    const int fq = lane >> 4;
+    bf8 bf[WC][NP];
+#pragma unroll
+    for (int j = 0; j < WC; j++)
+#pragma unroll
+      for (int p = 0; p < NP; p++)
+        bf[j][p] = *reinterpret_cast<const bf8 *>(tB + p * BSZ + lds_q((wc * WC + j) * 16 + (lane & 15), fq));
+#pragma unroll
+    for (int pa = NP - 1; pa >= 0; pa--) {
+      bf8 af[WR];
+#pragma unroll
+      for (int i = 0; i < WR; i++)
+        af[i] = *reinterpret_cast<const bf8 *>(tA + pa * ASZ + lds_q((wr * WR + i) * 16 + (lane & 15), fq));
+      // B pieces that pair with A piece pa: pa + pb <= 2 (split 6), <= 1 (split 3)
+#pragma unroll
+      for (int pb = NP - 1 - pa; pb >= 0; pb--)
+#pragma unroll
+        for (int i = 0; i < WR; i++)
+#pragma unroll
+          for (int j = 0; j < WC; j++)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bf[j][pb], acc[i][j], 0, 0, 0);
+    }
+    if (++nacc == PQ_FLUSH) {
+      nacc = 0;
+#pragma unroll
+      for (int i = 0; i < WR; i++)
+#pragma unroll
+        for (int j = 0; j < WC; j++) {
 #pragma unroll
           for (int v = 0; v < 4; v++) acc64[i][j][v] += (double)acc[i][j][v];
           acc[i][j] = f4{0, 0, 0, 0};
@@ -271,76 +399,87 @@ __global__ __launch_bounds__(PQ_THREADS, 2) void k_project_pair(ProjGeom g, cons
   // D of 16x16x32 (f32): lane l, value v -> row 4*(l/16) + v, column l%16
   const int64_t nreal = 2 * (int64_t)g.ncols;
 #pragma unroll
-  for (int i = 0; i < RB; i++)
+  for (int i = 0; i < WR; i++)
 #pragma unroll
-    for (int j = 0; j < NBW; j++)
+    for (int j = 0; j < WC; j++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int r = row0 + i * 16 + 4 * (lane >> 4) + v;
-        const int64_t n = 2 * cx0 + (wave * NBW + j) * 16 + (lane & 15);
+        const int r = row0 + (wr * WR + i) * 16 + 4 * (lane >> 4) + v;
+        const int jt = (wc * WC + j) * 16 + (lane & 15);  // tile row of the column
+        const int64_t n = 2 * (cx0 + (jt < NCX ? jt : jt - NCX)) + (jt < NCX ? 0 : 1);
         if (r < R && n < nreal) part[((int64_t)split * nreal + n) * R + r] = acc64[i][j][v] + (double)acc[i][j][v];
       }
 }
 
-// tile codes 100 + 10 RB + NBW (RB row blocks of 16 rows, 64 NBW real columns)
-#define PLG_PQ_CODES(X) X(9, 1) X(6, 1) X(5, 1) X(4, 1) X(4, 2) X(3, 2) X(2, 2) X(1, 2)
+// Tile codes 1000 + 100 WR + 10 WC + GR: each wave owns WR x WC blocks of
+// 16 x 16 outputs, the four waves form GR x (4 / GR) groups over rows x
+// columns; a workgroup covers WR*GR row blocks and 16*WC*(4/GR) real columns.
+#define PLG_PQ_CODES(X)                                                                                      \
+  X(6, 1, 1) X(5, 2, 1) X(4, 2, 1) X(3, 3, 1) X(3, 3, 2) X(4, 2, 2) X(2, 4, 2) X(3, 2, 2) X(4, 3, 2)          \
+  X(3, 4, 2) X(2, 2, 2) X(2, 4, 4) X(2, 6, 2)
+
+struct PqTile {
+  int wr, wc, gr;
+  int rb() const { return wr * gr; }
+  int ncx() const { return 8 * wc * (4 / gr); }
+};
+static PqTile pq_tile(int code) { return {(code / 100) % 10, (code / 10) % 10, code % 10}; }
 
 int launch_project_bf16(const ProjGeom &g, const cx<float> *c, double *part, int terms, const uint32_t *wtab,
                         int code, hipStream_t st) {
+  if (g.npairs <= 0 || !g.pidx) return PLG_ERR_INVALID;
   if (g.M > PQ_MMAX) return PLG_ERR_UNSUPPORTED;
-  if (g.npairs <= 0 || !g.pair_ms) return PLG_ERR_INVALID;
-  const int rb = (code - 100) / 10, nbw = code % 10;
-  if (code < 100 || rb <= 0 || nbw <= 0) return PLG_ERR_INVALID;
+  if (code < 1000 || code > 1999) return PLG_ERR_INVALID;
+  const PqTile t = pq_tile(code);
   const int nblk = (2 * g.npairs + 15) / 16;
-  dim3 block(PQ_THREADS);
-  dim3 grid((unsigned)((g.ncols + 32 * nbw - 1) / (32 * nbw)), (unsigned)g.nsplit, (unsigned)((nblk + rb - 1) / rb));
-  const size_t dyn = (size_t)g.M * 4 * sizeof(uint32_t);
-#define PLG_PQ(RB, NBW)                                                                    \
-  if (rb == RB && nbw == NBW) {                                                            \
-    if (terms == 6) k_project_pair<RB, NBW, 6><<<grid, block, dyn, st>>>(g, c, wtab, part); \
-    else k_project_pair<RB, NBW, 3><<<grid, block, dyn, st>>>(g, c, wtab, part);            \
-    return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;                         \
+  const int nrg = (nblk + t.rb() - 1) / t.rb();
+  const int64_t ncg = (g.ncols + t.ncx() - 1) / t.ncx();
+  if ((ncg + 7) / 8 * 8 * nrg > INT32_MAX) return PLG_ERR_UNSUPPORTED;
+  dim3 block(2 * PQ_THREADS);
+  dim3 grid((unsigned)((ncg + 7) / 8 * 8 * nrg), (unsigned)g.nsplit);
+#define PLG_PQ(WR, WC, GR)                                                                             \
+  if (t.wr == WR && t.wc == WC && t.gr == GR) {                                                        \
+    if (terms == 6) k_project_pair<WR, WC, GR, 6><<<grid, block, 0, st>>>(g, c, wtab, part, nrg, (int)ncg); \
+    else k_project_pair<WR, WC, GR, 3><<<grid, block, 0, st>>>(g, c, wtab, part, nrg, (int)ncg);            \
+    return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;                                     \
   }
   PLG_PQ_CODES(PLG_PQ)
 #undef PLG_PQ
   return PLG_ERR_INVALID;
 }
 
-#define PLG_PQ_CODE(RB, NBW) 100 + 10 * RB + NBW,
+#define PLG_PQ_CODE(WR, WC, GR) 1000 + 100 * WR + 10 * WC + GR,
 static const int k_pq_codes[] = {PLG_PQ_CODES(PLG_PQ_CODE)};
 #undef PLG_PQ_CODE
 
-// tile codes the autotuner tries for a pair count (no row tile taller than the rows)
+// tile codes the autotuner tries for a pair count: row tiles that waste at
+// most one row block per row group (all when none qualifies)
 std::vector<int> proj_candidates(int npairs) {
   const int nblk = (2 * npairs + 15) / 16;
   std::vector<int> v;
-  for (int code : k_pq_codes)
-    if ((code - 100) / 10 <= std::max(nblk, 1)) v.push_back(code);
+  for (int code : k_pq_codes) {
+    const int rb = pq_tile(code).rb();
+    const int groups = (nblk + rb - 1) / rb;
+    if (groups * rb - nblk <= std::max(1, groups - 1)) v.push_back(code);
+  }
+  if (v.empty()) v.assign(std::begin(k_pq_codes), std::end(k_pq_codes));
   return v;
 }
 
 // workgroups of one K split for a tile code
 int64_t proj_workgroups(int code, int64_t ncols, int npairs) {
-  const int rb = (code - 100) / 10, nbw = code % 10;
+  const PqTile t = pq_tile(code);
   const int nblk = (2 * npairs + 15) / 16;
-  return ((ncols + 32 * nbw - 1) / (32 * nbw)) * ((nblk + rb - 1) / rb);
+  return ((ncols + t.ncx() - 1) / t.ncx()) * ((nblk + t.rb() - 1) / t.rb());
 }
 
-// default tile: the fewest padded row blocks, then the tallest row tile
+// default tile: 3 x 3 blocks per wave in a 2 x 2 wave grid; square-ish wave
+// tiles read the fewest LDS bytes per MFMA
 int pick_proj(int npairs) {
   const int nblk = (2 * npairs + 15) / 16;
-  int best = 112, bestw = 1 << 30, bestrb = 0;
-  for (int code : k_pq_codes) {
-    const int rb = (code - 100) / 10;
-    if (rb > std::max(nblk, 1)) continue;
-    const int waste = ((nblk + rb - 1) / rb) * rb - nblk;
-    if (waste < bestw || (waste == bestw && rb > bestrb)) {
-      best = code;
-      bestw = waste;
-      bestrb = rb;
-    }
-  }
-  return best;
+  if (nblk <= 2) return 1222;
+  if (nblk <= 4) return 1242;
+  return 1332;
 }
 
 // Pairs (n, -n) of a momentum list, on the scaled momenta ms (nmom x 3,

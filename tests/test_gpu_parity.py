@@ -353,7 +353,7 @@ def test_meson_meson_mixed_two_flavours():
 
 
 # every projection tile the autotuner may pick, pinned one at a time
-PROJ_CODES = [191, 161, 151, 141, 142, 132, 122, 112]
+PROJ_CODES = [1611, 1521, 1421, 1331, 1332, 1422, 1242, 1322, 1432, 1342, 1222, 1244, 1262]
 
 
 @pytest.mark.parametrize("code", PROJ_CODES)
