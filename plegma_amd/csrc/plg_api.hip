@@ -175,6 +175,7 @@ struct Plan {
   std::vector<int> moms;
   std::vector<int> ms;                 // scaled momenta n_k (M / L_k) mod M
   int npairs = 0;                      // (n, -n) pairs of the mixed projection
+  int zero_pair = 0;                   // the last pair is n = 0 (summed, not a GEMM row)
   std::vector<int> pair_ms, mom_pair;
   std::vector<RowDev> rows;
   std::vector<OutRowDev> orows;
@@ -268,7 +269,7 @@ static int build_plan(Plan &P, int kind, int mask, const std::vector<plg_prop> &
   P.nmom = space == PLG_MOMENTUM_SPACE ? nmom : 0;
   P.moms.assign(moms, moms + 3 * P.nmom);
   P.ms = scaled_moms(moms, P.nmom, 1);
-  if (P.T == PLG_F32 && P.nmom > 0) P.npairs = make_pairs(P.ms, g_ctx.M, P.pair_ms, P.mom_pair);
+  if (P.T == PLG_F32 && P.nmom > 0) P.npairs = make_pairs(P.ms, g_ctx.M, P.pair_ms, P.mom_pair, &P.zero_pair);
   const int64_t V3 = g_ctx.V3;
   const int64_t inner = space == PLG_MOMENTUM_SPACE ? (int64_t)nmom : V3;  // per-row extent
 
@@ -499,10 +500,11 @@ static void choose_tiles(Plan &P) {
     P.mtile = Mreal <= 96 ? 32 : 64;
     tiles = ((ncols + 127) / 128) * ((Mreal + P.mtile - 1) / P.mtile);
   } else {
-    P.mbw = pick_proj(std::max(P.npairs, 1));
+    const int gp = std::max(P.npairs - P.zero_pair, 1);
+    P.mbw = pick_proj(gp);
     // PLG_PROJ_CODE pins the projection tile (tile studies and tests)
     if (const char *e = std::getenv("PLG_PROJ_CODE")) P.mbw = std::atoi(e);
-    tiles = proj_workgroups(P.mbw, ncols, std::max(P.npairs, 1));
+    tiles = proj_workgroups(P.mbw, ncols, gp);
   }
   int64_t ns = (1024 + tiles - 1) / tiles;
   ns = std::max<int64_t>(1, std::min<int64_t>(ns, g_ctx.V3 / 512));
@@ -569,6 +571,7 @@ static ProjGeom plan_geom(const Plan &P) {
   g.pair_ms = P.d_pair_ms;
   g.mom_pair = P.d_mom_pair;
   g.pidx = P.d_pidx;
+  g.zero_pair = P.zero_pair;
   return g;
 }
 
@@ -735,7 +738,7 @@ static int autotune(Plan &P, hipStream_t st) {
   P.tpb = best_tpb;
   // projection: tile code and K splits
   ProjGeom g = plan_geom(P);
-  std::vector<int> codes = P.T == PLG_F64 ? std::vector<int>{32, 64} : proj_candidates(P.npairs);
+  std::vector<int> codes = P.T == PLG_F64 ? std::vector<int>{32, 64} : proj_candidates(std::max(P.npairs - P.zero_pair, 1));
   const size_t per_split = part_bytes(P) / std::max(P.nsplit, 1);
   double best_p = 1e30;
   int best_code = P.T == PLG_F64 ? P.mtile : P.mbw, best_ns = P.nsplit;
@@ -930,7 +933,8 @@ int plg_ft_apply(const void *field, int precision, int dof, int nt, const int *m
   // exp(sign i 2pi n.x/L): the GEMM applies exp(-i ...), so sign = +1 negates the momenta
   const std::vector<int> ms = scaled_moms(moms, nmom, -sign);
   std::vector<int> pair_ms, mom_pair;
-  const int npairs = precision == PLG_F32 ? make_pairs(ms, g_ctx.M, pair_ms, mom_pair) : 0;
+  int zero_pair = 0;
+  const int npairs = precision == PLG_F32 ? make_pairs(ms, g_ctx.M, pair_ms, mom_pair, &zero_pair) : 0;
   ProjGeom g;
   g.V3 = g_ctx.V3;
   for (int i = 0; i < 3; i++) g.L[i] = g_ctx.L[i];
@@ -946,8 +950,8 @@ int plg_ft_apply(const void *field, int precision, int dof, int nt, const int *m
     code = Mreal <= 96 ? 32 : 64;
     tiles = ((g.ncols + 127) / 128) * ((Mreal + code - 1) / code);
   } else {
-    code = pick_proj(npairs);
-    tiles = proj_workgroups(code, g.ncols, npairs);
+    code = pick_proj(std::max(npairs - zero_pair, 1));
+    tiles = proj_workgroups(code, g.ncols, std::max(npairs - zero_pair, 1));
   }
   g.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>((1024 + tiles - 1) / tiles, g_ctx.V3 / 512));
   int *d_ms = nullptr, *d_pair_ms = nullptr, *d_mom_pair = nullptr;
@@ -959,6 +963,7 @@ int plg_ft_apply(const void *field, int precision, int dof, int nt, const int *m
   g.npairs = npairs;
   g.pair_ms = d_pair_ms;
   g.mom_pair = d_mom_pair;
+  g.zero_pair = zero_pair;
   g.ctile = 0;
   if (!rc && npairs > 0) {
     if (hipMalloc((void **)&d_pidx, phase_index_bytes(g_ctx.V3, npairs)) != hipSuccess)

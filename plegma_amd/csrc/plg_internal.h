@@ -83,6 +83,7 @@ struct ProjGeom {
   const int *pair_ms = nullptr;   // device, npairs*3: scaled representative momenta
   const int *mom_pair = nullptr;  // device, nmom: 2j (+1 for the -n member)
   const uint16_t *pidx = nullptr;  // device, phase indices n_j.x (M/L) mod M of the pairs
+  int zero_pair = 0;  // 1: the last pair is n = 0, summed directly instead of a GEMM row
   // layout of the K x N operand: 1 = site correlator, tile-major (below);
   // 0 = a field, column-major (column n, site x at n*V3 + x).  The mixed
   // projection reads the tile-major layout only.
@@ -113,11 +114,14 @@ size_t phase_index_bytes(int64_t V3, int npairs);
 int launch_phase_index(const ProjGeom &g, uint16_t *tab, hipStream_t st);
 // column-major FP32 field (column n at n * V3) -> tile-major layout (ctile_idx)
 int launch_to_tiles(const cx<float> *f, int64_t ncols, int64_t V3, cx<float> *t, hipStream_t st);
+// (npairs here: the pairs of the GEMM, without a zero-momentum pair)
 int64_t proj_workgroups(int code, int64_t ncols, int npairs);
 int pick_proj(int npairs);
 std::vector<int> proj_candidates(int npairs);
-// pairs (n, -n) of scaled momenta; returns the number of pairs
-int make_pairs(const std::vector<int> &ms, int M, std::vector<int> &pair_ms, std::vector<int> &mom_pair);
+// pairs (n, -n) of scaled momenta; returns the number of pairs.  A zero
+// momentum pair is moved to the end (*zero_last = 1).
+int make_pairs(const std::vector<int> &ms, int M, std::vector<int> &pair_ms, std::vector<int> &mom_pair,
+               int *zero_last);
 // the projection sum_s part[s] of column n, momentum m, in either layout
 __device__ __forceinline__ cx<double> part_value(const ProjGeom &g, const double *part, int64_t n, int m) {
   double re = 0, im = 0;

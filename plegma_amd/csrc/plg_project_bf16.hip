@@ -10,7 +10,10 @@
 // with rows {cos, sin} (2j, 2j + 1) of the representative momentum of pair j
 // and real columns {Re c, Im c} (2n, 2n + 1) of every site-correlator column:
 // half the arithmetic of projecting every momentum on its own.  A momentum
-// without a partner in the list is a pair of its own.
+// without a partner in the list is a pair of its own.  The zero momentum (all
+// cos 1, all sin 0) is not a GEMM row: when the list holds it, it is the last
+// pair and the producer waves of the first row group sum the columns
+// directly.
 //
 // Split-bf16 operands on v_mfma_f32_16x16x32_bf16 (K = 32 sites per step):
 //   a = a0 + a1 + a2, b = b0 + b1 + b2 (bf16 pieces, ~2^-24 relative)
@@ -128,12 +131,14 @@ __device__ __forceinline__ void split3(float a, float b, uint32_t *w) {
 // register sets (a load takes about a microsecond under load, several
 // steps).  A consumer and a producer wave share each SIMD, so the
 // producers' VALU work runs in the shadow of the consumers' MFMAs.
-template <int WR, int WC, int GR, int TERMS>
-__global__ __launch_bounds__(2 * PQ_THREADS, 1) void k_project_pair(ProjGeom g, const cx<float> *__restrict__ c,
+// NCW consumer waves (4: one per SIMD; 8: two per SIMD with half the
+// accumulators each) and 4 producer waves.
+template <int WR, int WC, int GR, int TERMS, int NCW>
+__global__ __launch_bounds__((NCW + 4) * 64, 1) void k_project_pair(ProjGeom g, const cx<float> *__restrict__ c,
                                                                     const uint32_t *__restrict__ wtab, double *__restrict__ part,
                                                                     int nrg, int ncg) {
   constexpr int NP = TERMS == 3 ? 2 : 3;
-  constexpr int GC = 4 / GR;              // consumer waves along the columns
+  constexpr int GC = NCW / GR;            // consumer waves along the columns
   constexpr int RB = WR * GR;             // row blocks per workgroup
   constexpr int NR = WC * GC * 16;        // real columns per workgroup
   constexpr int NCX = NR / 2;             // complex columns per workgroup
@@ -160,11 +165,13 @@ __global__ __launch_bounds__(2 * PQ_THREADS, 1) void k_project_pair(ProjGeom g, 
   const int rg = slot % nrg, cg = (slot / nrg) * 8 + xcd;
   if (cg >= ncg) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const bool producer = wave >= 4;
+  const bool producer = wave >= NCW;
   const int64_t cx0 = (int64_t)cg * NCX;  // first complex column
   const int split = blockIdx.y;
   const int row0 = rg * RB * 16;
-  const int R = 2 * g.npairs;
+  const int R = 2 * g.npairs;               // partial rows (pairs x cos/sin)
+  const int Rg = R - 2 * g.zero_pair;        // rows of the GEMM
+  const bool zsum = g.zero_pair && rg == 0;  // this workgroup sums the columns for n = 0
   const int64_t PP = pidx_pairs(g.npairs);
   // K range of the split, whole steps of 32 sites
   int64_t xb, xe;
@@ -174,11 +181,11 @@ __global__ __launch_bounds__(2 * PQ_THREADS, 1) void k_project_pair(ProjGeom g, 
     xe = xb + per < g.V3 ? xb + per : g.V3;
   }
   const int nk = xe > xb ? (int)((xe - xb + PQ_KT - 1) / PQ_KT) : 0;
-  for (int i = threadIdx.x; i < g.M; i += 2 * PQ_THREADS) sW[i] = u4{wtab[6 * i], wtab[6 * i + 1], wtab[6 * i + 2], 0};
+  for (int i = threadIdx.x; i < g.M; i += (NCW + 4) * 64) sW[i] = u4{wtab[6 * i], wtab[6 * i + 1], wtab[6 * i + 2], 0};
   __syncthreads();
 
   if (producer) {
-    const int ptid = threadIdx.x - PQ_THREADS;
+    const int ptid = threadIdx.x - NCW * 64;
     // phase item q: pair aj of the tile, sites 8 ao .. 8 ao + 7 (rows 2 aj
     // cos, 2 aj + 1 sin); reads clamped into the index table (pairs past it
     // feed outputs that are not stored), items past the tile store into junk
@@ -211,6 +218,9 @@ __global__ __launch_bounds__(2 * PQ_THREADS, 1) void k_project_pair(ProjGeom g, 
       bsrc[q] = c + ctile_idx(min(cx0 + bn, (int64_t)g.ncols - 1), xb + 8 * bo[q], g.ncols);
     }
     constexpr int NQ = PQ_INFLIGHT;
+    double zre[BPI], zim[BPI];  // column sums of the staged steps (n = 0)
+#pragma unroll
+    for (int q = 0; q < BPI; q++) zre[q] = zim[q] = 0;
     cx<float> bvr[NQ][BPI][8];
     u4 avr[NQ][API];  // 8 phase indices (uint16) per item
     // full step k into register set SL: unconditional loads, so that the
@@ -248,6 +258,16 @@ __global__ __launch_bounds__(2 * PQ_THREADS, 1) void k_project_pair(ProjGeom g, 
       constexpr int SL = decltype(slot)::value;
 #pragma unroll
       for (int q = 0; q < BPI; q++) {
+        if (zsum && boff[q] >= 0 && bin[q]) {
+          float sr = 0, si = 0;
+#pragma unroll
+          for (int s = 0; s < 8; s++) {
+            sr += bvr[SL][q][s].re;
+            si += bvr[SL][q][s].im;
+          }
+          zre[q] += (double)sr;
+          zim[q] += (double)si;
+        }
         u4 wre[3], wim[3];
 #pragma unroll
         for (int s = 0; s < 4; s++) {
@@ -340,6 +360,27 @@ __global__ __launch_bounds__(2 * PQ_THREADS, 1) void k_project_pair(ProjGeom g, 
       __syncthreads();
     }
     if (nk > 0) __syncthreads();
+    if (zsum) {
+      // the four octets of a column sit in lanes l, l ^ 16, l ^ 32, l ^ 48
+      const int64_t nreal = 2 * (int64_t)g.ncols;
+#pragma unroll
+      for (int q = 0; q < BPI; q++) {
+        double zr = zre[q], zi = zim[q];
+        zr += __shfl_xor(zr, 16, 64);
+        zi += __shfl_xor(zi, 16, 64);
+        zr += __shfl_xor(zr, 32, 64);
+        zi += __shfl_xor(zi, 32, 64);
+        const int it = ptid + PQ_THREADS * q;
+        const int64_t n = cx0 + (it & 15) + 16 * (it >> 6);
+        if (bo[q] == 0 && boff[q] >= 0 && bin[q]) {
+          double *pr = part + ((int64_t)split * nreal + 2 * n) * R + Rg;
+          pr[0] = zr;      // Re column, cos row
+          pr[1] = 0;       // Re column, sin row
+          pr[R] = zi;      // Im column, cos row
+          pr[R + 1] = 0;   // Im column, sin row
+        }
+      }
+    }
     return;
   }
 
@@ -407,48 +448,51 @@ __global__ __launch_bounds__(2 * PQ_THREADS, 1) void k_project_pair(ProjGeom g, 
         const int r = row0 + (wr * WR + i) * 16 + 4 * (lane >> 4) + v;
         const int jt = (wc * WC + j) * 16 + (lane & 15);  // tile row of the column
         const int64_t n = 2 * (cx0 + (jt < NCX ? jt : jt - NCX)) + (jt < NCX ? 0 : 1);
-        if (r < R && n < nreal) part[((int64_t)split * nreal + n) * R + r] = acc64[i][j][v] + (double)acc[i][j][v];
+        if (r < Rg && n < nreal) part[((int64_t)split * nreal + n) * R + r] = acc64[i][j][v] + (double)acc[i][j][v];
       }
 }
 
-// Tile codes 1000 + 100 WR + 10 WC + GR: each wave owns WR x WC blocks of
-// 16 x 16 outputs, the four waves form GR x (4 / GR) groups over rows x
-// columns; a workgroup covers WR*GR row blocks and 16*WC*(4/GR) real columns.
+// Tile codes 1000 (NCW / 4) + 100 WR + 10 WC + GR (1xxx: 4 consumer waves,
+// 2xxx: 8): each consumer wave owns WR x WC blocks of 16 x 16 outputs, the
+// consumer waves form GR x (NCW / GR) groups over rows x columns; a
+// workgroup covers WR*GR row blocks and 16*WC*(NCW/GR) real columns.
 #define PLG_PQ_CODES(X)                                                                                      \
-  X(6, 1, 1) X(5, 2, 1) X(4, 2, 1) X(3, 3, 1) X(3, 3, 2) X(4, 2, 2) X(2, 4, 2) X(3, 2, 2) X(4, 3, 2)          \
-  X(3, 4, 2) X(2, 2, 2) X(2, 4, 4) X(2, 6, 2)
+  X(6, 1, 1, 4) X(5, 2, 1, 4) X(4, 2, 1, 4) X(3, 3, 1, 4) X(3, 3, 2, 4) X(4, 2, 2, 4) X(2, 4, 2, 4)           \
+  X(3, 2, 2, 4) X(4, 3, 2, 4) X(3, 4, 2, 4) X(2, 2, 2, 4) X(2, 4, 4, 4) X(2, 6, 2, 4)                         \
+  X(3, 2, 2, 8) X(4, 2, 2, 8) X(2, 2, 4, 8) X(3, 2, 4, 8) X(2, 3, 4, 8) X(8, 1, 2, 8) X(4, 2, 4, 8)      \
+  X(4, 1, 4, 8) X(2, 2, 8, 8)
 
 struct PqTile {
-  int wr, wc, gr;
+  int wr, wc, gr, ncw;
   int rb() const { return wr * gr; }
-  int ncx() const { return 8 * wc * (4 / gr); }
+  int ncx() const { return 8 * wc * (ncw / gr); }
 };
-static PqTile pq_tile(int code) { return {(code / 100) % 10, (code / 10) % 10, code % 10}; }
+static PqTile pq_tile(int code) { return {(code / 100) % 10, (code / 10) % 10, code % 10, 4 * (code / 1000)}; }
 
 int launch_project_bf16(const ProjGeom &g, const cx<float> *c, double *part, int terms, const uint32_t *wtab,
                         int code, hipStream_t st) {
   if (g.npairs <= 0 || !g.pidx) return PLG_ERR_INVALID;
   if (g.M > PQ_MMAX) return PLG_ERR_UNSUPPORTED;
-  if (code < 1000 || code > 1999) return PLG_ERR_INVALID;
+  if (code < 1000 || code > 2999) return PLG_ERR_INVALID;
   const PqTile t = pq_tile(code);
-  const int nblk = (2 * g.npairs + 15) / 16;
+  const int nblk = max(1, (2 * (g.npairs - g.zero_pair) + 15) / 16);
   const int nrg = (nblk + t.rb() - 1) / t.rb();
   const int64_t ncg = (g.ncols + t.ncx() - 1) / t.ncx();
   if ((ncg + 7) / 8 * 8 * nrg > INT32_MAX) return PLG_ERR_UNSUPPORTED;
-  dim3 block(2 * PQ_THREADS);
+  dim3 block((t.ncw + 4) * 64);
   dim3 grid((unsigned)((ncg + 7) / 8 * 8 * nrg), (unsigned)g.nsplit);
-#define PLG_PQ(WR, WC, GR)                                                                             \
-  if (t.wr == WR && t.wc == WC && t.gr == GR) {                                                        \
-    if (terms == 6) k_project_pair<WR, WC, GR, 6><<<grid, block, 0, st>>>(g, c, wtab, part, nrg, (int)ncg); \
-    else k_project_pair<WR, WC, GR, 3><<<grid, block, 0, st>>>(g, c, wtab, part, nrg, (int)ncg);            \
-    return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;                                     \
+#define PLG_PQ(WR, WC, GR, NCW)                                                                            \
+  if (t.wr == WR && t.wc == WC && t.gr == GR && t.ncw == NCW) {                                          \
+    if (terms == 6) k_project_pair<WR, WC, GR, 6, NCW><<<grid, block, 0, st>>>(g, c, wtab, part, nrg, (int)ncg); \
+    else k_project_pair<WR, WC, GR, 3, NCW><<<grid, block, 0, st>>>(g, c, wtab, part, nrg, (int)ncg);            \
+    return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;                                       \
   }
   PLG_PQ_CODES(PLG_PQ)
 #undef PLG_PQ
   return PLG_ERR_INVALID;
 }
 
-#define PLG_PQ_CODE(WR, WC, GR) 1000 + 100 * WR + 10 * WC + GR,
+#define PLG_PQ_CODE(WR, WC, GR, NCW) 1000 * (NCW / 4) + 100 * WR + 10 * WC + GR,
 static const int k_pq_codes[] = {PLG_PQ_CODES(PLG_PQ_CODE)};
 #undef PLG_PQ_CODE
 
@@ -473,13 +517,14 @@ int64_t proj_workgroups(int code, int64_t ncols, int npairs) {
   return ((ncols + t.ncx() - 1) / t.ncx()) * ((nblk + t.rb() - 1) / t.rb());
 }
 
-// default tile: 3 x 3 blocks per wave in a 2 x 2 wave grid; square-ish wave
-// tiles read the fewest LDS bytes per MFMA
+// default tile (the fastest on the measured workloads): 8 consumer waves of
+// 4 x 2 blocks when 8 row blocks divide the rows, else 4 consumer waves of
+// 3 x 4 blocks
 int pick_proj(int npairs) {
   const int nblk = (2 * npairs + 15) / 16;
   if (nblk <= 2) return 1222;
   if (nblk <= 4) return 1242;
-  return 1332;
+  return nblk % 8 == 0 ? 2422 : 1342;
 }
 
 // Pairs (n, -n) of a momentum list, on the scaled momenta ms (nmom x 3,
@@ -488,7 +533,8 @@ int pick_proj(int npairs) {
 // momentum equal to its own negative (mod M: zero, or components L/2) has a
 // zero sine row and stays alone, as does one whose partner is absent or
 // already taken.
-int make_pairs(const std::vector<int> &ms, int M, std::vector<int> &pair_ms, std::vector<int> &mom_pair) {
+int make_pairs(const std::vector<int> &ms, int M, std::vector<int> &pair_ms, std::vector<int> &mom_pair,
+               int *zero_last) {
   const int nmom = (int)ms.size() / 3;
   pair_ms.clear();
   mom_pair.assign(nmom, -1);
@@ -508,7 +554,21 @@ int make_pairs(const std::vector<int> &ms, int M, std::vector<int> &pair_ms, std
     mom_pair[m] = 2 * j;
     if (nk != k) open[k].push_back(m);
   }
-  return (int)pair_ms.size() / 3;
+  const int np = (int)pair_ms.size() / 3;
+  // move a zero-momentum pair to the end
+  *zero_last = 0;
+  for (int j = 0; j < np; j++)
+    if (pair_ms[3 * j] == 0 && pair_ms[3 * j + 1] == 0 && pair_ms[3 * j + 2] == 0) {
+      const int l = np - 1;
+      for (int i = 0; i < 3; i++) std::swap(pair_ms[3 * j + i], pair_ms[3 * l + i]);
+      for (int &v : mom_pair) {
+        if (v >> 1 == j) v = 2 * l + (v & 1);
+        else if (v >> 1 == l) v = 2 * j + (v & 1);
+      }
+      *zero_last = 1;
+      break;
+    }
+  return np;
 }
 
 }  // namespace plg

@@ -353,7 +353,7 @@ def test_meson_meson_mixed_two_flavours():
 
 
 # every projection tile the autotuner may pick, pinned one at a time
-PROJ_CODES = [1611, 1521, 1421, 1331, 1332, 1422, 1242, 1322, 1432, 1342, 1222, 1244, 1262]
+PROJ_CODES = [1611, 1521, 1421, 1331, 1332, 1422, 1242, 1322, 1432, 1342, 1222, 1244, 1262, 2322, 2422, 2224, 2324, 2234, 2812, 2424, 2414, 2228]
 
 
 @pytest.mark.parametrize("code", PROJ_CODES)
