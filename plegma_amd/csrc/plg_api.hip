@@ -660,7 +660,7 @@ static std::string tune_key(const Plan &P) {
   k << prop.gcnArchName << " L=" << g_ctx.L[0] << "x" << g_ctx.L[1] << "x" << g_ctx.L[2] << "x" << g_ctx.L[3]
     << " nmom=" << P.nmom << " cols=" << P.rows.size() * P.nch_c << " rows=" << P.rows.size() << " mode=" << P.mode
     << " npairs=" << P.npairs << " terms=" << P.terms << " kind=" << P.kind << " mask=" << P.mask
-    << " proj=pair32t";
+    << " proj=pair32z";
   for (auto &l : P.launches) k << " L" << l.what << "." << l.set << "." << l.two << l.mes << "." << l.gmask;
   std::string s = k.str();
   for (char &ch : s)
@@ -736,8 +736,13 @@ static int autotune(Plan &P, hipStream_t st) {
     }
   }
   P.tpb = best_tpb;
-  // projection: tile code and K splits
+  // projection: tile code and K splits, timed together with the finalize
+  // (which sums the splits) into a scratch output
   ProjGeom g = plan_geom(P);
+  void *scratch = nullptr;
+  if (hipMalloc(&scratch, std::max<size_t>(1, (size_t)P.out_elems) * sizeof(cx<double>)) != hipSuccess)
+    return fail(PLG_ERR_HIP, "autotune scratch allocation failed");
+  const int nout = (int)P.csel.size();
   std::vector<int> codes = P.T == PLG_F64 ? std::vector<int>{32, 64} : proj_candidates(std::max(P.npairs - P.zero_pair, 1));
   const size_t per_split = part_bytes(P) / std::max(P.nsplit, 1);
   double best_p = 1e30;
@@ -752,6 +757,9 @@ static int autotune(Plan &P, hipStream_t st) {
             if (P.T == PLG_F64) lrc = launch_project_f64(g, (const cx<double> *)P.c, (cx<double> *)P.part, code, st);
             else
               lrc = launch_project_bf16(g, (const cx<float> *)P.c, (double *)P.part, P.terms, g_ctx.d_wtab, code, st);
+            if (lrc == PLG_OK)
+              lrc = launch_finalize(g, (const double *)P.part, P.d_orows, (int)P.orows.size(), P.nch_c, nout,
+                                    P.d_csel, P.d_choff, P.m_stride, scratch, PLG_F64, st);
           },
           st);
       if (lrc != PLG_OK) {  // a tile this geometry does not provide (LDS size)
@@ -767,6 +775,8 @@ static int autotune(Plan &P, hipStream_t st) {
   if (P.T == PLG_F64) P.mtile = best_code;
   else P.mbw = best_code;
   P.nsplit = best_ns;
+  hipStreamSynchronize(st);
+  hipFree(scratch);
   if (hipGetLastError() != hipSuccess) return fail(PLG_ERR_HIP, "autotune launch failed");
   tune_store(key, best_code, best_ns, best_tpb, best_c + best_p);
   return PLG_OK;
