@@ -86,6 +86,20 @@ template <typename T> struct cx {
   T re, im;
 };
 template <typename T> __host__ __device__ inline cx<T> cmake(T r, T i) { return cx<T>{r, i}; }
+
+// Loads and stores of HBM data through global (address space 1) pointers.
+// Through a generic pointer the compiler emits flat instructions, which also
+// count in the LDS wait counter: every LDS wait would then wait for the
+// prefetch of the next tile as well.
+template <typename T> __device__ __forceinline__ cx<T> gload(const cx<T> *p) {
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  const v2 v = *(const __attribute__((address_space(1))) v2 *)p;
+  return cmake<T>(v[0], v[1]);
+}
+template <typename T> __device__ __forceinline__ void gstore(cx<T> *p, const cx<T> &v) {
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  *(__attribute__((address_space(1))) v2 *)p = v2{v.re, v.im};
+}
 template <typename T> __host__ __device__ inline cx<T> operator+(cx<T> a, cx<T> b) {
   return {a.re + b.re, a.im + b.im};
 }

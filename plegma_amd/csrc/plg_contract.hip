@@ -101,7 +101,7 @@ __device__ __forceinline__ const cx<T> *site_base(const PropDev<T> &P, int t, in
   return P.data + (int64_t)plane * V3 + x;
 }
 
-template <typename T> __device__ __forceinline__ cx<T> ldg(const cx<T> *p) { return *p; }
+template <typename T> __device__ __forceinline__ cx<T> ldg(const cx<T> *p) { return gload(p); }
 
 // ------------------------------------------------------------------ mesons
 // Single propagator, Hermitian Gram form.  SET_M16 or SET_M10.
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(256) void k_meson_herm(const PropDev<T> *__restrict
       else v = cconj(gx[d - 1][herm_rank(d, tm.q)]);
       acc = acc + cmul_iph(v, tm.ph);
     });
-    out[ctile_idx(col0 + decltype(ch)::value, x, ncols)] = acc;
+    gstore(out + ctile_idx(col0 + decltype(ch)::value, x, ncols), acc);
   });
 }
 
@@ -194,7 +194,7 @@ __global__ __launch_bounds__(256) void k_meson_pair(const PropDev<T> *__restrict
   }
   const int64_t ncols = (int64_t)gridDim.y * nch_total, col0 = (int64_t)r * nch_total + ch0;
 #pragma unroll
-  for (int ch = C0; ch < C1; ch++) out[ctile_idx(col0 + ch, x, ncols)] = acc[ch - C0];
+  for (int ch = C0; ch < C1; ch++) gstore(out + ctile_idx(col0 + ch, x, ncols), acc[ch - C0]);
 }
 
 // ------------------------------------------------------------------ launchers

@@ -117,8 +117,8 @@ void k_mesmes(const PropDev<T> *__restrict__ props, const RowDev *__restrict__ r
 #pragma unroll
     for (int j = 0; j < NF; j++) {
       const int64_t i = (tid >> 4) + 16 * j;
-      pf1[j] = ok ? b1[i * P1.cs] : cmake<T>(0, 0);
-      pf2[j] = ok ? b2[i * P2.cs] : cmake<T>(0, 0);
+      pf1[j] = ok ? gload(b1 + i * P1.cs) : cmake<T>(0, 0);
+      pf2[j] = ok ? gload(b2 + i * P2.cs) : cmake<T>(0, 0);
     }
   };
   if (blockIdx.x < ntiles) fetch(blockIdx.x);
@@ -186,7 +186,7 @@ void k_mesmes(const PropDev<T> *__restrict__ props, const RowDev *__restrict__ r
       if (x < V3) {
         const int64_t ncols = (int64_t)gridDim.y * nch_total, col0 = (int64_t)blockIdx.y * nch_total + ch0;
         for (int ch = tid >> 4; ch < NCH; ch += BT_THREADS / BT_SITES)
-          out[ctile_idx(col0 + ch, x, ncols)] = stage[ls * BT_WS + (ch & 15) * 2 + (ch >> 4)];
+          gstore(out + ctile_idx(col0 + ch, x, ncols), stage[ls * BT_WS + (ch & 15) * 2 + (ch >> 4)]);
       }
     }
     __syncthreads();
