@@ -199,16 +199,28 @@ void k_baryon(const PropDev<T> *__restrict__ props,
   // contracted (coalesced: 16 consecutive sites per component).
   constexpr int NF = 144 / (BT_THREADS / BT_SITES);
   cx<T> pf1[NF], pf2[TWO ? NF : 1];
+  // one pointer per propagator, advanced by 16 components per load
+  const cx<T> *f1 = P1.data + plane1 + (int64_t)(tid >> 4) * P1.cs + (tid & 15);
+  const cx<T> *f2 = P2.data + plane2 + (int64_t)(tid >> 4) * P2.cs + (tid & 15);
+  const int64_t st1 = 16 * P1.cs, st2 = 16 * P2.cs;
   auto fetch = [&](int tl) {
-    const int64_t x = (int64_t)tl * BT_SITES + (tid & 15);
-    const bool ok = x < V3;
-    const cx<T> *b1 = P1.data + plane1 + (ok ? x : 0);
-    const cx<T> *b2 = P2.data + plane2 + (ok ? x : 0);
+    const int64_t x0 = (int64_t)tl * BT_SITES;
+    const cx<T> *b1 = f1 + x0, *b2 = f2 + x0;
+    if (x0 + BT_SITES <= V3) {
 #pragma unroll
-    for (int j = 0; j < NF; j++) {
-      const int64_t i = (tid >> 4) + 16 * j;
-      pf1[j] = ok ? gload(b1 + i * P1.cs) : cmake<T>(0, 0);
-      if constexpr (TWO) pf2[j] = ok ? gload(b2 + i * P2.cs) : cmake<T>(0, 0);
+      for (int j = 0; j < NF; j++) {
+        pf1[j] = gload(b1);
+        if constexpr (TWO) pf2[j] = gload(b2);
+        b1 += st1;
+        b2 += st2;
+      }
+    } else {
+      const bool ok = x0 + (tid & 15) < V3;
+#pragma unroll
+      for (int j = 0; j < NF; j++) {
+        pf1[j] = ok ? gload(b1 + j * st1) : cmake<T>(0, 0);
+        if constexpr (TWO) pf2[j] = ok ? gload(b2 + j * st2) : cmake<T>(0, 0);
+      }
     }
   };
   if (blockIdx.x < ntiles) fetch(blockIdx.x);
