@@ -49,6 +49,20 @@ public:
     return v;
   }
 
+  // ---- preparation in place (include/PLEGMA_Propagator.h:57-59)
+  // entries at global time t < t0 negated (lib/PLEGMA_Propagator.cu:98-100)
+  void applyBoundaries_device(int t0) {
+    plg_prop v = view();
+    plg_check(plg_prop_apply_boundaries(&v, t0, nullptr), "plg_prop_apply_boundaries");
+  }
+  // twisted-mass rotation to the physical basis, sign +-1 (lib/PLEGMA_Propagator.cu:103-106)
+  void rotateToPhysicalBase_device(int sign) {
+    if (sign != +1 && sign != -1) PLEGMA_error("The sign can be only +-1\n");
+    plg_prop v = view();
+    plg_check(plg_prop_rotate_physical(&v, sign, nullptr), "plg_prop_rotate_physical");
+    hip_check(hipDeviceSynchronize(), "rotateToPhysicalBase_device");
+  }
+
   // ---- inputs of benchmarks and tests (no reference counterpart)
   // seeded N(0,1) entries times exp(-lambda |t - t0|); FP32 propagators carry
   // per-time-slice exponents so that fast decays stay in range

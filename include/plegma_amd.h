@@ -69,7 +69,13 @@ enum plg_kind {
  * plane_exp (optional, device, nt ints): the stored entries of plane p are the
  * propagator times 2^-plane_exp[p] (an exact power-of-two rescaling that keeps
  * FP32 data of fast-decaying propagators inside the normal range).  Results
- * are returned unscaled. */
+ * are returned unscaled.
+ * rotate_sign (0, +1 or -1) and bc_t0 (0: none) make the contractions read
+ * the view as if plg_prop_rotate_physical(rotate_sign) and then
+ * plg_prop_apply_boundaries(bc_t0) had been applied to it, without changing
+ * it: the preparation PLEGMA's drivers do in place before contracting
+ * (plegma/mesons_2pt.cpp:158-165, plegma/PDFs_nucleon.cpp:382-385), fused
+ * into the contraction's load stage.  Zero-initialised views prepare nothing. */
 typedef struct {
   const void *data;    /* device pointer */
   int precision;       /* plg_precision */
@@ -78,6 +84,8 @@ typedef struct {
   int pad_;
   int64_t comp_stride; /* complex elements */
   const int *plane_exp;
+  int rotate_sign;     /* 0: none; +-1: twisted-mass rotation to the physical basis */
+  int bc_t0;           /* entries at global time t < bc_t0 enter negated */
 } plg_prop;
 
 /* ---- runtime ------------------------------------------------------------ */
@@ -173,6 +181,19 @@ int plg_synth_prop_perturbed(const plg_prop *dst, uint64_t seed, uint32_t base_s
  * is set, it is written first (e = floor(log2 max |entry|) of each plane) and
  * the copy stores entry * 2^-e */
 int plg_cast_prop(const plg_prop *dst, const plg_prop *src, void *stream);
+
+/* Propagator preparation, in place on every plane of a view (the view's
+ * rotate_sign and bc_t0 are ignored here).
+ * plg_prop_rotate_physical: S -> U S U with U = (1 + i sign g5) / sqrt(2),
+ *   S'[mu][nu] = (S[mu][nu] - S[mu+2][nu+2] + i sign (S[mu][nu+2] + S[mu+2][nu])) / 2
+ *   (spin indices mod 4, every colour pair), sign = +-1; replaces
+ *   PLEGMA_Propagator::rotateToPhysicalBase_device
+ *   (lib/PLEGMA_Propagator.cu:103-106, lib/kernels/PLEGMA_propagator_utils.cuh:89-124).
+ * plg_prop_apply_boundaries: negates every entry at global time t < t0;
+ *   replaces PLEGMA_Propagator::applyBoundaries_device
+ *   (lib/PLEGMA_Propagator.cu:98-100, lib/kernels/PLEGMA_propagator_utils.cuh:64-80). */
+int plg_prop_rotate_physical(const plg_prop *p, int sign, void *stream);
+int plg_prop_apply_boundaries(const plg_prop *p, int t0, void *stream);
 
 /* ---- learned part (no counterpart in the reference): a multilayer
  * perceptron x -> [Linear -> SiLU] x (nlayers - 1) -> Linear -> y with BF16

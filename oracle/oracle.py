@@ -78,6 +78,9 @@ def lib():
         L.orc_meson_table.argtypes = [C.c_int, i32p, f64p]
         L.orc_diquark_table.argtypes = [C.c_int, i32p, f64p]
         L.orc_gamma.argtypes = [C.c_int, f64p]
+        L.orc_rotate_site.argtypes = [f64p, C.c_int, f64p]
+        L.orc_prop_rotate.argtypes = [f64p, C.c_int64, C.c_int, C.c_int64, C.c_int]
+        L.orc_prop_boundaries.argtypes = [f64p, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int]
         L.orc_gamma.restype = None
         _lib = L
     return _lib
@@ -243,3 +246,27 @@ def prop_to_sites(prop: np.ndarray, L, t, comp_stride=None, t_first=0) -> np.nda
     p = prop.reshape(144, cs, 2)[:, plane * V3:(plane + 1) * V3, :]
     c = p[..., 0] + 1j * p[..., 1]
     return np.ascontiguousarray(c.T).reshape(V3, 4, 4, 3, 3)
+
+
+def rotate_site(S, sign):
+    """rotation to the physical basis of one site: S complex [144] -> [144]"""
+    a = np.ascontiguousarray(np.stack([np.real(S), np.imag(S)], -1), dtype=np.float64)
+    out = np.empty_like(a)
+    lib().orc_rotate_site(a.ctypes.data_as(C.POINTER(C.c_double)), int(sign),
+                          out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def prop_rotate(P, L, sign):
+    """in place on a float64 propagator array [144, nt*V3, 2]"""
+    assert P.dtype == np.float64 and P.flags.c_contiguous
+    V3 = L[0] * L[1] * L[2]
+    lib().orc_prop_rotate(P.ctypes.data_as(C.POINTER(C.c_double)), P.shape[1], P.shape[1] // V3, V3, int(sign))
+
+
+def prop_boundaries(P, L, t0, t_first=0):
+    """in place on a float64 propagator array [144, nt*V3, 2]"""
+    assert P.dtype == np.float64 and P.flags.c_contiguous
+    V3 = L[0] * L[1] * L[2]
+    lib().orc_prop_boundaries(P.ctypes.data_as(C.POINTER(C.c_double)), P.shape[1], t_first,
+                              P.shape[1] // V3, L[3], V3, int(t0))

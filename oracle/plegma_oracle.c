@@ -817,3 +817,53 @@ void orc_synth_prop(const int L[4], uint64_t seed, uint32_t stream, int t0, doub
       }
     }
 }
+
+/* ------------------------------------------------------------------ propagator preparation */
+/* PLEGMA_propagator_utils.cuh:103-118: per colour pair (c1, c2),
+ * PT[mu][nu] = 0.5 (P[mu][nu] - P[mu+2][nu+2] + (i sign) (P[mu][nu+2] + P[mu+2][nu])),
+ * spin indices mod 4; the product with i sign is the complex product of
+ * Float2 (re = 0 re - sign im, im = 0 im + sign re). */
+void orc_rotate_site(const double *S, int sign, double *out) {
+  for (int c = 0; c < 9; c++)
+    for (int mu = 0; mu < 4; mu++)
+      for (int nu = 0; nu < 4; nu++) {
+        const double *p = S + 2 * ((mu * 4 + nu) * 9 + c);
+        const double *q = S + 2 * ((((mu + 2) % 4) * 4 + (nu + 2) % 4) * 9 + c);
+        const double *u = S + 2 * ((mu * 4 + (nu + 2) % 4) * 9 + c);
+        const double *v = S + 2 * ((((mu + 2) % 4) * 4 + nu) * 9 + c);
+        const double sr = u[0] + v[0], si = u[1] + v[1];
+        const double tr = 0.0 * sr - sign * si, ti = 0.0 * si + sign * sr;
+        double *o = out + 2 * ((mu * 4 + nu) * 9 + c);
+        o[0] = 0.5 * ((p[0] - q[0]) + tr);
+        o[1] = 0.5 * ((p[1] - q[1]) + ti);
+      }
+}
+
+void orc_prop_rotate(double *data, int64_t comp_stride, int nt, int64_t V3, int sign) {
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < (int64_t)nt * V3; e++) {
+    double S[2 * NPROP], R[2 * NPROP];
+    for (int i = 0; i < NPROP; i++) {
+      S[2 * i] = data[2 * (i * comp_stride + e)];
+      S[2 * i + 1] = data[2 * (i * comp_stride + e) + 1];
+    }
+    orc_rotate_site(S, sign, R);
+    for (int i = 0; i < NPROP; i++) {
+      data[2 * (i * comp_stride + e)] = R[2 * i];
+      data[2 * (i * comp_stride + e) + 1] = R[2 * i + 1];
+    }
+  }
+}
+
+/* PLEGMA_propagator_utils.cuh:64-80: if (t < t0) every component *= -1 */
+void orc_prop_boundaries(double *data, int64_t comp_stride, int t_first, int nt, int T, int64_t V3, int t0) {
+  for (int pl = 0; pl < nt; pl++) {
+    if ((t_first + pl) % T >= t0) continue;
+    for (int i = 0; i < NPROP; i++)
+      for (int64_t x = 0; x < V3; x++) {
+        double *v = data + 2 * (i * comp_stride + (int64_t)pl * V3 + x);
+        v[0] *= -1.;
+        v[1] *= -1.;
+      }
+  }
+}

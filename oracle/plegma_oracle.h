@@ -81,6 +81,17 @@ void orc_site_delta32(int k, const double *S1, const double *S2, double *out); /
 void orc_site_delta12(int k, const double *S1, const double *S2, double *out); /* 32 */
 void orc_site_meson_meson(const double *SL, const double *SH, double *out);    /* 2*NMM */
 
+/* Propagator preparation.
+ * orc_rotate_site: rotation to the physical basis of one site (144 complex,
+ *   component ((mu*4+nu)*3+a)*3+b), sign +-1:
+ *   rotateToPhysicalBase_kernel, lib/kernels/PLEGMA_propagator_utils.cuh:89-124.
+ * orc_prop_rotate: the same over a view (nt planes of V3 sites), in place.
+ * orc_prop_boundaries: negates the planes of a view whose global time is
+ *   below t0: apply_boundaries_kernel, lib/kernels/PLEGMA_propagator_utils.cuh:64-80. */
+void orc_rotate_site(const double *S, int sign, double *out);
+void orc_prop_rotate(double *data, int64_t comp_stride, int nt, int64_t V3, int sign);
+void orc_prop_boundaries(double *data, int64_t comp_stride, int t_first, int nt, int T, int64_t V3, int t0);
+
 /* Full correlator in the reference H_elem layout
  *   H[((((g*nd + d)*localT + j)*nmom + m)*nch + c)*2 + ri]
  * Rows listed in `rows` (nrows>0) are computed, others left untouched;

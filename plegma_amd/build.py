@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libplegma_amd.so")
 SOURCES = ["plg_api.hip", "plg_contract.hip", "plg_baryon.hip", "plg_project.hip", "plg_project_bf16.hip",
-           "plg_synth.hip", "plg_mesmes.hip", "plg_mlp.hip"]
+           "plg_synth.hip", "plg_mesmes.hip", "plg_mlp.hip", "plg_prep.hip"]
 ARCH = "gfx950"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

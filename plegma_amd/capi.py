@@ -31,7 +31,8 @@ EXPORTS = ["plg_init", "plg_finalize", "plg_last_error", "plg_version", "plg_mom
            "plg_correlator_size", "plg_batch_create", "plg_batch_run", "plg_batch_info",
            "plg_batch_destroy", "plg_batch_profile", "plg_synth_prop", "plg_synth_prop_perturbed", "plg_cast_prop",
            "plg_mlp_create", "plg_mlp_nparams", "plg_mlp_set_params", "plg_mlp_get_params",
-           "plg_mlp_forward", "plg_mlp_gradient", "plg_mlp_train", "plg_mlp_destroy", "plg_ft_apply"]
+           "plg_mlp_forward", "plg_mlp_gradient", "plg_mlp_train", "plg_mlp_destroy", "plg_ft_apply",
+           "plg_prop_rotate_physical", "plg_prop_apply_boundaries"]
 
 
 class PlgError(RuntimeError):
@@ -43,7 +44,7 @@ class PlgError(RuntimeError):
 class plg_prop(C.Structure):
     _fields_ = [("data", C.c_void_p), ("precision", C.c_int), ("t_first", C.c_int),
                 ("nt", C.c_int), ("pad_", C.c_int), ("comp_stride", C.c_int64),
-                ("plane_exp", C.c_void_p)]
+                ("plane_exp", C.c_void_p), ("rotate_sign", C.c_int), ("bc_t0", C.c_int)]
 
 
 class plg_job(C.Structure):
@@ -244,6 +245,14 @@ def synth_prop_perturbed(view, seed, base_stream, pert_stream, eps, t0, lam, str
 
 def cast_prop(dst, src, stream=None):
     check(lib().plg_cast_prop(C.byref(dst), C.byref(src), _stream(stream)))
+
+
+def prop_rotate_physical(view, sign, stream=None):
+    check(lib().plg_prop_rotate_physical(C.byref(view), sign, _stream(stream)))
+
+
+def prop_apply_boundaries(view, t0, stream=None):
+    check(lib().plg_prop_apply_boundaries(C.byref(view), t0, _stream(stream)))
 
 
 class Mlp:

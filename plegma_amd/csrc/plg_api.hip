@@ -479,7 +479,8 @@ static int build_plan(Plan &P, int kind, int mask, const std::vector<plg_prop> &
 
 template <typename T> static void fill_propdev(const std::vector<plg_prop> &in, std::vector<PropDev<T>> &out) {
   out.clear();
-  for (auto &p : in) out.push_back(PropDev<T>{(const cx<T> *)p.data, p.comp_stride, p.t_first, p.nt});
+  for (auto &p : in)
+    out.push_back(PropDev<T>{(const cx<T> *)p.data, p.comp_stride, p.t_first, p.nt, p.rotate_sign, p.bc_t0});
 }
 
 // plan tables are small; copied synchronously at plan build time
@@ -1149,6 +1150,37 @@ int plg_cast_prop(const plg_prop *dst, const plg_prop *src, void *stream) {
   }
   int rc = launch_cast(*dst, *src, g_ctx.V3, g_ctx.L[3], (hipStream_t)stream);
   return rc ? fail(rc, "cast launch failed") : PLG_OK;
+}
+
+int plg_prop_rotate_physical(const plg_prop *p, int sign, void *stream) {
+  if (!g_ctx.init) return fail(PLG_ERR_NOT_INIT, "plg_init not called");
+  if (!p || !p->data) return fail(PLG_ERR_INVALID, "NULL propagator");
+  if (sign != 1 && sign != -1) return fail(PLG_ERR_INVALID, "the sign can be only +-1");
+  if (p->precision != PLG_F64 && p->precision != PLG_F32) return fail(PLG_ERR_INVALID, "bad precision");
+  if (p->nt <= 0 || p->nt > g_ctx.L[3] || p->comp_stride < (int64_t)p->nt * g_ctx.V3)
+    return fail(PLG_ERR_INVALID, "propagator view: bad nt/comp_stride");
+  const int rc = launch_prop_rotate(*p, g_ctx.V3, sign, (hipStream_t)stream);
+  return rc ? fail(rc, "rotation launch failed") : PLG_OK;
+}
+
+int plg_prop_apply_boundaries(const plg_prop *p, int t0, void *stream) {
+  if (!g_ctx.init) return fail(PLG_ERR_NOT_INIT, "plg_init not called");
+  if (!p || !p->data) return fail(PLG_ERR_INVALID, "NULL propagator");
+  if (p->precision != PLG_F64 && p->precision != PLG_F32) return fail(PLG_ERR_INVALID, "bad precision");
+  if (p->nt <= 0 || p->nt > g_ctx.L[3] || p->comp_stride < (int64_t)p->nt * g_ctx.V3)
+    return fail(PLG_ERR_INVALID, "propagator view: bad nt/comp_stride");
+  // planes whose global time is below t0
+  std::vector<int> planes;
+  for (int pl = 0; pl < p->nt; pl++)
+    if ((p->t_first + pl) % g_ctx.L[3] < t0) planes.push_back(pl);
+  if (planes.empty()) return PLG_OK;
+  hipStream_t st = (hipStream_t)stream;
+  int *d_planes = nullptr;
+  int rc = upload(&d_planes, planes, st);
+  if (!rc) rc = launch_prop_negate(*p, g_ctx.V3, d_planes, (int)planes.size(), st);
+  if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = PLG_ERR_HIP;
+  if (d_planes) hipFree(d_planes);
+  return rc ? fail(rc, "boundary launch failed") : PLG_OK;
 }
 
 }  // extern "C"

@@ -193,6 +193,9 @@ void k_baryon(const PropDev<T> *__restrict__ props,
   const int hi = lane >> 2, lo = lane & 3;
   const int64_t plane1 = (int64_t)((row.t - P1.t_first + T_lat) % T_lat) * V3;
   const int64_t plane2 = (int64_t)((row.t - P2.t_first + T_lat) % T_lat) * V3;
+  // preparation of the loaded entries (boundary sign, rotation)
+  const T sg1 = prop_sign(P1, row.t), sg2 = prop_sign(P2, row.t);
+  const bool prep_rot = P1.rot != 0 || (TWO && P2.rot != 0);
 
   // tile staging: thread -> site tid%16, components tid/16 + 16 j (j < 9).
   // The components of tile t+1 are fetched into registers while tile t is
@@ -228,10 +231,15 @@ void k_baryon(const PropDev<T> *__restrict__ props,
     const int64_t x0 = (int64_t)tile * BT_SITES;
 #pragma unroll
     for (int j = 0; j < NF; j++) {
-      sh.s1[(tid >> 4) + 16 * j][tid & 15] = pf1[j];
-      if constexpr (TWO) sh.s2[(tid >> 4) + 16 * j][tid & 15] = pf2[j];
+      sh.s1[(tid >> 4) + 16 * j][tid & 15] = cmake<T>(sg1 * pf1[j].re, sg1 * pf1[j].im);
+      if constexpr (TWO) sh.s2[(tid >> 4) + 16 * j][tid & 15] = cmake<T>(sg2 * pf2[j].re, sg2 * pf2[j].im);
     }
     __syncthreads();
+    if (prep_rot) {  // rotation to the physical basis on the staged tiles
+      if (P1.rot) rot_tile(sh.s1, P1.rot);
+      if (TWO && P2.rot) rot_tile(sh.s2, P2.rot);
+      __syncthreads();
+    }
     if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
     const cx<T>(*S1)[BT_SP] = sh.s1;
     const cx<T>(*S2)[BT_SP] = TWO ? sh.s2 : sh.s1;

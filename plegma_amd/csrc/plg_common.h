@@ -87,6 +87,33 @@ template <typename T> struct cx {
 };
 template <typename T> __host__ __device__ inline cx<T> cmake(T r, T i) { return cx<T>{r, i}; }
 
+// Twisted-mass rotation to the physical basis of the four spin entries
+// a = S[m][n], b = S[m][n+2], c = S[m+2][n], d = S[m+2][n+2] of one colour
+// pair (spin indices mod 4), in place:
+//   S'[mu][nu] = (S[mu][nu] - S[mu+2][nu+2] + i s (S[mu][nu+2] + S[mu+2][nu])) / 2
+// (lib/kernels/PLEGMA_propagator_utils.cuh:103-118, same evaluation order).
+template <typename T>
+__host__ __device__ __forceinline__ void rot_physical(cx<T> &a, cx<T> &b, cx<T> &c, cx<T> &d, int s) {
+  const T sg = (T)s, h = (T)0.5;
+  const cx<T> A = a, B = b, C = c, D = d;
+  auto term = [&](cx<T> p, cx<T> q, cx<T> r) {  // (p - q + i s r) / 2
+    return cmake<T>(h * ((p.re - q.re) + (-sg * r.im)), h * ((p.im - q.im) + sg * r.re));
+  };
+  a = term(A, D, B + C);
+  d = term(D, A, C + B);
+  b = term(B, C, A + D);
+  c = term(C, B, D + A);
+}
+
+// rotation to the physical basis of the 16 spin entries s[mu * 4 + nu] of one
+// colour pair, in registers
+template <typename T> __device__ __forceinline__ void rot_spins(cx<T> (&s)[16], int sign) {
+#pragma unroll
+  for (int m = 0; m < 2; m++)
+#pragma unroll
+    for (int n = 0; n < 2; n++) rot_physical(s[m * 4 + n], s[m * 4 + n + 2], s[(m + 2) * 4 + n], s[(m + 2) * 4 + n + 2], sign);
+}
+
 // Loads and stores of HBM data through global (address space 1) pointers.
 // Through a generic pointer the compiler emits flat instructions, which also
 // count in the LDS wait counter: every LDS wait would then wait for the

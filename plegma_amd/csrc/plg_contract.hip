@@ -132,6 +132,8 @@ __global__ __launch_bounds__(256) void k_meson_herm(const PropDev<T> *__restrict
     cx<T> s[16];
 #pragma unroll
     for (int p = 0; p < 16; p++) s[p] = ldg(base + (int64_t)(p * 9 + c) * cs);
+    // rotation to the physical basis (the boundary sign cancels in S x S^dag)
+    if (P.rot) rot_spins(s, P.rot);
 #pragma unroll
     for (int p = 0; p < 16; p++) n0[p] = fma(s[p].re, s[p].re, fma(s[p].im, s[p].im, n0[p]));
     static_for<1, 4>([&](auto d) {
@@ -183,6 +185,8 @@ __global__ __launch_bounds__(256) void k_meson_pair(const PropDev<T> *__restrict
       s1[p] = ldg(b1 + (int64_t)(p * 9 + c) * P1.cs);
       s2[p] = ldg(b2 + (int64_t)(p * 9 + c) * P2.cs);
     }
+    if (P1.rot) rot_spins(s1, P1.rot);
+    if (P2.rot) rot_spins(s2, P2.rot);
     static_for<C0, C1>([&](auto ch) {
       static_for<0, 16>([&](auto i) {
         constexpr MTerm tm = TableOf<SET>::v.t[decltype(ch)::value][decltype(i)::value];
@@ -193,8 +197,11 @@ __global__ __launch_bounds__(256) void k_meson_pair(const PropDev<T> *__restrict
     });
   }
   const int64_t ncols = (int64_t)gridDim.y * nch_total, col0 = (int64_t)r * nch_total + ch0;
+  // boundary signs: the channels are bilinear in S1 and S2
+  const T sg = prop_sign(P1, row.t) * prop_sign(P2, row.t);
 #pragma unroll
-  for (int ch = C0; ch < C1; ch++) gstore(out + ctile_idx(col0 + ch, x, ncols), acc[ch - C0]);
+  for (int ch = C0; ch < C1; ch++)
+    gstore(out + ctile_idx(col0 + ch, x, ncols), cmake<T>(sg * acc[ch - C0].re, sg * acc[ch - C0].im));
 }
 
 // ------------------------------------------------------------------ launchers

@@ -23,7 +23,13 @@ template <typename T> struct PropDev {
   int64_t cs;  // component stride (complex)
   int t_first;
   int nt;
+  int rot;   // 0, or the sign of the rotation to the physical basis applied on load
+  int bct0;  // entries at global time t < bct0 are negated on load
 };
+// sign of a propagator's entries at global time t after the boundary fix
+template <typename T> __device__ __forceinline__ T prop_sign(const PropDev<T> &P, int t) {
+  return t < P.bct0 ? (T)-1 : (T)1;
+}
 
 // one contraction row: a time slice of a job and its propagators
 struct RowDev {
@@ -174,6 +180,10 @@ int launch_scatter_pos(const void *c, int c_prec, int64_t V3, int64_t ncols, con
 int launch_synth(const plg_prop &dst, const int L[4], uint64_t seed, uint32_t sa, uint32_t sb,
                  double eps, int t0, double lambda, hipStream_t st);
 int launch_cast(const plg_prop &dst, const plg_prop &src, int64_t V3, int T_lat, hipStream_t st);
+
+// propagator preparation in place (all planes of a view)
+int launch_prop_rotate(const plg_prop &p, int64_t V3, int sign, hipStream_t st);
+int launch_prop_negate(const plg_prop &p, int64_t V3, const int *d_planes, int nplanes, hipStream_t st);
 
 // error reporting
 void set_error(const std::string &msg);

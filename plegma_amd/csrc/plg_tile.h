@@ -67,6 +67,21 @@ template <typename T> __device__ __forceinline__ cx<T> cmul_iph_rt(cx<T> a, int 
   return (ph & 2) ? cmake<T>(-re, -im) : cmake<T>(re, im);
 }
 
+// rotation to the physical basis of a staged tile S[component][site] in
+// place (all threads of the workgroup; the caller synchronises after it)
+template <typename T> __device__ __forceinline__ void rot_tile(cx<T> (*S)[BT_SP], int sign) {
+  for (int it = threadIdx.x; it < BT_SITES * 36; it += BT_THREADS) {
+    const int s = it % BT_SITES, cq = it / BT_SITES, cp = cq % 9, q = cq / 9, m = q >> 1, n = q & 1;
+    cx<T> a = S[(m * 4 + n) * 9 + cp][s], b = S[(m * 4 + n + 2) * 9 + cp][s];
+    cx<T> c = S[((m + 2) * 4 + n) * 9 + cp][s], d = S[((m + 2) * 4 + n + 2) * 9 + cp][s];
+    rot_physical(a, b, c, d, sign);
+    S[(m * 4 + n) * 9 + cp][s] = a;
+    S[(m * 4 + n + 2) * 9 + cp][s] = b;
+    S[((m + 2) * 4 + n) * 9 + cp][s] = c;
+    S[((m + 2) * 4 + n + 2) * 9 + cp][s] = d;
+  }
+}
+
 // entry i of a compile-time 4-vector at a runtime index (selects, no scratch)
 __device__ __forceinline__ int pick4(const int (&v)[4], int i) {
   int r = 0;
