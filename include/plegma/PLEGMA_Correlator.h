@@ -1,7 +1,9 @@
 // PLEGMA_Correlator.h -- the two-point correlator with the interface of
 // include/PLEGMA_Correlator.h:18-331 for one process on one GPU: window
 // accessors, dataset/group bookkeeping, H_elem in the reference's layout,
-// contract* overwriting the buffer, the ASCII writer.  Every contraction runs
+// contract* overwriting the buffer, the ASCII writer and the HDF5 writer
+// (objects of the reference's HDF5 file in the record file of
+// PLEGMA_h5twin.h, as this build has no HDF5 library).  Every contraction runs
 // in libplegma_amd.so through plg_correlator: Float = double is the FP64
 // path (PLG_MODE_F64), Float = float the mixed path (FP32 contraction,
 // split-bf16 MFMA projection, FP64 sums).
@@ -9,11 +11,13 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <memory>
 #include <string>
 
 #include "PLEGMA_FT.h"
 #include "PLEGMA_Propagator.h"
+#include "PLEGMA_h5twin.h"
 
 namespace plegma {
 
@@ -258,11 +262,121 @@ public:
     }
     std::fclose(f);
   }
-  void writeHDF5(std::string) const { PLEGMA_error("HDF5 output is not part of this library"); }
-  void writeFile(std::string filename, FILE_FORMAT format = DEFAULT_FORMAT) const {
-    if (format == ASCII_FORMAT || format == DEFAULT_FORMAT) writeASCII(filename);
-    else writeHDF5(filename);
+  // lib/PLEGMA_Correlator.cu:809-850 (fill_H5_shapes) and 893-946
+  // (writeHDF5) into the record file of PLEGMA_h5twin.h.  Per group
+  // /sxXXsyXXszXXstXX/<group>: "mvec" (Nmom x 3 int, momentum space) and one
+  // dataset per name of shape (totalT, Nmom | Lz, Ly, Lx, shape..., 2) with
+  // its "description" attribute.  The time axis counts from the source time
+  // (tau = t - t0 mod T) and the spatial axes from the source position,
+  // where the reference's start offsets with periodic wrap put them
+  // (PLEGMA_hdf5.h:332-379).
+  void writeHDF5(std::string filename) const {
+    const bool mom = corr_space == MOMENTUM_SPACE;
+    const int T = L(3), lt = localT();
+    const int Lx = L(0), Ly = L(1), Lz = L(2);
+    const size_t npt = mom ? (size_t)corr_mom_space->Nmoms() : (size_t)Lattice::V3();
+    const size_t sr = getSiteSize() / (nDatasets() * nGroups());
+    std::vector<uint64_t> dims = {(uint64_t)totalT};
+    std::string descr = "shape: ";
+    if (mom) {
+      descr += "/time/moms";
+      dims.push_back(npt);
+    } else {
+      descr += "/t/z/y/x";
+      dims.insert(dims.end(), {(uint64_t)Lz, (uint64_t)Ly, (uint64_t)Lx});
+    }
+    if (!shape.empty()) {
+      descr += "/" + description;
+      for (int n : shape) dims.push_back((uint64_t)n);
+    }
+    descr += "/re-im";
+    dims.push_back(2);
+    // destination of buffer row j and point x in the dataset
+    std::vector<size_t> row_dst(lt), pt_dst(npt);
+    for (int j = 0; j < lt; j++) {
+      int t = 0;
+      plg_check(plg_row_time(source.data(), totalT, j, &t), "plg_row_time");
+      const int tau = ((t - source[3]) % T + T) % T;
+      if (tau >= totalT) PLEGMA_error("row %d (t = %d) lies outside the correlator's time extent", j, t);
+      row_dst[j] = (size_t)tau;
+    }
+    for (size_t x = 0; x < npt; x++) {
+      if (mom) {
+        pt_dst[x] = x;
+      } else {
+        const int ix = (int)(x % Lx), iy = (int)(x / Lx % Ly), iz = (int)(x / ((size_t)Lx * Ly));
+        const int sx = (ix - source[0] + Lx) % Lx, sy = (iy - source[1] + Ly) % Ly, sz = (iz - source[2] + Lz) % Lz;
+        pt_dst[x] = ((size_t)sz * Ly + sy) * Lx + sx;
+      }
+    }
+    const size_t block = (size_t)totalT * npt * sr * 2;  // one dataset, in Floats
+    std::vector<Float> buf(block);
+    char top[64];
+    std::snprintf(top, sizeof(top), "/sx%02dsy%02dsz%02dst%02d/", source[0], source[1], source[2], source[3]);
+    std::vector<int32_t> mvec;
+    std::vector<uint64_t> mdims;
+    if (mom) {
+      for (const auto &m : corr_mom_space->MomList())
+        for (auto v : m) mvec.push_back((int32_t)std::lround(v));
+      // a list of one momentum stays a flat vector of 3 (PLEGMA_hdf5.h:683-691)
+      mdims = npt == 1 ? std::vector<uint64_t>{3} : std::vector<uint64_t>{(uint64_t)npt, 3};
+    }
+    // the HDF5 file name as include/io/PLEGMA_hdf5.h:541-560 forms it ("x.h5",
+    // "x.h5/<group path>" -> x.h5, anything else -> "<name>.h5"; the group path
+    // is unused as the correlator's paths are absolute); the record file is
+    // that name + ".twin"
+    std::string h5name = filename;
+    const size_t ext = filename.rfind(".h5");
+    if (ext == std::string::npos || ext + 3 < filename.size())
+      h5name = ext != std::string::npos && filename[ext + 3] == '/' ? filename.substr(0, ext + 3) : filename + ".h5";
+    try {
+      H5Twin w(h5name + ".twin");
+      const Float *H = H_elem();
+      for (size_t g = 0; g < nGroups(); g++) {
+        const std::string gpath = std::string(top) + (groups.empty() ? "" : groups[g]) + "/";
+        if (mom) w.write_dataset(gpath + "mvec", mvec.data(), mdims);
+        for (size_t d = 0; d < nDatasets(); d++) {
+          const Float *src = H + (g * nDatasets() + d) * (size_t)lt * npt * sr * 2;
+          if ((size_t)lt != (size_t)totalT) std::fill(buf.begin(), buf.end(), Float(0));
+          for (int j = 0; j < lt; j++)
+            for (size_t x = 0; x < npt; x++)
+              std::memcpy(&buf[((row_dst[j] * npt) + pt_dst[x]) * sr * 2], src + ((size_t)j * npt + x) * sr * 2,
+                          sr * 2 * sizeof(Float));
+          const std::string name = datasets.empty() ? "arr" : datasets[d];
+          w.write_dataset(gpath + name, buf.data(), dims, {{"description", descr}});
+        }
+      }
+    } catch (const std::exception &e) {
+      PLEGMA_error("writeHDF5: %s", e.what());
+    }
   }
+  // include/PLEGMA_io.h:8-34 and 53-73: the format from the file name's
+  // extension (.h5/.hdf5 also before a '/'; .txt/.dat/.ascii/.raw; .lime/.bin)
+  static FILE_FORMAT deduceFormat(const std::string &filename) {
+    for (std::string ext : {".h5", ".hdf5"}) {
+      const size_t pos = filename.rfind(ext);
+      if (pos == std::string::npos) continue;
+      if (pos + ext.size() == filename.size() || filename[pos + ext.size()] == '/') return HDF5_FORMAT;
+    }
+    auto ends = [&](const std::string &ext) {
+      return filename.size() >= ext.size() && filename.compare(filename.size() - ext.size(), ext.size(), ext) == 0;
+    };
+    for (std::string ext : {".txt", ".dat", ".ascii", ".raw"})
+      if (ends(ext)) return ASCII_FORMAT;
+    for (std::string ext : {".lime", ".bin"})
+      if (ends(ext)) return LIME_FORMAT;
+    std::fprintf(stderr, "PLEGMA WARNING: Deduce type of %s failed.\n", filename.c_str());
+    return DEFAULT_FORMAT;
+  }
+  void writeFile(std::string filename, FILE_FORMAT format) const {
+    switch (format) {
+    case ASCII_FORMAT: writeASCII(filename); break;
+    case HDF5_FORMAT: writeHDF5(filename); break;
+    case LIME_FORMAT: PLEGMA_error("Writing a correlator in LIME format is not supported");
+    default: PLEGMA_error("No default writing defined");
+    }
+  }
+  void writeFile(std::string filename) const { writeFile(filename, deduceFormat(filename)); }
 };
 
 }  // namespace plegma

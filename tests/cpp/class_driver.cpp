@@ -1,8 +1,8 @@
 // class_driver.cpp -- drives the C++ class layer (include/plegma/) the way a
 // PLEGMA driver does (plegma/Calc_2pt.cpp:121-132, 160-266): one lattice,
-// seeded propagators, a correlator per contraction, H_elem and the ASCII file.
-// Writes each H_elem as raw little-endian Float pairs to <outdir>/<name>.bin
-// for the GPU tests to compare with the CPU oracle.
+// seeded propagators, a correlator per contraction, H_elem, the ASCII file and
+// the HDF5 writer's record file.  Writes each H_elem as raw little-endian Float
+// pairs to <outdir>/<name>.bin for the GPU tests to compare with the CPU oracle.
 //
 //   class_driver <outdir> <f64|f32>
 #include <cstdio>
@@ -39,17 +39,34 @@ template <typename Float> static int run(const std::string &out) {
     c.contractMesons(u, d);
     dump(c, out + "/mesons_" + tag + ".bin");
     c.writeASCII(out + "/mesons_" + tag);
+    c.writeHDF5(out + "/corr_" + tag);  // -> corr_<tag>.h5.twin
   }
   {
     PLEGMA_Correlator<Float> c(MOMENTUM_SPACE, src, 4, 6);
     c.contractBaryons(u, d, PLG_G_NUCL | PLG_G_DELTAPP_ALL);
     dump(c, out + "/baryons_" + tag + ".bin");
+    c.writeFile(out + "/corr_" + tag + ".h5");  // format from the extension
   }
   {
     PLEGMA_Correlator<Float> c(MOMENTUM_SPACE, src, 1);  // full time extent
     c.contractMesonsAll(u, d);
     dump(c, out + "/mesons_all_" + tag + ".bin");
     c.writeASCII(out + "/mesons_all_" + tag);
+    // same group as contractMesons: its mvec stays, twop_meson is added
+    c.writeFile(out + "/corr_" + tag + ".h5/ignored/group", HDF5_FORMAT);
+  }
+  {
+    // a second write of a dataset already in the file is skipped
+    PLEGMA_Correlator<Float> c(MOMENTUM_SPACE, src, 4, 6);
+    c.contractMesons(d, u);
+    c.writeHDF5(out + "/corr_" + tag + ".h5");
+  }
+  {
+    // position space over a window that wraps the time boundary
+    PLEGMA_Correlator<Float> c(POSITION_SPACE, src, 0, 4);
+    c.contractMesonsNew(u, d);
+    dump(c, out + "/pos_mesons_" + tag + ".bin");
+    c.writeHDF5(out + "/pos_" + tag + ".h5");
   }
   {
     PLEGMA_Correlator<Float> c(MOMENTUM_SPACE, src, 4, 6);

@@ -92,3 +92,80 @@ def test_ft_apply_class(outputs, prec, tol):
     for t in (0, 5):
         ref = cf.dft(U[:, t, :].T, L, [0, 0, 0], moms)       # [mom][144]
         assert G.rel_err(got[t].T, ref, axis=0) < tol
+
+
+def _descr(body):
+    return "shape: " + body + "/re-im"
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-6)])
+def test_hdf5_record_file(outputs, prec, tol):
+    """writeHDF5 / writeFile(.h5): the objects of the reference's HDF5 file
+    (lib/PLEGMA_Correlator.cu:809-946) in the record file: paths
+    /sxXXsyXXszXXstXX/<group>/<dataset>, mvec, shape (totalT, Nmom, shape, 2)
+    with time counted from the source, the description attribute, and the
+    skip of datasets already present."""
+    from plegma_amd import h5twin
+    d, u, dn = outputs
+    ds = h5twin.read(os.path.join(d, f"corr_{prec}.h5.twin"))
+    top = "/sx01sy02sz03st05"
+    mdesc = "pseudoscalar, scalar, g5g1, g5g2, g5g3, g5g4, g1, g2, g3, g4"
+    bdesc = "1,g1,g2,g3,g4,g5,g5g1,g5g2,g5g3,g5g4,s12,s13,s23,s41,s42,s43"
+    bgroups = ["nucl_nucl", "deltapp_deltamm_11", "deltapp_deltamm_22", "deltapp_deltamm_33"]
+    want = [f"{top}/mesons/mvec", f"{top}/mesons/twop_meson_1", f"{top}/mesons/twop_meson_2"]
+    for g in bgroups:
+        want += [f"{top}/baryons/{g}/mvec", f"{top}/baryons/{g}/twop_baryon_1", f"{top}/baryons/{g}/twop_baryon_2"]
+    want += [f"{top}/mesons/twop_meson"]
+    assert list(ds) == want
+    dt = np.float64 if prec == "f64" else np.float32
+    moms = O.momlist(4)
+    np.testing.assert_array_equal(ds[f"{top}/mesons/mvec"].data, moms)   # first writer's list kept
+
+    def check(path, ref_block, totalT, descr):
+        obj = ds[path]
+        assert obj.data.dtype == dt
+        assert obj.attrs == {"description": descr}
+        got = obj.data.astype(np.float64)
+        got = got[..., 0] + 1j * got[..., 1]
+        assert got.shape[0] == totalT and got.shape[1:] == ref_block.shape[1:]
+        for j in range(totalT):
+            tau = (O.row_time(L, SRC, totalT, j) - SRC[3]) % L[3]
+            assert G.rel_err(got[tau], ref_block[j]) < tol
+
+    ref = O.correlator(O.MESONS, L, SRC, 6, 4, u, dn)                    # [g][d][j][m][c]
+    for k in range(2):
+        check(f"{top}/mesons/twop_meson_{k + 1}", ref[0, k], 6, _descr("/time/moms/" + mdesc))
+    ref = O.correlator(O.BARYONS, L, SRC, 6, 4, u, dn, mask=O.G_NUCL | sum(O.G_DELTA32))
+    for gi, g in enumerate(bgroups):
+        np.testing.assert_array_equal(ds[f"{top}/baryons/{g}/mvec"].data, moms)
+        for k in range(2):
+            check(f"{top}/baryons/{g}/twop_baryon_{k + 1}", ref[gi, k], 6, _descr("/time/moms/" + bdesc))
+    ref = O.correlator(O.MESONS_ALL, L, SRC, 8, 1, u, dn)
+    alld = "pseudoscalar, scalar" + "".join(f", ({v}g{i},{v}g{j})" for v in ("g5", "")
+                                            for i in range(1, 5) for j in range(1, 5))
+    check(f"{top}/mesons/twop_meson", ref[0, 0], 8, _descr("/time/moms/" + alld))
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-6)])
+def test_hdf5_position_space(outputs, prec, tol):
+    """position space: (totalT, Lz, Ly, Lx, 10, 2), every axis counted from
+    the source with periodic wrap (lib/PLEGMA_Correlator.cu:825-832)"""
+    from plegma_amd import h5twin
+    d, u, dn = outputs
+    ds = h5twin.read(os.path.join(d, f"pos_{prec}.h5.twin"))
+    path = "/sx01sy02sz03st05/mesons/twop_meson"
+    assert list(ds) == [path]
+    mdesc = "pseudoscalar, scalar, g5g1, g5g2, g5g3, g5g4, g1, g2, g3, g4"
+    assert ds[path].attrs == {"description": _descr("/t/z/y/x/" + mdesc)}
+    got = ds[path].data.astype(np.float64)
+    got = got[..., 0] + 1j * got[..., 1]
+    assert got.shape == (4, L[2], L[1], L[0], 10)
+    # the raw H_elem of the same correlator: rows j over x lexicographic
+    H = _load(d, "pos_mesons", prec).reshape(4, L[2], L[1], L[0], 10)
+    for j in range(4):
+        t = O.row_time(L, SRC, 4, j)
+        tau = (t - SRC[3]) % L[3]
+        ref = O.site_correlator(O.MESONS_NEW, L, t, u, dn).reshape(L[2], L[1], L[0], 10)
+        assert G.rel_err(H[j], ref, axis=(0, 1, 2)) < tol
+        sh = np.roll(ref, shift=(-SRC[2], -SRC[1], -SRC[0]), axis=(0, 1, 2))
+        assert G.rel_err(got[tau], sh, axis=(0, 1, 2)) < tol
