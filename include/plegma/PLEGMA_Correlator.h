@@ -228,6 +228,63 @@ public:
     run(PLG_MESON_MESON, 0, propLight, propHeavy);
   }
 
+  // lib/PLEGMA_Correlator.cu:136-188: every flavour product ("ucc" ... "sss")
+  // whose propagators are allocated (NONE = flavour absent), optionally only
+  // those holding s / c; datasets = the products' operator names in order
+  void contractBaryonsUDSC(PLEGMA_Propagator<Float> &propUP, PLEGMA_Propagator<Float> &propDN,
+                           PLEGMA_Propagator<Float> &propST, PLEGMA_Propagator<Float> &propCH,
+                           bool only_st = false, bool only_ch = false) {
+    shape = {};
+    description = "";
+    groups = {};
+    datasets.clear();
+    PLEGMA_Propagator<Float> *fl[4] = {&propUP, &propDN, &propST, &propCH};
+    plg_prop v[4];
+    const plg_prop *vp[4];
+    for (int i = 0; i < 4; i++) {
+      vp[i] = nullptr;
+      if (fl[i]->getAllocation() != NONE) {
+        v[i] = fl[i]->view();
+        vp[i] = &v[i];
+      }
+    }
+    int prods[64], np = 0, nd = 0;
+    plg_check(plg_baryons_udsc_select(vp[0], vp[1], vp[2], vp[3], only_st, only_ch, prods, 64, &np, &nd),
+              "plg_baryons_udsc_select");
+    for (int i = 0; i < np; i++) {
+      int nops = 0;
+      plg_check(plg_udsc_product(prods[i], nullptr, &nops), "plg_udsc_product");
+      for (int k = 0; k < nops; k++) {
+        char name[64];
+        plg_check(plg_udsc_op_name(prods[i], k, name, sizeof(name)), "plg_udsc_op_name");
+        datasets.push_back(name);
+      }
+    }
+    if (datasets.empty()) return;
+    initialize();
+    std::vector<int> moms = corr_space == MOMENTUM_SPACE ? corr_mom_space->intMomenta() : std::vector<int>{};
+    const int nmom = (int)moms.size() / 3;
+    const int space = corr_space == MOMENTUM_SPACE ? PLG_MOMENTUM_SPACE : PLG_POSITION_SPACE;
+    int64_t n = 0;
+    plg_check(plg_baryons_udsc_size(vp[0], vp[1], vp[2], vp[3], only_st, only_ch, source.data(), totalT, nmom,
+                                    space, &n),
+              "plg_baryons_udsc_size");
+    if ((size_t)n != getTotalSize()) PLEGMA_error("udsc correlator size mismatch (%lld vs %zu)", (long long)n,
+                                                  getTotalSize());
+    const size_t bytes = (size_t)n * 2 * sizeof(Float);
+    if (bytes > d_bytes) {
+      if (d_out) (void)hipFree(d_out);
+      hip_check(hipMalloc(&d_out, bytes), "correlator result buffer");
+      d_bytes = bytes;
+    }
+    const int mode = sizeof(Float) == 8 ? PLG_MODE_F64 : PLG_MODE_MIXED;
+    plg_check(plg_baryons_udsc(vp[0], vp[1], vp[2], vp[3], only_st, only_ch, source.data(), totalT,
+                               moms.empty() ? nullptr : moms.data(), nmom, mode, space, plg_precision_of<Float>(),
+                               d_out, nullptr),
+              "plg_baryons_udsc");
+    hip_check(hipMemcpy(H_elem(), d_out, bytes, hipMemcpyDeviceToHost), "result copy");
+  }
+
   // ---- output (lib/PLEGMA_Correlator.cu:721-806, momentum space)
   void writeASCII(std::string filename) const {
     if (corr_space != MOMENTUM_SPACE) PLEGMA_error("WriteASCII do not support writing in position space.");

@@ -128,6 +128,32 @@ int plg_correlator(int kind, int group_mask, const plg_prop *p1, const plg_prop 
 int plg_correlator_size(int kind, int group_mask, const int source[4], int totalT, int nmom,
                         int space, int64_t *elems);
 
+/* ---- udsc baryons ------------------------------------------------------
+ * Replaces contractBaryonsUDSC / contract_baryons_udsc
+ * (lib/PLEGMA_Correlator.cu:136-188, lib/kernels/PLEGMA_baryons_udsc.cu:267-300):
+ * the two-point functions of the udsc baryon interpolators for every flavour
+ * product ("ucc", "dcc", ..., "sss") whose propagators are given (NULL =
+ * absent), optionally only products holding s (only_st) or c (only_ch).
+ * Datasets: the product's operators in the reference's order, products in
+ * the reference's order; names from plg_udsc_op_name ("Lambda/Pp_Cg5_Cg5").
+ * out: device, [dataset][j][m] complex (momentum space) or [dataset][j][x]
+ * (position space) of out_precision; mode as plg_correlator (PLG_MODE_MIXED
+ * takes FP32 propagators).  Term tables: generated from the interpolating
+ * fields (python/baryons.py of the reference makes the same tables). */
+int plg_udsc_count(int *nprod);
+/* flavour letters (3 chars + NUL into flavours[4]) and operator count */
+int plg_udsc_product(int prod, char *flavours, int *nops);
+int plg_udsc_op_name(int prod, int k, char *buf, int cap);
+/* the products (indices) plg_baryons_udsc runs for these arguments */
+int plg_baryons_udsc_select(const plg_prop *up, const plg_prop *dn, const plg_prop *st, const plg_prop *ch,
+                            int only_st, int only_ch, int *prods, int cap, int *nprods, int *ndatasets);
+int plg_baryons_udsc_size(const plg_prop *up, const plg_prop *dn, const plg_prop *st, const plg_prop *ch,
+                          int only_st, int only_ch, const int source[4], int totalT, int nmom, int space,
+                          int64_t *elems);
+int plg_baryons_udsc(const plg_prop *up, const plg_prop *dn, const plg_prop *st, const plg_prop *ch, int only_st,
+                     int only_ch, const int source[4], int totalT, const int *moms, int nmom, int mode, int space,
+                     int out_precision, void *out, void *stream);
+
 /* Standalone momentum projection of a field, replacing PLEGMA_FT::apply with
  * FT_GEMV (lib/PLEGMA_FT.cu:144-162, lib/kernels/PLEGMA_FT.cuh:74-100):
  *   out[t][i][m] (+)= sum_x f_i(x, t) exp(sign i 2pi n_m . x / L)

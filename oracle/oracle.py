@@ -82,6 +82,10 @@ def lib():
         L.orc_prop_rotate.argtypes = [f64p, C.c_int64, C.c_int, C.c_int64, C.c_int]
         L.orc_prop_boundaries.argtypes = [f64p, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int]
         L.orc_gamma.restype = None
+        L.orc_site_udsc.argtypes = [f64p, f64p, f64p, i32p, f64p, i32p, i32p, C.c_int, f64p]
+        L.orc_site_udsc.restype = None
+        L.orc_udsc_field.argtypes = [i32p, C.c_int, C.POINTER(OrcProp), C.POINTER(OrcProp), C.POINTER(OrcProp),
+                                     i32p, f64p, i32p, i32p, C.c_int, f64p, C.c_int]
         _lib = L
     return _lib
 
@@ -270,3 +274,66 @@ def prop_boundaries(P, L, t0, t_first=0):
     V3 = L[0] * L[1] * L[2]
     lib().orc_prop_boundaries(P.ctypes.data_as(C.POINTER(C.c_double)), P.shape[1], t_first,
                               P.shape[1] // V3, L[3], V3, int(t0))
+
+
+def udsc_tables(prod_tables):
+    """flat arrays of one flavour product's operator tables [(name, {m6: v})]"""
+    idx, val, first, count = [], [], [], []
+    for _, t in prod_tables:
+        first.append(len(idx) // 6)
+        count.append(len(t))
+        for m in sorted(t):
+            idx.extend(m)
+            val.extend([t[m].real, t[m].imag])
+    return (np.array(idx, dtype=np.int32), np.array(val, dtype=np.float64), np.array(first, dtype=np.int32),
+            np.array(count, dtype=np.int32))
+
+
+def udsc_site(S1, S2, S3, prod_tables):
+    """operators of one product on one site; S*: complex [4,4,3,3]"""
+    idx, val, first, count = udsc_tables(prod_tables)
+    raw = [np.ascontiguousarray(np.stack([S.real, S.imag], -1).reshape(-1)) for S in (S1, S2, S3)]
+    out = np.zeros(2 * len(count))
+    lib().orc_site_udsc(_ptr(raw[0]), _ptr(raw[1]), _ptr(raw[2]), idx.ctypes.data_as(C.POINTER(C.c_int)),
+                        _ptr(val), first.ctypes.data_as(C.POINTER(C.c_int)),
+                        count.ctypes.data_as(C.POINTER(C.c_int)), len(count), _ptr(out))
+    return out[0::2] + 1j * out[1::2]
+
+
+def udsc_correlator(L, src, totalT, q2max, props, products, only_st=False, only_ch=False, space="momentum",
+                    nthreads=0):
+    """contractBaryonsUDSC (lib/PLEGMA_Correlator.cu:136-188) on the CPU.
+
+    props: flavour letter ('u', 'd', 's', 'c') -> float64 SoA propagator
+    (144, T*V3, 2) flattened, absent flavours missing; products: the
+    [(letters, [(name, {m6: v})])] term tables.  Returns (names, H) with
+    H[d][j][m] (momentum space) or H[d][j][x] (position space)."""
+    V3 = L[0] * L[1] * L[2]
+    s, e, lt = window(L, src, totalT)
+    names, blocks = [], []
+    for letters, tabs in products:
+        if any(f not in props for f in letters):
+            continue
+        if only_st and "s" not in letters:
+            continue
+        if only_ch and "c" not in letters:
+            continue
+        idx, val, first, count = udsc_tables(tabs)
+        P = [OrcProp(props[f].ctypes.data, L[3] * V3, 0, L[3]) for f in letters]
+        nops = len(count)
+        H = np.zeros((nops, lt) + ((len(momlist(q2max)),) if space == "momentum" else (V3,)), dtype=complex)
+        for j in range(lt):
+            t = row_time(L, src, totalT, j)
+            f = np.zeros(V3 * nops * 2)
+            rc = lib().orc_udsc_field(_ivec(L), t, C.byref(P[0]), C.byref(P[1]), C.byref(P[2]),
+                                      idx.ctypes.data_as(C.POINTER(C.c_int)), _ptr(val),
+                                      first.ctypes.data_as(C.POINTER(C.c_int)),
+                                      count.ctypes.data_as(C.POINTER(C.c_int)), nops, _ptr(f), nthreads)
+            if rc:
+                raise RuntimeError(f"orc_udsc_field failed: {rc}")
+            f = f.reshape(V3, nops, 2)
+            f = f[..., 0] + 1j * f[..., 1]
+            H[:, j] = (dft(L, src[:3], q2max, f, nthreads) if space == "momentum" else f).T
+        names += [n for n, _ in tabs]
+        blocks.append(H)
+    return names, (np.concatenate(blocks) if blocks else np.zeros((0,)))

@@ -867,3 +867,55 @@ void orc_prop_boundaries(double *data, int64_t comp_stride, int t_first, int nt,
       }
   }
 }
+
+/* ------------------------------------------------------------------ */
+/* udsc baryons: lib/kernels/PLEGMA_baryons_udsc.cu:86-121             */
+/*   O_k = sum_t v_t sum_{cc1,cc2} sgn sgn' S1[m0][m1][a][a1]          */
+/*         S2[m2][m3][b][b1] S3[m4][m5][c][c1]                         */
+/* with (a,b,c) = eps[cc1], (a1,b1,c1) = eps[cc2]; terms of operator k */
+/* are idx[6*t..], val[2*t..] for t in [first[k], first[k]+count[k]).  */
+/* ------------------------------------------------------------------ */
+void orc_site_udsc(const double *S1r, const double *S2r, const double *S3r, const int *idx, const double *val,
+                   const int *first, const int *count, int nops, double *out) {
+  cplx S1[NPROP], S2[NPROP], S3[NPROP];
+  load_site(S1r, S1);
+  load_site(S2r, S2);
+  load_site(S3r, S3);
+  for (int k = 0; k < nops; k++) {
+    cplx acc = 0;
+    for (int t = first[k]; t < first[k] + count[k]; t++) {
+      const int *mu = idx + 6 * t;
+      const cplx v = val[2 * t] + I * val[2 * t + 1];
+      for (int cc1 = 0; cc1 < 6; cc1++) {
+        const int a = EPS[cc1][0], b = EPS[cc1][1], c = EPS[cc1][2];
+        for (int cc2 = 0; cc2 < 6; cc2++) {
+          const int a1 = EPS[cc2][0], b1 = EPS[cc2][1], c1 = EPS[cc2][2];
+          acc += SGN_EPS[cc1] * SGN_EPS[cc2] * v * PS(S1, mu[0], mu[1], a, a1) * PS(S2, mu[2], mu[3], b, b1) *
+                 PS(S3, mu[4], mu[5], c, c1);
+        }
+      }
+    }
+    store(acc, out + 2 * k);
+  }
+}
+
+int orc_udsc_field(const int L[4], int t, const orc_prop *p1, const orc_prop *p2, const orc_prop *p3,
+                   const int *idx, const double *val, const int *first, const int *count, int nops, double *out,
+                   int nthreads) {
+  int64_t V3 = (int64_t)L[0] * L[1] * L[2];
+  int err = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static) reduction(| : err)
+#endif
+  for (int64_t x = 0; x < V3; x++) {
+    double S1[2 * NPROP], S2[2 * NPROP], S3[2 * NPROP];
+    if (gather_site(p1, L[3], V3, t, x, S1) || gather_site(p2, L[3], V3, t, x, S2) ||
+        gather_site(p3, L[3], V3, t, x, S3)) {
+      err |= 1;
+      continue;
+    }
+    orc_site_udsc(S1, S2, S3, idx, val, first, count, nops, out + x * nops * 2);
+  }
+  return err ? -2 : 0;
+}

@@ -107,6 +107,16 @@ int orc_site_correlator(int kind, int group_mask, const int L[4], int t,
                         const orc_prop *p1, const orc_prop *p2, double *out,
                         int nthreads);
 
+/* udsc baryons (lib/kernels/PLEGMA_baryons_udsc.cu:86-121): operators k of a
+ * term table (idx 6 ints, val re/im per term; operator k = terms
+ * [first[k], first[k]+count[k])) on one site, out[2*k + ri] */
+void orc_site_udsc(const double *S1, const double *S2, const double *S3, const int *idx, const double *val,
+                   const int *first, const int *count, int nops, double *out);
+/* the same over the V3 sites of time slice t: out[(x*nops + k)*2 + ri] */
+int orc_udsc_field(const int L[4], int t, const orc_prop *p1, const orc_prop *p2, const orc_prop *p3,
+                   const int *idx, const double *val, const int *first, const int *count, int nops, double *out,
+                   int nthreads);
+
 /* Naive DFT of a site field (reference fourier_transform_3D semantics, sign -1):
  * f[(x*ncomp + c)*2+ri] over V3 sites -> out[(m*ncomp + c)*2+ri]. */
 void orc_dft(const int L[4], const int src3[3], int q2max, const double *f,

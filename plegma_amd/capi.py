@@ -32,7 +32,8 @@ EXPORTS = ["plg_init", "plg_finalize", "plg_last_error", "plg_version", "plg_mom
            "plg_batch_destroy", "plg_batch_profile", "plg_synth_prop", "plg_synth_prop_perturbed", "plg_cast_prop",
            "plg_mlp_create", "plg_mlp_nparams", "plg_mlp_set_params", "plg_mlp_get_params",
            "plg_mlp_forward", "plg_mlp_gradient", "plg_mlp_train", "plg_mlp_destroy", "plg_ft_apply",
-           "plg_prop_rotate_physical", "plg_prop_apply_boundaries"]
+           "plg_prop_rotate_physical", "plg_prop_apply_boundaries", "plg_udsc_count", "plg_udsc_product",
+           "plg_udsc_op_name", "plg_baryons_udsc_select", "plg_baryons_udsc_size", "plg_baryons_udsc"]
 
 
 class PlgError(RuntimeError):
@@ -91,6 +92,15 @@ def lib():
         L.plg_ft_apply.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, ip, C.c_int, C.c_int, C.c_int,
                                    C.c_void_p, C.c_int, C.c_void_p]
         fp = C.POINTER(C.c_float)
+        pp = C.POINTER(plg_prop)
+        L.plg_udsc_count.argtypes = [ip]
+        L.plg_udsc_product.argtypes = [C.c_int, C.c_char_p, ip]
+        L.plg_udsc_op_name.argtypes = [C.c_int, C.c_int, C.c_char_p, C.c_int]
+        L.plg_baryons_udsc_select.argtypes = [pp, pp, pp, pp, C.c_int, C.c_int, ip, C.c_int, ip, ip]
+        L.plg_baryons_udsc_size.argtypes = [pp, pp, pp, pp, C.c_int, C.c_int, ip, C.c_int, C.c_int, C.c_int,
+                                            C.POINTER(C.c_int64)]
+        L.plg_baryons_udsc.argtypes = [pp, pp, pp, pp, C.c_int, C.c_int, ip, C.c_int, ip, C.c_int, C.c_int,
+                                       C.c_int, C.c_int, C.c_void_p, C.c_void_p]
         L.plg_mlp_create.argtypes = [ip, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         L.plg_mlp_nparams.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         L.plg_mlp_set_params.argtypes = [C.c_void_p, fp]
@@ -175,6 +185,53 @@ def ft_apply(field, precision, dof, nt, moms, sign, out, out_prec=F64, accumulat
     flat = [int(v) for m in moms for v in m]
     check(lib().plg_ft_apply(C.c_void_p(field.data_ptr()), precision, dof, nt, ivec(flat), len(moms), sign,
                              int(accumulate), C.c_void_p(out.data_ptr()), out_prec, _stream(stream)))
+
+
+def udsc_products():
+    """[(flavour letters, [dataset names])] of the udsc baryon products, in order"""
+    n = C.c_int()
+    check(lib().plg_udsc_count(C.byref(n)))
+    out = []
+    for p in range(n.value):
+        fl = C.create_string_buffer(4)
+        k = C.c_int()
+        check(lib().plg_udsc_product(p, fl, C.byref(k)))
+        names = []
+        for i in range(k.value):
+            b = C.create_string_buffer(64)
+            check(lib().plg_udsc_op_name(p, i, b, 64))
+            names.append(b.value.decode())
+        out.append((fl.value.decode(), names))
+    return out
+
+
+def _ref(v):
+    return C.byref(v) if v is not None else None
+
+
+def udsc_select(up, dn, st, ch, only_st=False, only_ch=False):
+    """(product indices, number of datasets) that baryons_udsc computes"""
+    prods = (C.c_int * 32)()
+    n, nd = C.c_int(), C.c_int()
+    check(lib().plg_baryons_udsc_select(_ref(up), _ref(dn), _ref(st), _ref(ch), int(only_st), int(only_ch),
+                                        prods, 32, C.byref(n), C.byref(nd)))
+    return list(prods[:n.value]), nd.value
+
+
+def baryons_udsc(up, dn, st, ch, source, totalT, moms, mode, space, out_prec, out, only_st=False, only_ch=False,
+                 stream=None):
+    """contractBaryonsUDSC: views up/dn/st/ch (None = absent) -> out [dataset][j][m]"""
+    flat = [int(v) for m in moms for v in m]
+    check(lib().plg_baryons_udsc(_ref(up), _ref(dn), _ref(st), _ref(ch), int(only_st), int(only_ch),
+                                 ivec(source), totalT, ivec(flat) if flat else None, len(moms), mode, space,
+                                 out_prec, C.c_void_p(out.data_ptr()), _stream(stream)))
+
+
+def baryons_udsc_size(up, dn, st, ch, source, totalT, nmom, space=MOMENTUM_SPACE, only_st=False, only_ch=False):
+    n = C.c_int64()
+    check(lib().plg_baryons_udsc_size(_ref(up), _ref(dn), _ref(st), _ref(ch), int(only_st), int(only_ch),
+                                      ivec(source), totalT, nmom, space, C.byref(n)))
+    return n.value
 
 
 def correlator_size(kind, mask, source, totalT, nmom, space=MOMENTUM_SPACE):

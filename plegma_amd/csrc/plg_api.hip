@@ -153,7 +153,7 @@ static int kind_shape(int kind, int mask, int &ng, int &nd, int &nch) {
 
 // ------------------------------------------------------------------ plans
 struct Launch {
-  enum { MESON, BARYON, MESMES } what;
+  enum { MESON, BARYON, MESMES, UDSC } what;
   int set;          // meson set
   bool two, mes;    // baryon
   int gmask;
@@ -196,6 +196,10 @@ struct Plan {
   int64_t *d_choff = nullptr;
   int *d_ms = nullptr, *d_pair_ms = nullptr, *d_mom_pair = nullptr;
   uint16_t *d_pidx = nullptr;  // phase indices of the mixed projection
+  // udsc baryons: the flavour product's schedule (device arrays owned here)
+  UdscSched udsc{};
+  short *d_uq = nullptr, *d_ut = nullptr;
+  UdscEntry *d_ue = nullptr;
   // batch-owned workspace (single calls use the context workspace)
   void *c = nullptr, *part = nullptr;
   size_t c_bytes = 0, part_bytes = 0;
@@ -211,7 +215,8 @@ struct Plan {
     if (graph) hipGraphDestroy(graph);
     if (cap_stream) hipStreamDestroy(cap_stream);
     for (void *p : {(void *)d_props, (void *)d_rows, (void *)d_orows, (void *)d_csel, (void *)d_choff, (void *)d_ms,
-                    (void *)d_pair_ms, (void *)d_mom_pair, (void *)d_pidx})
+                    (void *)d_pair_ms, (void *)d_mom_pair, (void *)d_pidx, (void *)d_uq, (void *)d_ut,
+                    (void *)d_ue})
       if (p) hipFree(p);
     if (own_ws) {
       if (c) hipFree(c);
@@ -594,6 +599,7 @@ static std::string launch_name(const Plan &P, const Launch &l) {
   std::string t = P.T == PLG_F64 ? "f64" : "f32";
   if (l.what == Launch::MESON) return "contract_mesons_" + t;
   if (l.what == Launch::MESMES) return "contract_mesmes_" + t;
+  if (l.what == Launch::UDSC) return "contract_udsc_" + t;
   return std::string(l.mes ? "contract_twopt_" : "contract_baryons_") + t;
 }
 
@@ -605,6 +611,8 @@ template <typename T> static int launch_contract_t(const Plan &P, const Launch &
     return launch_meson<T>(l.set, pd, P.d_rows, nrows, (cx<T> *)c, g_ctx.V3, g_ctx.L[3], P.nch_c, l.ch0, st);
   case Launch::MESMES:
     return launch_mesmes<T>(pd, P.d_rows, nrows, (cx<T> *)c, g_ctx.V3, g_ctx.L[3], P.nch_c, l.ch0, P.tpb, st);
+  case Launch::UDSC:
+    return launch_udsc<T>(P.udsc, pd, P.d_rows, nrows, (cx<T> *)c, g_ctx.V3, g_ctx.L[3], P.nch_c, l.ch0, P.tpb, st);
   default:
     return launch_baryon<T>(l.two, l.mes, l.gmask, pd, P.d_rows, nrows, (cx<T> *)c, g_ctx.V3, g_ctx.L[3],
                             P.nch_c, l.ch0, P.tpb, st);
@@ -929,6 +937,165 @@ int plg_correlator(int kind, int group_mask, const plg_prop *p1, const plg_prop 
   if (rc) return rc;
   // the plan's device tables are freed on return: finish this correlator first
   HIPCHK(hipStreamSynchronize(st));
+  return PLG_OK;
+}
+
+// ------------------------------------------------------------------ udsc baryons
+// lib/PLEGMA_Correlator.cu:136-188 (product selection, datasets) and
+// lib/kernels/PLEGMA_baryons_udsc.cu:267-300 (one product after the other)
+static const char UDSC_LETTERS[4] = {'u', 'd', 's', 'c'};
+
+static void udsc_select(const plg_prop *const fl[4], int only_st, int only_ch, std::vector<int> &prods, int &nd) {
+  prods.clear();
+  nd = 0;
+  for (int i = 0; i < udsc_products(); i++) {
+    const std::string f = udsc_flavours(i);
+    bool ok = true;
+    for (int k = 0; k < 4; k++)
+      if (!fl[k] && f.find(UDSC_LETTERS[k]) != std::string::npos) ok = false;
+    if (only_st && f.find('s') == std::string::npos) ok = false;
+    if (only_ch && f.find('c') == std::string::npos) ok = false;
+    if (ok) {
+      prods.push_back(i);
+      nd += udsc_nops(i);
+    }
+  }
+}
+
+int plg_udsc_count(int *nprod) {
+  if (!nprod) return fail(PLG_ERR_INVALID, "NULL argument");
+  *nprod = udsc_products();
+  return PLG_OK;
+}
+
+int plg_udsc_product(int prod, char *flavours, int *nops) {
+  if (prod < 0 || prod >= udsc_products()) return fail(PLG_ERR_INVALID, "no such flavour product");
+  if (flavours) std::memcpy(flavours, udsc_flavours(prod), 4);
+  if (nops) *nops = udsc_nops(prod);
+  return PLG_OK;
+}
+
+int plg_udsc_op_name(int prod, int k, char *buf, int cap) {
+  if (prod < 0 || prod >= udsc_products() || k < 0 || k >= udsc_nops(prod) || !buf || cap <= 0)
+    return fail(PLG_ERR_INVALID, "bad product, operator or buffer");
+  const char *n = udsc_op_name(prod, k);
+  if ((int)std::strlen(n) >= cap) return fail(PLG_ERR_RANGE, "name buffer too small");
+  std::strcpy(buf, n);
+  return PLG_OK;
+}
+
+int plg_baryons_udsc_select(const plg_prop *up, const plg_prop *dn, const plg_prop *st, const plg_prop *ch,
+                            int only_st, int only_ch, int *prods, int cap, int *nprods, int *ndatasets) {
+  const plg_prop *fl[4] = {up, dn, st, ch};
+  std::vector<int> v;
+  int nd;
+  udsc_select(fl, only_st, only_ch, v, nd);
+  if (prods && (int)v.size() > cap) return fail(PLG_ERR_RANGE, "product buffer too small");
+  if (prods) std::copy(v.begin(), v.end(), prods);
+  if (nprods) *nprods = (int)v.size();
+  if (ndatasets) *ndatasets = nd;
+  return PLG_OK;
+}
+
+int plg_baryons_udsc_size(const plg_prop *up, const plg_prop *dn, const plg_prop *st, const plg_prop *ch,
+                          int only_st, int only_ch, const int source[4], int totalT, int nmom, int space,
+                          int64_t *elems) {
+  int lt, nd, rc;
+  if (!elems) return fail(PLG_ERR_INVALID, "NULL argument");
+  if ((rc = plg_window(source, totalT, nullptr, nullptr, &lt))) return rc;
+  const plg_prop *fl[4] = {up, dn, st, ch};
+  std::vector<int> v;
+  udsc_select(fl, only_st, only_ch, v, nd);
+  *elems = (int64_t)nd * lt * (space == PLG_MOMENTUM_SPACE ? nmom : g_ctx.V3);
+  return PLG_OK;
+}
+
+int plg_baryons_udsc(const plg_prop *up, const plg_prop *dn, const plg_prop *st, const plg_prop *ch, int only_st,
+                     int only_ch, const int source[4], int totalT, const int *moms, int nmom, int mode, int space,
+                     int out_precision, void *out, void *stream) {
+  if (!g_ctx.init) return fail(PLG_ERR_NOT_INIT, "plg_init not called");
+  if (!out || !source) return fail(PLG_ERR_INVALID, "NULL argument");
+  if (out_precision != PLG_F64 && out_precision != PLG_F32) return fail(PLG_ERR_INVALID, "bad out precision");
+  if (mode != PLG_MODE_F64 && mode != PLG_MODE_MIXED) return fail(PLG_ERR_INVALID, "bad mode");
+  if (space != PLG_MOMENTUM_SPACE && space != PLG_POSITION_SPACE) return fail(PLG_ERR_INVALID, "bad space");
+  if (space == PLG_MOMENTUM_SPACE && (nmom <= 0 || !moms)) return fail(PLG_ERR_INVALID, "empty momentum list");
+  int lt, rc;
+  if ((rc = plg_window(source, totalT, nullptr, nullptr, &lt))) return rc;
+  const Window w = window_of(g_ctx.L, source, totalT);
+  const plg_prop *fl[4] = {up, dn, st, ch};
+  std::vector<int> prods;
+  int nd_total;
+  udsc_select(fl, only_st, only_ch, prods, nd_total);
+  hipStream_t hs = (hipStream_t)stream;
+  const int T = g_ctx.L[3];
+  const int64_t inner = space == PLG_MOMENTUM_SPACE ? (int64_t)nmom : g_ctx.V3;
+  int d0 = 0;
+  for (int prod : prods) {
+    Plan P;
+    P.kind = -1;
+    P.mode = mode;
+    P.terms = PLG_SPLIT_BF16X6;
+    P.space = space;
+    P.T = mode == PLG_MODE_F64 ? PLG_F64 : PLG_F32;
+    const char *f = udsc_flavours(prod);
+    for (int i = 0; i < 3; i++) {
+      const int k = (int)(std::find(UDSC_LETTERS, UDSC_LETTERS + 4, f[i]) - UDSC_LETTERS);
+      P.props.push_back(*fl[k]);
+    }
+    P.nmom = space == PLG_MOMENTUM_SPACE ? nmom : 0;
+    if (P.nmom) P.moms.assign(moms, moms + 3 * P.nmom);
+    P.ms = scaled_moms(P.moms.data(), P.nmom, 1);
+    if (P.T == PLG_F32 && P.nmom > 0) P.npairs = make_pairs(P.ms, g_ctx.M, P.pair_ms, P.mom_pair, &P.zero_pair);
+    const int nops = udsc_nops(prod);
+    P.nch_c = nops;
+    for (int j = 0; j < lt; j++) {
+      const int t = row_time(g_ctx.L, source, w, j);
+      for (const plg_prop &v : P.props)
+        if ((rc = check_prop(v, P.T, t))) return rc;
+      P.rows.push_back({t, 0, 1, 0, 2});
+    }
+    for (int k = 0; k < nops; k++)
+      for (int j = 0; j < lt; j++) {
+        const int t = P.rows[j].t;
+        OutRowDev o{};
+        o.crow = j;
+        o.ch_base = k;
+        o.nch_row = 1;
+        o.sx = source[0]; o.sy = source[1]; o.sz = source[2];
+        o.base = ((int64_t)(d0 + k) * lt + j) * inner;
+        // trilinear in the three views: one plane exponent of each
+        auto plane = [&](const plg_prop &v) { return ((t - v.t_first) % T + T) % T; };
+        o.ea = P.props[0].plane_exp; o.pla = plane(P.props[0]); o.ka = 1;
+        o.eb = P.props[1].plane_exp; o.plb = plane(P.props[1]); o.kb = 1;
+        o.ec = P.props[2].plane_exp; o.plc = plane(P.props[2]); o.kc = 1;
+        P.orows.push_back(o);
+      }
+    P.csel = {0};
+    P.ch_off = {0};
+    P.m_stride = 1;
+    P.out_elems = (int64_t)nd_total * lt * inner;
+    P.nsites = (int64_t)lt * g_ctx.V3;
+    P.launches.push_back({Launch::UDSC, prod, false, false, 0, 0});
+    UdscSchedHost h;
+    if ((rc = udsc_schedule(prod, h))) return fail(rc, "udsc schedule");
+    if ((rc = upload(&P.d_uq, h.quads, hs)) || (rc = upload(&P.d_ut, h.terms, hs)) || (rc = upload(&P.d_ue, h.ents, hs)))
+      return rc;
+    P.udsc = h.s;
+    P.udsc.quads = P.d_uq;
+    P.udsc.terms = P.d_ut;
+    P.udsc.ents = P.d_ue;
+    choose_tiles(P);
+    P.tpb = 8;  // sites per wave of the udsc kernel
+    if ((rc = upload_plan(P, hs))) return rc;
+    {
+      std::lock_guard<std::mutex> lk(g_mu);
+      if ((rc = ensure(&g_ctx.ws_c, &g_ctx.ws_c_bytes, c_bytes(P)))) return rc;
+      if ((rc = ensure(&g_ctx.ws_part, &g_ctx.ws_part_bytes, part_bytes(P)))) return rc;
+    }
+    if ((rc = enqueue(P, g_ctx.ws_c, g_ctx.ws_part, out, out_precision, hs))) return rc;
+    HIPCHK(hipStreamSynchronize(hs));  // the plan's tables are freed at the end of the iteration
+    d0 += nops;
+  }
   return PLG_OK;
 }
 

@@ -36,6 +36,7 @@ struct RowDev {
   int t;
   int pa, pb;
   int job;
+  int pc;  // third propagator (udsc baryons)
 };
 
 // a finalize row: computed row, source, output base (complex elements).
@@ -47,17 +48,50 @@ struct OutRowDev {
   int sx, sy, sz;
   int nch_row;   // channels of this row (0: the launch-wide count)
   int64_t base;  // H offset of (m = 0, k = 0)
-  // result scale 2^(ka*ea[pla] + kb*eb[plb]) undoes the propagators' plane
-  // exponents (ea/eb NULL: no scaling)
+  // result scale 2^(ka*ea[pla] + kb*eb[plb] + kc*ec[plc]) undoes the
+  // propagators' plane exponents (ea/eb/ec NULL: no scaling)
   const int *ea, *eb;
   int pla, plb, ka, kb;
+  const int *ec;
+  int plc, kc;
 };
 __device__ __forceinline__ double row_scale(const OutRowDev &R) {
   int e = 0;
   if (R.ea) e += R.ka * R.ea[R.pla];
   if (R.eb) e += R.kb * R.eb[R.plb];
+  if (R.ec) e += R.kc * R.ec[R.plc];
   return ldexp(1.0, e);
 }
+
+// udsc baryons: chunked schedule of one flavour product's term tables
+constexpr int UD_MAXC = 4;      // chunks of 64 (m0m1, m2m3) quads
+constexpr int UD_MAXT = 256;    // distinct terms of a chunk
+constexpr int UD_MAXOPS = 16;   // operators of a product
+struct UdscEntry {
+  int t;          // chunk-local term
+  float re, im;   // coefficient (float, as the reference's tables)
+};
+struct UdscSched {
+  int nops, nchunks;
+  int quad_off[UD_MAXC], term_off[UD_MAXC], term_n[UD_MAXC];
+  int ent_off[UD_MAXC][UD_MAXOPS], ent_n[UD_MAXC][UD_MAXOPS];
+  const short *quads;  // m0m1 | m2m3 << 4 per lane, -1 = none
+  const short *terms;  // chunk-local quad | (m4m5) << 6, -1 = padding
+  const UdscEntry *ents;
+};
+struct UdscSchedHost {
+  UdscSched s{};
+  std::vector<short> quads, terms;
+  std::vector<UdscEntry> ents;
+};
+int udsc_schedule(int prod, UdscSchedHost &h);
+int udsc_products();
+const char *udsc_flavours(int prod);
+int udsc_nops(int prod);
+const char *udsc_op_name(int prod, int k);
+template <typename T>
+int launch_udsc(const UdscSched &S, const PropDev<T> *props, const RowDev *rows, int nrows, cx<T> *out, int64_t V3,
+                int T_lat, int nch_total, int ch0, int spw, hipStream_t st);
 
 template <typename T>
 int launch_meson(int set, const PropDev<T> *props, const RowDev *rows, int nrows, cx<T> *out,

@@ -74,16 +74,18 @@ def _perm_sign(p):
     return s
 
 
-def baryon_wick(props, sink, source, A):
+def baryon_wick(props, sink, source, A, Ab=None):
     """Spin-open baryon 2pt from Wick's theorem.
 
     Interpolator chi_mu = sum_terms w * eps_abc (q1_{nu a} A_{nu lam} q2_{lam b}) q3_{mu c}
-    with flavours (q1,q2,q3) per term; the conjugate uses Abar = g4 A^dag g4.
+    with flavours (q1,q2,q3) per term; the conjugate uses Abar = g4 A^dag g4
+    (or the given source matrix Ab).
     Every pairing of sink quark i with a source antiquark sigma(i) of the same
     flavour contributes sign(sigma) * prod_i S_f[s_i][s'_sigma(i)][c_i][c'_sigma(i)].
     props: dict flavour -> S[..., 4,4,3,3]."""
     import itertools
-    Ab = bar(A)
+    if Ab is None:
+        Ab = bar(A)
     sp, spp = 'nlm', 'qpM'      # sink spins (nu,lam,mu), source spins (nu',lam',mu')
     co, cop = 'abc', 'xyz'
     total = 0

@@ -74,6 +74,23 @@ template <typename Float> static int run(const std::string &out) {
     dump(c, out + "/mesmes_" + tag + ".bin");
   }
   {
+    // udsc baryons with the charm propagator absent: the products of u, d, s
+    PLEGMA_Propagator<Float> s, none(NONE);
+    if (sizeof(Float) == 8) {
+      s.synthesize(1234, 3, 0, 0.8);
+    } else {
+      PLEGMA_Propagator<double> s64;
+      s64.synthesize(1234, 3, 0, 0.8);
+      s.castFrom(s64);
+    }
+    PLEGMA_Correlator<Float> c(MOMENTUM_SPACE, src, 2, 6);
+    c.contractBaryonsUDSC(u, d, s, none);
+    dump(c, out + "/udsc_" + tag + ".bin");
+    FILE *f = std::fopen((out + "/udsc_" + tag + ".names").c_str(), "w");
+    for (auto &n : c.getDatasets()) std::fprintf(f, "%s\n", n.c_str());
+    std::fclose(f);
+  }
+  {
     // a propagator as a 144-component field through PLEGMA_FT::apply
     PLEGMA_FT<Float> ft(2);
     ft.apply(u, FT_GEMV, -1);

@@ -169,3 +169,17 @@ def test_hdf5_position_space(outputs, prec, tol):
         assert G.rel_err(H[j], ref, axis=(0, 1, 2)) < tol
         sh = np.roll(ref, shift=(-SRC[2], -SRC[1], -SRC[0]), axis=(0, 1, 2))
         assert G.rel_err(got[tau], sh, axis=(0, 1, 2)) < tol
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-6)])
+def test_udsc_class(outputs, prec, tol):
+    """contractBaryonsUDSC(u, d, s, absent charm): datasets of the u/d/s
+    products, H_elem [dataset][t][mom], against the oracle"""
+    from plegma_amd import gen_udsc
+    d, u, dn = outputs
+    s = O.synth_prop(L, 1234, 3, 0, 0.8)
+    names, ref = O.udsc_correlator(L, SRC, 6, 2, {"u": u, "d": dn, "s": s}, gen_udsc.products())
+    got_names = open(os.path.join(d, f"udsc_{prec}.names")).read().split()
+    assert got_names == names
+    got = _load(d, "udsc", prec).reshape(ref.shape)
+    assert G.rel_err(got, ref, axis=-1) < tol

@@ -1,0 +1,110 @@
+"""udsc baryons on the GPU (plg_baryons_udsc, the counterpart of
+contractBaryonsUDSC, lib/PLEGMA_Correlator.cu:136-188) against the CPU oracle
+(orc_udsc_field: the term sums of lib/kernels/PLEGMA_baryons_udsc.cu:86-121
+with the tables pinned to the reference's in test_udsc_cpu.py)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from plegma_amd import capi, gen_udsc
+
+import gpu_util as G
+
+pytestmark = pytest.mark.gpu
+
+L = [4, 4, 4, 8]
+SRC = [1, 2, 3, 5]
+TOTALT = 6
+Q2 = 2
+
+
+@pytest.fixture(scope="module", autouse=True)
+def ctx():
+    G.torch_cuda()
+    print("\n[gpu] plg_init", L, flush=True)
+    capi.init(L, 0)
+    yield
+    capi.finalize()
+
+
+@pytest.fixture(scope="module")
+def props(ctx):
+    out = {}
+    for i, (f, lam) in enumerate(zip("udsc", (0.5, 0.6, 0.8, 1.2))):
+        t, v = G.make_prop(L, 77, 20 + i, SRC[3], lam)
+        out[f] = (t, v, G.host(t))
+    return out
+
+
+@pytest.fixture(scope="module")
+def tables():
+    return gen_udsc.products()
+
+
+def _run(views, mode=capi.MODE_F64, space=capi.MOMENTUM_SPACE, out_prec=capi.F64, only_st=False, only_ch=False,
+         src=SRC, totalT=TOTALT):
+    torch = G.torch_cuda()
+    moms = capi.momlist(Q2) if space == capi.MOMENTUM_SPACE else []
+    n = capi.baryons_udsc_size(*views, src, totalT, len(moms), space, only_st, only_ch)
+    dt = torch.float64 if out_prec == capi.F64 else torch.float32
+    out = torch.full((max(n, 1), 2), float("nan"), dtype=dt, device="cuda")
+    print(f"[gpu] udsc mode={mode} space={space} only_st={only_st} only_ch={only_ch}", flush=True)
+    capi.baryons_udsc(*views, src, totalT, moms, mode, space, out_prec, out, only_st, only_ch)
+    torch.cuda.synchronize()
+    h = out[:n].to("cpu", torch.float64).numpy()
+    return h[:, 0] + 1j * h[:, 1]
+
+
+def _oracle(props, tables, flavours="udsc", space="momentum", only_st=False, only_ch=False, hosts=None):
+    hp = {f: (hosts or {}).get(f, props[f][2]) for f in flavours}
+    return O.udsc_correlator(L, SRC, TOTALT, Q2, hp, tables, only_st, only_ch, space)
+
+
+def test_udsc_f64(props, tables):
+    names, ref = _oracle(props, tables)
+    assert names == [n for _, ops in capi.udsc_products() for n in ops]
+    got = _run([props[f][1] for f in "udsc"]).reshape(ref.shape)
+    assert G.rel_err(got, ref, axis=-1) < 1e-12
+
+
+def test_udsc_mixed(props, tables):
+    """FP32 propagators with per-plane exponents, FP32 contraction, split-bf16 projection"""
+    names, ref = _oracle(props, tables)
+    keep = [G.cast_f32(props[f][0], props[f][1], scaled=True) for f in "udsc"]
+    got = _run([v for _, v in keep], mode=capi.MODE_MIXED).reshape(ref.shape)
+    assert G.rel_err(got, ref, axis=-1) < 1e-6
+
+
+@pytest.mark.parametrize("absent,only_st,only_ch", [("c", True, False), ("s", False, True), ("", False, True)])
+def test_udsc_selection(props, tables, absent, only_st, only_ch):
+    fl = "".join(f for f in "udsc" if f not in absent)
+    names, ref = _oracle(props, tables, fl, only_st=only_st, only_ch=only_ch)
+    views = [None if f in absent else props[f][1] for f in "udsc"]
+    got = _run(views, only_st=only_st, only_ch=only_ch)
+    assert got.size == ref.size
+    assert G.rel_err(got.reshape(ref.shape), ref, axis=-1) < 1e-12
+
+
+def test_udsc_position_space(props, tables):
+    names, ref = _oracle(props, tables, space="position", only_ch=True)
+    got = _run([props[f][1] for f in "udsc"], space=capi.POSITION_SPACE, only_ch=True).reshape(ref.shape)
+    assert G.rel_err(got, ref, axis=-1) < 1e-12
+
+
+def test_udsc_fused_prep(props, tables):
+    """rotation to the physical basis and the boundary sign applied on load,
+    against the oracle on prepared copies"""
+    hosts, views = {}, []
+    for f, sign in zip("udsc", (1, -1, 1, -1)):
+        t, v, h = props[f]
+        w = capi.plg_prop(v.data, v.precision, v.t_first, v.nt, 0, v.comp_stride)
+        w.rotate_sign = sign
+        w.bc_t0 = SRC[3]
+        views.append(w)
+        hp = h.copy().reshape(144, -1, 2)
+        O.prop_rotate(hp, L, sign)
+        O.prop_boundaries(hp, L, SRC[3])
+        hosts[f] = hp
+    names, ref = _oracle(props, tables, only_st=True, hosts=hosts)
+    got = _run(views, only_st=True).reshape(ref.shape)
+    assert G.rel_err(got, ref, axis=-1) < 1e-12
