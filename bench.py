@@ -27,6 +27,12 @@ oracle (reference per-site bodies, FP64, OpenMP) on a bounded sample.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload largest|learned]
 
+Workload "udsc" (--workload udsc): lattice 32^3 x 64, one source, four
+flavour propagators (u, d, s, c), all 20 flavour products of the udsc baryon
+interpolators (200 operators), all 64 time slices, |n|^2 <= 9 (123 momenta);
+value = sites/s of the mixed path (FP32 contraction, split-bf16 projection),
+FP64 path and the mixed path's error beside it.
+
 Workload "learned": see run_learned.
 """
 from __future__ import annotations
@@ -390,6 +396,142 @@ def run_largest(args, torch, capi, dev, stream):
     return res
 
 
+UDSC_SRC = [3, 5, 7, 10]
+UDSC_LAMBDAS = (0.5, 0.55, 0.8, 1.2)
+
+
+def udsc_work_per_site():
+    """implemented work of the udsc kernel per site, summed over the 20
+    products: complex MACs (36 per (m0m1, m2m3) quad for the colour diquark,
+    9 per distinct term, 1 per operator entry) and the compulsory bytes
+    (three propagators read per product)"""
+    from plegma_amd import gen_udsc
+    cmac = 0
+    nops = 0
+    for _, ops in gen_udsc.products():
+        terms = set(m for _, t in ops for m in t)
+        quads = set(m[:4] for m in terms)
+        cmac += 36 * len(quads) + 9 * len(terms) + sum(len(t) for _, t in ops)
+        nops += len(ops)
+    return cmac, nops
+
+
+def cpu_baseline_udsc(budget_s=12.0):
+    """Oracle udsc (the reference's per-term epsilon sums in FP64 + per-site
+    DFT, OpenMP) on an 8^3 spatial lattice; the per-site work does not depend
+    on the lattice size"""
+    from oracle import oracle as O
+    from plegma_amd import gen_udsc
+    nth = min(os.cpu_count() or 1, 16)
+    Ls = [8, 8, 8, 16]
+    V3 = Ls[0] * Ls[1] * Ls[2]
+    props = {f: O.synth_prop(Ls, SEED, i, 0, lam, nthreads=nth) for i, (f, lam) in enumerate(zip("udsc", UDSC_LAMBDAS))}
+    tables = gen_udsc.products()
+    done, used = 0, 0.0
+    for totalT in (1, 2, 4, 8, 16):
+        t0 = time.perf_counter()
+        O.udsc_correlator(Ls, [1, 2, 3, 0], totalT, Q2MAX4, props, tables, nthreads=nth)
+        used = time.perf_counter() - t0
+        done = totalT
+        if used * 2.2 > budget_s:
+            break
+    return {"value": done * V3 / used, "unit": "sites/s", "cores": nth, "kind": "port",
+            "sample": f"{done} time slice(s) x {V3} sites of an 8^3x16 lattice, 20 flavour products "
+                      f"(200 operators), 123 momenta, FP64 per-term epsilon sums + per-site DFT, {used:.1f} s"}
+
+
+def run_udsc(args, torch, capi, dev, stream):
+    capi.init(L, 0)
+    V3 = L[0] * L[1] * L[2]
+    T = L[3]
+    moms = capi.momlist(Q2MAX4)
+    log("synthesising four FP64 flavour propagators and their FP32 copies")
+    v64, v32, keep = [], [], []
+    for fl, lam in enumerate(UDSC_LAMBDAS):
+        t64 = torch.empty((144, T * V3, 2), dtype=torch.float64, device=dev)
+        v = capi.plg_prop(t64.data_ptr(), capi.F64, 0, T, 0, T * V3)
+        capi.synth_prop(v, SEED, 40 + fl, UDSC_SRC[3], lam)
+        t32 = torch.empty((144, T * V3, 2), dtype=torch.float32, device=dev)
+        e32 = torch.empty(T, dtype=torch.int32, device=dev)
+        w = capi.plg_prop(t32.data_ptr(), capi.F32, 0, T, 0, T * V3, e32.data_ptr())
+        capi.cast_prop(w, v)
+        keep += [t64, t32, e32]
+        v64.append(v)
+        v32.append(w)
+    torch.cuda.synchronize()
+    n = capi.baryons_udsc_size(*v32, UDSC_SRC, T, len(moms))
+    out_mx = torch.zeros((n, 2), dtype=torch.float64, device=dev)
+    out64 = torch.zeros((n, 2), dtype=torch.float64, device=dev)
+
+    def mixed():
+        capi.baryons_udsc(*v32, UDSC_SRC, T, moms, capi.MODE_MIXED, capi.MOMENTUM_SPACE, capi.F64, out_mx,
+                          stream=stream)
+
+    def plain():
+        capi.baryons_udsc(*v64, UDSC_SRC, T, moms, capi.MODE_F64, capi.MOMENTUM_SPACE, capi.F64, out64,
+                          stream=stream)
+
+    nsites = V3 * T
+    log(f"mixed: {nsites} sites x 20 products; warmup {args.warmup}")
+    for _ in range(args.warmup):
+        mixed()
+    ms_mx, _ = timed(torch, stream, mixed, args.steps)
+    log(f"mixed: {ms_mx:.3f} ms/step")
+    st_ms = capi.baryons_udsc_profile(*v32, UDSC_SRC, T, moms, capi.MODE_MIXED, capi.MOMENTUM_SPACE, capi.F64,
+                                      out_mx, stream=stream)
+    cmac, nops = udsc_work_per_site()
+    res = {"metric": "udsc baryon 2pt sites/s (20 flavour products, 200 operators), mixed precision",
+           "value": nsites / (ms_mx / 1e3), "unit": "sites/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_mx,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": f"f32 contraction + bf16x6 MFMA projection (fp32 acc, fp64 sums)",
+           "data": "synthetic",
+           "config": {"workload": "udsc: lattice 32^3x64, one source, u/d/s/c propagators, all 20 flavour "
+                                  "products (200 operators), 64 time slices, |n|^2<=9 (123 momenta)",
+                      "lattice": L, "sources": 1, "sites": nsites, "operators": nops, "nmom": len(moms),
+                      "parallelism": "single GPU"},
+           "stages_ms": st_ms}
+    if not args.no_fp64:
+        plain()
+        ms_64, _ = timed(torch, stream, plain, max(1, args.steps // 2))
+        st64 = capi.baryons_udsc_profile(*v64, UDSC_SRC, T, moms, capi.MODE_F64, capi.MOMENTUM_SPACE, capi.F64,
+                                         out64, stream=stream)
+        torch.cuda.synchronize()
+        a = out_mx.cpu().numpy()
+        b = out64.cpu().numpy()
+        A = (a[:, 0] + 1j * a[:, 1]).reshape(nops, T, len(moms))
+        B = (b[:, 0] + 1j * b[:, 1]).reshape(nops, T, len(moms))
+        den = np.abs(B).max(axis=2, keepdims=True)
+        E = np.abs(A - B) / np.where(den == 0, 1.0, den)
+        res["fp64_plain"] = {"value": nsites / (ms_64 / 1e3), "unit": "sites/s", "ms_per_step": ms_64,
+                             "stages_ms": st64}
+        res["speedup_vs_fp64"] = res["value"] / res["fp64_plain"]["value"]
+        res["rel_err_vs_fp64"] = {"max": float(E.max()), "median": float(np.median(E)),
+                                  "definition": "max over operators, time slices and momenta of "
+                                                "|C_mixed - C_fp64| / max_momenta |C_fp64|"}
+    dom = max(st_ms, key=st_ms.get)
+    t = st_ms[dom] / 1e3
+    prop_bytes = 20 * 3 * 144 * 8
+    byts = nsites * (prop_bytes + nops * 8)
+    r = {"kernel": dom, "bound": "hbm", "achieved": byts / t / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "traffic": None,
+         "algorithmic": f"20 products x 3 x 1152 B FP32 propagator reads + {nops} x 8 B site correlator "
+                        f"writes per site",
+         "valu": {"achieved": nsites * 8 * cmac / t / 1e12, "peak": FP32_VALU_PEAK_TFS, "unit": "TFLOP/s",
+                  "flops_per_site": 8 * cmac}}
+    r["frac"] = r["achieved"] / HBM_PEAK_GBS
+    r["valu"]["frac"] = r["valu"]["achieved"] / FP32_VALU_PEAK_TFS
+    tr = measured_traffic("udsc", dom)
+    if tr:
+        r["traffic"] = tr[0]
+        r["traffic_source"] = f"rocprofv3 FETCH_SIZE/WRITE_SIZE of {tr[1]} (profiles/traffic.json)"
+    res["roofline"] = r
+    if not args.no_cpu:
+        log("cpu baseline")
+        res["cpu_baseline"] = cpu_baseline_udsc()
+    return res
+
+
 def run_learned(args, torch, capi, dev, stream):
     """32^3 x 64, 256 sources with propagators base + 5% perturbation: exact
     two-point results (16 mesons + nucleon, |n|^2 <= 16) of every source by the
@@ -491,7 +633,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--split", type=int, default=6)
-    ap.add_argument("--workload", choices=["batching", "largest", "learned"], default="batching")
+    ap.add_argument("--workload", choices=["batching", "largest", "learned", "udsc"], default="batching")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
@@ -517,7 +659,7 @@ def main():
         torch.cuda.synchronize()
         del a, b
     stream = torch.cuda.Stream()
-    run = {"largest": run_largest, "learned": run_learned}.get(args.workload, run_batching)
+    run = {"largest": run_largest, "learned": run_learned, "udsc": run_udsc}.get(args.workload, run_batching)
     res = run(args, torch, capi, dev, stream)
     capi.finalize()
     print(json.dumps(res), flush=True)

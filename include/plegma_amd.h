@@ -153,6 +153,12 @@ int plg_baryons_udsc_size(const plg_prop *up, const plg_prop *dn, const plg_prop
 int plg_baryons_udsc(const plg_prop *up, const plg_prop *dn, const plg_prop *st, const plg_prop *ch, int only_st,
                      int only_ch, const int source[4], int totalT, const int *moms, int nmom, int mode, int space,
                      int out_precision, void *out, void *stream);
+/* the same, with HIP events between the stages of every product: nstages
+ * stage names (32 bytes each) and their milliseconds summed over the products */
+int plg_baryons_udsc_profile(const plg_prop *up, const plg_prop *dn, const plg_prop *st, const plg_prop *ch,
+                             int only_st, int only_ch, const int source[4], int totalT, const int *moms, int nmom,
+                             int mode, int space, int out_precision, void *out, void *stream, int cap, int *nstages,
+                             double *ms, char *names);
 
 /* Standalone momentum projection of a field, replacing PLEGMA_FT::apply with
  * FT_GEMV (lib/PLEGMA_FT.cu:144-162, lib/kernels/PLEGMA_FT.cuh:74-100):

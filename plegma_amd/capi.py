@@ -33,7 +33,8 @@ EXPORTS = ["plg_init", "plg_finalize", "plg_last_error", "plg_version", "plg_mom
            "plg_mlp_create", "plg_mlp_nparams", "plg_mlp_set_params", "plg_mlp_get_params",
            "plg_mlp_forward", "plg_mlp_gradient", "plg_mlp_train", "plg_mlp_destroy", "plg_ft_apply",
            "plg_prop_rotate_physical", "plg_prop_apply_boundaries", "plg_udsc_count", "plg_udsc_product",
-           "plg_udsc_op_name", "plg_baryons_udsc_select", "plg_baryons_udsc_size", "plg_baryons_udsc"]
+           "plg_udsc_op_name", "plg_baryons_udsc_select", "plg_baryons_udsc_size", "plg_baryons_udsc",
+           "plg_baryons_udsc_profile"]
 
 
 class PlgError(RuntimeError):
@@ -101,6 +102,9 @@ def lib():
                                             C.POINTER(C.c_int64)]
         L.plg_baryons_udsc.argtypes = [pp, pp, pp, pp, C.c_int, C.c_int, ip, C.c_int, ip, C.c_int, C.c_int,
                                        C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        L.plg_baryons_udsc_profile.argtypes = [pp, pp, pp, pp, C.c_int, C.c_int, ip, C.c_int, ip, C.c_int,
+                                               C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, ip,
+                                               C.POINTER(C.c_double), C.c_char_p]
         L.plg_mlp_create.argtypes = [ip, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         L.plg_mlp_nparams.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         L.plg_mlp_set_params.argtypes = [C.c_void_p, fp]
@@ -225,6 +229,21 @@ def baryons_udsc(up, dn, st, ch, source, totalT, moms, mode, space, out_prec, ou
     check(lib().plg_baryons_udsc(_ref(up), _ref(dn), _ref(st), _ref(ch), int(only_st), int(only_ch),
                                  ivec(source), totalT, ivec(flat) if flat else None, len(moms), mode, space,
                                  out_prec, C.c_void_p(out.data_ptr()), _stream(stream)))
+
+
+def baryons_udsc_profile(up, dn, st, ch, source, totalT, moms, mode, space, out_prec, out, only_st=False,
+                         only_ch=False, stream=None, cap=16):
+    """one baryons_udsc pass with events between stages: {stage: ms summed over products}"""
+    flat = [int(v) for m in moms for v in m]
+    n = C.c_int()
+    ms = (C.c_double * cap)()
+    names = C.create_string_buffer(32 * cap)
+    check(lib().plg_baryons_udsc_profile(_ref(up), _ref(dn), _ref(st), _ref(ch), int(only_st), int(only_ch),
+                                         ivec(source), totalT, ivec(flat) if flat else None, len(moms), mode, space,
+                                         out_prec, C.c_void_p(out.data_ptr()), _stream(stream), cap, C.byref(n), ms,
+                                         names))
+    raw = names.raw
+    return {raw[32 * i:32 * i + 32].split(b"\0")[0].decode(): ms[i] for i in range(min(n.value, cap))}
 
 
 def baryons_udsc_size(up, dn, st, ch, source, totalT, nmom, space=MOMENTUM_SPACE, only_st=False, only_ch=False):

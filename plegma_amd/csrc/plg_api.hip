@@ -198,8 +198,10 @@ struct Plan {
   uint16_t *d_pidx = nullptr;  // phase indices of the mixed projection
   // udsc baryons: the flavour product's schedule (device arrays owned here)
   UdscSched udsc{};
-  short *d_uq = nullptr, *d_ut = nullptr;
-  UdscEntry *d_ue = nullptr;
+  short *d_uq = nullptr;
+  signed char *d_ut = nullptr;
+  uint16_t *d_ue = nullptr;
+  float *d_uc = nullptr;
   // batch-owned workspace (single calls use the context workspace)
   void *c = nullptr, *part = nullptr;
   size_t c_bytes = 0, part_bytes = 0;
@@ -216,7 +218,7 @@ struct Plan {
     if (cap_stream) hipStreamDestroy(cap_stream);
     for (void *p : {(void *)d_props, (void *)d_rows, (void *)d_orows, (void *)d_csel, (void *)d_choff, (void *)d_ms,
                     (void *)d_pair_ms, (void *)d_mom_pair, (void *)d_pidx, (void *)d_uq, (void *)d_ut,
-                    (void *)d_ue})
+                    (void *)d_ue, (void *)d_uc})
       if (p) hipFree(p);
     if (own_ws) {
       if (c) hipFree(c);
@@ -1010,9 +1012,10 @@ int plg_baryons_udsc_size(const plg_prop *up, const plg_prop *dn, const plg_prop
   return PLG_OK;
 }
 
-int plg_baryons_udsc(const plg_prop *up, const plg_prop *dn, const plg_prop *st, const plg_prop *ch, int only_st,
-                     int only_ch, const int source[4], int totalT, const int *moms, int nmom, int mode, int space,
-                     int out_precision, void *out, void *stream) {
+// stage_ms (optional): summed per-stage milliseconds over the products
+static int udsc_run(const plg_prop *up, const plg_prop *dn, const plg_prop *st, const plg_prop *ch, int only_st,
+                    int only_ch, const int source[4], int totalT, const int *moms, int nmom, int mode, int space,
+                    int out_precision, void *out, void *stream, std::map<std::string, double> *stage_ms) {
   if (!g_ctx.init) return fail(PLG_ERR_NOT_INIT, "plg_init not called");
   if (!out || !source) return fail(PLG_ERR_INVALID, "NULL argument");
   if (out_precision != PLG_F64 && out_precision != PLG_F32) return fail(PLG_ERR_INVALID, "bad out precision");
@@ -1078,23 +1081,65 @@ int plg_baryons_udsc(const plg_prop *up, const plg_prop *dn, const plg_prop *st,
     P.launches.push_back({Launch::UDSC, prod, false, false, 0, 0});
     UdscSchedHost h;
     if ((rc = udsc_schedule(prod, h))) return fail(rc, "udsc schedule");
-    if ((rc = upload(&P.d_uq, h.quads, hs)) || (rc = upload(&P.d_ut, h.terms, hs)) || (rc = upload(&P.d_ue, h.ents, hs)))
+    if ((rc = upload(&P.d_uq, h.quads, hs)) || (rc = upload(&P.d_ut, h.tp, hs)) ||
+        (rc = upload(&P.d_ue, h.ents, hs)) || (rc = upload(&P.d_uc, h.coef, hs)))
       return rc;
     P.udsc = h.s;
     P.udsc.quads = P.d_uq;
-    P.udsc.terms = P.d_ut;
+    P.udsc.tp = P.d_ut;
     P.udsc.ents = P.d_ue;
+    P.udsc.coef = P.d_uc;
     choose_tiles(P);
-    P.tpb = 8;  // sites per wave of the udsc kernel
+    P.tpb = 32;  // sites per wave of the udsc kernel
     if ((rc = upload_plan(P, hs))) return rc;
     {
       std::lock_guard<std::mutex> lk(g_mu);
       if ((rc = ensure(&g_ctx.ws_c, &g_ctx.ws_c_bytes, c_bytes(P)))) return rc;
       if ((rc = ensure(&g_ctx.ws_part, &g_ctx.ws_part_bytes, part_bytes(P)))) return rc;
     }
-    if ((rc = enqueue(P, g_ctx.ws_c, g_ctx.ws_part, out, out_precision, hs))) return rc;
-    HIPCHK(hipStreamSynchronize(hs));  // the plan's tables are freed at the end of the iteration
+    StageLog log;
+    rc = enqueue(P, g_ctx.ws_c, g_ctx.ws_part, out, out_precision, hs, stage_ms ? &log : nullptr);
+    // the plan's tables are freed at the end of the iteration
+    if (!rc && hipStreamSynchronize(hs) != hipSuccess) rc = fail(PLG_ERR_HIP, "udsc sync failed");
+    for (size_t i = 0; i + 1 < log.ev.size(); i++) {
+      float t = 0;
+      if (!rc) hipEventElapsedTime(&t, log.ev[i], log.ev[i + 1]);
+      (*stage_ms)[log.name[i + 1]] += t;
+    }
+    for (hipEvent_t e : log.ev) hipEventDestroy(e);
+    if (rc) return rc;
     d0 += nops;
+  }
+  return PLG_OK;
+}
+
+int plg_baryons_udsc(const plg_prop *up, const plg_prop *dn, const plg_prop *st, const plg_prop *ch, int only_st,
+                     int only_ch, const int source[4], int totalT, const int *moms, int nmom, int mode, int space,
+                     int out_precision, void *out, void *stream) {
+  return udsc_run(up, dn, st, ch, only_st, only_ch, source, totalT, moms, nmom, mode, space, out_precision, out,
+                  stream, nullptr);
+}
+
+int plg_baryons_udsc_profile(const plg_prop *up, const plg_prop *dn, const plg_prop *st, const plg_prop *ch,
+                             int only_st, int only_ch, const int source[4], int totalT, const int *moms, int nmom,
+                             int mode, int space, int out_precision, void *out, void *stream, int cap, int *nstages,
+                             double *ms, char *names) {
+  if (!nstages) return fail(PLG_ERR_INVALID, "NULL argument");
+  std::map<std::string, double> sm;
+  const int rc = udsc_run(up, dn, st, ch, only_st, only_ch, source, totalT, moms, nmom, mode, space, out_precision,
+                          out, stream, &sm);
+  if (rc) return rc;
+  *nstages = (int)sm.size();
+  int i = 0;
+  for (auto &kv : sm) {
+    if (i < cap) {
+      if (ms) ms[i] = kv.second;
+      if (names) {
+        std::memset(names + 32 * i, 0, 32);
+        std::strncpy(names + 32 * i, kv.first.c_str(), 31);
+      }
+    }
+    i++;
   }
   return PLG_OK;
 }

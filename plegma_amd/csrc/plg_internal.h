@@ -63,26 +63,30 @@ __device__ __forceinline__ double row_scale(const OutRowDev &R) {
   return ldexp(1.0, e);
 }
 
-// udsc baryons: chunked schedule of one flavour product's term tables
-constexpr int UD_MAXC = 4;      // chunks of 64 (m0m1, m2m3) quads
-constexpr int UD_MAXT = 256;    // distinct terms of a chunk
-constexpr int UD_MAXOPS = 16;   // operators of a product
-struct UdscEntry {
-  int t;          // chunk-local term
-  float re, im;   // coefficient (float, as the reference's tables)
-};
+// udsc baryons: chunked schedule of one flavour product's term tables.
+// Chunk c holds up to 64 (m0m1, m2m3) quads, one per lane; lane l's quad has
+// up to UD_TPQ distinct terms, kept in the chunk-local slots l*UD_TPQ + e.
+constexpr int UD_MAXC = 4;        // chunks
+constexpr int UD_TPQ = 4;         // distinct terms per quad
+constexpr int UD_MAXOPS = 16;     // operators of a product
+constexpr int UD_MAXENT = 2816;   // operator entries of a product (padded)
+constexpr int UD_MAXCOEF = 64;    // distinct coefficients of a product
 struct UdscSched {
-  int nops, nchunks;
-  int quad_off[UD_MAXC], term_off[UD_MAXC], term_n[UD_MAXC];
-  int ent_off[UD_MAXC][UD_MAXOPS], ent_n[UD_MAXC][UD_MAXOPS];
-  const short *quads;  // m0m1 | m2m3 << 4 per lane, -1 = none
-  const short *terms;  // chunk-local quad | (m4m5) << 6, -1 = padding
-  const UdscEntry *ents;
+  int nops, nchunks, nent, ncoef;
+  // entries of operator k in chunk c: four sublists of ent_len entries (a
+  // multiple of 4) from ent_off (a multiple of 4), one per lane of the operator
+  int ent_off[UD_MAXC][UD_MAXOPS], ent_len[UD_MAXC][UD_MAXOPS];
+  const short *quads;       // [chunk][lane]: m0m1 | m2m3 << 4, -1 = none
+  const signed char *tp;    // [chunk][lane][UD_TPQ]: m4m5 of each term, -1 = none
+  const uint16_t *ents;     // chunk-local term slot | coefficient index << 8
+  const float *coef;        // [ncoef][2] coefficients (float, as the reference's tables); 0 is zero
 };
 struct UdscSchedHost {
   UdscSched s{};
-  std::vector<short> quads, terms;
-  std::vector<UdscEntry> ents;
+  std::vector<short> quads;
+  std::vector<signed char> tp;
+  std::vector<uint16_t> ents;
+  std::vector<float> coef;
 };
 int udsc_schedule(int prod, UdscSchedHost &h);
 int udsc_products();
