@@ -1090,7 +1090,7 @@ static int udsc_run(const plg_prop *up, const plg_prop *dn, const plg_prop *st, 
     P.udsc.ents = P.d_ue;
     P.udsc.coef = P.d_uc;
     choose_tiles(P);
-    P.tpb = 32;  // sites per wave of the udsc kernel
+    P.tpb = 64;  // sites per workgroup of the udsc kernel
     if ((rc = upload_plan(P, hs))) return rc;
     {
       std::lock_guard<std::mutex> lk(g_mu);

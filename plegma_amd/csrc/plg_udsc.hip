@@ -16,8 +16,8 @@
 // registers and the (at most four) distinct terms of its quad, which go to
 // LDS slots; then each operator sums its entries over those slots (4 lanes per
 // operator).  D costs 36 complex products per quad, a term 9, an entry 1.
-// The next site's propagator entries are loaded into registers while the
-// current site is contracted.
+// A workgroup stages whole 128-byte lines of the propagators (16 FP32 or 8
+// FP64 consecutive sites), so every line is read from HBM once.
 #include "plg_internal.h"
 #include "plg_udsc_tables.h"
 
@@ -119,16 +119,31 @@ int udsc_nops(int prod) { return UDSC_PRODS[prod].nops; }
 const char *udsc_op_name(int prod, int k) { return UDSC_OPS[UDSC_PRODS[prod].first_op + k].name; }
 
 // ------------------------------------------------------------------ kernel
-// one wave per workgroup; the wave walks `spw` consecutive sites of one row
+// A workgroup of 4 waves stages the three propagators of G consecutive sites
+// (one 128-byte line of every component) into LDS with coalesced loads; each
+// wave then contracts G/4 of those sites, one at a time.
+// LDS written and read by lanes of one wave only: a wavefront-scope fence
+// orders the accesses (a wave's LDS operations are performed in order)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <typename T> struct UdscGeom {
+  static constexpr int G = 128 / (int)sizeof(cx<T>);  // sites per line
+  static constexpr int SPAD = 433;                    // LDS stride of a site (432 + 1)
+};
+
 template <typename T>
-__global__ __launch_bounds__(64) void k_udsc(UdscSched S, const PropDev<T> *__restrict__ props,
-                                             const RowDev *__restrict__ rows, cx<T> *__restrict__ out, int64_t V3,
-                                             int T_lat, int nch_total, int ch0, int spw) {
-  __shared__ cx<T> sp[3 * 144];           // the site's propagators [f][m01][c c']
-  __shared__ cx<T> sterm[64 * UD_TPQ];    // the chunk's distinct terms
-  __shared__ __align__(16) uint16_t sent[UD_MAXENT];    // operator entries
+__global__ __launch_bounds__(256) void k_udsc(UdscSched S, const PropDev<T> *__restrict__ props,
+                                              const RowDev *__restrict__ rows, cx<T> *__restrict__ out, int64_t V3,
+                                              int T_lat, int nch_total, int ch0, int groups_per_block) {
+  constexpr int G = UdscGeom<T>::G, SPAD = UdscGeom<T>::SPAD;
+  __shared__ cx<T> sp[G * SPAD];              // [site][f][m01][c c']
+  __shared__ cx<T> sterm[4][64 * UD_TPQ];     // each wave's chunk terms
+  __shared__ __align__(16) uint16_t sent[UD_MAXENT];
   __shared__ float scoef[2 * UD_MAXCOEF];
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const RowDev row = rows[blockIdx.y];
   const PropDev<T> P0 = props[row.pa], P1 = props[row.pb], P2 = props[row.pc];
   auto plane_base = [&](const PropDev<T> &Q) {
@@ -136,13 +151,13 @@ __global__ __launch_bounds__(64) void k_udsc(UdscSched S, const PropDev<T> *__re
     plane += plane < 0 ? T_lat : 0;
     return Q.data + (int64_t)plane * V3;
   };
+  const cx<T> *b0 = plane_base(P0), *b1 = plane_base(P1), *b2 = plane_base(P2);
   const T sg = prop_sign(P0, row.t) * prop_sign(P1, row.t) * prop_sign(P2, row.t);
   const int rot_any = P0.rot | P1.rot | P2.rot;
   const int64_t ncols = (int64_t)gridDim.y * nch_total, col0 = (int64_t)blockIdx.y * nch_total + ch0;
   const int kop = lane >> 2, rr = lane & 3;
   const bool op_lane = kop < S.nops;
-  // the lane's schedule in registers; the entries and coefficients in LDS:
-  // no memory loads but the propagators inside the site loop
+  // the lane's schedule in registers; the entries and coefficients in LDS
   int qv[UD_MAXC], tpv[UD_MAXC], eoff[UD_MAXC], en[UD_MAXC];
 #pragma unroll
   for (int c = 0; c < UD_MAXC; c++) {
@@ -152,117 +167,116 @@ __global__ __launch_bounds__(64) void k_udsc(UdscSched S, const PropDev<T> *__re
     en[c] = live && op_lane ? S.ent_len[c][kop] : 0;
     eoff[c] = live && op_lane ? S.ent_off[c][kop] + rr * en[c] : 0;
   }
-  for (int i = lane; i < S.nent; i += 64) sent[i] = S.ents[i];
-  for (int i = lane; i < 2 * S.ncoef; i += 64) scoef[i] = S.coef[i];
-  // this lane's 7 staging entries e = lane + 64 k: propagator f, component comp
-  const cx<T> *src[7];
-#pragma unroll
-  for (int k = 0; k < 7; k++) {
-    const int e = min(lane + 64 * k, 431);
-    const int f = e / 144, comp = e - f * 144;
-    const PropDev<T> &Q = f == 0 ? P0 : (f == 1 ? P1 : P2);
-    src[k] = plane_base(Q) + (int64_t)comp * Q.cs;
-  }
-  const int64_t x0 = (int64_t)blockIdx.x * spw;
-  const int nsite = (int)min((int64_t)spw, V3 - x0);
-  cx<T> pre[7];
-#pragma unroll
-  for (int k = 0; k < 7; k++) pre[k] = gload(src[k] + x0);
+  for (int i = tid; i < S.nent; i += 256) sent[i] = S.ents[i];
+  for (int i = tid; i < 2 * S.ncoef; i += 256) scoef[i] = S.coef[i];
+  cx<T> *st = sterm[wave];
 
-  for (int i = 0; i < nsite; i++) {
-    const int64_t x = x0 + i;
-#pragma unroll
-    for (int k = 0; k < 7; k++)
-      if (k < 6 || lane < 48) sp[lane + 64 * k] = pre[k];
-    // the next site's entries, in flight during this site's contraction
-    if (i + 1 < nsite) {
-#pragma unroll
-      for (int k = 0; k < 7; k++) pre[k] = gload(src[k] + x + 1);
+  for (int gi = 0; gi < groups_per_block; gi++) {
+    const int64_t x0 = ((int64_t)blockIdx.x * groups_per_block + gi) * G;
+    if (x0 >= V3) break;  // uniform over the workgroup
+    __syncthreads();      // the previous group's sites are done
+    // stage: element idx = comp * G + s, lanes of a component read G consecutive sites
+    for (int idx = tid; idx < 432 * G; idx += 256) {
+      const int comp = idx / G, s = idx - comp * G;
+      const int f = comp / 144, cl = comp - f * 144;
+      const cx<T> *b = f == 0 ? b0 : (f == 1 ? b1 : b2);
+      const int64_t cs = f == 0 ? P0.cs : (f == 1 ? P1.cs : P2.cs);
+      const int64_t x = x0 + s;
+      sp[s * SPAD + comp] = x < V3 ? gload(b + (int64_t)cl * cs + x) : cmake<T>(0, 0);
     }
     __syncthreads();
-    if (rot_any) {
-      if (lane < 27) {  // rotation to the physical basis, per (propagator, colour pair)
-        const int f = lane / 9, cc = lane % 9;
+    if (rot_any) {  // rotation to the physical basis, per (site, propagator, colour pair)
+      for (int k = tid; k < G * 27; k += 256) {
+        const int s = k / 27, f = (k % 27) / 9, cc = k % 9;
         const int rot = f == 0 ? P0.rot : (f == 1 ? P1.rot : P2.rot);
         if (rot) {
-          cx<T> s[16];
+          cx<T> *b = sp + s * SPAD + f * 144 + cc;
+          cx<T> v[16];
 #pragma unroll
-          for (int m = 0; m < 16; m++) s[m] = sp[f * 144 + m * 9 + cc];
-          rot_spins(s, rot);
+          for (int m = 0; m < 16; m++) v[m] = b[m * 9];
+          rot_spins(v, rot);
 #pragma unroll
-          for (int m = 0; m < 16; m++) sp[f * 144 + m * 9 + cc] = s[m];
+          for (int m = 0; m < 16; m++) b[m * 9] = v[m];
         }
       }
       __syncthreads();
     }
-    cx<T> acc = cmake<T>(0, 0);
+    for (int s = wave; s < G; s += 4) {
+      const cx<T> *spx = sp + s * SPAD;
+      cx<T> acc = cmake<T>(0, 0);
 #pragma unroll
-    for (int c = 0; c < UD_MAXC; c++) {
-      if (c >= S.nchunks) break;
-      const int q = qv[c];
-      if (q >= 0) {
-        // D of this lane's quad, in registers
-        const cx<T> *A = sp + (q & 15) * 9, *B = sp + 144 + (q >> 4) * 9;
-        cx<T> a[9], b[9], d[9];
+      for (int c = 0; c < UD_MAXC; c++) {
+        if (c >= S.nchunks) break;
+        const int q = qv[c];
+        if (q >= 0) {
+          // D of this lane's quad, in registers
+          const cx<T> *A = spx + (q & 15) * 9, *B = spx + 144 + (q >> 4) * 9;
+          cx<T> a[9], b[9], d[9];
 #pragma unroll
-        for (int j = 0; j < 9; j++) {
-          a[j] = A[j];
-          b[j] = B[j];
-        }
+          for (int j = 0; j < 9; j++) {
+            a[j] = A[j];
+            b[j] = B[j];
+          }
 #pragma unroll
-        for (int cc = 0; cc < 3; cc++) {
-          const int a1 = (cc + 1) % 3, a2 = (cc + 2) % 3;
+          for (int cc = 0; cc < 3; cc++) {
+            const int a1 = (cc + 1) % 3, a2 = (cc + 2) % 3;
 #pragma unroll
-          for (int dd = 0; dd < 3; dd++) {
-            const int b1 = (dd + 1) % 3, b2 = (dd + 2) % 3;
-            cx<T> v = a[a1 * 3 + b1] * b[a2 * 3 + b2];
-            v = v - a[a1 * 3 + b2] * b[a2 * 3 + b1];
-            v = v - a[a2 * 3 + b1] * b[a1 * 3 + b2];
-            cfma(v, a[a2 * 3 + b2], b[a1 * 3 + b1]);
-            d[cc * 3 + dd] = v;
+            for (int dd = 0; dd < 3; dd++) {
+              const int e1 = (dd + 1) % 3, e2 = (dd + 2) % 3;
+              cx<T> v = a[a1 * 3 + e1] * b[a2 * 3 + e2];
+              v = v - a[a1 * 3 + e2] * b[a2 * 3 + e1];
+              v = v - a[a2 * 3 + e1] * b[a1 * 3 + e2];
+              cfma(v, a[a2 * 3 + e2], b[a1 * 3 + e1]);
+              d[cc * 3 + dd] = v;
+            }
+          }
+          // its distinct terms T = sum_cc' D[c][c'] S3[m4m5][c][c']
+#pragma unroll
+          for (int e = 0; e < UD_TPQ; e++) {
+            const int p = (signed char)(tpv[c] >> (8 * e));
+            if (p >= 0) {
+              const cx<T> *C3 = spx + 288 + p * 9;
+              cx<T> v = cmake<T>(0, 0);
+#pragma unroll
+              for (int j = 0; j < 9; j++) cfma(v, d[j], C3[j]);
+              st[lane * UD_TPQ + e] = v;
+            }
           }
         }
-        // its distinct terms T = sum_cc' D[c][c'] S3[m4m5][c][c']
+        wave_sync();
+        // operator entries: 4 lanes per operator, each a quarter of the list,
+        // four entries per LDS read
+        for (int e = 0; e < en[c]; e += 4) {
+          const uint2 w = *(const uint2 *)(sent + eoff[c] + e);
+          const int u[4] = {(int)(w.x & 0xffff), (int)(w.x >> 16), (int)(w.y & 0xffff), (int)(w.y >> 16)};
 #pragma unroll
-        for (int e = 0; e < UD_TPQ; e++) {
-          const int p = (signed char)(tpv[c] >> (8 * e));
-          if (p >= 0) {
-            const cx<T> *C3 = sp + 288 + p * 9;
-            cx<T> v = cmake<T>(0, 0);
-#pragma unroll
-            for (int j = 0; j < 9; j++) cfma(v, d[j], C3[j]);
-            sterm[lane * UD_TPQ + e] = v;
+          for (int j = 0; j < 4; j++) {
+            const int ci = u[j] >> 8;
+            cfma(acc, cmake<T>((T)scoef[2 * ci], (T)scoef[2 * ci + 1]), st[u[j] & 255]);
           }
         }
+        wave_sync();
       }
-      __syncthreads();
-      // operator entries: 4 lanes per operator, each a quarter of the list,
-      // four entries per LDS read
-      for (int e = 0; e < en[c]; e += 4) {
-        const uint2 w = *(const uint2 *)(sent + eoff[c] + e);
-        const int u[4] = {(int)(w.x & 0xffff), (int)(w.x >> 16), (int)(w.y & 0xffff), (int)(w.y >> 16)};
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          const int ci = u[j] >> 8;
-          cfma(acc, cmake<T>((T)scoef[2 * ci], (T)scoef[2 * ci + 1]), sterm[u[j] & 255]);
-        }
-      }
-      __syncthreads();
+      acc.re += __shfl_xor(acc.re, 1);
+      acc.im += __shfl_xor(acc.im, 1);
+      acc.re += __shfl_xor(acc.re, 2);
+      acc.im += __shfl_xor(acc.im, 2);
+      const int64_t x = x0 + s;
+      if (rr == 0 && op_lane && x < V3)
+        gstore(out + ctile_idx(col0 + kop, x, ncols), cmake<T>(sg * acc.re, sg * acc.im));
     }
-    acc.re += __shfl_xor(acc.re, 1);
-    acc.im += __shfl_xor(acc.im, 1);
-    acc.re += __shfl_xor(acc.re, 2);
-    acc.im += __shfl_xor(acc.im, 2);
-    if (rr == 0 && op_lane) gstore(out + ctile_idx(col0 + kop, x, ncols), cmake<T>(sg * acc.re, sg * acc.im));
   }
 }
 
 template <typename T>
 int launch_udsc(const UdscSched &S, const PropDev<T> *props, const RowDev *rows, int nrows, cx<T> *out, int64_t V3,
                 int T_lat, int nch_total, int ch0, int spw, hipStream_t st) {
-  spw = std::max(1, spw);
-  dim3 block(64), grid((unsigned)((V3 + spw - 1) / spw), (unsigned)nrows);
-  k_udsc<T><<<grid, block, 0, st>>>(S, props, rows, out, V3, T_lat, nch_total, ch0, spw);
+  // spw: sites per workgroup, rounded to whole groups
+  constexpr int G = UdscGeom<T>::G;
+  const int gpb = std::max(1, spw / G);
+  const int64_t ngroups = (V3 + G - 1) / G;
+  dim3 block(256), grid((unsigned)((ngroups + gpb - 1) / gpb), (unsigned)nrows);
+  k_udsc<T><<<grid, block, 0, st>>>(S, props, rows, out, V3, T_lat, nch_total, ch0, gpb);
   return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;
 }
 
