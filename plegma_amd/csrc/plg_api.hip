@@ -44,6 +44,9 @@ struct Context {
   int M = 1;                     // lcm(Lx,Ly,Lz)
   double *d_cos = nullptr, *d_sin = nullptr;
   uint32_t *d_wtab = nullptr;    // [M][6] packed bf16 operand pairs (see plg_init)
+  // udsc baryon schedules on the device, built on first use per product
+  std::map<int, UdscSched> udsc;
+  std::vector<void *> udsc_mem;
   // grow-only workspace
   void *ws_c = nullptr;
   size_t ws_c_bytes = 0;
@@ -196,12 +199,8 @@ struct Plan {
   int64_t *d_choff = nullptr;
   int *d_ms = nullptr, *d_pair_ms = nullptr, *d_mom_pair = nullptr;
   uint16_t *d_pidx = nullptr;  // phase indices of the mixed projection
-  // udsc baryons: the flavour product's schedule (device arrays owned here)
+  // udsc baryons: the flavour product's schedule (device arrays owned by the context)
   UdscSched udsc{};
-  short *d_uq = nullptr;
-  signed char *d_ut = nullptr;
-  uint16_t *d_ue = nullptr;
-  float *d_uc = nullptr;
   // batch-owned workspace (single calls use the context workspace)
   void *c = nullptr, *part = nullptr;
   size_t c_bytes = 0, part_bytes = 0;
@@ -217,8 +216,7 @@ struct Plan {
     if (graph) hipGraphDestroy(graph);
     if (cap_stream) hipStreamDestroy(cap_stream);
     for (void *p : {(void *)d_props, (void *)d_rows, (void *)d_orows, (void *)d_csel, (void *)d_choff, (void *)d_ms,
-                    (void *)d_pair_ms, (void *)d_mom_pair, (void *)d_pidx, (void *)d_uq, (void *)d_ut,
-                    (void *)d_ue, (void *)d_uc})
+                    (void *)d_pair_ms, (void *)d_mom_pair, (void *)d_pidx})
       if (p) hipFree(p);
     if (own_ws) {
       if (c) hipFree(c);
@@ -855,6 +853,7 @@ int plg_finalize(void) {
   hipDeviceSynchronize();
   for (void *p : {(void *)g_ctx.d_cos, (void *)g_ctx.d_sin, (void *)g_ctx.d_wtab, g_ctx.ws_c, g_ctx.ws_part})
     if (p) hipFree(p);
+  for (void *p : g_ctx.udsc_mem) hipFree(p);
   g_ctx = Context();
   return PLG_OK;
 }
@@ -1012,6 +1011,38 @@ int plg_baryons_udsc_size(const plg_prop *up, const plg_prop *dn, const plg_prop
   return PLG_OK;
 }
 
+// the device schedule of a flavour product, uploaded once per context
+static int udsc_device_schedule(int prod, UdscSched &out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_ctx.udsc.find(prod);
+  if (it != g_ctx.udsc.end()) {
+    out = it->second;
+    return PLG_OK;
+  }
+  UdscSchedHost h;
+  int rc = udsc_schedule(prod, h);
+  if (rc) return fail(rc, "udsc schedule");
+  short *q = nullptr;
+  signed char *t = nullptr;
+  uint16_t *e = nullptr;
+  float *c = nullptr;
+  if ((rc = upload(&q, h.quads, nullptr))) return rc;
+  g_ctx.udsc_mem.push_back(q);
+  if ((rc = upload(&t, h.tp, nullptr))) return rc;
+  g_ctx.udsc_mem.push_back(t);
+  if ((rc = upload(&e, h.ents, nullptr))) return rc;
+  g_ctx.udsc_mem.push_back(e);
+  if ((rc = upload(&c, h.coef, nullptr))) return rc;
+  g_ctx.udsc_mem.push_back(c);
+  h.s.quads = q;
+  h.s.tp = t;
+  h.s.ents = e;
+  h.s.coef = c;
+  g_ctx.udsc[prod] = h.s;
+  out = h.s;
+  return PLG_OK;
+}
+
 // stage_ms (optional): summed per-stage milliseconds over the products
 static int udsc_run(const plg_prop *up, const plg_prop *dn, const plg_prop *st, const plg_prop *ch, int only_st,
                     int only_ch, const int source[4], int totalT, const int *moms, int nmom, int mode, int space,
@@ -1079,16 +1110,7 @@ static int udsc_run(const plg_prop *up, const plg_prop *dn, const plg_prop *st, 
     P.out_elems = (int64_t)nd_total * lt * inner;
     P.nsites = (int64_t)lt * g_ctx.V3;
     P.launches.push_back({Launch::UDSC, prod, false, false, 0, 0});
-    UdscSchedHost h;
-    if ((rc = udsc_schedule(prod, h))) return fail(rc, "udsc schedule");
-    if ((rc = upload(&P.d_uq, h.quads, hs)) || (rc = upload(&P.d_ut, h.tp, hs)) ||
-        (rc = upload(&P.d_ue, h.ents, hs)) || (rc = upload(&P.d_uc, h.coef, hs)))
-      return rc;
-    P.udsc = h.s;
-    P.udsc.quads = P.d_uq;
-    P.udsc.tp = P.d_ut;
-    P.udsc.ents = P.d_ue;
-    P.udsc.coef = P.d_uc;
+    if ((rc = udsc_device_schedule(prod, P.udsc))) return rc;
     choose_tiles(P);
     P.tpb = 64;  // sites per workgroup of the udsc kernel
     if ((rc = upload_plan(P, hs))) return rc;

@@ -86,6 +86,7 @@ template <typename Float> static int run(const std::string &out) {
     PLEGMA_Correlator<Float> c(MOMENTUM_SPACE, src, 2, 6);
     c.contractBaryonsUDSC(u, d, s, none);
     dump(c, out + "/udsc_" + tag + ".bin");
+    c.writeHDF5(out + "/udsc_" + tag + ".h5");
     FILE *f = std::fopen((out + "/udsc_" + tag + ".names").c_str(), "w");
     for (auto &n : c.getDatasets()) std::fprintf(f, "%s\n", n.c_str());
     std::fclose(f);

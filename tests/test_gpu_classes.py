@@ -183,3 +183,26 @@ def test_udsc_class(outputs, prec, tol):
     assert got_names == names
     got = _load(d, "udsc", prec).reshape(ref.shape)
     assert G.rel_err(got, ref, axis=-1) < tol
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-6)])
+def test_udsc_hdf5(outputs, prec, tol):
+    """udsc datasets in the HDF5 record file: no groups and no shape, so every
+    operator is /sxXXsyXXszXXstXX/<baryon>/<operator> of shape (totalT, Nmom, 2)
+    and mvec sits at the source level"""
+    from plegma_amd import gen_udsc, h5twin
+    d, u, dn = outputs
+    s = O.synth_prop(L, 1234, 3, 0, 0.8)
+    names, ref = O.udsc_correlator(L, SRC, 6, 2, {"u": u, "d": dn, "s": s}, gen_udsc.products())
+    ds = h5twin.read(os.path.join(d, f"udsc_{prec}.h5.twin"))
+    top = "/sx01sy02sz03st05"
+    assert list(ds) == [f"{top}/mvec"] + [f"{top}/{n}" for n in names]
+    np.testing.assert_array_equal(ds[f"{top}/mvec"].data, O.momlist(2))
+    for k, n in enumerate(names):
+        obj = ds[f"{top}/{n}"]
+        assert obj.attrs == {"description": "shape: /time/moms/re-im"}
+        got = obj.data.astype(np.float64)
+        got = got[..., 0] + 1j * got[..., 1]
+        for j in range(6):
+            tau = (O.row_time(L, SRC, 6, j) - SRC[3]) % L[3]
+            assert G.rel_err(got[tau][:, None], ref[k, j][:, None], axis=0) < tol

@@ -108,3 +108,18 @@ def test_udsc_fused_prep(props, tables):
     names, ref = _oracle(props, tables, only_st=True, hosts=hosts)
     got = _run(views, only_st=True).reshape(ref.shape)
     assert G.rel_err(got, ref, axis=-1) < 1e-12
+
+
+def test_udsc_f32_output(props, tables):
+    """FP64 contraction written as FP32 (out_precision PLG_F32)"""
+    names, ref = _oracle(props, tables, only_ch=True)
+    got = _run([props[f][1] for f in "udsc"], out_prec=capi.F32, only_ch=True).reshape(ref.shape)
+    assert G.rel_err(got, ref, axis=-1) < 1e-6
+
+
+def test_udsc_repeat(props, tables):
+    """a second call reuses the cached schedules and gives the same bits"""
+    views = [props[f][1] for f in "udsc"]
+    a = _run(views, only_st=True)
+    b = _run(views, only_st=True)
+    assert np.array_equal(a, b)
