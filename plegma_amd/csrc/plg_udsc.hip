@@ -223,11 +223,26 @@ __global__ __launch_bounds__(256) void k_udsc(UdscSched S, const PropDev<T> *__r
 #pragma unroll
             for (int dd = 0; dd < 3; dd++) {
               const int e1 = (dd + 1) % 3, e2 = (dd + 2) % 3;
-              cx<T> v = a[a1 * 3 + e1] * b[a2 * 3 + e2];
-              v = v - a[a1 * 3 + e2] * b[a2 * 3 + e1];
-              v = v - a[a2 * 3 + e1] * b[a1 * 3 + e2];
-              cfma(v, a[a2 * 3 + e2], b[a1 * 3 + e1]);
-              d[cc * 3 + dd] = v;
+              // + a[a1][e1] b[a2][e2] - a[a1][e2] b[a2][e1] - a[a2][e1] b[a1][e2] + a[a2][e2] b[a1][e1]
+              const cx<T> p0 = a[a1 * 3 + e1], q0 = b[a2 * 3 + e2], p1 = a[a1 * 3 + e2], q1 = b[a2 * 3 + e1];
+              const cx<T> p2 = a[a2 * 3 + e1], q2 = b[a1 * 3 + e2], p3 = a[a2 * 3 + e2], q3 = b[a1 * 3 + e1];
+              T re = p0.re * q0.re;
+              re = fma(-p0.im, q0.im, re);
+              re = fma(-p1.re, q1.re, re);
+              re = fma(p1.im, q1.im, re);
+              re = fma(-p2.re, q2.re, re);
+              re = fma(p2.im, q2.im, re);
+              re = fma(p3.re, q3.re, re);
+              re = fma(-p3.im, q3.im, re);
+              T im = p0.re * q0.im;
+              im = fma(p0.im, q0.re, im);
+              im = fma(-p1.re, q1.im, im);
+              im = fma(-p1.im, q1.re, im);
+              im = fma(-p2.re, q2.im, im);
+              im = fma(-p2.im, q2.re, im);
+              im = fma(p3.re, q3.im, im);
+              im = fma(p3.im, q3.re, im);
+              d[cc * 3 + dd] = cmake<T>(re, im);
             }
           }
           // its distinct terms T = sum_cc' D[c][c'] S3[m4m5][c][c']
