@@ -641,7 +641,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-              f"(sources and time slices shard without communication; see the design notes)",
+              f"(sources and time slices would shard over GPUs without communication)",
               file=sys.stderr)
         sys.exit(2)
 

@@ -16,7 +16,7 @@ import gpu_util as G
 
 pytestmark = pytest.mark.gpu
 
-L1 = [8, 8, 8, 16]      # BASELINE configuration 1 geometry
+L1 = [8, 8, 8, 16]      # the 8^3x16 lattice of the small two-point workload
 Q1 = 4                   # |n|^2 <= 4 : 33 momenta
 TOL64 = 1e-12
 
