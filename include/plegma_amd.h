@@ -153,6 +153,18 @@ int plg_baryons_udsc_size(const plg_prop *up, const plg_prop *dn, const plg_prop
 int plg_baryons_udsc(const plg_prop *up, const plg_prop *dn, const plg_prop *st, const plg_prop *ch, int only_st,
                      int only_ch, const int source[4], int totalT, const int *moms, int nmom, int mode, int space,
                      int out_precision, void *out, void *stream);
+/* A three-propagator table contraction in the format of the reference's
+ * generated tables (contract_props, lib/kernels/PLEGMA_baryons_udsc.cu:86-121,
+ * tables BP_prop_prods_idxs / _vals / _count of
+ * lib/kernels/PLEGMA_baryons_udsc_arrays.cuh):
+ *   O_k = sum_t vals_t sum_{eps, eps'} sgn sgn' S1[i0][i1][a][a'] S2[i2][i3][b][b'] S3[i4][i5][c][c']
+ * with operator k the next counts[k] terms; idxs: 6 spin indices per term,
+ * vals: re, im (float) per term.  Limits: 16 operators, 4 distinct terms per
+ * (i0 i1, i2 i3), 64 distinct coefficients (PLG_ERR_UNSUPPORTED beyond).
+ * out: [k][j][m] (or [k][j][x]) as plg_baryons_udsc. */
+int plg_baryons_table(const plg_prop *p1, const plg_prop *p2, const plg_prop *p3, const int16_t *idxs,
+                      const float *vals, const int16_t *counts, int nops, const int source[4], int totalT,
+                      const int *moms, int nmom, int mode, int space, int out_precision, void *out, void *stream);
 /* the same, with HIP events between the stages of every product: nstages
  * stage names (32 bytes each) and their milliseconds summed over the products */
 int plg_baryons_udsc_profile(const plg_prop *up, const plg_prop *dn, const plg_prop *st, const plg_prop *ch,

@@ -34,7 +34,7 @@ EXPORTS = ["plg_init", "plg_finalize", "plg_last_error", "plg_version", "plg_mom
            "plg_mlp_forward", "plg_mlp_gradient", "plg_mlp_train", "plg_mlp_destroy", "plg_ft_apply",
            "plg_prop_rotate_physical", "plg_prop_apply_boundaries", "plg_udsc_count", "plg_udsc_product",
            "plg_udsc_op_name", "plg_baryons_udsc_select", "plg_baryons_udsc_size", "plg_baryons_udsc",
-           "plg_baryons_udsc_profile"]
+           "plg_baryons_udsc_profile", "plg_baryons_table"]
 
 
 class PlgError(RuntimeError):
@@ -105,6 +105,8 @@ def lib():
         L.plg_baryons_udsc_profile.argtypes = [pp, pp, pp, pp, C.c_int, C.c_int, ip, C.c_int, ip, C.c_int,
                                                C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, ip,
                                                C.POINTER(C.c_double), C.c_char_p]
+        L.plg_baryons_table.argtypes = [pp, pp, pp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, ip, C.c_int, ip,
+                                        C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
         L.plg_mlp_create.argtypes = [ip, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         L.plg_mlp_nparams.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         L.plg_mlp_set_params.argtypes = [C.c_void_p, fp]
@@ -244,6 +246,21 @@ def baryons_udsc_profile(up, dn, st, ch, source, totalT, moms, mode, space, out_
                                          names))
     raw = names.raw
     return {raw[32 * i:32 * i + 32].split(b"\0")[0].decode(): ms[i] for i in range(min(n.value, cap))}
+
+
+def baryons_table(p1, p2, p3, idxs, vals, counts, source, totalT, moms, mode, space, out_prec, out, stream=None):
+    """table-driven three-propagator contraction (the reference's udsc table
+    format): idxs int16 [n, 6], vals float32 [n, 2], counts int16 [nops];
+    out [nops][j][m] complex"""
+    import numpy as np
+    idxs = np.ascontiguousarray(idxs, dtype=np.int16)
+    vals = np.ascontiguousarray(vals, dtype=np.float32)
+    counts = np.ascontiguousarray(counts, dtype=np.int16)
+    flat = [int(v) for m in moms for v in m]
+    check(lib().plg_baryons_table(C.byref(p1), C.byref(p2), C.byref(p3), idxs.ctypes.data, vals.ctypes.data,
+                                  counts.ctypes.data, len(counts), ivec(source), totalT,
+                                  ivec(flat) if flat else None, len(moms), mode, space, out_prec,
+                                  C.c_void_p(out.data_ptr()), _stream(stream)))
 
 
 def baryons_udsc_size(up, dn, st, ch, source, totalT, nmom, space=MOMENTUM_SPACE, only_st=False, only_ch=False):

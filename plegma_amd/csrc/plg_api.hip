@@ -1044,95 +1044,167 @@ static int udsc_device_schedule(int prod, UdscSched &out) {
 }
 
 // stage_ms (optional): summed per-stage milliseconds over the products
-static int udsc_run(const plg_prop *up, const plg_prop *dn, const plg_prop *st, const plg_prop *ch, int only_st,
-                    int only_ch, const int source[4], int totalT, const int *moms, int nmom, int mode, int space,
-                    int out_precision, void *out, void *stream, std::map<std::string, double> *stage_ms) {
+static int udsc_check_args(const void *out, const int *source, int out_precision, int mode, int space,
+                           const int *moms, int nmom) {
   if (!g_ctx.init) return fail(PLG_ERR_NOT_INIT, "plg_init not called");
   if (!out || !source) return fail(PLG_ERR_INVALID, "NULL argument");
   if (out_precision != PLG_F64 && out_precision != PLG_F32) return fail(PLG_ERR_INVALID, "bad out precision");
   if (mode != PLG_MODE_F64 && mode != PLG_MODE_MIXED) return fail(PLG_ERR_INVALID, "bad mode");
   if (space != PLG_MOMENTUM_SPACE && space != PLG_POSITION_SPACE) return fail(PLG_ERR_INVALID, "bad space");
   if (space == PLG_MOMENTUM_SPACE && (nmom <= 0 || !moms)) return fail(PLG_ERR_INVALID, "empty momentum list");
+  return PLG_OK;
+}
+
+// one three-propagator table contraction (views S1, S2, S3 of the table),
+// its operators written as datasets d0 .. d0 + nops - 1 of the output
+static int udsc_one(const plg_prop *const views[3], const UdscSched &S, int nops, int d0, int nd_total,
+                    const int source[4], int totalT, const int *moms, int nmom, int mode, int space, int out_precision,
+                    void *out, hipStream_t hs, std::map<std::string, double> *stage_ms) {
   int lt, rc;
   if ((rc = plg_window(source, totalT, nullptr, nullptr, &lt))) return rc;
   const Window w = window_of(g_ctx.L, source, totalT);
+  const int T = g_ctx.L[3];
+  const int64_t inner = space == PLG_MOMENTUM_SPACE ? (int64_t)nmom : g_ctx.V3;
+  Plan P;
+  P.kind = -1;
+  P.mode = mode;
+  P.terms = PLG_SPLIT_BF16X6;
+  P.space = space;
+  P.T = mode == PLG_MODE_F64 ? PLG_F64 : PLG_F32;
+  for (int i = 0; i < 3; i++) {
+    if (!views[i]) return fail(PLG_ERR_INVALID, "NULL propagator view");
+    P.props.push_back(*views[i]);
+  }
+  P.nmom = space == PLG_MOMENTUM_SPACE ? nmom : 0;
+  if (P.nmom) P.moms.assign(moms, moms + 3 * P.nmom);
+  P.ms = scaled_moms(P.moms.data(), P.nmom, 1);
+  if (P.T == PLG_F32 && P.nmom > 0) P.npairs = make_pairs(P.ms, g_ctx.M, P.pair_ms, P.mom_pair, &P.zero_pair);
+  P.nch_c = nops;
+  for (int j = 0; j < lt; j++) {
+    const int t = row_time(g_ctx.L, source, w, j);
+    for (const plg_prop &v : P.props)
+      if ((rc = check_prop(v, P.T, t))) return rc;
+    P.rows.push_back({t, 0, 1, 0, 2});
+  }
+  for (int k = 0; k < nops; k++)
+    for (int j = 0; j < lt; j++) {
+      const int t = P.rows[j].t;
+      OutRowDev o{};
+      o.crow = j;
+      o.ch_base = k;
+      o.nch_row = 1;
+      o.sx = source[0]; o.sy = source[1]; o.sz = source[2];
+      o.base = ((int64_t)(d0 + k) * lt + j) * inner;
+      // trilinear in the three views: one plane exponent of each
+      auto plane = [&](const plg_prop &v) { return ((t - v.t_first) % T + T) % T; };
+      o.ea = P.props[0].plane_exp; o.pla = plane(P.props[0]); o.ka = 1;
+      o.eb = P.props[1].plane_exp; o.plb = plane(P.props[1]); o.kb = 1;
+      o.ec = P.props[2].plane_exp; o.plc = plane(P.props[2]); o.kc = 1;
+      P.orows.push_back(o);
+    }
+  P.csel = {0};
+  P.ch_off = {0};
+  P.m_stride = 1;
+  P.out_elems = (int64_t)nd_total * lt * inner;
+  P.nsites = (int64_t)lt * g_ctx.V3;
+  P.launches.push_back({Launch::UDSC, 0, false, false, 0, 0});
+  P.udsc = S;
+  choose_tiles(P);
+  P.tpb = 64;  // sites per workgroup of the udsc kernel
+  if ((rc = upload_plan(P, hs))) return rc;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if ((rc = ensure(&g_ctx.ws_c, &g_ctx.ws_c_bytes, c_bytes(P)))) return rc;
+    if ((rc = ensure(&g_ctx.ws_part, &g_ctx.ws_part_bytes, part_bytes(P)))) return rc;
+  }
+  StageLog log;
+  rc = enqueue(P, g_ctx.ws_c, g_ctx.ws_part, out, out_precision, hs, stage_ms ? &log : nullptr);
+  // the plan's tables are freed on return
+  if (!rc && hipStreamSynchronize(hs) != hipSuccess) rc = fail(PLG_ERR_HIP, "udsc sync failed");
+  for (size_t i = 0; i + 1 < log.ev.size(); i++) {
+    float t = 0;
+    if (!rc) hipEventElapsedTime(&t, log.ev[i], log.ev[i + 1]);
+    (*stage_ms)[log.name[i + 1]] += t;
+  }
+  for (hipEvent_t e : log.ev) hipEventDestroy(e);
+  return rc;
+}
+
+static int udsc_run(const plg_prop *up, const plg_prop *dn, const plg_prop *st, const plg_prop *ch, int only_st,
+                    int only_ch, const int source[4], int totalT, const int *moms, int nmom, int mode, int space,
+                    int out_precision, void *out, void *stream, std::map<std::string, double> *stage_ms) {
+  int rc;
+  if ((rc = udsc_check_args(out, source, out_precision, mode, space, moms, nmom))) return rc;
   const plg_prop *fl[4] = {up, dn, st, ch};
   std::vector<int> prods;
   int nd_total;
   udsc_select(fl, only_st, only_ch, prods, nd_total);
-  hipStream_t hs = (hipStream_t)stream;
-  const int T = g_ctx.L[3];
-  const int64_t inner = space == PLG_MOMENTUM_SPACE ? (int64_t)nmom : g_ctx.V3;
   int d0 = 0;
   for (int prod : prods) {
-    Plan P;
-    P.kind = -1;
-    P.mode = mode;
-    P.terms = PLG_SPLIT_BF16X6;
-    P.space = space;
-    P.T = mode == PLG_MODE_F64 ? PLG_F64 : PLG_F32;
     const char *f = udsc_flavours(prod);
-    for (int i = 0; i < 3; i++) {
-      const int k = (int)(std::find(UDSC_LETTERS, UDSC_LETTERS + 4, f[i]) - UDSC_LETTERS);
-      P.props.push_back(*fl[k]);
-    }
-    P.nmom = space == PLG_MOMENTUM_SPACE ? nmom : 0;
-    if (P.nmom) P.moms.assign(moms, moms + 3 * P.nmom);
-    P.ms = scaled_moms(P.moms.data(), P.nmom, 1);
-    if (P.T == PLG_F32 && P.nmom > 0) P.npairs = make_pairs(P.ms, g_ctx.M, P.pair_ms, P.mom_pair, &P.zero_pair);
+    const plg_prop *views[3];
+    for (int i = 0; i < 3; i++)
+      views[i] = fl[std::find(UDSC_LETTERS, UDSC_LETTERS + 4, f[i]) - UDSC_LETTERS];
+    UdscSched S;
+    if ((rc = udsc_device_schedule(prod, S))) return rc;
     const int nops = udsc_nops(prod);
-    P.nch_c = nops;
-    for (int j = 0; j < lt; j++) {
-      const int t = row_time(g_ctx.L, source, w, j);
-      for (const plg_prop &v : P.props)
-        if ((rc = check_prop(v, P.T, t))) return rc;
-      P.rows.push_back({t, 0, 1, 0, 2});
-    }
-    for (int k = 0; k < nops; k++)
-      for (int j = 0; j < lt; j++) {
-        const int t = P.rows[j].t;
-        OutRowDev o{};
-        o.crow = j;
-        o.ch_base = k;
-        o.nch_row = 1;
-        o.sx = source[0]; o.sy = source[1]; o.sz = source[2];
-        o.base = ((int64_t)(d0 + k) * lt + j) * inner;
-        // trilinear in the three views: one plane exponent of each
-        auto plane = [&](const plg_prop &v) { return ((t - v.t_first) % T + T) % T; };
-        o.ea = P.props[0].plane_exp; o.pla = plane(P.props[0]); o.ka = 1;
-        o.eb = P.props[1].plane_exp; o.plb = plane(P.props[1]); o.kb = 1;
-        o.ec = P.props[2].plane_exp; o.plc = plane(P.props[2]); o.kc = 1;
-        P.orows.push_back(o);
-      }
-    P.csel = {0};
-    P.ch_off = {0};
-    P.m_stride = 1;
-    P.out_elems = (int64_t)nd_total * lt * inner;
-    P.nsites = (int64_t)lt * g_ctx.V3;
-    P.launches.push_back({Launch::UDSC, prod, false, false, 0, 0});
-    if ((rc = udsc_device_schedule(prod, P.udsc))) return rc;
-    choose_tiles(P);
-    P.tpb = 64;  // sites per workgroup of the udsc kernel
-    if ((rc = upload_plan(P, hs))) return rc;
-    {
-      std::lock_guard<std::mutex> lk(g_mu);
-      if ((rc = ensure(&g_ctx.ws_c, &g_ctx.ws_c_bytes, c_bytes(P)))) return rc;
-      if ((rc = ensure(&g_ctx.ws_part, &g_ctx.ws_part_bytes, part_bytes(P)))) return rc;
-    }
-    StageLog log;
-    rc = enqueue(P, g_ctx.ws_c, g_ctx.ws_part, out, out_precision, hs, stage_ms ? &log : nullptr);
-    // the plan's tables are freed at the end of the iteration
-    if (!rc && hipStreamSynchronize(hs) != hipSuccess) rc = fail(PLG_ERR_HIP, "udsc sync failed");
-    for (size_t i = 0; i + 1 < log.ev.size(); i++) {
-      float t = 0;
-      if (!rc) hipEventElapsedTime(&t, log.ev[i], log.ev[i + 1]);
-      (*stage_ms)[log.name[i + 1]] += t;
-    }
-    for (hipEvent_t e : log.ev) hipEventDestroy(e);
-    if (rc) return rc;
+    if ((rc = udsc_one(views, S, nops, d0, nd_total, source, totalT, moms, nmom, mode, space, out_precision, out,
+                       (hipStream_t)stream, stage_ms)))
+      return rc;
     d0 += nops;
   }
   return PLG_OK;
+}
+
+int plg_baryons_table(const plg_prop *p1, const plg_prop *p2, const plg_prop *p3, const int16_t *idxs,
+                      const float *vals, const int16_t *counts, int nops, const int source[4], int totalT,
+                      const int *moms, int nmom, int mode, int space, int out_precision, void *out, void *stream) {
+  int rc;
+  if ((rc = udsc_check_args(out, source, out_precision, mode, space, moms, nmom))) return rc;
+  if (!idxs || !vals || !counts || nops <= 0) return fail(PLG_ERR_INVALID, "empty table");
+  std::vector<UdscTerm> terms;
+  std::vector<int> first(nops), count(nops);
+  for (int k = 0; k < nops; k++) {
+    if (counts[k] < 0) return fail(PLG_ERR_INVALID, "negative term count");
+    first[k] = (int)terms.size();
+    count[k] = counts[k];
+    for (int i = 0; i < counts[k]; i++) {
+      const size_t t = terms.size();
+      UdscTerm u;
+      for (int j = 0; j < 6; j++) u.m[j] = (signed char)idxs[6 * t + j];
+      u.re = vals[2 * t];
+      u.im = vals[2 * t + 1];
+      terms.push_back(u);
+    }
+  }
+  UdscSchedHost h;
+  rc = udsc_schedule_terms(terms.data(), first.data(), count.data(), nops, h);
+  if (rc == PLG_ERR_INVALID) return fail(rc, "spin index outside 0..3");
+  if (rc)
+    return fail(rc, "table exceeds the kernel's limits (16 operators, 256 spin quads, 4 terms per quad, "
+                    "64 distinct coefficients)");
+  hipStream_t hs = (hipStream_t)stream;
+  std::vector<void *> mem;
+  auto put = [&](auto &v, auto **d) {
+    const int r = upload(d, v, hs);
+    if (!r) mem.push_back(*d);
+    return r;
+  };
+  short *q = nullptr;
+  signed char *tp = nullptr;
+  uint16_t *e = nullptr;
+  float *c = nullptr;
+  if (!(rc = put(h.quads, &q)) && !(rc = put(h.tp, &tp)) && !(rc = put(h.ents, &e)) && !(rc = put(h.coef, &c))) {
+    h.s.quads = q;
+    h.s.tp = tp;
+    h.s.ents = e;
+    h.s.coef = c;
+    const plg_prop *views[3] = {p1, p2, p3};
+    rc = udsc_one(views, h.s, nops, 0, nops, source, totalT, moms, nmom, mode, space, out_precision, out, hs,
+                  nullptr);
+  }
+  for (void *p : mem) hipFree(p);
+  return rc;
 }
 
 int plg_baryons_udsc(const plg_prop *up, const plg_prop *dn, const plg_prop *st, const plg_prop *ch, int only_st,

@@ -89,6 +89,13 @@ struct UdscSchedHost {
   std::vector<float> coef;
 };
 int udsc_schedule(int prod, UdscSchedHost &h);
+// one term of a table: six spin indices and a coefficient (the layout of the
+// generated tables, plg_udsc_tables.h)
+struct UdscTerm {
+  signed char m[6];
+  float re, im;
+};
+int udsc_schedule_terms(const UdscTerm *terms, const int *first, const int *count, int nops, UdscSchedHost &h);
 int udsc_products();
 const char *udsc_flavours(int prod);
 int udsc_nops(int prod);

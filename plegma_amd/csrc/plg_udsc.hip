@@ -28,11 +28,10 @@
 namespace plg {
 
 // ------------------------------------------------------------------ schedule
-// host: build the chunked schedule of one flavour product from the tables
-int udsc_schedule(int prod, UdscSchedHost &h) {
-  if (prod < 0 || prod >= UDSC_NPROD) return PLG_ERR_INVALID;
-  const UdscProd &P = UDSC_PRODS[prod];
-  if (P.nops > UD_MAXOPS) return PLG_ERR_UNSUPPORTED;
+// host: the chunked schedule of nops operators, operator k being the terms
+// [first[k], first[k] + count[k]) of `terms`
+int udsc_schedule_terms(const UdscTerm *terms, const int *first, const int *count, int nops, UdscSchedHost &h) {
+  if (nops <= 0 || nops > UD_MAXOPS) return PLG_ERR_UNSUPPORTED;
   auto key6 = [](const UdscTerm &t) {
     int k = 0;
     for (int i = 0; i < 6; i++) k = k * 4 + t.m[i];
@@ -42,16 +41,18 @@ int udsc_schedule(int prod, UdscSchedHost &h) {
     const int m0 = (k6 >> 10) & 3, m1 = (k6 >> 8) & 3, m2 = (k6 >> 6) & 3, m3 = (k6 >> 4) & 3;
     return (m0 * 4 + m1) | ((m2 * 4 + m3) << 4);
   };
+  for (int k = 0; k < nops; k++)
+    for (int i = 0; i < count[k]; i++)
+      for (int j = 0; j < 6; j++)
+        if (terms[first[k] + i].m[j] < 0 || terms[first[k] + i].m[j] > 3) return PLG_ERR_INVALID;
   // distinct terms grouped by quad
   std::map<int, std::vector<int>> by_quad;  // quad -> distinct key6 terms
-  for (int k = 0; k < P.nops; k++) {
-    const UdscOp &o = UDSC_OPS[P.first_op + k];
-    for (int i = 0; i < o.count; i++) {
-      const int k6 = key6(UDSC_TERMS[o.first + i]);
+  for (int k = 0; k < nops; k++)
+    for (int i = 0; i < count[k]; i++) {
+      const int k6 = key6(terms[first[k] + i]);
       auto &v = by_quad[quad_of(k6)];
       if (std::find(v.begin(), v.end(), k6) == v.end()) v.push_back(k6);
     }
-  }
   std::vector<int> quads;
   for (auto &kv : by_quad) {
     if ((int)kv.second.size() > UD_TPQ) return PLG_ERR_UNSUPPORTED;
@@ -60,11 +61,11 @@ int udsc_schedule(int prod, UdscSchedHost &h) {
   std::sort(quads.begin(), quads.end(), [](int a, int b) {
     return (a & 15) != (b & 15) ? (a & 15) < (b & 15) : (a >> 4) < (b >> 4);
   });
-  const int nch = ((int)quads.size() + 63) / 64;
+  const int nch = std::max(1, ((int)quads.size() + 63) / 64);
   if (nch > UD_MAXC) return PLG_ERR_UNSUPPORTED;
   std::map<int, std::pair<int, int>> slot;  // key6 -> (chunk, chunk-local slot)
   h = UdscSchedHost{};
-  h.s.nops = P.nops;
+  h.s.nops = nops;
   h.s.nchunks = nch;
   h.quads.assign((size_t)nch * 64, (short)-1);
   h.tp.assign((size_t)nch * 64 * UD_TPQ, (signed char)-1);
@@ -83,15 +84,15 @@ int udsc_schedule(int prod, UdscSchedHost &h) {
   h.coef = {0.f, 0.f};
   cix[{0.f, 0.f}] = 0;
   for (int c = 0; c < nch; c++)
-    for (int k = 0; k < P.nops; k++) {
+    for (int k = 0; k < nops; k++) {
       std::vector<uint16_t> v;
-      const UdscOp &o = UDSC_OPS[P.first_op + k];
-      for (int i = 0; i < o.count; i++) {
-        const UdscTerm &t = UDSC_TERMS[o.first + i];
+      for (int i = 0; i < count[k]; i++) {
+        const UdscTerm &t = terms[first[k] + i];
         const auto &sl = slot[key6(t)];
         if (sl.first != c) continue;
         auto it = cix.find({t.re, t.im});
         if (it == cix.end()) {
+          if ((int)h.coef.size() / 2 >= UD_MAXCOEF) return PLG_ERR_UNSUPPORTED;
           it = cix.insert({{t.re, t.im}, (int)h.coef.size() / 2}).first;
           h.coef.push_back(t.re);
           h.coef.push_back(t.im);
@@ -111,6 +112,18 @@ int udsc_schedule(int prod, UdscSchedHost &h) {
   h.s.ncoef = (int)h.coef.size() / 2;
   if (h.s.nent > UD_MAXENT || h.s.ncoef > UD_MAXCOEF || 64 * UD_TPQ > 256) return PLG_ERR_UNSUPPORTED;
   return PLG_OK;
+}
+
+// the schedule of a flavour product of the built-in tables
+int udsc_schedule(int prod, UdscSchedHost &h) {
+  if (prod < 0 || prod >= UDSC_NPROD) return PLG_ERR_INVALID;
+  const UdscProd &P = UDSC_PRODS[prod];
+  std::vector<int> first(P.nops), count(P.nops);
+  for (int k = 0; k < P.nops; k++) {
+    first[k] = UDSC_OPS[P.first_op + k].first;
+    count[k] = UDSC_OPS[P.first_op + k].count;
+  }
+  return udsc_schedule_terms(UDSC_TERMS, first.data(), count.data(), P.nops, h);
 }
 
 int udsc_products() { return UDSC_NPROD; }

@@ -208,7 +208,8 @@ def write_header(path):
     lines = ["// Generated by plegma_amd/gen_udsc.py from the udsc baryon interpolating fields.",
              "// Per flavour product (propagators S1 S2 S3 in the order of its letters):",
              "// operators, each a list of terms {m0, m1, m2, m3, m4, m5, re, im}.",
-             "#pragma once", "namespace plg {", "struct UdscTerm { signed char m[6]; float re, im; };",
+             "// UdscTerm {m[6], re, im} is declared in plg_internal.h.",
+             "#pragma once", "namespace plg {",
              "struct UdscOp { const char *name; int first, count; };",
              "struct UdscProd { const char *flavours; int first_op, nops; };"]
     terms, ops, prs = [], [], []

@@ -123,3 +123,78 @@ def test_udsc_repeat(props, tables):
     a = _run(views, only_st=True)
     b = _run(views, only_st=True)
     assert np.array_equal(a, b)
+
+
+def _flat_tables(tabs):
+    idx, val, cnt = [], [], []
+    for _, t in tabs:
+        cnt.append(len(t))
+        for m in sorted(t):
+            idx.append(m)
+            val.append([t[m].real, t[m].imag])
+    return np.array(idx, dtype=np.int16), np.array(val, dtype=np.float32), np.array(cnt, dtype=np.int16)
+
+
+def _run_table(views, tabs, mode=capi.MODE_F64):
+    torch = G.torch_cuda()
+    moms = capi.momlist(Q2)
+    n = len(tabs) * TOTALT * len(moms)
+    out = torch.full((n, 2), float("nan"), dtype=torch.float64, device="cuda")
+    idx, val, cnt = _flat_tables(tabs)
+    print(f"[gpu] table contraction: {len(tabs)} operators, {len(idx)} terms", flush=True)
+    capi.baryons_table(*views, idx, val, cnt, SRC, TOTALT, moms, mode, capi.MOMENTUM_SPACE, capi.F64, out)
+    torch.cuda.synchronize()
+    h = out.to("cpu", torch.float64).numpy()
+    return (h[:, 0] + 1j * h[:, 1]).reshape(len(tabs), TOTALT, len(moms))
+
+
+def test_table_builtin_product(props, tables):
+    """the 'uds' tables passed in the reference's table format"""
+    tabs = dict(tables)["uds"]
+    names, ref = O.udsc_correlator(L, SRC, TOTALT, Q2, {f: props[f][2] for f in "uds"}, [("uds", tabs)])
+    got = _run_table([props[f][1] for f in "uds"], tabs)
+    assert G.rel_err(got, ref, axis=-1) < 1e-12
+
+
+def test_table_random(props):
+    """random tables (spins, coefficients, shared terms) on three flavours, FP64 and mixed"""
+    rng = np.random.default_rng(11)
+    coefs = [1, -1, 1j, -1j, 0.5, -0.25j, 2 + 1j, 1 / 3]
+    # at most 4 distinct (m4 m5) per spin quad over all operators, the kernel's limit
+    allowed = [rng.choice(16, size=rng.integers(1, 5), replace=False) for _ in range(256)]
+    tabs = []
+    for k in range(7):
+        t = {}
+        for quad in rng.choice(256, size=40, replace=False):
+            m01, m23 = quad % 16, quad // 16
+            pick = allowed[quad]
+            for p in rng.choice(pick, size=rng.integers(1, len(pick) + 1), replace=False):
+                m = (m01 // 4, m01 % 4, m23 // 4, m23 % 4, p // 4, p % 4)
+                v = complex(coefs[rng.integers(len(coefs))])
+                t[m] = complex(np.float32(v.real), np.float32(v.imag))
+        tabs.append((f"op{k}", t))
+    hp = {"u": props["c"][2], "d": props["s"][2], "s": props["u"][2]}
+    names, ref = O.udsc_correlator(L, SRC, TOTALT, Q2, hp, [("uds", tabs)])
+    views = [props[f][1] for f in "csu"]
+    got = _run_table(views, tabs)
+    assert G.rel_err(got, ref, axis=-1) < 1e-12
+    keep = {f: G.cast_f32(props[f][0], props[f][1], scaled=True) for f in "csu"}
+    got = _run_table([keep[f][1] for f in "csu"], tabs, mode=capi.MODE_MIXED)
+    assert G.rel_err(got, ref, axis=-1) < 1e-6
+
+
+def test_table_limits(props):
+    """tables beyond the kernel's limits are refused, not truncated"""
+    torch = G.torch_cuda()
+    out = torch.zeros((17 * TOTALT * 33, 2), dtype=torch.float64, device="cuda")
+    views = [props[f][1] for f in "uds"]
+    idx = np.zeros((17, 6), dtype=np.int16)
+    val = np.ones((17, 2), dtype=np.float32)
+    with pytest.raises(capi.PlgError, match="UNSUPPORTED"):
+        capi.baryons_table(*views, idx, val, np.ones(17, dtype=np.int16), SRC, TOTALT, capi.momlist(Q2),
+                           capi.MODE_F64, capi.MOMENTUM_SPACE, capi.F64, out)
+    bad = idx[:1].copy()
+    bad[0, 3] = 4
+    with pytest.raises(capi.PlgError, match="INVALID"):
+        capi.baryons_table(*views, bad, val[:1], np.ones(1, dtype=np.int16), SRC, TOTALT, capi.momlist(Q2),
+                           capi.MODE_F64, capi.MOMENTUM_SPACE, capi.F64, out)
