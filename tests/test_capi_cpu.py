@@ -71,3 +71,33 @@ def test_class_headers_compile_and_driver_is_built():
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     assert os.access(build.DRIVER, os.X_OK)
+
+
+def test_c_header_is_plain_c99():
+    """the C ABI header compiles as C99 (the cgo / ctypes side of a binding)"""
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "t.c")
+        with open(src, "w") as f:
+            f.write('#include "plegma_amd.h"\nint main(void) { return plg_version() > 0 ? 0 : 1; }\n')
+        r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-fsyntax-only",
+                            "-I" + os.path.join(ROOT, "include"), src], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+
+
+def test_udsc_and_table_calls_before_init_fail_cleanly():
+    import numpy as np
+    with pytest.raises(capi.PlgError) as e:
+        capi.baryons_udsc_size(None, None, None, None, [0, 0, 0, 0], 4, 1)
+    assert e.value.code == -2
+    v = capi.plg_prop()
+    with pytest.raises(capi.PlgError) as e:
+        capi.baryons_table(v, v, v, np.zeros((1, 6)), np.ones((1, 2)), np.ones(1), [0, 0, 0, 0], 4, [[0, 0, 0]],
+                           capi.MODE_F64, capi.MOMENTUM_SPACE, capi.F64, _FakeTensor())
+    assert e.value.code == -2
+
+
+class _FakeTensor:
+    def data_ptr(self):
+        return 0
