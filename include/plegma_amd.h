@@ -165,6 +165,23 @@ int plg_baryons_udsc(const plg_prop *up, const plg_prop *dn, const plg_prop *st,
 int plg_baryons_table(const plg_prop *p1, const plg_prop *p2, const plg_prop *p3, const int16_t *idxs,
                       const float *vals, const int16_t *counts, int nops, const int source[4], int totalT,
                       const int *moms, int nmom, int mode, int space, int out_precision, void *out, void *stream);
+/* A four-propagator table contraction in the format of the reference's
+ * tetraquark tables (contract_props,
+ * lib/kernels/PLEGMA_heavy_light_tetraquarks.cu:6-62, tables
+ * TETRA_prop_prods_idxs / _col_contr / _vals / _count of
+ * PLEGMA_heavy_light_tetraquarks_arrays.cuh):
+ *   O_k = sum_t vals_t sum_{a0..a3} P1[i0][i1][a[c0]][a[c1]] P2[i2][i3][a[c2]][a[c3]]
+ *         conj(P3[i5+2][i4+2][a[c5]][a[c4]]) conj(P4[i7+2][i6+2][a[c7]][a[c6]])
+ * (spin indices + 2 mod 4), operator k the next counts[k] terms; idxs i0..i7
+ * and col_contr c0..c7: 8 int16 per term; vals: re, im (float) per term.
+ * Supported: up to 32 operators; c0..c3 a permutation of 0..3 closing either
+ * as (c0,c1) = (c5,c4), (c2,c3) = (c7,c6) or as c0 = c5, c1 = c6, c2 = c7,
+ * c3 = c4 (the forms of the reference's tables); PLG_ERR_UNSUPPORTED else.
+ * out: [k][j][m] (or [k][j][x]) as plg_baryons_udsc. */
+int plg_tetraquark_table(const plg_prop *p1, const plg_prop *p2, const plg_prop *p3, const plg_prop *p4,
+                         const int16_t *idxs, const int16_t *col_contr, const float *vals, const int16_t *counts,
+                         int nops, const int source[4], int totalT, const int *moms, int nmom, int mode, int space,
+                         int out_precision, void *out, void *stream);
 /* the same, with HIP events between the stages of every product: nstages
  * stage names (32 bytes each) and their milliseconds summed over the products */
 int plg_baryons_udsc_profile(const plg_prop *up, const plg_prop *dn, const plg_prop *st, const plg_prop *ch,

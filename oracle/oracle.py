@@ -86,6 +86,10 @@ def lib():
         L.orc_site_udsc.restype = None
         L.orc_udsc_field.argtypes = [i32p, C.c_int, C.POINTER(OrcProp), C.POINTER(OrcProp), C.POINTER(OrcProp),
                                      i32p, f64p, i32p, i32p, C.c_int, f64p, C.c_int]
+        L.orc_site_tetra.argtypes = [f64p, f64p, f64p, f64p, i32p, i32p, f64p, i32p, i32p, C.c_int, f64p]
+        L.orc_site_tetra.restype = None
+        L.orc_tetra_field.argtypes = [i32p, C.c_int] + [C.POINTER(OrcProp)] * 4 + [i32p, i32p, f64p, i32p, i32p,
+                                                                                   C.c_int, f64p, C.c_int]
         _lib = L
     return _lib
 
@@ -337,3 +341,31 @@ def udsc_correlator(L, src, totalT, q2max, props, products, only_st=False, only_
         names += [n for n, _ in tabs]
         blocks.append(H)
     return names, (np.concatenate(blocks) if blocks else np.zeros((0,)))
+
+
+def tetra_correlator(L, src, totalT, q2max, props, idxs, cols, vals, counts, nthreads=0):
+    """contract_tetraquarks' term sums (lib/kernels/PLEGMA_heavy_light_tetraquarks.cu:26-49)
+    on the CPU for one table: props = 4 float64 SoA propagators; idxs, cols
+    int [n, 8]; vals [n] complex; counts [nops].  Returns H[k][j][m]."""
+    V3 = L[0] * L[1] * L[2]
+    s, e, lt = window(L, src, totalT)
+    idx = np.ascontiguousarray(np.asarray(idxs, dtype=np.int32).reshape(-1))
+    col = np.ascontiguousarray(np.asarray(cols, dtype=np.int32).reshape(-1))
+    val = np.ascontiguousarray(np.stack([np.real(vals), np.imag(vals)], -1).astype(np.float64).reshape(-1))
+    count = np.asarray(counts, dtype=np.int32)
+    first = np.concatenate([[0], np.cumsum(count)[:-1]]).astype(np.int32)
+    nops = len(count)
+    P = [OrcProp(p.ctypes.data, L[3] * V3, 0, L[3]) for p in props]
+    ip = C.POINTER(C.c_int)
+    H = np.zeros((nops, lt, len(momlist(q2max))), dtype=complex)
+    for j in range(lt):
+        t = row_time(L, src, totalT, j)
+        f = np.zeros(V3 * nops * 2)
+        rc = lib().orc_tetra_field(_ivec(L), t, *[C.byref(p) for p in P], idx.ctypes.data_as(ip),
+                                   col.ctypes.data_as(ip), _ptr(val), first.ctypes.data_as(ip),
+                                   count.ctypes.data_as(ip), nops, _ptr(f), nthreads)
+        if rc:
+            raise RuntimeError(f"orc_tetra_field failed: {rc}")
+        f = f.reshape(V3, nops, 2)
+        H[:, j] = dft(L, src[:3], q2max, f[..., 0] + 1j * f[..., 1], nthreads).T
+    return H

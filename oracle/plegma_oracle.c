@@ -919,3 +919,57 @@ int orc_udsc_field(const int L[4], int t, const orc_prop *p1, const orc_prop *p2
   }
   return err ? -2 : 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* four-propagator tables: lib/kernels/PLEGMA_heavy_light_tetraquarks */
+/* .cu:26-49.  Term t of operator k (terms [first[k], first[k] +       */
+/* count[k])): spins mu = idx[8t..], colour slots c = col[8t..],       */
+/*   v sum_{a0..a3} P1[mu0][mu1][a[c0]][a[c1]] P2[mu2][mu3][a[c2]][a[c3]] */
+/*   conj(P3[(mu5+2)%4][(mu4+2)%4][a[c5]][a[c4]])                       */
+/*   conj(P4[(mu7+2)%4][(mu6+2)%4][a[c7]][a[c6]])                       */
+/* ------------------------------------------------------------------ */
+void orc_site_tetra(const double *S1r, const double *S2r, const double *S3r, const double *S4r, const int *idx,
+                    const int *col, const double *val, const int *first, const int *count, int nops, double *out) {
+  cplx S1[NPROP], S2[NPROP], S3[NPROP], S4[NPROP];
+  load_site(S1r, S1);
+  load_site(S2r, S2);
+  load_site(S3r, S3);
+  load_site(S4r, S4);
+  for (int k = 0; k < nops; k++) {
+    cplx acc = 0;
+    for (int t = first[k]; t < first[k] + count[k]; t++) {
+      const int *mu = idx + 8 * t, *c = col + 8 * t;
+      const cplx v = val[2 * t] + I * val[2 * t + 1];
+      int a[4];
+      for (a[0] = 0; a[0] < 3; a[0]++)
+        for (a[1] = 0; a[1] < 3; a[1]++)
+          for (a[2] = 0; a[2] < 3; a[2]++)
+            for (a[3] = 0; a[3] < 3; a[3]++)
+              acc += v * PS(S1, mu[0], mu[1], a[c[0]], a[c[1]]) * PS(S2, mu[2], mu[3], a[c[2]], a[c[3]]) *
+                     conj(PS(S3, (mu[5] + 2) % 4, (mu[4] + 2) % 4, a[c[5]], a[c[4]])) *
+                     conj(PS(S4, (mu[7] + 2) % 4, (mu[6] + 2) % 4, a[c[7]], a[c[6]]));
+    }
+    store(acc, out + 2 * k);
+  }
+}
+
+int orc_tetra_field(const int L[4], int t, const orc_prop *p1, const orc_prop *p2, const orc_prop *p3,
+                    const orc_prop *p4, const int *idx, const int *col, const double *val, const int *first,
+                    const int *count, int nops, double *out, int nthreads) {
+  int64_t V3 = (int64_t)L[0] * L[1] * L[2];
+  int err = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static) reduction(| : err)
+#endif
+  for (int64_t x = 0; x < V3; x++) {
+    double S1[2 * NPROP], S2[2 * NPROP], S3[2 * NPROP], S4[2 * NPROP];
+    if (gather_site(p1, L[3], V3, t, x, S1) || gather_site(p2, L[3], V3, t, x, S2) ||
+        gather_site(p3, L[3], V3, t, x, S3) || gather_site(p4, L[3], V3, t, x, S4)) {
+      err |= 1;
+      continue;
+    }
+    orc_site_tetra(S1, S2, S3, S4, idx, col, val, first, count, nops, out + x * nops * 2);
+  }
+  return err ? -2 : 0;
+}

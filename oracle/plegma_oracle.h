@@ -117,6 +117,16 @@ int orc_udsc_field(const int L[4], int t, const orc_prop *p1, const orc_prop *p2
                    const int *idx, const double *val, const int *first, const int *count, int nops, double *out,
                    int nthreads);
 
+/* four-propagator tables (lib/kernels/PLEGMA_heavy_light_tetraquarks.cu:26-49):
+ * 8 spin indices and 8 colour slots per term, operator k = terms
+ * [first[k], first[k]+count[k]); one site, out[2*k + ri] */
+void orc_site_tetra(const double *S1, const double *S2, const double *S3, const double *S4, const int *idx,
+                    const int *col, const double *val, const int *first, const int *count, int nops, double *out);
+/* the same over the V3 sites of time slice t: out[(x*nops + k)*2 + ri] */
+int orc_tetra_field(const int L[4], int t, const orc_prop *p1, const orc_prop *p2, const orc_prop *p3,
+                    const orc_prop *p4, const int *idx, const int *col, const double *val, const int *first,
+                    const int *count, int nops, double *out, int nthreads);
+
 /* Naive DFT of a site field (reference fourier_transform_3D semantics, sign -1):
  * f[(x*ncomp + c)*2+ri] over V3 sites -> out[(m*ncomp + c)*2+ri]. */
 void orc_dft(const int L[4], const int src3[3], int q2max, const double *f,

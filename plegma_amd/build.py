@@ -19,7 +19,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libplegma_amd.so")
 SOURCES = ["plg_api.hip", "plg_contract.hip", "plg_baryon.hip", "plg_project.hip", "plg_project_bf16.hip",
-           "plg_synth.hip", "plg_mesmes.hip", "plg_mlp.hip", "plg_prep.hip", "plg_udsc.hip"]
+           "plg_synth.hip", "plg_mesmes.hip", "plg_mlp.hip", "plg_prep.hip", "plg_udsc.hip",
+           "plg_tetra.hip"]
 # generated from the udsc baryon interpolating fields by plegma_amd/gen_udsc.py
 UDSC_TABLES = os.path.join(CSRC, "plg_udsc_tables.h")
 ARCH = "gfx950"

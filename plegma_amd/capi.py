@@ -34,7 +34,8 @@ EXPORTS = ["plg_init", "plg_finalize", "plg_last_error", "plg_version", "plg_mom
            "plg_mlp_forward", "plg_mlp_gradient", "plg_mlp_train", "plg_mlp_destroy", "plg_ft_apply",
            "plg_prop_rotate_physical", "plg_prop_apply_boundaries", "plg_udsc_count", "plg_udsc_product",
            "plg_udsc_op_name", "plg_baryons_udsc_select", "plg_baryons_udsc_size", "plg_baryons_udsc",
-           "plg_baryons_udsc_profile", "plg_baryons_table"]
+           "plg_baryons_udsc_profile", "plg_baryons_table",
+           "plg_tetraquark_table"]
 
 
 class PlgError(RuntimeError):
@@ -107,6 +108,9 @@ def lib():
                                                C.POINTER(C.c_double), C.c_char_p]
         L.plg_baryons_table.argtypes = [pp, pp, pp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, ip, C.c_int, ip,
                                         C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        L.plg_tetraquark_table.argtypes = [pp, pp, pp, pp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                           ip, C.c_int, ip, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                           C.c_void_p]
         L.plg_mlp_create.argtypes = [ip, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         L.plg_mlp_nparams.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         L.plg_mlp_set_params.argtypes = [C.c_void_p, fp]
@@ -261,6 +265,23 @@ def baryons_table(p1, p2, p3, idxs, vals, counts, source, totalT, moms, mode, sp
                                   counts.ctypes.data, len(counts), ivec(source), totalT,
                                   ivec(flat) if flat else None, len(moms), mode, space, out_prec,
                                   C.c_void_p(out.data_ptr()), _stream(stream)))
+
+
+def tetraquark_table(p1, p2, p3, p4, idxs, cols, vals, counts, source, totalT, moms, mode, space, out_prec, out,
+                     stream=None):
+    """four-propagator table contraction (the reference's tetraquark table
+    format): idxs, cols int16 [n, 8], vals float32 [n, 2], counts int16
+    [nops]; out [nops][j][m] complex"""
+    import numpy as np
+    idxs = np.ascontiguousarray(idxs, dtype=np.int16)
+    cols = np.ascontiguousarray(cols, dtype=np.int16)
+    vals = np.ascontiguousarray(vals, dtype=np.float32)
+    counts = np.ascontiguousarray(counts, dtype=np.int16)
+    flat = [int(v) for m in moms for v in m]
+    check(lib().plg_tetraquark_table(C.byref(p1), C.byref(p2), C.byref(p3), C.byref(p4), idxs.ctypes.data,
+                                     cols.ctypes.data, vals.ctypes.data, counts.ctypes.data, len(counts),
+                                     ivec(source), totalT, ivec(flat) if flat else None, len(moms), mode, space,
+                                     out_prec, C.c_void_p(out.data_ptr()), _stream(stream)))
 
 
 def baryons_udsc_size(up, dn, st, ch, source, totalT, nmom, space=MOMENTUM_SPACE, only_st=False, only_ch=False):

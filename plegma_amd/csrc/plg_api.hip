@@ -156,7 +156,7 @@ static int kind_shape(int kind, int mask, int &ng, int &nd, int &nch) {
 
 // ------------------------------------------------------------------ plans
 struct Launch {
-  enum { MESON, BARYON, MESMES, UDSC } what;
+  enum { MESON, BARYON, MESMES, UDSC, TETRA } what;
   int set;          // meson set
   bool two, mes;    // baryon
   int gmask;
@@ -201,6 +201,10 @@ struct Plan {
   uint16_t *d_pidx = nullptr;  // phase indices of the mixed projection
   // udsc baryons: the flavour product's schedule (device arrays owned by the context)
   UdscSched udsc{};
+  // four-propagator tables: terms (owned by the caller of the plan) and rows
+  TetraSched tetra{};
+  std::vector<RowDev4> rows4;
+  RowDev4 *d_rows4 = nullptr;
   // batch-owned workspace (single calls use the context workspace)
   void *c = nullptr, *part = nullptr;
   size_t c_bytes = 0, part_bytes = 0;
@@ -216,7 +220,7 @@ struct Plan {
     if (graph) hipGraphDestroy(graph);
     if (cap_stream) hipStreamDestroy(cap_stream);
     for (void *p : {(void *)d_props, (void *)d_rows, (void *)d_orows, (void *)d_csel, (void *)d_choff, (void *)d_ms,
-                    (void *)d_pair_ms, (void *)d_mom_pair, (void *)d_pidx})
+                    (void *)d_pair_ms, (void *)d_mom_pair, (void *)d_pidx, (void *)d_rows4})
       if (p) hipFree(p);
     if (own_ws) {
       if (c) hipFree(c);
@@ -600,6 +604,7 @@ static std::string launch_name(const Plan &P, const Launch &l) {
   if (l.what == Launch::MESON) return "contract_mesons_" + t;
   if (l.what == Launch::MESMES) return "contract_mesmes_" + t;
   if (l.what == Launch::UDSC) return "contract_udsc_" + t;
+  if (l.what == Launch::TETRA) return "contract_tetra_" + t;
   return std::string(l.mes ? "contract_twopt_" : "contract_baryons_") + t;
 }
 
@@ -613,6 +618,8 @@ template <typename T> static int launch_contract_t(const Plan &P, const Launch &
     return launch_mesmes<T>(pd, P.d_rows, nrows, (cx<T> *)c, g_ctx.V3, g_ctx.L[3], P.nch_c, l.ch0, P.tpb, st);
   case Launch::UDSC:
     return launch_udsc<T>(P.udsc, pd, P.d_rows, nrows, (cx<T> *)c, g_ctx.V3, g_ctx.L[3], P.nch_c, l.ch0, P.tpb, st);
+  case Launch::TETRA:
+    return launch_tetra<T>(P.tetra, pd, P.d_rows4, nrows, (cx<T> *)c, g_ctx.V3, g_ctx.L[3], P.nch_c, P.tpb, st);
   default:
     return launch_baryon<T>(l.two, l.mes, l.gmask, pd, P.d_rows, nrows, (cx<T> *)c, g_ctx.V3, g_ctx.L[3],
                             P.nch_c, l.ch0, P.tpb, st);
@@ -1154,6 +1161,95 @@ static int udsc_run(const plg_prop *up, const plg_prop *dn, const plg_prop *st, 
     d0 += nops;
   }
   return PLG_OK;
+}
+
+int plg_tetraquark_table(const plg_prop *p1, const plg_prop *p2, const plg_prop *p3, const plg_prop *p4,
+                         const int16_t *idxs, const int16_t *col_contr, const float *vals, const int16_t *counts,
+                         int nops, const int source[4], int totalT, const int *moms, int nmom, int mode, int space,
+                         int out_precision, void *out, void *stream) {
+  int rc;
+  if ((rc = udsc_check_args(out, source, out_precision, mode, space, moms, nmom))) return rc;
+  if (!idxs || !col_contr || !vals || !counts || nops <= 0) return fail(PLG_ERR_INVALID, "empty table");
+  const plg_prop *views[4] = {p1, p2, p3, p4};
+  for (const plg_prop *v : views)
+    if (!v) return fail(PLG_ERR_INVALID, "NULL propagator view");
+  std::vector<TetraTerm> terms;
+  std::vector<int> first, count;
+  rc = tetra_pack(idxs, col_contr, vals, counts, nops, terms, first, count);
+  if (rc == PLG_ERR_INVALID) return fail(rc, "spin or colour index outside 0..3");
+  if (rc)
+    return fail(rc, "table outside the supported form (32 operators; colour slots a permutation of the four "
+                    "sums, closing as <P1,P3><P2,P4> or Tr[(P3^dag P1)(P4^dag P2)])");
+  int lt;
+  if ((rc = plg_window(source, totalT, nullptr, nullptr, &lt))) return rc;
+  const Window w = window_of(g_ctx.L, source, totalT);
+  const int T = g_ctx.L[3];
+  const int64_t inner = space == PLG_MOMENTUM_SPACE ? (int64_t)nmom : g_ctx.V3;
+  hipStream_t hs = (hipStream_t)stream;
+  Plan P;
+  P.kind = -1;
+  P.mode = mode;
+  P.terms = PLG_SPLIT_BF16X6;
+  P.space = space;
+  P.T = mode == PLG_MODE_F64 ? PLG_F64 : PLG_F32;
+  for (const plg_prop *v : views) P.props.push_back(*v);
+  P.nmom = space == PLG_MOMENTUM_SPACE ? nmom : 0;
+  if (P.nmom) P.moms.assign(moms, moms + 3 * P.nmom);
+  P.ms = scaled_moms(P.moms.data(), P.nmom, 1);
+  if (P.T == PLG_F32 && P.nmom > 0) P.npairs = make_pairs(P.ms, g_ctx.M, P.pair_ms, P.mom_pair, &P.zero_pair);
+  P.nch_c = nops;
+  for (int j = 0; j < lt; j++) {
+    const int t = row_time(g_ctx.L, source, w, j);
+    for (const plg_prop &v : P.props)
+      if ((rc = check_prop(v, P.T, t))) return rc;
+    P.rows.push_back({t, 0, 1, 0, 2});
+    P.rows4.push_back({t, {0, 1, 2, 3}});
+  }
+  for (int k = 0; k < nops; k++)
+    for (int j = 0; j < lt; j++) {
+      const int t = P.rows[j].t;
+      OutRowDev o{};
+      o.crow = j;
+      o.ch_base = k;
+      o.nch_row = 1;
+      o.sx = source[0]; o.sy = source[1]; o.sz = source[2];
+      o.base = ((int64_t)k * lt + j) * inner;
+      // multilinear in the four views: one plane exponent of each
+      auto plane = [&](const plg_prop &v) { return ((t - v.t_first) % T + T) % T; };
+      o.ea = P.props[0].plane_exp; o.pla = plane(P.props[0]); o.ka = 1;
+      o.eb = P.props[1].plane_exp; o.plb = plane(P.props[1]); o.kb = 1;
+      o.ec = P.props[2].plane_exp; o.plc = plane(P.props[2]); o.kc = 1;
+      o.ed = P.props[3].plane_exp; o.pld = plane(P.props[3]); o.kd = 1;
+      P.orows.push_back(o);
+    }
+  P.csel = {0};
+  P.ch_off = {0};
+  P.m_stride = 1;
+  P.out_elems = (int64_t)nops * lt * inner;
+  P.nsites = (int64_t)lt * g_ctx.V3;
+  P.launches.push_back({Launch::TETRA, 0, false, false, 0, 0});
+  TetraTerm *d_terms = nullptr;
+  if ((rc = upload(&d_terms, terms, hs))) return rc;
+  P.tetra.nops = nops;
+  for (int k = 0; k < nops; k++) {
+    P.tetra.first[k] = first[k];
+    P.tetra.count[k] = count[k];
+  }
+  P.tetra.terms = d_terms;
+  choose_tiles(P);
+  P.tpb = 64;  // sites per workgroup of the kernel
+  if (!(rc = upload_plan(P, hs)) && !(rc = upload(&P.d_rows4, P.rows4, hs))) {
+    {
+      std::lock_guard<std::mutex> lk(g_mu);
+      if (!(rc = ensure(&g_ctx.ws_c, &g_ctx.ws_c_bytes, c_bytes(P))))
+        rc = ensure(&g_ctx.ws_part, &g_ctx.ws_part_bytes, part_bytes(P));
+    }
+    if (!rc) rc = enqueue(P, g_ctx.ws_c, g_ctx.ws_part, out, out_precision, hs);
+    if (!rc && hipStreamSynchronize(hs) != hipSuccess) rc = fail(PLG_ERR_HIP, "tetraquark table sync failed");
+  }
+  if (rc) hipStreamSynchronize(hs);
+  hipFree(d_terms);
+  return rc;
 }
 
 int plg_baryons_table(const plg_prop *p1, const plg_prop *p2, const plg_prop *p3, const int16_t *idxs,

@@ -54,12 +54,15 @@ struct OutRowDev {
   int pla, plb, ka, kb;
   const int *ec;
   int plc, kc;
+  const int *ed;
+  int pld, kd;
 };
 __device__ __forceinline__ double row_scale(const OutRowDev &R) {
   int e = 0;
   if (R.ea) e += R.ka * R.ea[R.pla];
   if (R.eb) e += R.kb * R.eb[R.plb];
   if (R.ec) e += R.kc * R.ec[R.plc];
+  if (R.ed) e += R.kd * R.ed[R.pld];
   return ldexp(1.0, e);
 }
 
@@ -96,6 +99,28 @@ struct UdscTerm {
   float re, im;
 };
 int udsc_schedule_terms(const UdscTerm *terms, const int *first, const int *count, int nops, UdscSchedHost &h);
+
+// four-propagator table contractions (tetraquark table format)
+constexpr int TETRA_MAXOPS = 32;
+struct TetraTerm {
+  uint32_t spins;  // 4 bits per block: P1 (i0 i1), P2 (i2 i3), P3 (i5+2, i4+2), P4 (i7+2, i6+2)
+  int kind;        // 0 direct <B1,B3><B2,B4>, 1 crossed Tr[(B3^dag B1)(B4^dag B2)]
+  float re, im;
+};
+struct TetraSched {
+  int nops;
+  int first[TETRA_MAXOPS], count[TETRA_MAXOPS];
+  const TetraTerm *terms;
+};
+struct RowDev4 {
+  int t;
+  int p[4];
+};
+int tetra_pack(const int16_t *idxs, const int16_t *cols, const float *vals, const int16_t *counts, int nops,
+               std::vector<TetraTerm> &terms, std::vector<int> &first, std::vector<int> &count);
+template <typename T>
+int launch_tetra(const TetraSched &S, const PropDev<T> *props, const RowDev4 *rows, int nrows, cx<T> *out,
+                 int64_t V3, int T_lat, int nch_total, int spw, hipStream_t st);
 int udsc_products();
 const char *udsc_flavours(int prod);
 int udsc_nops(int prod);

@@ -1,0 +1,108 @@
+"""Four-propagator table contractions on the GPU (plg_tetraquark_table, the
+counterpart of contract_props in lib/kernels/PLEGMA_heavy_light_tetraquarks.cu:6-62
+with the reference's table format) against the oracle's literal 81-term
+colour sums, on random tables in the reference's four colour patterns."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from plegma_amd import capi
+
+import gpu_util as G
+
+pytestmark = pytest.mark.gpu
+
+L = [4, 4, 4, 8]
+SRC = [2, 1, 3, 6]
+TOTALT = 5
+Q2 = 2
+PATTERNS = [(0, 2, 1, 3, 2, 0, 3, 1), (0, 3, 1, 2, 3, 0, 2, 1), (0, 2, 1, 3, 3, 0, 2, 1), (0, 3, 1, 2, 2, 0, 3, 1)]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def ctx():
+    G.torch_cuda()
+    print("\n[gpu] plg_init", L, flush=True)
+    capi.init(L, 0)
+    yield
+    capi.finalize()
+
+
+@pytest.fixture(scope="module")
+def props(ctx):
+    out = []
+    for i, lam in enumerate((0.5, 0.6, 1.1, 1.3)):
+        t, v = G.make_prop(L, 91, 30 + i, SRC[3], lam)
+        out.append((t, v, G.host(t)))
+    return out
+
+
+def _table(rng, nops=5, nterm=60):
+    idxs, cols, vals, counts = [], [], [], []
+    coefs = [1, -1, 1j, -0.5, 0.25 + 0.5j, 1 / 3]
+    for k in range(nops):
+        n = int(rng.integers(nterm // 2, nterm))
+        counts.append(n)
+        for _ in range(n):
+            idxs.append(rng.integers(0, 4, size=8))
+            cols.append(PATTERNS[rng.integers(4)])
+            v = complex(coefs[rng.integers(len(coefs))])
+            vals.append([np.float32(v.real), np.float32(v.imag)])
+    return (np.array(idxs, dtype=np.int16), np.array(cols, dtype=np.int16), np.array(vals, dtype=np.float32),
+            np.array(counts, dtype=np.int16))
+
+
+def _run(views, tab, mode=capi.MODE_F64):
+    torch = G.torch_cuda()
+    moms = capi.momlist(Q2)
+    idxs, cols, vals, counts = tab
+    out = torch.full((len(counts) * TOTALT * len(moms), 2), float("nan"), dtype=torch.float64, device="cuda")
+    print(f"[gpu] tetraquark table: {len(counts)} operators, {len(idxs)} terms, mode {mode}", flush=True)
+    capi.tetraquark_table(*views, idxs, cols, vals, counts, SRC, TOTALT, moms, mode, capi.MOMENTUM_SPACE, capi.F64,
+                          out)
+    torch.cuda.synchronize()
+    h = out.to("cpu", torch.float64).numpy()
+    return (h[:, 0] + 1j * h[:, 1]).reshape(len(counts), TOTALT, len(moms))
+
+
+def _oracle(props, tab):
+    idxs, cols, vals, counts = tab
+    v = vals[:, 0].astype(np.float64) + 1j * vals[:, 1].astype(np.float64)
+    return O.tetra_correlator(L, SRC, TOTALT, Q2, [p[2] for p in props], idxs, cols, v, counts)
+
+
+def test_tetra_random_f64(props):
+    tab = _table(np.random.default_rng(4))
+    ref = _oracle(props, tab)
+    got = _run([p[1] for p in props], tab)
+    assert G.rel_err(got, ref, axis=-1) < 1e-12
+
+
+def test_tetra_repeated_views(props):
+    """one propagator in two slots (as propLT serves u and d)"""
+    tab = _table(np.random.default_rng(5), nops=3)
+    P = [props[0], props[0], props[2], props[2]]
+    ref = _oracle(P, tab)
+    got = _run([p[1] for p in P], tab)
+    assert G.rel_err(got, ref, axis=-1) < 1e-12
+
+
+def test_tetra_mixed(props):
+    tab = _table(np.random.default_rng(6), nops=4)
+    ref = _oracle(props, tab)
+    keep = [G.cast_f32(p[0], p[1], scaled=True) for p in props]
+    got = _run([k[1] for k in keep], tab, mode=capi.MODE_MIXED)
+    assert G.rel_err(got, ref, axis=-1) < 1e-6
+
+
+def test_tetra_refuses_other_patterns(props):
+    torch = G.torch_cuda()
+    out = torch.zeros((TOTALT * 33, 2), dtype=torch.float64, device="cuda")
+    idxs = np.zeros((1, 8), dtype=np.int16)
+    vals = np.ones((1, 2), dtype=np.float32)
+    for cols, code in (((0, 0, 1, 1, 2, 0, 3, 1), "UNSUPPORTED"), ((0, 2, 1, 3, 1, 0, 3, 2), "UNSUPPORTED"),
+                       ((0, 2, 1, 3, 2, 0, 3, 5), "INVALID")):
+        with pytest.raises(capi.PlgError, match=code):
+            capi.tetraquark_table(*[p[1] for p in props], idxs, np.array([cols], dtype=np.int16), vals,
+                                  np.ones(1, dtype=np.int16), SRC, TOTALT, capi.momlist(Q2), capi.MODE_F64,
+                                  capi.MOMENTUM_SPACE, capi.F64, out)

@@ -1,0 +1,62 @@
+"""Four-propagator table contractions (the tetraquark table format of
+lib/kernels/PLEGMA_heavy_light_tetraquarks.cu:26-49): the oracle's literal
+81-term colour sums against the closed forms the colour slots close into,
+for the four colour patterns the reference's tables use."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+# (c0..c7) of the reference's tables: two close as <B1,B3><B2,B4>, two as
+# Tr[(B3^dag B1)(B4^dag B2)]
+DIRECT = [(0, 2, 1, 3, 2, 0, 3, 1), (0, 3, 1, 2, 3, 0, 2, 1)]
+CROSSED = [(0, 2, 1, 3, 3, 0, 2, 1), (0, 3, 1, 2, 2, 0, 3, 1)]
+
+
+def _closed(S, m, pattern_direct):
+    B1 = S[0][m[0], m[1]]
+    B2 = S[1][m[2], m[3]]
+    B3 = S[2][(m[5] + 2) % 4, (m[4] + 2) % 4]
+    B4 = S[3][(m[7] + 2) % 4, (m[6] + 2) % 4]
+    if pattern_direct:
+        return np.sum(B1 * B3.conj()) * np.sum(B2 * B4.conj())
+    return np.trace((B3.conj().T @ B1) @ (B4.conj().T @ B2))
+
+
+@pytest.mark.parametrize("cols,direct", [(c, True) for c in DIRECT] + [(c, False) for c in CROSSED])
+def test_colour_patterns_close(cols, direct):
+    rng = np.random.default_rng(3)
+    S = [rng.standard_normal((4, 4, 3, 3)) + 1j * rng.standard_normal((4, 4, 3, 3)) for _ in range(4)]
+    raw = [np.ascontiguousarray(np.stack([s.real, s.imag], -1).reshape(-1)) for s in S]
+    for _ in range(6):
+        m = rng.integers(0, 4, size=8)
+        v = complex(rng.standard_normal(), rng.standard_normal())
+        idx = np.array(m, dtype=np.int32)
+        col = np.array(cols, dtype=np.int32)
+        val = np.array([v.real, v.imag])
+        out = np.zeros(2)
+        ip = O.C.POINTER(O.C.c_int)
+        O.lib().orc_site_tetra(*[O._ptr(r) for r in raw], idx.ctypes.data_as(ip), col.ctypes.data_as(ip),
+                               O._ptr(val), np.array([0], dtype=np.int32).ctypes.data_as(ip),
+                               np.array([1], dtype=np.int32).ctypes.data_as(ip), 1, O._ptr(out))
+        got = complex(out[0], out[1])
+        ref = v * _closed(S, m, direct)
+        assert abs(got - ref) <= 1e-12 * abs(ref)
+
+
+def test_unit_propagators():
+    """S = 1 (spin x colour): a direct term is 3 x 3 = 9 on the spin diagonal,
+    a crossed one Tr[1] = 3; off-diagonal spins give 0"""
+    S = np.zeros((4, 4, 3, 3), dtype=complex)
+    for mu in range(4):
+        S[mu, mu] = np.eye(3)
+    raw = np.ascontiguousarray(np.stack([S.real, S.imag], -1).reshape(-1))
+    ip = O.C.POINTER(O.C.c_int)
+    for cols, want in ((DIRECT[0], 9), (CROSSED[1], 3)):
+        for m, on in (((0, 0, 1, 1, 2, 2, 3, 3), True), ((0, 1, 1, 1, 2, 2, 3, 3), False)):
+            out = np.zeros(2)
+            O.lib().orc_site_tetra(*[O._ptr(raw)] * 4, np.array(m, dtype=np.int32).ctypes.data_as(ip),
+                                   np.array(cols, dtype=np.int32).ctypes.data_as(ip), O._ptr(np.array([1.0, 0.0])),
+                                   np.array([0], dtype=np.int32).ctypes.data_as(ip),
+                                   np.array([1], dtype=np.int32).ctypes.data_as(ip), 1, O._ptr(out))
+            assert complex(out[0], out[1]) == (want if on else 0)
