@@ -33,6 +33,12 @@ interpolators (200 operators), all 64 time slices, |n|^2 <= 9 (123 momenta);
 value = sites/s of the mixed path (FP32 contraction, split-bf16 projection),
 FP64 path and the mixed path's error beside it.
 
+Workload "tetra" (--workload tetra): the four-propagator table engine on
+32^3 x 64, one source, light / light / heavy / heavy propagators (as the
+reference's "udbb" product), a synthetic table of the reference's shape (16
+operators x 768 terms, its four colour patterns in its proportions), 64 time
+slices, 123 momenta; FP64 and mixed.
+
 Workload "learned": see run_learned.
 """
 from __future__ import annotations
@@ -532,6 +538,123 @@ def run_udsc(args, torch, capi, dev, stream):
     return res
 
 
+TETRA_PATTERNS = [((0, 2, 1, 3, 2, 0, 3, 1), 0), ((0, 2, 1, 3, 3, 0, 2, 1), 1), ((0, 3, 1, 2, 2, 0, 3, 1), 1),
+                  ((0, 3, 1, 2, 3, 0, 2, 1), 0)]
+TETRA_WEIGHTS = [9600, 9600, 3072, 3072]   # pattern counts of the reference's tables
+
+
+def tetra_table(nops=16, nterm=768):
+    """synthetic table with the reference's operator count, term count and
+    colour-pattern proportions"""
+    rng = np.random.default_rng(SEED)
+    w = np.array(TETRA_WEIGHTS, dtype=float) / sum(TETRA_WEIGHTS)
+    n = nops * nterm
+    pat = rng.choice(4, size=n, p=w)
+    cols = np.array([TETRA_PATTERNS[k][0] for k in pat], dtype=np.int16)
+    idxs = rng.integers(0, 4, size=(n, 8)).astype(np.int16)
+    vals = np.stack([rng.choice([-1.0, 1.0, 0.5], size=n), rng.choice([0.0, 1.0, -1.0], size=n)], -1).astype(np.float32)
+    counts = np.full(nops, nterm, dtype=np.int16)
+    direct = int(sum(1 for k in pat if TETRA_PATTERNS[k][1] == 0))
+    return idxs, cols, vals, counts, direct
+
+
+def cpu_baseline_tetra(tab, budget_s=12.0):
+    """Oracle (the reference's 81-term colour sums per term, FP64, + per-site
+    DFT, OpenMP) on an 8^3 spatial lattice: the per-site work does not depend
+    on the lattice size"""
+    from oracle import oracle as O
+    nth = min(os.cpu_count() or 1, 16)
+    Ls = [8, 8, 8, 16]
+    V3 = Ls[0] * Ls[1] * Ls[2]
+    P = [O.synth_prop(Ls, SEED, i, 0, lam, nthreads=nth) for i, lam in enumerate((0.5, 0.5, 1.2, 1.2))]
+    P[1] = P[0]
+    P[3] = P[2]
+    idxs, cols, vals, counts, _ = tab
+    v = vals[:, 0].astype(np.float64) + 1j * vals[:, 1]
+    t0 = time.perf_counter()
+    O.tetra_correlator(Ls, [1, 2, 3, 0], 1, Q2MAX4, P, idxs, cols, v, counts, nthreads=nth)
+    used = time.perf_counter() - t0
+    return {"value": V3 / used, "unit": "sites/s", "cores": nth, "kind": "port",
+            "sample": f"1 time slice x {V3} sites of an 8^3x16 lattice, 16 operators x 768 terms, 123 momenta, "
+                      f"FP64 81-term colour sums + per-site DFT, {used:.1f} s"}
+
+
+def run_tetra(args, torch, capi, dev, stream):
+    capi.init(L, 0)
+    V3 = L[0] * L[1] * L[2]
+    T = L[3]
+    src = [3, 5, 7, 10]
+    moms = capi.momlist(Q2MAX4)
+    log("synthesising light and heavy FP64 propagators and FP32 copies")
+    keep, v64, v32 = [], [], []
+    for fl, lam in enumerate((LAMBDA, LAMBDA_HEAVY)):
+        t64 = torch.empty((144, T * V3, 2), dtype=torch.float64, device=dev)
+        v = capi.plg_prop(t64.data_ptr(), capi.F64, 0, T, 0, T * V3)
+        capi.synth_prop(v, SEED, 60 + fl, src[3], lam)
+        t32 = torch.empty((144, T * V3, 2), dtype=torch.float32, device=dev)
+        e32 = torch.empty(T, dtype=torch.int32, device=dev)
+        w = capi.plg_prop(t32.data_ptr(), capi.F32, 0, T, 0, T * V3, e32.data_ptr())
+        capi.cast_prop(w, v)
+        keep += [t64, t32, e32]
+        v64.append(v)
+        v32.append(w)
+    torch.cuda.synchronize()
+    tab = tetra_table()
+    idxs, cols, vals, counts, ndirect = tab
+    nops = len(counts)
+    out_mx = torch.zeros((nops * T * len(moms), 2), dtype=torch.float64, device=dev)
+    out64 = torch.zeros_like(out_mx)
+    views32 = [v32[0], v32[0], v32[1], v32[1]]
+    views64 = [v64[0], v64[0], v64[1], v64[1]]
+
+    def mixed():
+        capi.tetraquark_table(*views32, idxs, cols, vals, counts, src, T, moms, capi.MODE_MIXED,
+                              capi.MOMENTUM_SPACE, capi.F64, out_mx, stream=stream)
+
+    def plain():
+        capi.tetraquark_table(*views64, idxs, cols, vals, counts, src, T, moms, capi.MODE_F64,
+                              capi.MOMENTUM_SPACE, capi.F64, out64, stream=stream)
+
+    nsites = V3 * T
+    log(f"mixed: {nsites} sites, {len(idxs)} terms; warmup {args.warmup}")
+    for _ in range(args.warmup):
+        mixed()
+    ms_mx, _ = timed(torch, stream, mixed, args.steps)
+    log(f"mixed: {ms_mx:.3f} ms/step")
+    # implemented complex MACs per site: 19 per direct term, 63 per crossed, 1 per coefficient
+    cmac = 19 * ndirect + 63 * (len(idxs) - ndirect) + len(idxs)
+    res = {"metric": "tetraquark table 2pt sites/s (16 operators x 768 terms), mixed precision",
+           "value": nsites / (ms_mx / 1e3), "unit": "sites/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms_mx, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32 contraction + bf16x6 MFMA projection (fp32 acc, fp64 sums)",
+           "data": "synthetic",
+           "config": {"workload": "tetra: lattice 32^3x64, one source, light/light/heavy/heavy propagators, "
+                                  "synthetic 16 x 768-term table in the reference's colour patterns, 64 time "
+                                  "slices, |n|^2<=9 (123 momenta)",
+                      "lattice": L, "sites": nsites, "operators": nops, "terms": int(len(idxs)),
+                      "nmom": len(moms), "parallelism": "single GPU"},
+           "contraction_valu": {"flops_per_site": 8 * cmac,
+                                "achieved_upper_bound_TFLOPs": nsites * 8 * cmac / (ms_mx / 1e3) / 1e12,
+                                "note": "whole step (contraction + projection) in the denominator"}}
+    if not args.no_fp64:
+        plain()
+        ms_64, _ = timed(torch, stream, plain, max(1, args.steps // 2))
+        torch.cuda.synchronize()
+        a = out_mx.cpu().numpy()
+        b = out64.cpu().numpy()
+        A = (a[:, 0] + 1j * a[:, 1]).reshape(nops, T, len(moms))
+        B = (b[:, 0] + 1j * b[:, 1]).reshape(nops, T, len(moms))
+        den = np.abs(B).max(axis=2, keepdims=True)
+        E = np.abs(A - B) / np.where(den == 0, 1.0, den)
+        res["fp64_plain"] = {"value": nsites / (ms_64 / 1e3), "unit": "sites/s", "ms_per_step": ms_64}
+        res["speedup_vs_fp64"] = res["value"] / res["fp64_plain"]["value"]
+        res["rel_err_vs_fp64"] = {"max": float(E.max()), "median": float(np.median(E))}
+    if not args.no_cpu:
+        log("cpu baseline")
+        res["cpu_baseline"] = cpu_baseline_tetra(tab)
+    return res
+
+
 def run_learned(args, torch, capi, dev, stream):
     """32^3 x 64, 256 sources with propagators base + 5% perturbation: exact
     two-point results (16 mesons + nucleon, |n|^2 <= 16) of every source by the
@@ -633,7 +756,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--split", type=int, default=6)
-    ap.add_argument("--workload", choices=["batching", "largest", "learned", "udsc"], default="batching")
+    ap.add_argument("--workload", choices=["batching", "largest", "learned", "udsc", "tetra"], default="batching")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
@@ -659,7 +782,7 @@ def main():
         torch.cuda.synchronize()
         del a, b
     stream = torch.cuda.Stream()
-    run = {"largest": run_largest, "learned": run_learned, "udsc": run_udsc}.get(args.workload, run_batching)
+    run = {"largest": run_largest, "learned": run_learned, "udsc": run_udsc, "tetra": run_tetra}.get(args.workload, run_batching)
     res = run(args, torch, capi, dev, stream)
     capi.finalize()
     print(json.dumps(res), flush=True)
