@@ -1247,7 +1247,7 @@ int plg_tetraquark_table(const plg_prop *p1, const plg_prop *p2, const plg_prop 
     if (!rc) rc = enqueue(P, g_ctx.ws_c, g_ctx.ws_part, out, out_precision, hs);
     if (!rc && hipStreamSynchronize(hs) != hipSuccess) rc = fail(PLG_ERR_HIP, "tetraquark table sync failed");
   }
-  if (rc) hipStreamSynchronize(hs);
+  hipStreamSynchronize(hs);  // no launch may still read the terms
   hipFree(d_terms);
   return rc;
 }
@@ -1299,6 +1299,7 @@ int plg_baryons_table(const plg_prop *p1, const plg_prop *p2, const plg_prop *p3
     rc = udsc_one(views, h.s, nops, 0, nops, source, totalT, moms, nmom, mode, space, out_precision, out, hs,
                   nullptr);
   }
+  hipStreamSynchronize(hs);  // no launch may still read the schedule
   for (void *p : mem) hipFree(p);
   return rc;
 }
