@@ -942,9 +942,11 @@ int plg_correlator(int kind, int group_mask, const plg_prop *p1, const plg_prop 
     if ((rc = ensure(&g_ctx.ws_part, &g_ctx.ws_part_bytes, part_bytes(P)))) return rc;
   }
   rc = enqueue(P, g_ctx.ws_c, g_ctx.ws_part, out, out_precision, st);
+  // the plan's device tables are freed on return: finish this correlator
+  // first, also after an error
+  const hipError_t se = hipStreamSynchronize(st);
   if (rc) return rc;
-  // the plan's device tables are freed on return: finish this correlator first
-  HIPCHK(hipStreamSynchronize(st));
+  if (se != hipSuccess) return fail(PLG_ERR_HIP, std::string("correlator sync: ") + hipGetErrorString(se));
   return PLG_OK;
 }
 
@@ -1126,8 +1128,9 @@ static int udsc_one(const plg_prop *const views[3], const UdscSched &S, int nops
   }
   StageLog log;
   rc = enqueue(P, g_ctx.ws_c, g_ctx.ws_part, out, out_precision, hs, stage_ms ? &log : nullptr);
-  // the plan's tables are freed on return
-  if (!rc && hipStreamSynchronize(hs) != hipSuccess) rc = fail(PLG_ERR_HIP, "udsc sync failed");
+  // the plan's tables are freed on return: wait for every launch, also after an error
+  const hipError_t se = hipStreamSynchronize(hs);
+  if (!rc && se != hipSuccess) rc = fail(PLG_ERR_HIP, "udsc sync failed");
   for (size_t i = 0; i + 1 < log.ev.size(); i++) {
     float t = 0;
     if (!rc) hipEventElapsedTime(&t, log.ev[i], log.ev[i + 1]);
@@ -1406,7 +1409,9 @@ int plg_ft_apply(const void *field, int precision, int dof, int nt, const int *m
     if (!rc) rc = launch_ft_out(g, (const double *)g_ctx.ws_part, dof, nt, accumulate, out, out_precision, st);
     if (rc) rc = fail(rc, "FT launch failed");
   }
-  if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(PLG_ERR_HIP, "FT synchronisation failed");
+  // the tables are freed below: wait for every launch, also after an error
+  const hipError_t se = hipStreamSynchronize(st);
+  if (!rc && se != hipSuccess) rc = fail(PLG_ERR_HIP, "FT synchronisation failed");
   for (void *p : {(void *)d_ms, (void *)d_pair_ms, (void *)d_mom_pair, (void *)d_pidx})
     if (p) hipFree(p);
   return rc;
