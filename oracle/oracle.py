@@ -343,7 +343,8 @@ def udsc_correlator(L, src, totalT, q2max, props, products, only_st=False, only_
     return names, (np.concatenate(blocks) if blocks else np.zeros((0,)))
 
 
-def tetra_correlator(L, src, totalT, q2max, props, idxs, cols, vals, counts, nthreads=0):
+def tetra_correlator(L, src, totalT, q2max, props, idxs, cols, vals, counts, nthreads=0, space="momentum",
+                     views=None):
     """contract_tetraquarks' term sums (lib/kernels/PLEGMA_heavy_light_tetraquarks.cu:26-49)
     on the CPU for one table: props = 4 float64 SoA propagators; idxs, cols
     int [n, 8]; vals [n] complex; counts [nops].  Returns H[k][j][m]."""
@@ -355,9 +356,10 @@ def tetra_correlator(L, src, totalT, q2max, props, idxs, cols, vals, counts, nth
     count = np.asarray(counts, dtype=np.int32)
     first = np.concatenate([[0], np.cumsum(count)[:-1]]).astype(np.int32)
     nops = len(count)
-    P = [OrcProp(p.ctypes.data, L[3] * V3, 0, L[3]) for p in props]
+    views = views or [(L[3] * V3, 0, L[3])] * 4
+    P = [OrcProp(p.ctypes.data, v[0], v[1], v[2]) for p, v in zip(props, views)]
     ip = C.POINTER(C.c_int)
-    H = np.zeros((nops, lt, len(momlist(q2max))), dtype=complex)
+    H = np.zeros((nops, lt, len(momlist(q2max)) if space == "momentum" else V3), dtype=complex)
     for j in range(lt):
         t = row_time(L, src, totalT, j)
         f = np.zeros(V3 * nops * 2)
@@ -367,5 +369,6 @@ def tetra_correlator(L, src, totalT, q2max, props, idxs, cols, vals, counts, nth
         if rc:
             raise RuntimeError(f"orc_tetra_field failed: {rc}")
         f = f.reshape(V3, nops, 2)
-        H[:, j] = dft(L, src[:3], q2max, f[..., 0] + 1j * f[..., 1], nthreads).T
+        f = f[..., 0] + 1j * f[..., 1]
+        H[:, j] = (dft(L, src[:3], q2max, f, nthreads) if space == "momentum" else f).T
     return H

@@ -52,23 +52,26 @@ def _table(rng, nops=5, nterm=60):
             np.array(counts, dtype=np.int16))
 
 
-def _run(views, tab, mode=capi.MODE_F64):
+def _run(views, tab, mode=capi.MODE_F64, space=capi.MOMENTUM_SPACE, out_prec=capi.F64):
     torch = G.torch_cuda()
-    moms = capi.momlist(Q2)
+    moms = capi.momlist(Q2) if space == capi.MOMENTUM_SPACE else []
+    inner = len(moms) if space == capi.MOMENTUM_SPACE else L[0] * L[1] * L[2]
     idxs, cols, vals, counts = tab
-    out = torch.full((len(counts) * TOTALT * len(moms), 2), float("nan"), dtype=torch.float64, device="cuda")
-    print(f"[gpu] tetraquark table: {len(counts)} operators, {len(idxs)} terms, mode {mode}", flush=True)
-    capi.tetraquark_table(*views, idxs, cols, vals, counts, SRC, TOTALT, moms, mode, capi.MOMENTUM_SPACE, capi.F64,
-                          out)
+    dt = torch.float64 if out_prec == capi.F64 else torch.float32
+    out = torch.full((len(counts) * TOTALT * inner, 2), float("nan"), dtype=dt, device="cuda")
+    print(f"[gpu] tetraquark table: {len(counts)} operators, {len(idxs)} terms, mode {mode}, space {space}",
+          flush=True)
+    capi.tetraquark_table(*views, idxs, cols, vals, counts, SRC, TOTALT, moms, mode, space, out_prec, out)
     torch.cuda.synchronize()
     h = out.to("cpu", torch.float64).numpy()
-    return (h[:, 0] + 1j * h[:, 1]).reshape(len(counts), TOTALT, len(moms))
+    return (h[:, 0] + 1j * h[:, 1]).reshape(len(counts), TOTALT, inner)
 
 
-def _oracle(props, tab):
+def _oracle(props, tab, space="momentum", hosts=None, views=None):
     idxs, cols, vals, counts = tab
     v = vals[:, 0].astype(np.float64) + 1j * vals[:, 1].astype(np.float64)
-    return O.tetra_correlator(L, SRC, TOTALT, Q2, [p[2] for p in props], idxs, cols, v, counts)
+    return O.tetra_correlator(L, SRC, TOTALT, Q2, hosts or [p[2] for p in props], idxs, cols, v, counts,
+                              space=space, views=views)
 
 
 def test_tetra_random_f64(props):
@@ -106,3 +109,30 @@ def test_tetra_refuses_other_patterns(props):
             capi.tetraquark_table(*[p[1] for p in props], idxs, np.array([cols], dtype=np.int16), vals,
                                   np.ones(1, dtype=np.int16), SRC, TOTALT, capi.momlist(Q2), capi.MODE_F64,
                                   capi.MOMENTUM_SPACE, capi.F64, out)
+
+
+def test_tetra_position_space_f32_output(props):
+    tab = _table(np.random.default_rng(7), nops=3, nterm=30)
+    ref = _oracle(props, tab, space="position")
+    got = _run([p[1] for p in props], tab, space=capi.POSITION_SPACE, out_prec=capi.F32)
+    assert G.rel_err(got, ref, axis=-1) < 1e-6
+
+
+def test_tetra_window_views(props):
+    """propagators stored over the source's window only (t_first, nt): the
+    rows of the correlator are t = 6, 7, 0, 1, 2"""
+    torch = G.torch_cuda()
+    tab = _table(np.random.default_rng(8), nops=3, nterm=30)
+    V3 = L[0] * L[1] * L[2]
+    nt, tf = 5, 6
+    views, hosts, keep = [], [], []
+    for i, lam in enumerate((0.5, 0.6, 1.1, 1.3)):
+        t = torch.empty((144, nt * V3, 2), dtype=torch.float64, device="cuda")
+        v = capi.plg_prop(t.data_ptr(), capi.F64, tf, nt, 0, nt * V3)
+        capi.synth_prop(v, 91, 30 + i, SRC[3], lam)
+        keep.append(t)
+        views.append(v)
+        hosts.append(G.host(t))
+    ref = _oracle(props, tab, hosts=hosts, views=[(nt * V3, tf, nt)] * 4)
+    got = _run(views, tab)
+    assert G.rel_err(got, ref, axis=-1) < 1e-12
