@@ -44,6 +44,7 @@ Workload "learned": see run_learned.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -64,6 +65,9 @@ HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFS = 2500.0
 FP64_PEAK_TFS = 78.6
 FP32_VALU_PEAK_TFS = 157.3
+# FP32 FMA rate without packed (two-wide) FP32 instructions, which this build
+# does not emit
+FP32_VALU_UNPACKED_TFS = 78.6
 
 # implemented arithmetic per site of the fused mixed contraction kernel, in
 # complex MACs per lane (16 lanes per site), 8 flop per CMAC:
@@ -92,39 +96,60 @@ def sources(rng):
     return src
 
 
-def cpu_baseline(budget_s=12.0, max_rows=8):
+def baseline_threads():
+    """threads of the CPU baseline: the process's CPU share (OMP_NUM_THREADS
+    when set, which the GPU host sets to the cores of one GPU's share, else
+    the CPUs this process may run on)"""
+    e = os.environ.get("OMP_NUM_THREADS", "")
+    if e.isdigit() and int(e) > 0:
+        return int(e)
+    return max(1, len(os.sched_getaffinity(0)))
+
+
+def cpu_info():
+    """what the CPU baseline ran on: nproc, this process's CPUs, OMP_NUM_THREADS
+    and the CPU model (/proc/cpuinfo)"""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "model": model}
+
+
+def cpu_baseline(P, view, x0, w, rows):
     """Oracle (reference per-site bodies in FP64 + naive per-site DFT, OpenMP)
-    on time slices of the first source until ~budget_s of CPU work."""
+    on the given rows of the first source, from the host copy P of the source's
+    FP64 propagator slices; returns the baseline and the oracle's rows (the
+    reference of the error check against the oracle)"""
     from oracle import oracle as O
-    nth = min(os.cpu_count() or 1, 16)
-    rng = np.random.default_rng(SEED)
-    (x0, w) = sources(rng)[0]
+    nth = baseline_threads()
     mask = O.G_NUCL | sum(O.G_DELTA32)
-    P = O.synth_prop(L, SEED, 0, x0[3], LAMBDA, t_first=x0[3], nt=min(w, max_rows),
-                     nthreads=nth)
-    view = (min(w, max_rows) * L[0] * L[1] * L[2], x0[3], min(w, max_rows))
     V3 = L[0] * L[1] * L[2]
-    done, t_used = 0, 0.0
-    for j in range(min(w, max_rows)):
+    R, t_used = {}, 0.0
+    for j in rows:
         t0 = time.perf_counter()
-        O.correlator(O.TWOPT, L, x0, w, Q2MAX, P, P, mask=mask, rows=[j], nthreads=nth,
-                     view1=view, view2=view)
+        H = O.correlator(O.TWOPT, L, x0, w, Q2MAX, P, P, mask=mask, rows=[j], nthreads=nth,
+                         view1=view, view2=view)
         t_used += time.perf_counter() - t0
-        done += 1
-        if t_used >= budget_s:
-            break
-    rate = done * V3 / t_used
-    return {"value": rate, "unit": "sites/s", "cores": nth, "kind": "port",
-            "sample": f"source 0 of the workload, {done} time slice(s) x {V3} sites, "
+        R[j] = H[:, :, j]
+    rate = len(rows) * V3 / t_used
+    return {"value": rate, "unit": "sites/s", "cores": nth, "kind": "port", **cpu_info(),
+            "sample": f"source 0 of the workload, {len(rows)} time slice(s) x {V3} sites, "
                       f"16 mesons + proton + Delta++ x3, 257 momenta, FP64 reference "
-                      f"bodies + per-site DFT, {t_used:.1f} s"}
+                      f"bodies + per-site DFT, {t_used:.1f} s on {nth} threads"}, R
 
 
 def cpu_baseline_largest(budget_s=12.0):
     """Oracle meson-meson (FP64 12x12 products per site + per-site DFT,
     OpenMP) on one time slice of the 64^3 x 128 workload"""
     from oracle import oracle as O
-    nth = min(os.cpu_count() or 1, 16)
+    nth = baseline_threads()
     V3 = L4[0] * L4[1] * L4[2]
     t = 3
     PL = O.synth_prop(L4, SEED, 0, 0, LAMBDA, t_first=t, nt=1, nthreads=nth)
@@ -134,7 +159,7 @@ def cpu_baseline_largest(budget_s=12.0):
     O.correlator(O.MESON_MESON, L4, [0, 0, 0, 0], L4[3], Q2MAX4, PL, PH, rows=[t], nthreads=nth,
                  view1=view, view2=view)
     used = time.perf_counter() - t0
-    return {"value": V3 / used, "unit": "sites/s", "cores": nth, "kind": "port",
+    return {"value": V3 / used, "unit": "sites/s", "cores": nth, "kind": "port", **cpu_info(),
             "sample": f"time slice t={t} of the 64^3x128 workload ({V3} sites), 16 mesons + "
                       f"4 meson-meson channels, 123 momenta, FP64 12x12 products + per-site "
                       f"DFT, {used:.1f} s"}
@@ -160,43 +185,81 @@ def stage_ms(batch, out, stream, reps=3):
     return {name: float(np.median([dict(r)[name] for r in runs])) for name, _ in runs[0]}
 
 
+def csrc_sha():
+    """hash of the kernel sources and build recipe (plegma_amd/csrc, the udsc
+    table generator, the build flags): counter readings taken on another
+    build are not used"""
+    h = hashlib.sha256()
+    d = os.path.join(ROOT, "plegma_amd")
+    files = sorted(f for f in os.listdir(os.path.join(d, "csrc"))
+                   if f.endswith((".hip", ".h")) and f != "plg_udsc_tables.h")
+    for f in [os.path.join("csrc", f) for f in files] + ["gen_udsc.py", "build.py"]:
+        h.update(f.encode())
+        with open(os.path.join(d, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(workload, stage):
     """HBM bytes per launch of a stage from the committed counter summary
-    (profiles/traffic.json), or None"""
+    (profiles/traffic.json): (bytes, kernel, note), bytes None when the
+    summary was taken on other kernel sources than these"""
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
-            e = json.load(f).get(workload, {}).get(stage)
-        return (e["bytes"], e["kernel"]) if e else None
-    except (OSError, ValueError, KeyError):
-        return None
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None, None, "no counter summary"
+    e = d.get(workload, {}).get(stage)
+    if not e:
+        return None, None, "stage not in the counter summary"
+    if d.get("csrc_sha") != csrc_sha():
+        return None, e.get("kernel"), (f"stale: counters of kernel sources {d.get('csrc_sha')}, "
+                                       f"this build is {csrc_sha()}")
+    return e["bytes"], e["kernel"], f"rocprofv3 FETCH_SIZE/WRITE_SIZE of {e['kernel']} " \
+                                    f"(profiles/traffic.json, kernel sources {d.get('csrc_sha')})"
+
+
+def roof(kernel, t_s, nsites, flops_site, bytes_site, bound_compute, cpeak, workload, algorithmic):
+    """a kernel against its roofline.  The binding roof is the one the
+    kernel's arithmetic intensity (flops per algorithmic HBM byte) puts it
+    under: compute when the intensity exceeds the ridge peak/HBM, else HBM;
+    the other roof is reported beside it."""
+    tf = nsites * flops_site / t_s / 1e12
+    gbs = nsites * bytes_site / t_s / 1e9
+    ai = flops_site / bytes_site
+    ridge = cpeak * 1e12 / (HBM_PEAK_GBS * 1e9)
+    comp = {"achieved": tf, "peak": cpeak, "unit": "TFLOP/s", "frac": tf / cpeak, "flops_per_site": flops_site}
+    if bound_compute == "valu":
+        comp["peak_unpacked_fp32"] = FP32_VALU_UNPACKED_TFS
+        comp["frac_unpacked_fp32"] = tf / FP32_VALU_UNPACKED_TFS
+    hbm = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+           "bytes_per_site": bytes_site}
+    tr, kname, note = measured_traffic(workload, kernel)
+    if ai > ridge:
+        r = {"kernel": kernel, "bound": bound_compute, **comp, "hbm": hbm}
+    else:
+        r = {"kernel": kernel, "bound": "hbm", **hbm, bound_compute: comp}
+    r.update({"traffic": tr, "traffic_source": note,
+              "arithmetic_intensity": ai, "ridge": ridge, "algorithmic": algorithmic,
+              "time_ms": t_s * 1e3})
+    if kname:
+        r["kernel_symbol"] = kname
+    return r
 
 
 def roofline(st_ms, nsites, nmom, nch, split, contract_flops, prop_bytes, workload):
-    """the dominant stage against its roofline: the contraction against HBM
-    (compulsory propagator read + site-correlator write) with its VALU rate
-    beside it, the projection against BF16 MFMA (useful flops)"""
+    """the dominant stage against its roofline: a contraction (FP32 VALU; HBM:
+    compulsory propagator read + site-correlator write) or the projection
+    (BF16 MFMA, useful flops; HBM: site-correlator read)"""
     dom = max(st_ms, key=st_ms.get)
     t = st_ms[dom] / 1e3
-    tr = measured_traffic(workload, dom)
     if dom.startswith("contract"):
-        byts = nsites * (prop_bytes + nch * 8)
-        achieved = byts / t / 1e9
-        r = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": tr[0] if tr else None,
-             "algorithmic": f"{prop_bytes} B FP32 propagator read + {nch * 8} B site "
-                            f"correlator write per site",
-             "valu": {"achieved": nsites * contract_flops / t / 1e12, "peak": FP32_VALU_PEAK_TFS,
-                      "unit": "TFLOP/s", "flops_per_site": contract_flops}}
-        r["valu"]["frac"] = r["valu"]["achieved"] / FP32_VALU_PEAK_TFS
-    else:
-        useful = nsites * 8 * nmom * nch
-        achieved = useful / t / 1e12
-        r = {"kernel": dom, "bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFS,
-             "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFS, "traffic": tr[0] if tr else None,
-             "algorithmic": f"8 flop x {nmom} momenta x {nch} channels per site (useful)",
-             "executed_frac": achieved * split / BF16_PEAK_TFS}
-    if tr:
-        r["traffic_source"] = f"rocprofv3 FETCH_SIZE/WRITE_SIZE of {tr[1]} (profiles/traffic.json)"
+        return roof(dom, t, nsites, contract_flops, prop_bytes + nch * 8, "valu", FP32_VALU_PEAK_TFS, workload,
+                    f"{contract_flops} implemented FP32 flop; {prop_bytes} B FP32 propagator read + "
+                    f"{nch * 8} B site correlator write per site")
+    r = roof(dom, t, nsites, 8 * nmom * nch, nch * 8, "mfma", BF16_PEAK_TFS, workload,
+             f"useful 8 flop x {nmom} momenta x {nch} channels per site; {nch * 8} B site correlator read")
+    r["executed_frac"] = r.get("frac", 0) * split
     return r
 
 
@@ -206,6 +269,45 @@ def rel_err_block(A, B):
     den = np.abs(B).max(axis=2, keepdims=True)
     den = np.where(den == 0, 1.0, den)
     return (np.abs(A - B) / den).max(axis=(1, 3))
+
+
+def shell_err_block(A, B, moms):
+    """[group, momentum] max over time slices and channels of |A - B| / rms
+    of |B| over the momenta of the same |n|^2 shell: the error per channel and
+    momentum at the scale of a correlator at that |n| (the n = 0 value does
+    not set the scale at |n| > 0, an accidental zero at one momentum does not
+    either); A, B: [group, row, momentum, channel]"""
+    q2 = (np.asarray(moms).reshape(-1, 3) ** 2).sum(1)
+    den = np.empty(B.shape)
+    for q in np.unique(q2):
+        sel = q2 == q
+        den[:, :, sel, :] = np.sqrt((np.abs(B[:, :, sel, :]) ** 2).mean(axis=2, keepdims=True))
+    den = np.where(den == 0, 1.0, den)
+    return (np.abs(A - B) / den).max(axis=(1, 3))
+
+
+def point_err(A, B):
+    """pointwise |A - B| / |B| over nonzero B: median, 99.9th percentile, max"""
+    b = np.abs(B).reshape(-1)
+    q = np.abs(A - B).reshape(-1)[b > 0] / b[b > 0]
+    return {"median": float(np.median(q)), "p99.9": float(np.quantile(q, 0.999)), "max": float(q.max())}
+
+
+def err_summary(A, B, moms):
+    """normwise, per (channel, momentum) at the shell scale, and pointwise
+    errors of A against the reference B ([group, row, momentum, channel])"""
+    E = rel_err_block(A, B)
+    S = shell_err_block(A, B, moms)
+    return {"normwise_max": float(E.max()), "normwise_median": float(np.median(E)),
+            "per_channel_momentum_max": float(S.max()),
+            "per_channel_momentum_median": float(np.median(S)),
+            "pointwise": point_err(A, B)}
+
+
+ERR_DEFS = {"normwise": "|C - C_ref| / max over momenta |C_ref| per (group, row, channel)",
+            "per_channel_momentum": "|C - C_ref| per (group, row, channel, momentum) / rms of |C_ref| "
+                                    "over the momenta of the same |n|^2 shell",
+            "pointwise": "|C - C_ref| / |C_ref| per element (unbounded at accidental zeros of C_ref)"}
 
 
 def run_batching(args, torch, capi, dev, stream):
@@ -281,32 +383,76 @@ def run_batching(args, torch, capi, dev, stream):
         st64 = {name: ms for name, ms in plain[0].profile(outs64[0], stream)}
         torch.cuda.synchronize()
         hm = out_mx.cpu().numpy()
-        errs = []
+        hm = hm[:, 0] + 1j * hm[:, 1]
+        As, Bs = [], []
         for (x0, w, s, o), n in zip(jobs, sizes):
-            a = hm[o:o + n]
             b = outs64[s].cpu().numpy()
-            A = (a[:, 0] + 1j * a[:, 1]).reshape(5, w, len(moms), 16)
-            B = (b[:, 0] + 1j * b[:, 1]).reshape(5, w, len(moms), 16)
-            errs.append(rel_err_block(A, B))
-        E = np.max(np.stack(errs), axis=0)
+            As.append(hm[o:o + n].reshape(5, w, len(moms), 16))
+            Bs.append((b[:, 0] + 1j * b[:, 1]).reshape(5, w, len(moms), 16))
+        A = np.concatenate(As, axis=1)
+        B = np.concatenate(Bs, axis=1)
         res["fp64_plain"] = {"value": nsites / (ms_64 / 1e3), "unit": "sites/s",
                              "ms_per_step": ms_64, "steps": k64,
-                             "stages_ms_source0": st64, "sites_source0": srcs[0][1] * V3}
+                             "stages_ms_source0": st64, "sites_source0": srcs[0][1] * V3,
+                             "algorithm": "the mixed path's: factorised contraction, projection on "
+                                          "(n, -n) momentum pairs; FP64 throughout"}
         res["speedup_vs_fp64"] = ms_64 / ms_mx
-        res["rel_err_vs_fp64"] = {"max": float(E.max()), "median": float(np.median(E)),
-                                  "per_group_max": [float(x) for x in E.max(axis=1)],
-                                  "definition": "max over sources, time slices and channels of "
-                                                "|C_mixed - C_fp64| / max_momenta |C_fp64|, "
-                                                "per group and momentum"}
-        del plain
+        res["rel_err_vs_fp64"] = {**err_summary(A, B, moms),
+                                  "per_group_max": [float(x) for x in rel_err_block(A, B).max(axis=1)],
+                                  "scope": "every source, time slice, group, channel and momentum",
+                                  "definitions": ERR_DEFS}
+        del plain, A, B, As, Bs
+        res["dropin"] = run_dropin(args, torch, capi, stream, srcs, v32, v64, sizes, moms, kind, mask, dev)
 
-    res["roofline"] = roofline(st_ms, nsites, len(moms), 80, args.split, CONTRACT_FLOPS_PER_SITE,
-                               144 * 8, "batching")
     res["stages_ms"] = st_ms
     if not args.no_cpu:
-        log("cpu baseline")
-        res["cpu_baseline"] = cpu_baseline()
+        # the oracle on rows of source 0, from the same FP64 propagator bytes
+        x0, w = srcs[0]
+        rows = list(range(min(w, 6)))
+        planes = sorted({(capi.row_time(x0, w, j) - x0[3]) % L[3] for j in rows})
+        assert planes == list(range(len(planes))), planes
+        np_ = len(planes)
+        P = np.ascontiguousarray(keep[0].view(144, w, V3, 2)[:, :np_].cpu().numpy().reshape(-1))
+        log("cpu baseline (oracle on source 0)")
+        res["cpu_baseline"], R = cpu_baseline(P, (np_ * V3, x0[3], np_), x0, w, rows)
+        Ro = np.stack([R[j] for j in rows], axis=2)[:, 0]            # [group, row, mom, ch]
+        hmx = out_mx.cpu().numpy()
+        hmx = (hmx[:, 0] + 1j * hmx[:, 1])[:sizes[0]].reshape(5, w, len(moms), 16)[:, rows]
+        err = {"rows": rows, "source": 0, "mixed": err_summary(hmx, Ro, moms), "definitions": ERR_DEFS}
+        if not args.no_fp64:
+            h64 = outs64[0].cpu().numpy()
+            h64 = (h64[:, 0] + 1j * h64[:, 1]).reshape(5, w, len(moms), 16)[:, rows]
+            err["fp64_normwise_max"] = float(rel_err_block(h64, Ro).max())
+        res["rel_err_vs_oracle"] = err
+    res["roofline"] = roofline(st_ms, nsites, len(moms), 80, args.split, CONTRACT_FLOPS_PER_SITE,
+                               144 * 8, "batching")
     mixed.close()
+    return res
+
+
+def run_dropin(args, torch, capi, stream, srcs, v32, v64, sizes, moms, kind, mask, dev):
+    """the path a PLEGMA driver takes: one plg_correlator call per source (the
+    C ABI under PLEGMA_Correlator::contract*), as plegma/Calc_2pt.cpp:121-132
+    loops over sources; each call synchronises its stream, plans are cached"""
+    V3 = L[0] * L[1] * L[2]
+    outs = [torch.empty((n, 2), dtype=torch.float64, device=dev) for n in sizes]
+    nsites = sum(w for _, w in srcs) * V3
+    res = {"note": "plg_correlator source by source (cached plans, autotuned tiles, one stream "
+                   "synchronisation per call)"}
+    for name, views, mode in (("mixed", v32, capi.MODE_MIXED), ("fp64", v64, capi.MODE_F64)):
+        def step():
+            for (x0, w), v, o in zip(srcs, views, outs):
+                capi.correlator(kind, mask, v, v, x0, w, moms, mode, capi.MOMENTUM_SPACE, capi.F64, o, stream)
+        step()  # plans and tuning of every source
+        k = max(1, args.steps // 2)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / k
+        log(f"drop-in {name}: {ms:.3f} ms/step")
+        res[name] = {"value": nsites / (ms / 1e3), "unit": "sites/s", "ms_per_step": ms, "steps": k}
     return res
 
 
@@ -428,7 +574,7 @@ def cpu_baseline_udsc(budget_s=12.0):
     on the lattice size"""
     from oracle import oracle as O
     from plegma_amd import gen_udsc
-    nth = min(os.cpu_count() or 1, 16)
+    nth = baseline_threads()
     Ls = [8, 8, 8, 16]
     V3 = Ls[0] * Ls[1] * Ls[2]
     props = {f: O.synth_prop(Ls, SEED, i, 0, lam, nthreads=nth) for i, (f, lam) in enumerate(zip("udsc", UDSC_LAMBDAS))}
@@ -441,7 +587,7 @@ def cpu_baseline_udsc(budget_s=12.0):
         done = totalT
         if used * 2.2 > budget_s:
             break
-    return {"value": done * V3 / used, "unit": "sites/s", "cores": nth, "kind": "port",
+    return {"value": done * V3 / used, "unit": "sites/s", "cores": nth, "kind": "port", **cpu_info(),
             "sample": f"{done} time slice(s) x {V3} sites of an 8^3x16 lattice, 20 flavour products "
                       f"(200 operators), 123 momenta, FP64 per-term epsilon sums + per-site DFT, {used:.1f} s"}
 
@@ -516,21 +662,13 @@ def run_udsc(args, torch, capi, dev, stream):
                                   "definition": "max over operators, time slices and momenta of "
                                                 "|C_mixed - C_fp64| / max_momenta |C_fp64|"}
     dom = max(st_ms, key=st_ms.get)
-    t = st_ms[dom] / 1e3
     prop_bytes = 20 * 3 * 144 * 8
-    byts = nsites * (prop_bytes + nops * 8)
-    r = {"kernel": dom, "bound": "hbm", "achieved": byts / t / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-         "traffic": None,
-         "algorithmic": f"20 products x 3 x 1152 B FP32 propagator reads + {nops} x 8 B site correlator "
-                        f"writes per site",
-         "valu": {"achieved": nsites * 8 * cmac / t / 1e12, "peak": FP32_VALU_PEAK_TFS, "unit": "TFLOP/s",
-                  "flops_per_site": 8 * cmac}}
-    r["frac"] = r["achieved"] / HBM_PEAK_GBS
-    r["valu"]["frac"] = r["valu"]["achieved"] / FP32_VALU_PEAK_TFS
-    tr = measured_traffic("udsc", dom)
-    if tr:
-        r["traffic"] = tr[0]
-        r["traffic_source"] = f"rocprofv3 FETCH_SIZE/WRITE_SIZE of {tr[1]} (profiles/traffic.json)"
+    if dom.startswith("contract"):
+        r = roof(dom, st_ms[dom] / 1e3, nsites, 8 * cmac, prop_bytes + nops * 8, "valu", FP32_VALU_PEAK_TFS,
+                 "udsc", f"{8 * cmac} implemented FP32 flop over the 20 products; 20 products x 3 x 1152 B "
+                         f"FP32 propagator reads + {nops} x 8 B site correlator writes per site")
+    else:
+        r = roofline(st_ms, nsites, len(moms), nops, 6, 0, 0, "udsc")
     res["roofline"] = r
     if not args.no_cpu:
         log("cpu baseline")
@@ -563,7 +701,7 @@ def cpu_baseline_tetra(tab, budget_s=12.0):
     DFT, OpenMP) on an 8^3 spatial lattice: the per-site work does not depend
     on the lattice size"""
     from oracle import oracle as O
-    nth = min(os.cpu_count() or 1, 16)
+    nth = baseline_threads()
     Ls = [8, 8, 8, 16]
     V3 = Ls[0] * Ls[1] * Ls[2]
     P = [O.synth_prop(Ls, SEED, i, 0, lam, nthreads=nth) for i, lam in enumerate((0.5, 0.5, 1.2, 1.2))]
@@ -574,7 +712,7 @@ def cpu_baseline_tetra(tab, budget_s=12.0):
     t0 = time.perf_counter()
     O.tetra_correlator(Ls, [1, 2, 3, 0], 1, Q2MAX4, P, idxs, cols, v, counts, nthreads=nth)
     used = time.perf_counter() - t0
-    return {"value": V3 / used, "unit": "sites/s", "cores": nth, "kind": "port",
+    return {"value": V3 / used, "unit": "sites/s", "cores": nth, "kind": "port", **cpu_info(),
             "sample": f"1 time slice x {V3} sites of an 8^3x16 lattice, 16 operators x 768 terms, 123 momenta, "
                       f"FP64 81-term colour sums + per-site DFT, {used:.1f} s"}
 
@@ -655,21 +793,33 @@ def run_tetra(args, torch, capi, dev, stream):
     return res
 
 
-def run_learned(args, torch, capi, dev, stream):
-    """32^3 x 64, 256 sources with propagators base + 5% perturbation: exact
-    two-point results (16 mesons + nucleon, |n|^2 <= 16) of every source by the
-    plain FP64 path, then the learned flow: cheap inputs (zero momentum) of all
-    sources, exact labels of 64, a network trained on 32, bias-corrected on 32,
-    predicting 192; errors against the exact results of the predicted sources"""
+def learned_flow(torch, capi, dev, stream, Lat=L, nsrc=256, split=(32, 32), steps=2000, seed=SEED + 5,
+                 log_every=64):
+    """The learned workload on lattice Lat with nsrc sources, propagators a
+    shared seeded base plus a 5% seeded per-source perturbation:
+      exact:  the plain FP64 path (16 mesons + nucleon, |n|^2 <= 16) of every
+              source (the labels, the truth the estimators are judged by, and
+              the fallback);
+      cheap:  the mixed path at n = 0 only (16 mesons + nucleon in FP32
+              contraction), every source;
+      learned flow: cheap inputs of all sources + exact labels of the first
+              split[0] + split[1] sources, the network trained on the first
+              split[0] (its time slices are the samples), bias-corrected on
+              the next split[1], predicting the rest.
+    Features and targets are in source-relative time; the targets have the
+    source phase removed.  Returns (result dict, arrays)."""
     from plegma_amd import learned
-    capi.init(L, 0)
-    V3, T = L[0] * L[1] * L[2], L[3]
-    rng = np.random.default_rng(SEED + 5)
-    srcs = [[int(rng.integers(0, L[i])) for i in range(4)] for _ in range(256)]
+    capi.init(Lat, 0)
+    V3, T = Lat[0] * Lat[1] * Lat[2], Lat[3]
+    rng = np.random.default_rng(seed)
+    srcs = [[int(rng.integers(0, Lat[i])) for i in range(4)] for _ in range(nsrc)]
     moms = capi.momlist(Q2MAX)
     kind, mask = capi.TWOPT, capi.G_NUCL
     t64 = torch.empty((144, T * V3, 2), dtype=torch.float64, device=dev)
     v = capi.plg_prop(t64.data_ptr(), capi.F64, 0, T, 0, T * V3)
+    t32 = torch.empty((144, T * V3, 2), dtype=torch.float32, device=dev)
+    e32 = torch.empty(T, dtype=torch.int32, device=dev)
+    w = capi.plg_prop(t32.data_ptr(), capi.F32, 0, T, 0, T * V3, e32.data_ptr())
     n = capi.correlator_size(kind, mask, srcs[0], T, len(moms))
     out = torch.empty((n, 2), dtype=torch.float64, device=dev)
     n0 = capi.correlator_size(kind, mask, srcs[0], T, 1)
@@ -683,69 +833,92 @@ def run_learned(args, torch, capi, dev, stream):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1)
 
-    log("exact FP64 two-point results of 256 sources (base + 5% perturbation)")
     X, Y, t_exact, t_cheap = [], [], 0.0, 0.0
     for s, src in enumerate(srcs):
-        capi.synth_prop_perturbed(v, SEED + 5, 1000, 1 + s, 0.05, src[3], LAMBDA, stream)
+        capi.synth_prop_perturbed(v, seed, 1000, 1 + s, 0.05, src[3], LAMBDA, stream)
         t_exact += ev_ms(lambda: capi.correlator(kind, mask, v, v, src, T, moms, capi.MODE_F64,
                                                  capi.MOMENTUM_SPACE, capi.F64, out, stream))
-        if s < 8:
-            t_cheap += ev_ms(lambda: capi.correlator(kind, mask, v, v, src, T, [(0, 0, 0)],
-                                                     capi.MODE_F64, capi.MOMENTUM_SPACE, capi.F64,
-                                                     out0, stream))
+        capi.cast_prop(w, v, stream)
+        t_cheap += ev_ms(lambda: capi.correlator(kind, mask, w, w, src, T, [(0, 0, 0)], capi.MODE_MIXED,
+                                                 capi.MOMENTUM_SPACE, capi.F64, out0, stream))
+        times = [capi.row_time(src, T, j) for j in range(T)]
         h = out.cpu().numpy()
-        x, y = learned.features_targets(h[:, 0] + 1j * h[:, 1], moms, T)
-        X.append(x)
-        Y.append(y)
-        if s % 64 == 63:
+        h0 = out0.cpu().numpy()
+        x, a = learned.cheap_features(h0[:, 0] + 1j * h0[:, 1], times, src, T)
+        X.append(learned.RowwiseNet.pack(x, a))
+        Y.append(learned.shell_targets(h[:, 0] + 1j * h[:, 1], moms, times, src, Lat).reshape(-1))
+        if (s + 1) % log_every == 0:
             log(f"  {s + 1} sources")
+    del t64, t32, e32
     X, Y = np.array(X), np.array(Y)
-    t_cheap = t_cheap / 8 * 256
-    tr, bc, pr = slice(0, 32), slice(32, 64), slice(64, 256)
-    log("training the network on 32 sources")
-    net = learned.CorrelatorNet(X.shape[1], Y.shape[1], hidden=(256, 256), seed=SEED, max_batch=256)
+    ntr, nbc = split
+    nlab = ntr + nbc
+    npred = nsrc - nlab
+    tr, bc, pr = slice(0, ntr), slice(ntr, nlab), slice(nlab, nsrc)
+    log(f"training the network on the {ntr * T} time slices of {ntr} sources")
+    net = learned.RowwiseNet(T, hidden=(256, 256), seed=SEED, max_batch=max(4096, ntr * T))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    final_loss = net.fit(X[tr], Y[tr], steps=2000, lr=1e-3)
+    final_loss = net.fit(X[tr], Y[tr], steps=steps, lr=1e-3)
     est = learned.bias_corrected(net, X[bc], Y[bc], X[pr])
+    torch.cuda.synchronize()
     t_ml = (time.perf_counter() - t0) * 1e3
     net.close()
-    exact = Y[pr].reshape(192, T, len(learned.SHELLS))
-    pred = est.pred.reshape(192, T, len(learned.SHELLS))
-    den = np.abs(exact).max(axis=1, keepdims=True)
-    per_src = (np.abs(pred - exact) / den).max(axis=1)                   # [source][shell]
+    gain_est = learned.expected_gain(est, Y[:nlab], nlab, nbc, npred)
+    # the error of each estimator of the mean over all sources, per output
+    # (time slice, shell) relative to the spread of the output over sources
+    scale = Y.std(axis=0)
+    scale = np.where(scale == 0, 1.0, scale)
     truth = Y.mean(axis=0)
-    m_ml = (Y[:64].sum(axis=0) + est.pred.sum(axis=0)) / 256
-    m_lab = Y[:64].mean(axis=0)
-    e_ml = float(np.sum((m_ml - truth) ** 2))
-    e_lab = float(np.sum((m_lab - truth) ** 2))
-    # the plain FP64 path is the fallback: where the held-out residuals put the
-    # predictions' error above 5%, the predicted sources are computed exactly
-    fb = learned.with_fallback(est, lambda: Y[pr], tol=0.05)
+    m_ml = (Y[:nlab].sum(axis=0) + est.pred.sum(axis=0)) / nsrc
+    m_lab = Y[:nlab].mean(axis=0)
+    e_ml = float(np.mean(((m_ml - truth) / scale) ** 2))
+    e_lab = float(np.mean(((m_lab - truth) / scale) ** 2))
+    per = np.abs(est.pred - Y[pr]) / scale                        # [source][time slice x shell]
+    # the plain FP64 path is the fallback: when the held-out sources say the
+    # bias-corrected estimator would not beat the labelled sources alone, the
+    # predicted sources are computed exactly
+    fb = learned.with_fallback(est, lambda: Y[pr], gain=gain_est, min_gain=1.0)
     cost_exact = t_exact
-    cost_ml = t_cheap + t_exact * 64 / 256 + t_ml + (t_exact * 192 / 256 if fb.used_fallback else 0.0)
-    nsites = 256 * V3 * T
-    res = {"metric": METRIC, "value": nsites / (cost_ml / 1e3), "unit": "sites/s", "n_gpus": 1,
-           "steps": 1, "warmup": 0, "ms_per_step": cost_ml, "higher_is_better": True,
+    cost_ml = t_cheap + t_exact * nlab / nsrc + t_ml + (t_exact * npred / nsrc if fb.used_fallback else 0.0)
+    nsites = nsrc * V3 * T
+    res = {"exact_fp64_ms_all_sources": cost_exact,
+           "ml_flow_ms": {f"cheap_inputs_{nsrc}": t_cheap, f"exact_labels_{nlab}": t_exact * nlab / nsrc,
+                          "train_and_predict": t_ml},
+           "sites": nsites, "flow_ms": cost_ml,
+           "network": f"17-256-256-11 per time slice (weights shared over the {T} time slices of a source), "
+                      f"BF16 weights, FP32 accumulation, {steps} full-batch Adam steps on {ntr * T} samples",
+           "train_loss_standardised": final_loss,
+           "pred_err_per_source": {"median": float(np.median(per)), "p90": float(np.quantile(per, 0.9)),
+                                   "max": float(per.max()),
+                                   "definition": "|pred - exact| / spread over sources, per source, time "
+                                                 "slice and shell"},
+           "heldout_rel_resid_median": float(np.median(est.rel_error_estimate())),
+           "expected_gain_from_heldout": gain_est, "fallback_used": bool(fb.used_fallback),
+           "fallback_rule": "compute the predicted sources exactly unless the held-out sources put the "
+                            "bias-corrected estimator's expected gain above 1",
+           "speedup_vs_exact_all": cost_exact / cost_ml,
+           "ensemble_mean_sq_err": {"bias_corrected": e_ml, "labelled_only": e_lab,
+                                    "gain": e_lab / e_ml if e_ml > 0 else None,
+                                    "definition": "mean over (time slice, shell) of ((estimate - mean over "
+                                                  "all sources) / spread over sources)^2"}}
+    return res, {"X": X, "Y": Y, "est": est}
+
+
+def run_learned(args, torch, capi, dev, stream):
+    """32^3 x 64, 256 sources (config of the learned part); see learned_flow"""
+    log("learned flow: exact FP64 and cheap mixed two-point results of 256 sources")
+    r, _ = learned_flow(torch, capi, dev, stream)
+    res = {"metric": METRIC, "value": r["sites"] / (r["flow_ms"] / 1e3), "unit": "sites/s", "n_gpus": 1,
+           "steps": 1, "warmup": 0, "ms_per_step": r["flow_ms"], "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None,
-           "dtype": "f64 two-point path; bf16 MLP (fp32 accumulation)", "data": "synthetic",
+           "dtype": "f64 two-point path (labels); f32 + bf16x6 at n = 0 (inputs); bf16 MLP (fp32 accumulation)",
+           "data": "synthetic",
            "config": {"workload": "learned: lattice 32^3x64, 256 sources, propagators base + 5% "
-                                  "perturbation, MLP 1088-256-256-704, train 32 / bias-correct 32 / "
+                                  "perturbation, MLP 17-256-256-11 per time slice, train 32 / bias-correct 32 / "
                                   "predict 192, nucleon at |n|^2 in 4..16 (11 shells)",
-                      "lattice": L, "sources": 256, "sites": nsites, "parallelism": "single GPU"},
-           "learned": {"exact_fp64_ms_all_sources": cost_exact,
-                       "ml_flow_ms": {"cheap_inputs_256": t_cheap, "exact_labels_64": t_exact / 4,
-                                      "train_and_predict": t_ml},
-                       "train_loss_standardised": final_loss,
-                       "pred_rel_err_per_source": {"median": float(np.median(per_src)),
-                                                   "max": float(per_src.max()),
-                                                   "definition": "max over t of |pred - exact| / "
-                                                                 "max_t |exact|, per source and shell"},
-                       "heldout_rel_err_estimate": float(np.median(est.rel_error_estimate())),
-                       "fallback_used": bool(fb.used_fallback), "fallback_tol": 0.05,
-                       "speedup_vs_exact_all": cost_exact / cost_ml,
-                       "ensemble_mean_sq_err": {"bias_corrected": e_ml, "labelled_only": e_lab,
-                                                "gain": e_lab / e_ml if e_ml > 0 else None}}}
+                      "lattice": L, "sources": 256, "sites": r["sites"], "parallelism": "single GPU"},
+           "learned": r}
     return res
 
 

@@ -15,6 +15,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <list>
+#include <memory>
 #include <map>
 #include <mutex>
 #include <numeric>
@@ -54,7 +56,13 @@ struct Context {
   size_t ws_part_bytes = 0;
 };
 static Context g_ctx;
+// g_mu guards the context (tables, workspace sizes, udsc schedules).
+// g_ws_mu is held by a single call (plg_correlator, the udsc and table
+// contractions, plg_ft_apply) from the moment it sizes the shared workspace
+// until its stream has finished, so two threads' single calls never share
+// or free each other's workspace; lock order g_ws_mu, then g_mu.
 static std::mutex g_mu;
+static std::mutex g_ws_mu;
 
 static uint16_t host_bf16_rn(float f) {
   uint32_t u;
@@ -278,7 +286,7 @@ static int build_plan(Plan &P, int kind, int mask, const std::vector<plg_prop> &
   P.nmom = space == PLG_MOMENTUM_SPACE ? nmom : 0;
   P.moms.assign(moms, moms + 3 * P.nmom);
   P.ms = scaled_moms(moms, P.nmom, 1);
-  if (P.T == PLG_F32 && P.nmom > 0) P.npairs = make_pairs(P.ms, g_ctx.M, P.pair_ms, P.mom_pair, &P.zero_pair);
+  if (P.nmom > 0) P.npairs = make_pairs(P.ms, g_ctx.M, P.pair_ms, P.mom_pair, &P.zero_pair);
   const int64_t V3 = g_ctx.V3;
   const int64_t inner = space == PLG_MOMENTUM_SPACE ? (int64_t)nmom : V3;  // per-row extent
 
@@ -504,11 +512,11 @@ template <typename A> static int upload(A **dst, const std::vector<A> &v, hipStr
 
 static void choose_tiles(Plan &P) {
   const int64_t ncols = (int64_t)P.rows.size() * P.nch_c;
-  const int Mreal = 2 * std::max(P.nmom, 1);
   int64_t tiles;
   if (P.T == PLG_F64) {
-    P.mtile = Mreal <= 96 ? 32 : 64;
-    tiles = ((ncols + 127) / 128) * ((Mreal + P.mtile - 1) / P.mtile);
+    const int R = 2 * std::max(P.npairs, 1);  // real rows {cos, sin} per momentum pair
+    P.mtile = R <= 96 ? 32 : 64;
+    tiles = ((ncols + 63) / 64) * ((R + P.mtile - 1) / P.mtile);
   } else {
     const int gp = std::max(P.npairs - P.zero_pair, 1);
     P.mbw = pick_proj(gp);
@@ -542,7 +550,7 @@ static int upload_plan(Plan &P, hipStream_t st) {
   if ((rc = upload(&P.d_ms, P.ms, st))) return rc;
   if ((rc = upload(&P.d_pair_ms, P.pair_ms, st))) return rc;
   if ((rc = upload(&P.d_mom_pair, P.mom_pair, st))) return rc;
-  if (P.npairs > 0) {
+  if (P.npairs > 0 && P.T == PLG_F32) {  // phase indices of the mixed projection
     if (P.d_pidx) hipFree(P.d_pidx);
     P.d_pidx = nullptr;
     HIPCHK(hipMalloc((void **)&P.d_pidx, phase_index_bytes(g_ctx.V3, P.npairs)));
@@ -798,6 +806,80 @@ static int autotune(Plan &P, hipStream_t st) {
   return PLG_OK;
 }
 
+// ------------------------------------------------------------------ single-call plans
+// plg_correlator keeps the plans of its recent calls (device tables, tiles),
+// keyed by everything the plan depends on: a driver that contracts source
+// after source into the same propagator buffers plans each geometry once.
+// Used under g_ws_mu only; the plans' device tables are read by launches the
+// call has synchronised before it returns, so evicting one is safe.
+static std::list<std::pair<std::string, std::unique_ptr<Plan>>> g_single;
+constexpr size_t SINGLE_PLANS = 32;
+
+static std::string single_key(int kind, int mask, const std::vector<plg_prop> &props, const int src[4], int totalT,
+                              const int *moms, int nmom, int mode, int space) {
+  std::ostringstream k;
+  k << kind << ' ' << mask << ' ' << mode << ' ' << space << ' ' << totalT;
+  for (int i = 0; i < 4; i++) k << ' ' << src[i];
+  for (const plg_prop &p : props)
+    k << " | " << p.data << ' ' << p.precision << ' ' << p.t_first << ' ' << p.nt << ' ' << p.comp_stride << ' '
+      << (const void *)p.plane_exp << ' ' << p.rotate_sign << ' ' << p.bc_t0;
+  k << " | " << nmom;
+  if (moms && space == PLG_MOMENTUM_SPACE)
+    for (int i = 0; i < 3 * nmom; i++) k << ' ' << moms[i];
+  // a pinned projection tile (PLG_PROJ_CODE) is part of the plan
+  if (const char *e = std::getenv("PLG_PROJ_CODE")) k << " code=" << e;
+  return k.str();
+}
+
+static Plan *single_lookup(const std::string &key) {
+  for (auto it = g_single.begin(); it != g_single.end(); ++it)
+    if (it->first == key) {
+      g_single.splice(g_single.begin(), g_single, it);  // most recent first
+      return g_single.front().second.get();
+    }
+  return nullptr;
+}
+
+static Plan *single_insert(const std::string &key, std::unique_ptr<Plan> P) {
+  g_single.emplace_front(key, std::move(P));
+  while (g_single.size() > SINGLE_PLANS) g_single.pop_back();
+  return g_single.front().second.get();
+}
+
+// tiles of a single-call plan: the tuning cache's entry for the plan's
+// geometry, else (PLG_TUNE_SINGLE unset or 1) a tuning run on the shared
+// workspace whose result goes to the cache, as the first call of a
+// geometry tunes in PLEGMA (lib/kernels/PLEGMA_kernel_tuner.cuh:240-292)
+static int tune_single(Plan &P, hipStream_t st) {
+  if (std::getenv("PLG_PROJ_CODE") || P.space != PLG_MOMENTUM_SPACE) return PLG_OK;
+  int a, b, c;
+  if (tune_lookup(tune_key(P), a, b, c)) {
+    if (P.T == PLG_F64) P.mtile = a;
+    else P.mbw = a;
+    P.nsplit = b;
+    P.tpb = c;
+    return PLG_OK;
+  }
+  const char *e = std::getenv("PLG_TUNE_SINGLE");
+  if (e && std::string(e) == "0") return PLG_OK;
+  const int ns0 = P.nsplit;
+  P.nsplit = (int)std::max<int64_t>(ns0, std::min<int64_t>(32, g_ctx.V3 / 256));
+  int rc;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if ((rc = ensure(&g_ctx.ws_c, &g_ctx.ws_c_bytes, c_bytes(P)))) return rc;
+    if ((rc = ensure(&g_ctx.ws_part, &g_ctx.ws_part_bytes, part_bytes(P)))) return rc;
+  }
+  P.nsplit = ns0;
+  P.c = g_ctx.ws_c;
+  P.part = g_ctx.ws_part;
+  P.part_bytes = g_ctx.ws_part_bytes;
+  rc = autotune(P, st);
+  P.c = P.part = nullptr;
+  P.part_bytes = 0;
+  return rc;
+}
+
 }  // namespace plg
 
 using namespace plg;
@@ -811,7 +893,10 @@ extern "C" {
 int plg_version(void) { return 100; }
 const char *plg_last_error(void) { return g_err.c_str(); }
 
+static void finalize_locked();
+
 int plg_init(const int L[4], int device) {
+  std::lock_guard<std::mutex> wl(g_ws_mu);
   std::lock_guard<std::mutex> lk(g_mu);
   if (!L) return fail(PLG_ERR_INVALID, "L is NULL");
   for (int i = 0; i < 4; i++)
@@ -822,7 +907,7 @@ int plg_init(const int L[4], int device) {
   HIPCHK(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) return fail(PLG_ERR_INVALID, "no such device");
   HIPCHK(hipSetDevice(device));
-  if (g_ctx.init) plg_finalize();
+  if (g_ctx.init) finalize_locked();
   std::memcpy(g_ctx.L, L, sizeof(int) * 4);
   g_ctx.V3 = (int64_t)L[0] * L[1] * L[2];
   g_ctx.device = device;
@@ -856,13 +941,21 @@ int plg_init(const int L[4], int device) {
 }
 
 int plg_finalize(void) {
-  if (!g_ctx.init) return PLG_OK;
+  std::lock_guard<std::mutex> wl(g_ws_mu);
+  std::lock_guard<std::mutex> lk(g_mu);
+  finalize_locked();
+  return PLG_OK;
+}
+
+// with g_ws_mu and g_mu held
+static void finalize_locked() {
+  if (!g_ctx.init) return;
   hipDeviceSynchronize();
+  g_single.clear();
   for (void *p : {(void *)g_ctx.d_cos, (void *)g_ctx.d_sin, (void *)g_ctx.d_wtab, g_ctx.ws_c, g_ctx.ws_part})
     if (p) hipFree(p);
   for (void *p : g_ctx.udsc_mem) hipFree(p);
   g_ctx = Context();
-  return PLG_OK;
 }
 
 int plg_momlist(int q2max, int *nmom, int *xyz, int cap) {
@@ -928,22 +1021,29 @@ int plg_correlator(int kind, int group_mask, const plg_prop *p1, const plg_prop 
   int rc = plg_window(source, totalT, nullptr, nullptr, &lt);
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
-  Plan P;
   std::vector<plg_prop> props = {*p1, p2 ? *p2 : *p1};
   if (!same_view(props[0], props[1]) && (kind == PLG_TWOPT))
     return fail(PLG_ERR_INVALID, "twopt takes one propagator");
-  std::vector<JobIn> jobs = {{{source[0], source[1], source[2], source[3]}, totalT, 0, 1, 0}};
-  if ((rc = build_plan(P, kind, group_mask, props, jobs, moms, nmom, mode, PLG_SPLIT_BF16X6, space))) return rc;
-  choose_tiles(P);
-  if ((rc = upload_plan(P, st))) return rc;
+  std::lock_guard<std::mutex> wl(g_ws_mu);
+  const std::string key = single_key(kind, group_mask, props, source, totalT, moms, nmom, mode, space);
+  Plan *P = single_lookup(key);
+  if (!P) {
+    auto np = std::make_unique<Plan>();
+    std::vector<JobIn> jobs = {{{source[0], source[1], source[2], source[3]}, totalT, 0, 1, 0}};
+    if ((rc = build_plan(*np, kind, group_mask, props, jobs, moms, nmom, mode, PLG_SPLIT_BF16X6, space))) return rc;
+    choose_tiles(*np);
+    if ((rc = upload_plan(*np, st))) return rc;
+    if ((rc = tune_single(*np, st))) return rc;
+    P = single_insert(key, std::move(np));
+  }
   {
     std::lock_guard<std::mutex> lk(g_mu);
-    if ((rc = ensure(&g_ctx.ws_c, &g_ctx.ws_c_bytes, c_bytes(P)))) return rc;
-    if ((rc = ensure(&g_ctx.ws_part, &g_ctx.ws_part_bytes, part_bytes(P)))) return rc;
+    if ((rc = ensure(&g_ctx.ws_c, &g_ctx.ws_c_bytes, c_bytes(*P)))) return rc;
+    if ((rc = ensure(&g_ctx.ws_part, &g_ctx.ws_part_bytes, part_bytes(*P)))) return rc;
   }
-  rc = enqueue(P, g_ctx.ws_c, g_ctx.ws_part, out, out_precision, st);
-  // the plan's device tables are freed on return: finish this correlator
-  // first, also after an error
+  rc = enqueue(*P, g_ctx.ws_c, g_ctx.ws_part, out, out_precision, st);
+  // the workspace is shared by single calls: finish this correlator before
+  // the next call may use it, also after an error
   const hipError_t se = hipStreamSynchronize(st);
   if (rc) return rc;
   if (se != hipSuccess) return fail(PLG_ERR_HIP, std::string("correlator sync: ") + hipGetErrorString(se));
@@ -1087,7 +1187,7 @@ static int udsc_one(const plg_prop *const views[3], const UdscSched &S, int nops
   P.nmom = space == PLG_MOMENTUM_SPACE ? nmom : 0;
   if (P.nmom) P.moms.assign(moms, moms + 3 * P.nmom);
   P.ms = scaled_moms(P.moms.data(), P.nmom, 1);
-  if (P.T == PLG_F32 && P.nmom > 0) P.npairs = make_pairs(P.ms, g_ctx.M, P.pair_ms, P.mom_pair, &P.zero_pair);
+  if (P.nmom > 0) P.npairs = make_pairs(P.ms, g_ctx.M, P.pair_ms, P.mom_pair, &P.zero_pair);
   P.nch_c = nops;
   for (int j = 0; j < lt; j++) {
     const int t = row_time(g_ctx.L, source, w, j);
@@ -1121,6 +1221,7 @@ static int udsc_one(const plg_prop *const views[3], const UdscSched &S, int nops
   choose_tiles(P);
   P.tpb = 64;  // sites per workgroup of the udsc kernel
   if ((rc = upload_plan(P, hs))) return rc;
+  std::lock_guard<std::mutex> wl(g_ws_mu);  // until the stream is synchronised below
   {
     std::lock_guard<std::mutex> lk(g_mu);
     if ((rc = ensure(&g_ctx.ws_c, &g_ctx.ws_c_bytes, c_bytes(P)))) return rc;
@@ -1199,7 +1300,7 @@ int plg_tetraquark_table(const plg_prop *p1, const plg_prop *p2, const plg_prop 
   P.nmom = space == PLG_MOMENTUM_SPACE ? nmom : 0;
   if (P.nmom) P.moms.assign(moms, moms + 3 * P.nmom);
   P.ms = scaled_moms(P.moms.data(), P.nmom, 1);
-  if (P.T == PLG_F32 && P.nmom > 0) P.npairs = make_pairs(P.ms, g_ctx.M, P.pair_ms, P.mom_pair, &P.zero_pair);
+  if (P.nmom > 0) P.npairs = make_pairs(P.ms, g_ctx.M, P.pair_ms, P.mom_pair, &P.zero_pair);
   P.nch_c = nops;
   for (int j = 0; j < lt; j++) {
     const int t = row_time(g_ctx.L, source, w, j);
@@ -1242,13 +1343,15 @@ int plg_tetraquark_table(const plg_prop *p1, const plg_prop *p2, const plg_prop 
   choose_tiles(P);
   P.tpb = 64;  // sites per workgroup of the kernel
   if (!(rc = upload_plan(P, hs)) && !(rc = upload(&P.d_rows4, P.rows4, hs))) {
+    std::lock_guard<std::mutex> wl(g_ws_mu);  // until the stream is synchronised below
     {
       std::lock_guard<std::mutex> lk(g_mu);
       if (!(rc = ensure(&g_ctx.ws_c, &g_ctx.ws_c_bytes, c_bytes(P))))
         rc = ensure(&g_ctx.ws_part, &g_ctx.ws_part_bytes, part_bytes(P));
     }
     if (!rc) rc = enqueue(P, g_ctx.ws_c, g_ctx.ws_part, out, out_precision, hs);
-    if (!rc && hipStreamSynchronize(hs) != hipSuccess) rc = fail(PLG_ERR_HIP, "tetraquark table sync failed");
+    const hipError_t se = hipStreamSynchronize(hs);  // also after an error
+    if (!rc && se != hipSuccess) rc = fail(PLG_ERR_HIP, "tetraquark table sync failed");
   }
   hipStreamSynchronize(hs);  // no launch may still read the terms
   hipFree(d_terms);
@@ -1351,7 +1454,7 @@ int plg_ft_apply(const void *field, int precision, int dof, int nt, const int *m
   const std::vector<int> ms = scaled_moms(moms, nmom, -sign);
   std::vector<int> pair_ms, mom_pair;
   int zero_pair = 0;
-  const int npairs = precision == PLG_F32 ? make_pairs(ms, g_ctx.M, pair_ms, mom_pair, &zero_pair) : 0;
+  const int npairs = make_pairs(ms, g_ctx.M, pair_ms, mom_pair, &zero_pair);
   ProjGeom g;
   g.V3 = g_ctx.V3;
   for (int i = 0; i < 3; i++) g.L[i] = g_ctx.L[i];
@@ -1360,12 +1463,12 @@ int plg_ft_apply(const void *field, int precision, int dof, int nt, const int *m
   g.ncols = dof * nt;
   g.cos_tab = g_ctx.d_cos;
   g.sin_tab = g_ctx.d_sin;
-  const int Mreal = 2 * nmom;
   int code;
   int64_t tiles;
   if (precision == PLG_F64) {
-    code = Mreal <= 96 ? 32 : 64;
-    tiles = ((g.ncols + 127) / 128) * ((Mreal + code - 1) / code);
+    const int R = 2 * npairs;
+    code = R <= 96 ? 32 : 64;
+    tiles = ((g.ncols + 63) / 64) * ((R + code - 1) / code);
   } else {
     code = pick_proj(std::max(npairs - zero_pair, 1));
     tiles = proj_workgroups(code, g.ncols, std::max(npairs - zero_pair, 1));
@@ -1382,13 +1485,15 @@ int plg_ft_apply(const void *field, int precision, int dof, int nt, const int *m
   g.mom_pair = d_mom_pair;
   g.zero_pair = zero_pair;
   g.ctile = 0;
-  if (!rc && npairs > 0) {
+  if (!rc && npairs > 0 && precision == PLG_F32) {
     if (hipMalloc((void **)&d_pidx, phase_index_bytes(g_ctx.V3, npairs)) != hipSuccess)
       rc = fail(PLG_ERR_HIP, "phase index allocation failed");
     else if ((rc = launch_phase_index(g, d_pidx, st)))
       rc = fail(rc, "phase index launch failed");
     g.pidx = d_pidx;
   }
+  // the shared workspace is this call's until the stream is synchronised below
+  std::unique_lock<std::mutex> wl(g_ws_mu);
   if (!rc) {
     std::lock_guard<std::mutex> lk(g_mu);
     rc = ensure(&g_ctx.ws_part, &g_ctx.ws_part_bytes, proj_part_bytes(g.nsplit, g.ncols, nmom, npairs));

@@ -141,10 +141,10 @@ int launch_baryon(bool two, bool mes, int gmask, const PropDev<T> *props, const 
                   int tiles_per_block, hipStream_t st);
 
 // projection: partial sums over each K split's site range of
-// sum_x exp(-i 2pi n.x / L) c[n][x], in one of two layouts:
-//   npairs == 0 (FP64 path): part[split][n][m], complex double;
-//   npairs > 0 (mixed path): part[split][2n + {re, im}][2j + {cos, sin}],
-//     real double, j = pair of momenta (n, -n) (see part_value)
+// sum_x exp(-i 2pi n.x / L) c[n][x].  Both paths (FP64 and mixed) write the
+// paired layout part[split][2n + {re, im}][2j + {cos, sin}], real double,
+// j = pair of momenta (n, -n) (see part_value); npairs == 0 (no momentum
+// list) leaves the complex layout part[split][n][m].
 struct ProjGeom {
   int64_t V3;
   int L[3];

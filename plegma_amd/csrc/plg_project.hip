@@ -7,11 +7,9 @@
 //
 //   part[s][n][m] = sum_{x in split s} exp(-i 2 pi n_m . x / L) c[n][x]
 //
-// with n = (row, channel) column of the site correlator.  The complex product
-// is a real GEMM with interleaved K (2x + re/im) and interleaved M (2m + re/im):
-//   A[2m  ][2x] =  cos   A[2m  ][2x+1] = sin
-//   A[2m+1][2x] = -sin   A[2m+1][2x+1] = cos      (theta = 2 pi k / M)
-// A is generated per K tile from the exact integer phase index
+// with n = (row, channel) column of the site correlator, computed on the
+// (n, -n) momentum pairs as a real GEMM (k_project_f64 below).  The phase
+// rows are generated per K tile from the exact integer phase index
 //   k = sum_i n_i x_i (M / L_i) mod M,  M = lcm(Lx, Ly, Lz),
 // and the FP64 table cos/sin(2 pi k / M); the source offset is applied later
 // in FP64 as a per-momentum constant phase (the finalize kernel).
@@ -51,8 +49,16 @@ __device__ __forceinline__ void split_range(const ProjGeom &g, int s, int64_t &b
   e = b + per < g.V3 ? b + per : g.V3;
 }
 
-constexpr int F64_KT = 16;              // sites per K tile
-constexpr int F64_LD = 2 * F64_KT + 1;  // padded LDS row (doubles)
+// Paired form (the same as the mixed path's, plg_project_bf16.hip): a
+// createMom list holds n and -n, and with
+//   Cr + i Ci = sum_x cos(theta_n x) c(x),  Sr + i Si = sum_x sin(theta_n x) c(x)
+// P(+-n) = (Cr +- Si) + i (Ci -+ Sr).  The GEMM has the real rows {cos, sin}
+// of one representative momentum per pair (n, -n), K = sites and the real
+// columns {Re c, Im c}: part[split][2n + {re, im}][2j + {cos, sin}], half the
+// MFMA work of the interleaved complex GEMM over every momentum.  The zero
+// momentum is an ordinary pair here (its sin row is zero).
+constexpr int F64_KT = 32;              // sites per K tile
+constexpr int F64_LD = F64_KT + 1;      // padded LDS row (doubles)
 
 template <int MT>
 __global__ __launch_bounds__(PJ_THREADS) void k_project_f64(ProjGeom g, const cx<double> *__restrict__ c,
@@ -61,12 +67,13 @@ __global__ __launch_bounds__(PJ_THREADS) void k_project_f64(ProjGeom g, const cx
   __shared__ double sB[PJ_NT][F64_LD];
   __shared__ double sA[MT][F64_LD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t n0 = (int64_t)blockIdx.x * PJ_NT;
-  const int mr0 = blockIdx.y * MT;
+  const int64_t nc0 = (int64_t)blockIdx.x * (PJ_NT / 2);  // first complex column
+  const int mr0 = blockIdx.y * MT;                        // first real row (2j + cos/sin)
   const int split = blockIdx.z;
   int64_t xb, xe;
   split_range(g, split, xb, xe);
-  const int Mreal = 2 * g.nmom;
+  const int R = 2 * g.npairs;
+  const int64_t nreal = 2 * (int64_t)g.ncols;
 
   d4 acc[MB][2];
 #pragma unroll
@@ -75,40 +82,39 @@ __global__ __launch_bounds__(PJ_THREADS) void k_project_f64(ProjGeom g, const cx
     for (int j = 0; j < 2; j++) acc[i][j] = d4{0, 0, 0, 0};
 
   for (int64_t xt = xb; xt < xe; xt += F64_KT) {
-    // stage B: 128 columns x 16 sites
+    // stage B: 64 complex columns x 32 sites as 128 real columns {re, im}
     {
-      const int s = tid & 15;
+      const int s = tid & 31;
       const int64_t x = xt + s;
 #pragma unroll
-      for (int i = 0; i < PJ_NT / 16; i++) {
-        const int col = (tid >> 4) + 16 * i;
-        const int64_t n = n0 + col;
+      for (int i = 0; i < (PJ_NT / 2) / 8; i++) {
+        const int col = (tid >> 5) + 8 * i;
+        const int64_t n = nc0 + col;
         cx<double> v = cmake<double>(0, 0);
         if (x < xe && n < g.ncols) v = c[cidx(g, n, x)];
-        sB[col][2 * s] = v.re;
-        sB[col][2 * s + 1] = v.im;
+        sB[2 * col][s] = v.re;
+        sB[2 * col + 1][s] = v.im;
       }
     }
-    // stage A: MT real rows x 16 sites
+    // stage A: MT/2 pairs x 32 sites, rows {cos, sin} of one phase index
 #pragma unroll
-    for (int i = 0; i < (MT * F64_KT) / PJ_THREADS; i++) {
+    for (int i = 0; i < (MT / 2 * F64_KT) / PJ_THREADS; i++) {
       const int e = tid + PJ_THREADS * i;
-      const int mr = e / F64_KT, s = e % F64_KT;
-      const int m = (mr0 + mr) >> 1, ri = (mr0 + mr) & 1;
+      const int jl = e / F64_KT, s = e % F64_KT;
+      const int j = (mr0 >> 1) + jl;
       const int64_t x = xt + s;
-      double a0 = 0, a1 = 0;
-      if (m < g.nmom && x < xe) {
-        const int k = phase_index(g.mom_scaled, m, site_xyz(x, g), g.M);
-        const double cs = g.cos_tab[k], sn = g.sin_tab[k];
-        a0 = ri ? -sn : cs;
-        a1 = ri ? cs : sn;
+      double cs = 0, sn = 0;
+      if (j < g.npairs && x < xe) {
+        const int k = phase_index(g.pair_ms, j, site_xyz(x, g), g.M);
+        cs = g.cos_tab[k];
+        sn = g.sin_tab[k];
       }
-      sA[mr][2 * s] = a0;
-      sA[mr][2 * s + 1] = a1;
+      sA[2 * jl][s] = cs;
+      sA[2 * jl + 1][s] = sn;
     }
     __syncthreads();
 #pragma unroll
-    for (int ks = 0; ks < 2 * F64_KT / 4; ks++) {
+    for (int ks = 0; ks < F64_KT / 4; ks++) {
       const int kk = ks * 4 + (lane >> 4);
       double bf[2];
 #pragma unroll
@@ -131,20 +137,22 @@ __global__ __launch_bounds__(PJ_THREADS) void k_project_f64(ProjGeom g, const cx
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const int mr = mr0 + i * 16 + (lane >> 4) + 4 * v;
-        const int64_t n = n0 + wave * 32 + j * 16 + (lane & 15);
-        if (mr < Mreal && n < g.ncols) part[((int64_t)split * g.ncols + n) * Mreal + mr] = acc[i][j][v];
+        const int64_t nr = 2 * nc0 + wave * 32 + j * 16 + (lane & 15);
+        if (mr < R && nr < nreal) part[((int64_t)split * nreal + nr) * R + mr] = acc[i][j][v];
       }
 }
 
 int launch_project_f64(const ProjGeom &g, const cx<double> *c, cx<double> *part, int mtile,
                        hipStream_t st) {
-  const int Mreal = 2 * g.nmom;
+  if (g.npairs <= 0) return PLG_ERR_INVALID;
+  const int R = 2 * g.npairs;
   dim3 block(PJ_THREADS);
+  const unsigned gx = (unsigned)((g.ncols + PJ_NT / 2 - 1) / (PJ_NT / 2));
   if (mtile == 32) {
-    dim3 grid((unsigned)((g.ncols + PJ_NT - 1) / PJ_NT), (unsigned)((Mreal + 31) / 32), (unsigned)g.nsplit);
+    dim3 grid(gx, (unsigned)((R + 31) / 32), (unsigned)g.nsplit);
     k_project_f64<32><<<grid, block, 0, st>>>(g, c, reinterpret_cast<double *>(part));
   } else {
-    dim3 grid((unsigned)((g.ncols + PJ_NT - 1) / PJ_NT), (unsigned)((Mreal + 63) / 64), (unsigned)g.nsplit);
+    dim3 grid(gx, (unsigned)((R + 63) / 64), (unsigned)g.nsplit);
     k_project_f64<64><<<grid, block, 0, st>>>(g, c, reinterpret_cast<double *>(part));
   }
   return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;

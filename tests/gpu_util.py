@@ -73,3 +73,43 @@ def rel_err(got, ref, axis=-2):
 
 def oracle_H(kind, L, src, totalT, q2, P1, P2, mask=O.G_NUCL, rows=None, nthreads=0):
     return O.correlator(kind, L, src, totalT, q2, P1, P2, mask=mask, rows=rows, nthreads=nthreads)
+
+
+def shell_rel_err(got, ref, moms, axis=-2):
+    """error per (channel, momentum): |got - ref| / rms of |ref| over the
+    momenta of the same |n|^2 shell (same group, dataset, row and channel).
+    The shell rms is the scale of a correlator at that |n|, so the n = 0 value
+    (which dominates a row's maximum) does not set the scale at |n| > 0, and
+    an accidental near-zero value at one momentum does not blow the error up.
+    Returns the max over all elements."""
+    moms = np.asarray(moms).reshape(-1, 3)
+    q2 = (moms ** 2).sum(1)
+    got = np.moveaxis(got, axis, -1)
+    ref = np.moveaxis(ref, axis, -1)
+    err = np.abs(got - ref)
+    den = np.empty(err.shape)
+    for q in np.unique(q2):
+        sel = q2 == q
+        den[..., sel] = np.sqrt((np.abs(ref[..., sel]) ** 2).mean(-1, keepdims=True))
+    den = np.where(den == 0, 1.0, den)
+    return float((err / den).max())
+
+
+def point_rel_err(got, ref):
+    """pointwise |got - ref| / |ref| over all nonzero reference elements:
+    (median, 99.9th percentile, max)"""
+    r = np.abs(ref).reshape(-1)
+    e = np.abs(got - ref).reshape(-1)
+    sel = r > 0
+    q = e[sel] / r[sel]
+    return float(np.median(q)), float(np.quantile(q, 0.999)), float(q.max())
+
+
+def host_slice(t, nt, V3, plane):
+    """the 144 components of one stored time slice of a device propagator
+    (tensor (144, nt*V3, 2)) as a float64 host array of an oracle view with
+    comp_stride V3 and nt = 1"""
+    import torch
+    torch.cuda.synchronize()
+    s = t.view(144, nt, V3, 2)[:, plane].to(torch.float64).cpu().numpy()
+    return np.ascontiguousarray(s.reshape(-1))

@@ -1,0 +1,166 @@
+"""GPU parity at the full 32^3 x 64 workloads, against the CPU oracle.
+
+Precision workload: one source, FP64 propagator and its FP32 copy (with
+per-time-slice exponents), 16 mesons + proton + Delta++ (Cg1, Cg2, Cg3),
+|n|^2 <= 16 (257 momenta), full and wrapped windows, through plg_correlator
+(the single-call path a PLEGMA driver reaches).  Batching workload: the 16
+seeded sources with windows of 8..32 slices of the benchmark, one mixed
+batch with autotuned tiles replayed as a hipGraph, and the FP64 path source
+by source.
+
+The oracle costs about half a second per time slice at this size, so each
+check compares sampled rows (time slices) in full: every group, channel and
+momentum of the row.  Tolerances:
+  FP64:  normwise over momenta (per group, dataset, row and channel) 1e-12;
+  mixed: normwise 1e-6, and per (channel, momentum) against the rms of the
+         reference over the momentum's |n|^2 shell 5e-6 (the pointwise
+         relative error is printed; it is unbounded at accidental zeros).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from plegma_amd import capi
+
+import gpu_util as G
+
+pytestmark = pytest.mark.gpu
+
+L = [32, 32, 32, 64]
+V3 = L[0] * L[1] * L[2]
+Q = 16
+MASK = capi.G_NUCL | capi.G_DELTAPP_ALL
+OMASK = O.G_NUCL | sum(O.G_DELTA32)
+NTH = min(os.cpu_count() or 1, 16)
+TOL64, TOLMX, TOLSHELL = 1e-12, 1e-6, 5e-6
+
+
+@pytest.fixture(scope="module", autouse=True)
+def ctx():
+    G.torch_cuda()
+    print("\n[gpu] plg_init", L, flush=True)
+    capi.init(L, 0)
+    yield
+    capi.finalize()
+
+
+def oracle_row(h_slice, t, src, totalT, j, mask=OMASK, kind=O.TWOPT):
+    """the oracle's row j (lattice time t) of a correlator from one host slice"""
+    view = (V3, t, 1)
+    R = O.correlator(kind, L, src, totalT, Q, h_slice, h_slice, mask=mask, rows=[j], nthreads=NTH,
+                     view1=view, view2=view)
+    return R[:, :, j]
+
+
+def check_rows(H64, Hmx, t64, nt, t_first, src, totalT, rows, label):
+    moms = O.momlist(Q)
+    for j in rows:
+        t = capi.row_time(src, totalT, j)
+        plane = (t - t_first) % L[3]
+        print(f"[gpu] {label}: oracle row j={j} t={t}", flush=True)
+        R = oracle_row(G.host_slice(t64, nt, V3, plane), t, src, totalT, j)
+        e64 = G.rel_err(H64[:, :, j], R)
+        emx = G.rel_err(Hmx[:, :, j], R)
+        esh = G.shell_rel_err(Hmx[:, :, j], R, moms)
+        pt = G.point_rel_err(Hmx[:, :, j], R)
+        print(f"[gpu] {label} j={j}: fp64 {e64:.2e}, mixed normwise {emx:.2e}, per (channel, momentum) "
+              f"shell-rms {esh:.2e}, pointwise median/p99.9/max {pt[0]:.1e}/{pt[1]:.1e}/{pt[2]:.1e}",
+              flush=True)
+        assert e64 < TOL64, (j, e64)
+        assert emx < TOLMX, (j, emx)
+        assert esh < TOLSHELL, (j, esh)
+
+
+@pytest.mark.parametrize("src,totalT,nt,t_first,rows", [
+    # full window: rows are the lattice times 0..63; the source slice, a
+    # near one and the farthest one (|t - t0| = 41)
+    ([5, 17, 30, 41], 64, 64, 0, [41, 45, 0]),
+    # wrapped window t0 = 60, totalT = 16: rows are t = 0..11, then 60..63,
+    # stored in a view of 16 slices starting at t = 60
+    ([12, 3, 27, 60], 16, 16, 60, [0, 11, 12]),
+])
+def test_precision_workload(src, totalT, nt, t_first, rows):
+    torch = G.torch_cuda()
+    print(f"[gpu] precision workload src={src} totalT={totalT}", flush=True)
+    t64, v64 = G.make_prop(L, 4242, 7, src[3], 0.5, t_first=t_first, nt=nt)
+    (t32, e32), v32 = G.cast_f32(t64, v64, scaled=True)
+    shp = lambda H: G.shaped(H, capi.TWOPT, MASK, src, totalT, Q)
+    H64 = shp(G.run_correlator(capi.TWOPT, MASK, v64, v64, src, totalT, Q))
+    Hmx = shp(G.run_correlator(capi.TWOPT, MASK, v32, v32, src, totalT, Q, mode=capi.MODE_MIXED))
+    del t32, e32
+    check_rows(H64, Hmx, t64, nt, t_first, src, totalT, rows, "precision")
+    # a second call reuses the cached plan: identical bits
+    H64b = shp(G.run_correlator(capi.TWOPT, MASK, v64, v64, src, totalT, Q))
+    assert np.array_equal(H64, H64b)
+    del t64
+    torch.cuda.empty_cache()
+
+
+def bench_sources():
+    """the 16 sources (position, window) of the batching benchmark"""
+    import bench
+    return bench.sources(np.random.default_rng(bench.SEED)), bench.SEED, bench.LAMBDA
+
+
+def test_batching_workload():
+    torch = G.torch_cuda()
+    srcs, seed, lam = bench_sources()
+    moms = capi.momlist(Q)
+    keep, v64, v32 = [], [], []
+    for s, (x0, w) in enumerate(srcs):
+        t = torch.empty((144, w * V3, 2), dtype=torch.float64, device="cuda")
+        a = capi.plg_prop(t.data_ptr(), capi.F64, x0[3], w, 0, w * V3)
+        capi.synth_prop(a, seed, s, x0[3], lam)
+        (t32, e32), b = G.cast_f32(t, a, scaled=True)
+        keep.append((t, t32, e32))
+        v64.append(a)
+        v32.append(b)
+    jobs, off, sizes = [], 0, []
+    for s, (x0, w) in enumerate(srcs):
+        n = capi.correlator_size(capi.TWOPT, MASK, x0, w, len(moms))
+        jobs.append((x0, w, s, off))
+        sizes.append(n)
+        off += n
+    print("[gpu] batching: mixed batch, graph + autotune", flush=True)
+    b = capi.Batch(capi.TWOPT, MASK, v32, jobs, moms, capi.MODE_MIXED,
+                   flags=capi.BATCH_GRAPH | capi.BATCH_TUNE)
+    nout, nsites, nrows = b.info()
+    assert nsites == sum(w for _, w in srcs) * V3
+    out = torch.full((nout, 2), float("nan"), dtype=torch.float64, device="cuda")
+    b.run(out)
+    first = G.host(out)
+    b.run(out)  # a graph replay
+    hm = G.host(out)
+    assert np.array_equal(first, hm)
+    b.close()
+    hm = hm.reshape(-1, 2)
+    hm = hm[:, 0] + 1j * hm[:, 1]
+    for s, ((x0, w), (_, _, _, o), n) in enumerate(zip(srcs, jobs, sizes)):
+        H64 = G.shaped(G.run_correlator(capi.TWOPT, MASK, v64[s], v64[s], x0, w, Q), capi.TWOPT, MASK, x0, w, Q)
+        Hmx = G.shaped(hm[o:o + n], capi.TWOPT, MASK, x0, w, Q)
+        j = (5 * s) % w  # one row per source, spread over the windows
+        check_rows(H64, Hmx, keep[s][0], w, x0[3], x0, w, [j], f"batching source {s}")
+
+
+def test_learned_workload():
+    """The learned workload at its configuration (256 sources, base + 5%
+    perturbation, train 32 / bias-correct 32 / predict 192): the network is
+    trained on the device by the library's kernels; the fallback decision
+    follows the gain the held-out sources predict, and when the learned
+    estimator is used it beats the labelled sources alone."""
+    import bench
+    torch = G.torch_cuda()
+    stream = torch.cuda.Stream()
+    print("[gpu] learned workload: 256 sources", flush=True)
+    r, arr = bench.learned_flow(torch, capi, "cuda", stream)
+    g = r["ensemble_mean_sq_err"]
+    print(f"[gpu] learned: expected gain {r['expected_gain_from_heldout']:.2f}, measured gain {g['gain']:.2f}, "
+          f"fallback {r['fallback_used']}, per-source error median {r['pred_err_per_source']['median']:.3f}",
+          flush=True)
+    assert r["fallback_used"] == (r["expected_gain_from_heldout"] < 1.0)
+    if not r["fallback_used"]:
+        assert g["gain"] > 1.0
+    else:
+        assert np.array_equal(arr["est"].pred, arr["Y"][64:])

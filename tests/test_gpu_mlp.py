@@ -52,7 +52,10 @@ def test_forward_matches_oracle(dims, B):
     net.close()
 
 
-@pytest.mark.parametrize("dims,B", [([40, 64, 48, 24], 16), ([1088, 256, 256, 704], 32)])
+# the learned workload's network (17 cheap values -> 2 x 256 -> 11 shells per
+# time slice) at its training batch (32 sources x 64 time slices)
+@pytest.mark.parametrize("dims,B", [([40, 64, 48, 24], 16), ([1088, 256, 256, 704], 32),
+                                    ([17, 256, 256, 11], 2048)])
 def test_gradient_matches_oracle(dims, B):
     """device loss and gradient vs the oracle with the device's bf16 rounding
     points (differences: FP32 vs FP64 sums), and vs the all-FP64 network

@@ -424,23 +424,67 @@ def test_ft_apply_irregular_momenta(prec, tol):
 
 
 def test_mixed_correlator_irregular_momenta(props):
-    """Baryon + meson correlator in the mixed path on the irregular list."""
+    """Baryon + meson correlator, FP64 and mixed, on the irregular list,
+    against the oracle's site correlator projected by the exact-phase DFT of
+    the closed forms (the oracle's own DFT takes |n|^2 lists only)."""
+    import closed_forms as cf
     (t1, v1, h1), _ = props
     t32, v32 = G.cast_f32(t1, v1)
     src, totalT = [1, 2, 3, 4], 16
     mask = capi.G_NUCL | capi.G_DELTAPP_ALL
     n = capi.correlator_size(capi.TWOPT, mask, src, totalT, len(IRREGULAR))
     torch = G.torch_cuda()
-    out = torch.zeros((n, 2), dtype=torch.float64, device="cuda")
-    capi.correlator(capi.TWOPT, mask, v32, v32, src, totalT, IRREGULAR, capi.MODE_MIXED,
-                    capi.MOMENTUM_SPACE, capi.F64, out)
-    h = G.host(out).reshape(n, 2)
-    got = h[:, 0] + 1j * h[:, 1]
-    out64 = torch.zeros((n, 2), dtype=torch.float64, device="cuda")
-    capi.correlator(capi.TWOPT, mask, v1, v1, src, totalT, IRREGULAR, capi.MODE_F64,
-                    capi.MOMENTUM_SPACE, capi.F64, out64)
-    h = G.host(out64).reshape(n, 2)
-    ref = h[:, 0] + 1j * h[:, 1]
-    got = got.reshape(-1, len(IRREGULAR), 16)
-    ref = ref.reshape(-1, len(IRREGULAR), 16)
-    assert G.rel_err(got, ref, axis=1) < 1e-6
+    got = {}
+    for mode, v in ((capi.MODE_MIXED, v32), (capi.MODE_F64, v1)):
+        out = torch.zeros((n, 2), dtype=torch.float64, device="cuda")
+        capi.correlator(capi.TWOPT, mask, v, v, src, totalT, IRREGULAR, mode, capi.MOMENTUM_SPACE, capi.F64, out)
+        h = G.host(out).reshape(n, 2)
+        got[mode] = (h[:, 0] + 1j * h[:, 1]).reshape(5, -1, len(IRREGULAR), 16)
+    _, _, lt = capi.window(src, totalT)
+    for j in (0, 5, lt - 1):
+        t = capi.row_time(src, totalT, j)
+        f = O.site_correlator(O.TWOPT, L1, t, h1, h1, mask=O.G_NUCL | sum(O.G_DELTA32))   # [V3, 5 * 16]
+        ref = cf.dft(f, L1, src[:3], np.array(IRREGULAR)).reshape(len(IRREGULAR), 5, 16).transpose(1, 0, 2)
+        assert G.rel_err(got[capi.MODE_F64][:, j], ref) < TOL64, j
+        assert G.rel_err(got[capi.MODE_MIXED][:, j], ref) < 1e-6, j
+
+
+def test_concurrent_single_calls(props):
+    """plg_correlator from two host threads at once (distinct streams, shared
+    workspace, different sizes so the workspace grows under the other call):
+    every result equals the one of a call made alone"""
+    import threading
+    torch = G.torch_cuda()
+    (t1, v1, h1), (t2, v2, h2) = props
+    moms = capi.momlist(Q1)
+    cases = [(capi.TWOPT, capi.G_NUCL | capi.G_DELTAPP_ALL, v1, v1, [0, 0, 0, 0], 16),
+             (capi.MESONS16, 0, v1, v2, [3, 5, 1, 13], 8)]
+    alone, outs = [], []
+    for kind, mask, a, b, src, tt in cases:
+        n = capi.correlator_size(kind, mask, src, tt, len(moms))
+        o = torch.zeros((n, 2), dtype=torch.float64, device="cuda")
+        capi.correlator(kind, mask, a, b, src, tt, moms, capi.MODE_F64, capi.MOMENTUM_SPACE, capi.F64, o)
+        alone.append(G.host(o))
+        outs.append(torch.zeros_like(o))
+    errors = []
+
+    def worker(i):
+        kind, mask, a, b, src, tt = cases[i]
+        st = torch.cuda.Stream()
+        try:
+            for _ in range(20):
+                outs[i].zero_()
+                torch.cuda.synchronize()
+                capi.correlator(kind, mask, a, b, src, tt, moms, capi.MODE_F64, capi.MOMENTUM_SPACE, capi.F64,
+                                outs[i], st)
+                if not np.array_equal(G.host(outs[i]), alone[i]):
+                    errors.append(i)
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    th = [threading.Thread(target=worker, args=(i,)) for i in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errors, errors

@@ -128,3 +128,70 @@ def test_features_targets_layout():
     sel = [i for i, m in enumerate(moms) if m[0] ** 2 + m[1] ** 2 + m[2] ** 2 == 9]
     assert len(sel) == 30
     assert np.allclose(y[:, learned.SHELLS.index(9)], (H[1][:, sel, 0] + H[1][:, sel, 5]).real.mean(1))
+
+
+def test_source_relative_features_drop_the_decay():
+    """cheap_features: rows go to source-relative time, and per-time-slice
+    decay factors (d^2 on the mesons, d^3 on the nucleon) cancel"""
+    T, t0 = 8, 5
+    rng = np.random.default_rng(10)
+    times = list(range(T))
+    H0 = rng.normal(size=(2, T, 1, 16)) + 1j * rng.normal(size=(2, T, 1, 16))
+    H0[0, :, 0, learned.PION] = np.abs(H0[0, :, 0, learned.PION]) + 1.0
+    x, a = learned.cheap_features(H0.reshape(-1), times, [0, 0, 0, t0], T)
+    order = [(t0 + k) % T for k in range(T)]
+    np.testing.assert_allclose(a, H0[0, order, 0, learned.PION].real)
+    np.testing.assert_allclose(x[:, learned.PION], 1.0)
+    d = np.exp(-0.7 * np.abs(np.arange(T) - t0))
+    Hd = H0.copy()
+    Hd[0] *= (d ** 2)[:, None, None]
+    Hd[1] *= (d ** 3)[:, None, None]
+    xd, ad = learned.cheap_features(Hd.reshape(-1), times, [0, 0, 0, t0], T)
+    np.testing.assert_allclose(xd, x, rtol=1e-12)
+    np.testing.assert_allclose(ad, a * (d ** 2)[order], rtol=1e-12)
+
+
+def test_shell_targets_remove_the_source_phase():
+    """C(t, n) = exp(+i 2pi n.x0/L) D(t, n) for a source at x0: the targets
+    are the shell means of Re D, the same for every x0"""
+    from plegma_amd import capi
+    L = [4, 6, 8, 4]
+    moms = capi.momlist(16)
+    m = np.array(moms)
+    rng = np.random.default_rng(11)
+    D = rng.normal(size=(2, L[3], len(moms), 16)) + 1j * rng.normal(size=(2, L[3], len(moms), 16))
+    times = list(range(L[3]))
+    ys = []
+    for x0 in ([0, 0, 0], [1, 5, 3], [3, 2, 7]):
+        ph = np.exp(2j * np.pi * (m * (np.array(x0) / np.array(L[:3]))).sum(1))
+        C = D * ph[None, None, :, None]
+        ys.append(learned.shell_targets(C.reshape(-1), moms, times, x0 + [0], L))
+    for y in ys[1:]:
+        np.testing.assert_allclose(y, ys[0], rtol=1e-12, atol=1e-12)
+    nuc = D[1][:, :, 0] + D[1][:, :, 5]
+    sel = (m ** 2).sum(1) == 9
+    np.testing.assert_allclose(ys[0][:, learned.SHELLS.index(9)], nuc[:, sel].real.mean(1))
+
+
+def test_expected_gain_matches_monte_carlo():
+    """expected_gain from one draw's labelled and held-out sources agrees with
+    the mean-squared-error ratio of the two estimators over many draws"""
+    rng = np.random.default_rng(12)
+    nlab, nbc, npred = 64, 32, 192
+    N = nlab + npred
+    sy, sr = 1.0, 0.3
+    el, em, gains = [], [], []
+    for _ in range(300):
+        u = rng.normal(size=(N, 3)) * sy                 # outputs of N sources
+        pred = u + rng.normal(size=(N, 3)) * sr + 0.2    # a biased predictor
+        est = learned.bias_corrected(_Fixed(lambda v: v, 0.0), pred[nlab - nbc:nlab], u[nlab - nbc:nlab],
+                                     pred[nlab:])
+        truth = u.mean(0)
+        m_ml = (u[:nlab].sum(0) + est.pred.sum(0)) / N
+        m_lab = u[:nlab].mean(0)
+        el.append(((m_lab - truth) ** 2).sum())
+        em.append(((m_ml - truth) ** 2).sum())
+        gains.append(learned.expected_gain(est, u[:nlab], nlab, nbc, npred))
+    mc = np.mean(el) / np.mean(em)
+    assert abs(np.median(gains) / mc - 1) < 0.25, (np.median(gains), mc)
+    assert mc > 3          # sr = 0.3 sy: the corrected estimator wins

@@ -158,6 +158,30 @@ def test_unit_propagator_closed_forms():
         np.testing.assert_allclose(d, cf.delta32(S[None], k)[0], atol=1e-12)
 
 
+def test_delta12_matches_wick_closed_form():
+    """orc_site_delta12 (restating PLEGMA_baryons_deltas.cu:3-60: the 8-term
+    iso-1/2 sum with weights -4, 2, 2, -2, -2, -1, 1, 4 over 3) against the
+    Wick engine of (2 (u Cg_k d) u + (u Cg_k u) d) / sqrt(3), with u and d
+    distinct random propagators, so that a dropped term, a wrong weight or a
+    swapped flavour changes the result; dataset 2 swaps the flavours."""
+    rng = np.random.default_rng(12)
+    S1 = rng.normal(size=(4, 4, 3, 3)) + 1j * rng.normal(size=(4, 4, 3, 3))
+    S2 = rng.normal(size=(4, 4, 3, 3)) + 1j * rng.normal(size=(4, 4, 3, 3))
+    for k in range(3):
+        d = O.site_body("delta12", S1, S2, k)
+        ref1 = cf.delta12(S1[None], S2[None], k)[0]
+        ref2 = cf.delta12(S2[None], S1[None], k)[0]
+        scale = max(np.abs(ref1).max(), np.abs(ref2).max())
+        np.testing.assert_allclose(d[:16].reshape(4, 4), ref1, atol=1e-12 * scale)
+        np.testing.assert_allclose(d[16:].reshape(4, 4), ref2, atol=1e-12 * scale)
+    # u = d: the iso-1/2 and iso-3/2 states are made of the same quarks, and
+    # (2 udu + uud)/sqrt(3) with u = d is sqrt(3) (uuu), so Delta+ = 3 Delta++ / 3
+    for k in range(3):
+        d12 = O.site_body("delta12", S1, S1, k)[:16]
+        d32 = O.site_body("delta32", S1, S1, k)[:16]
+        np.testing.assert_allclose(d12, d32, atol=1e-12 * np.abs(d32).max())
+
+
 def test_pion_positive():
     rng = np.random.default_rng(3)
     S = rng.normal(size=(4, 4, 3, 3)) + 1j * rng.normal(size=(4, 4, 3, 3))
