@@ -285,6 +285,67 @@ public:
     hip_check(hipMemcpy(H_elem(), d_out, bytes, hipMemcpyDeviceToHost), "result copy");
   }
 
+  // lib/PLEGMA_Correlator.cu:192-250: every heavy-light tetraquark product
+  // ("udbb", "usbb") whose propagators are allocated (NONE = flavour absent:
+  // u, d read propLT, s propST, c propCH, b propBT), optionally only those
+  // holding s / c; datasets = the products' operator names in order.  A
+  // double propagator runs the FP64 path, a float one the mixed path; the
+  // result is stored in Float.
+  template <typename Float2>
+  void contractTetraquarks(PLEGMA_Propagator<Float2> &propLT, PLEGMA_Propagator<Float2> &propST,
+                           PLEGMA_Propagator<Float2> &propCH, PLEGMA_Propagator<Float2> &propBT,
+                           bool only_st = false, bool only_ch = false) {
+    shape = {};
+    description = "";
+    groups = {};
+    datasets.clear();
+    PLEGMA_Propagator<Float2> *fl[4] = {&propLT, &propST, &propCH, &propBT};
+    plg_prop v[4];
+    const plg_prop *vp[4];
+    for (int i = 0; i < 4; i++) {
+      vp[i] = nullptr;
+      if (fl[i]->getAllocation() != NONE) {
+        v[i] = fl[i]->view();
+        vp[i] = &v[i];
+      }
+    }
+    int prods[32], np = 0, nd = 0;
+    plg_check(plg_tetraquarks_select(vp[0], vp[1], vp[2], vp[3], only_st, only_ch, prods, 32, &np, &nd),
+              "plg_tetraquarks_select");
+    for (int i = 0; i < np; i++) {
+      int nops = 0;
+      plg_check(plg_tetra_product(prods[i], nullptr, &nops), "plg_tetra_product");
+      for (int k = 0; k < nops; k++) {
+        char name[64];
+        plg_check(plg_tetra_op_name(prods[i], k, name, sizeof(name)), "plg_tetra_op_name");
+        datasets.push_back(name);
+      }
+    }
+    if (datasets.empty()) return;
+    initialize();
+    std::vector<int> moms = corr_space == MOMENTUM_SPACE ? corr_mom_space->intMomenta() : std::vector<int>{};
+    const int nmom = (int)moms.size() / 3;
+    const int space = corr_space == MOMENTUM_SPACE ? PLG_MOMENTUM_SPACE : PLG_POSITION_SPACE;
+    int64_t n = 0;
+    plg_check(plg_tetraquarks_size(vp[0], vp[1], vp[2], vp[3], only_st, only_ch, source.data(), totalT, nmom, space,
+                                   &n),
+              "plg_tetraquarks_size");
+    if ((size_t)n != getTotalSize()) PLEGMA_error("tetraquark correlator size mismatch (%lld vs %zu)", (long long)n,
+                                                  getTotalSize());
+    const size_t bytes = (size_t)n * 2 * sizeof(Float);
+    if (bytes > d_bytes) {
+      if (d_out) (void)hipFree(d_out);
+      hip_check(hipMalloc(&d_out, bytes), "correlator result buffer");
+      d_bytes = bytes;
+    }
+    const int mode = sizeof(Float2) == 8 ? PLG_MODE_F64 : PLG_MODE_MIXED;
+    plg_check(plg_tetraquarks(vp[0], vp[1], vp[2], vp[3], only_st, only_ch, source.data(), totalT,
+                              moms.empty() ? nullptr : moms.data(), nmom, mode, space, plg_precision_of<Float>(),
+                              d_out, nullptr),
+              "plg_tetraquarks");
+    hip_check(hipMemcpy(H_elem(), d_out, bytes, hipMemcpyDeviceToHost), "result copy");
+  }
+
   // ---- output (lib/PLEGMA_Correlator.cu:721-806, momentum space)
   void writeASCII(std::string filename) const {
     if (corr_space != MOMENTUM_SPACE) PLEGMA_error("WriteASCII do not support writing in position space.");

@@ -182,6 +182,32 @@ int plg_tetraquark_table(const plg_prop *p1, const plg_prop *p2, const plg_prop 
                          const int16_t *idxs, const int16_t *col_contr, const float *vals, const int16_t *counts,
                          int nops, const int source[4], int totalT, const int *moms, int nmom, int mode, int space,
                          int out_precision, void *out, void *stream);
+/* ---- heavy-light tetraquarks --------------------------------------------
+ * Replaces contractTetraquarks / contract_tetraquarks
+ * (lib/PLEGMA_Correlator.cu:192-250,
+ * lib/kernels/PLEGMA_heavy_light_tetraquarks.cu:163-212): the two-point
+ * functions of the heavy-light tetraquark operators (meson-meson,
+ * meson-diquark, diquark-meson and diquark-diquark forms; 16 per product) for
+ * every flavour product ("udbb", "usbb") whose propagators are given (NULL =
+ * absent): u and d read lt, s reads st, c reads ch, b reads bt; optionally
+ * only products holding s (only_st) or c (only_ch).  Datasets: the product's
+ * operators in the reference's order ("bbud/g5_gj_g5_gj_ll", ...), products
+ * in the reference's order; out as plg_baryons_udsc.  Term tables: generated
+ * from the interpolating fields (python/tetraquarks_v3.py of the reference
+ * makes the same tables). */
+int plg_tetra_count(int *nprod);
+/* flavour letters (4 chars + NUL into flavours[5]) and operator count */
+int plg_tetra_product(int prod, char *flavours, int *nops);
+int plg_tetra_op_name(int prod, int k, char *buf, int cap);
+/* the products (indices) plg_tetraquarks runs for these arguments */
+int plg_tetraquarks_select(const plg_prop *lt, const plg_prop *st, const plg_prop *ch, const plg_prop *bt,
+                           int only_st, int only_ch, int *prods, int cap, int *nprods, int *ndatasets);
+int plg_tetraquarks_size(const plg_prop *lt, const plg_prop *st, const plg_prop *ch, const plg_prop *bt,
+                         int only_st, int only_ch, const int source[4], int totalT, int nmom, int space,
+                         int64_t *elems);
+int plg_tetraquarks(const plg_prop *lt, const plg_prop *st, const plg_prop *ch, const plg_prop *bt, int only_st,
+                    int only_ch, const int source[4], int totalT, const int *moms, int nmom, int mode, int space,
+                    int out_precision, void *out, void *stream);
 /* the same, with HIP events between the stages of every product: nstages
  * stage names (32 bytes each) and their milliseconds summed over the products */
 int plg_baryons_udsc_profile(const plg_prop *up, const plg_prop *dn, const plg_prop *st, const plg_prop *ch,

@@ -21,8 +21,10 @@ LIB = os.path.join(HERE, "libplegma_amd.so")
 SOURCES = ["plg_api.hip", "plg_contract.hip", "plg_baryon.hip", "plg_project.hip", "plg_project_bf16.hip",
            "plg_synth.hip", "plg_mesmes.hip", "plg_mlp.hip", "plg_prep.hip", "plg_udsc.hip",
            "plg_tetra.hip"]
-# generated from the udsc baryon interpolating fields by plegma_amd/gen_udsc.py
+# generated from the udsc baryon and the heavy-light tetraquark interpolating
+# fields by plegma_amd/gen_udsc.py and plegma_amd/gen_tetra.py
 UDSC_TABLES = os.path.join(CSRC, "plg_udsc_tables.h")
+TETRA_TABLES = os.path.join(CSRC, "plg_tetra_tables.h")
 ARCH = "gfx950"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -46,8 +48,9 @@ def _compile(src: str) -> str:
 
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    from plegma_amd import gen_udsc
+    from plegma_amd import gen_tetra, gen_udsc
     gen_udsc.write_header(UDSC_TABLES)
+    gen_tetra.write_header(TETRA_TABLES)
     with cf.ThreadPoolExecutor(max_workers=min(4, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"

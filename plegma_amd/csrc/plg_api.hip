@@ -8,6 +8,7 @@
 // contract* calls) or once for many sources (plg_batch_*, replayed from a
 // captured hipGraph).
 #include "plg_internal.h"
+#include "plg_tetra_tables.h"
 
 #include <algorithm>
 #include <cmath>
@@ -1267,16 +1268,16 @@ static int udsc_run(const plg_prop *up, const plg_prop *dn, const plg_prop *st, 
   return PLG_OK;
 }
 
-int plg_tetraquark_table(const plg_prop *p1, const plg_prop *p2, const plg_prop *p3, const plg_prop *p4,
-                         const int16_t *idxs, const int16_t *col_contr, const float *vals, const int16_t *counts,
-                         int nops, const int source[4], int totalT, const int *moms, int nmom, int mode, int space,
-                         int out_precision, void *out, void *stream) {
+// one four-propagator table (views P1..P4), its operators written as datasets
+// d0 .. d0 + nops - 1 of the output
+static int tetra_run(const plg_prop *const views[4], const int16_t *idxs, const int16_t *col_contr, const float *vals,
+                     const int16_t *counts, int nops, int d0, const int source[4], int totalT, const int *moms,
+                     int nmom, int mode, int space, int out_precision, void *out, void *stream) {
   int rc;
   if ((rc = udsc_check_args(out, source, out_precision, mode, space, moms, nmom))) return rc;
   if (!idxs || !col_contr || !vals || !counts || nops <= 0) return fail(PLG_ERR_INVALID, "empty table");
-  const plg_prop *views[4] = {p1, p2, p3, p4};
-  for (const plg_prop *v : views)
-    if (!v) return fail(PLG_ERR_INVALID, "NULL propagator view");
+  for (int i = 0; i < 4; i++)
+    if (!views[i]) return fail(PLG_ERR_INVALID, "NULL propagator view");
   std::vector<TetraTerm> terms;
   std::vector<int> first, count;
   rc = tetra_pack(idxs, col_contr, vals, counts, nops, terms, first, count);
@@ -1296,7 +1297,7 @@ int plg_tetraquark_table(const plg_prop *p1, const plg_prop *p2, const plg_prop 
   P.terms = PLG_SPLIT_BF16X6;
   P.space = space;
   P.T = mode == PLG_MODE_F64 ? PLG_F64 : PLG_F32;
-  for (const plg_prop *v : views) P.props.push_back(*v);
+  for (int i = 0; i < 4; i++) P.props.push_back(*views[i]);
   P.nmom = space == PLG_MOMENTUM_SPACE ? nmom : 0;
   if (P.nmom) P.moms.assign(moms, moms + 3 * P.nmom);
   P.ms = scaled_moms(P.moms.data(), P.nmom, 1);
@@ -1317,7 +1318,7 @@ int plg_tetraquark_table(const plg_prop *p1, const plg_prop *p2, const plg_prop 
       o.ch_base = k;
       o.nch_row = 1;
       o.sx = source[0]; o.sy = source[1]; o.sz = source[2];
-      o.base = ((int64_t)k * lt + j) * inner;
+      o.base = ((int64_t)(d0 + k) * lt + j) * inner;
       // multilinear in the four views: one plane exponent of each
       auto plane = [&](const plg_prop &v) { return ((t - v.t_first) % T + T) % T; };
       o.ea = P.props[0].plane_exp; o.pla = plane(P.props[0]); o.ka = 1;
@@ -1329,7 +1330,7 @@ int plg_tetraquark_table(const plg_prop *p1, const plg_prop *p2, const plg_prop 
   P.csel = {0};
   P.ch_off = {0};
   P.m_stride = 1;
-  P.out_elems = (int64_t)nops * lt * inner;
+  P.out_elems = (int64_t)(d0 + nops) * lt * inner;
   P.nsites = (int64_t)lt * g_ctx.V3;
   P.launches.push_back({Launch::TETRA, 0, false, false, 0, 0});
   TetraTerm *d_terms = nullptr;
@@ -1356,6 +1357,109 @@ int plg_tetraquark_table(const plg_prop *p1, const plg_prop *p2, const plg_prop 
   hipStreamSynchronize(hs);  // no launch may still read the terms
   hipFree(d_terms);
   return rc;
+}
+
+int plg_tetraquark_table(const plg_prop *p1, const plg_prop *p2, const plg_prop *p3, const plg_prop *p4,
+                         const int16_t *idxs, const int16_t *col_contr, const float *vals, const int16_t *counts,
+                         int nops, const int source[4], int totalT, const int *moms, int nmom, int mode, int space,
+                         int out_precision, void *out, void *stream) {
+  const plg_prop *views[4] = {p1, p2, p3, p4};
+  return tetra_run(views, idxs, col_contr, vals, counts, nops, 0, source, totalT, moms, nmom, mode, space,
+                   out_precision, out, stream);
+}
+
+// ------------------------------------------------------------------ heavy-light tetraquarks
+// lib/PLEGMA_Correlator.cu:192-250 (product selection, datasets) and
+// lib/kernels/PLEGMA_heavy_light_tetraquarks.cu:163-212 (one product after the
+// other, propagator per flavour letter: u, d -> LT, s -> ST, c -> CH, b -> BT)
+static const char TT_LETTERS[5] = {'u', 'd', 's', 'c', 'b'};
+static const int TT_VIEW_OF[5] = {0, 0, 1, 2, 3};  // letter -> LT, ST, CH, BT argument
+
+static void tetra_select(const plg_prop *const fl[4], int only_st, int only_ch, std::vector<int> &prods, int &nd) {
+  prods.clear();
+  nd = 0;
+  for (int i = 0; i < TT_NPROD; i++) {
+    const std::string f = TT_PROD[i];
+    bool ok = true;
+    for (int k = 0; k < 5; k++)
+      if (!fl[TT_VIEW_OF[k]] && f.find(TT_LETTERS[k]) != std::string::npos) ok = false;
+    if (only_st && f.find('s') == std::string::npos) ok = false;
+    if (only_ch && f.find('c') == std::string::npos) ok = false;
+    if (ok) {
+      prods.push_back(i);
+      nd += TT_NOPS[i];
+    }
+  }
+}
+
+int plg_tetra_count(int *nprod) {
+  if (!nprod) return fail(PLG_ERR_INVALID, "NULL argument");
+  *nprod = TT_NPROD;
+  return PLG_OK;
+}
+
+int plg_tetra_product(int prod, char *flavours, int *nops) {
+  if (prod < 0 || prod >= TT_NPROD) return fail(PLG_ERR_INVALID, "no such flavour product");
+  if (flavours) std::memcpy(flavours, TT_PROD[prod], 5);
+  if (nops) *nops = TT_NOPS[prod];
+  return PLG_OK;
+}
+
+int plg_tetra_op_name(int prod, int k, char *buf, int cap) {
+  if (prod < 0 || prod >= TT_NPROD || k < 0 || k >= TT_NOPS[prod] || !buf || cap <= 0)
+    return fail(PLG_ERR_INVALID, "bad product, operator or buffer");
+  const char *n = TT_NAMES[prod][k];
+  if ((int)std::strlen(n) >= cap) return fail(PLG_ERR_RANGE, "name buffer too small");
+  std::strcpy(buf, n);
+  return PLG_OK;
+}
+
+int plg_tetraquarks_select(const plg_prop *lt, const plg_prop *st, const plg_prop *ch, const plg_prop *bt,
+                           int only_st, int only_ch, int *prods, int cap, int *nprods, int *ndatasets) {
+  const plg_prop *fl[4] = {lt, st, ch, bt};
+  std::vector<int> v;
+  int nd;
+  tetra_select(fl, only_st, only_ch, v, nd);
+  if (prods && (int)v.size() > cap) return fail(PLG_ERR_RANGE, "product buffer too small");
+  if (prods) std::copy(v.begin(), v.end(), prods);
+  if (nprods) *nprods = (int)v.size();
+  if (ndatasets) *ndatasets = nd;
+  return PLG_OK;
+}
+
+int plg_tetraquarks_size(const plg_prop *lt, const plg_prop *st, const plg_prop *ch, const plg_prop *bt,
+                         int only_st, int only_ch, const int source[4], int totalT, int nmom, int space,
+                         int64_t *elems) {
+  int ltT, nd, rc;
+  if (!elems) return fail(PLG_ERR_INVALID, "NULL argument");
+  if ((rc = plg_window(source, totalT, nullptr, nullptr, &ltT))) return rc;
+  const plg_prop *fl[4] = {lt, st, ch, bt};
+  std::vector<int> v;
+  tetra_select(fl, only_st, only_ch, v, nd);
+  *elems = (int64_t)nd * ltT * (space == PLG_MOMENTUM_SPACE ? nmom : g_ctx.V3);
+  return PLG_OK;
+}
+
+int plg_tetraquarks(const plg_prop *lt, const plg_prop *st, const plg_prop *ch, const plg_prop *bt, int only_st,
+                    int only_ch, const int source[4], int totalT, const int *moms, int nmom, int mode, int space,
+                    int out_precision, void *out, void *stream) {
+  int rc;
+  if ((rc = udsc_check_args(out, source, out_precision, mode, space, moms, nmom))) return rc;
+  const plg_prop *fl[4] = {lt, st, ch, bt};
+  std::vector<int> prods;
+  int nd_total;
+  tetra_select(fl, only_st, only_ch, prods, nd_total);
+  int d0 = 0;
+  for (int prod : prods) {
+    const char *f = TT_PROD[prod];
+    const plg_prop *views[4];
+    for (int i = 0; i < 4; i++) views[i] = fl[TT_VIEW_OF[std::find(TT_LETTERS, TT_LETTERS + 5, f[i]) - TT_LETTERS]];
+    if ((rc = tetra_run(views, TT_IDX[prod], TT_COL[prod], TT_VAL[prod], TT_COUNT[prod], TT_NOPS[prod], d0, source,
+                        totalT, moms, nmom, mode, space, out_precision, out, stream)))
+      return rc;
+    d0 += TT_NOPS[prod];
+  }
+  return PLG_OK;
 }
 
 int plg_baryons_table(const plg_prop *p1, const plg_prop *p2, const plg_prop *p3, const int16_t *idxs,

@@ -92,6 +92,28 @@ template <typename Float> static int run(const std::string &out) {
     std::fclose(f);
   }
   {
+    // heavy-light tetraquarks with the charm propagator absent and the light
+    // one as LT: products "udbb" and "usbb"; u (seed stream 1) as LT, the
+    // strange stream 3 as ST, a heavy bottom (stream 4, decay 1.2) as BT
+    PLEGMA_Propagator<Float> s, b, none(NONE);
+    if (sizeof(Float) == 8) {
+      s.synthesize(1234, 3, 0, 0.8);
+      b.synthesize(1234, 4, 0, 1.2);
+    } else {
+      PLEGMA_Propagator<double> s64, b64;
+      s64.synthesize(1234, 3, 0, 0.8);
+      b64.synthesize(1234, 4, 0, 1.2);
+      s.castFrom(s64);
+      b.castFrom(b64);
+    }
+    PLEGMA_Correlator<double> c(MOMENTUM_SPACE, src, 2, 6);
+    c.contractTetraquarks(u, s, none, b);
+    dump(c, out + "/tetra_" + tag + ".bin");
+    FILE *f = std::fopen((out + "/tetra_" + tag + ".names").c_str(), "w");
+    for (auto &n : c.getDatasets()) std::fprintf(f, "%s\n", n.c_str());
+    std::fclose(f);
+  }
+  {
     // a propagator as a 144-component field through PLEGMA_FT::apply
     PLEGMA_FT<Float> ft(2);
     ft.apply(u, FT_GEMV, -1);

@@ -206,3 +206,28 @@ def test_udsc_hdf5(outputs, prec, tol):
         for j in range(6):
             tau = (O.row_time(L, SRC, 6, j) - SRC[3]) % L[3]
             assert G.rel_err(got[tau][:, None], ref[k, j][:, None], axis=0) < tol
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-6)])
+def test_tetraquarks_class(outputs, prec, tol):
+    """contractTetraquarks(LT = u, ST = s, CH absent, BT = b) into a double
+    correlator: the datasets of "udbb" and "usbb" (reference order, names),
+    H_elem [dataset][t][mom], against the oracle's 81-term sums over the
+    generated tables"""
+    from plegma_amd import gen_tetra
+    d, u, dn = outputs
+    s = O.synth_prop(L, 1234, 3, 0, 0.8)
+    b = O.synth_prop(L, 1234, 4, 0, 1.2)
+    fl = {"u": u, "d": u, "s": s, "b": b}
+    names, refs = [], []
+    for p, ops in gen_tetra.tables():
+        idx = np.concatenate([o[1] for o in ops])
+        col = np.concatenate([o[2] for o in ops])
+        val = np.concatenate([o[3] for o in ops]).astype(np.complex128)
+        refs.append(O.tetra_correlator(L, SRC, 6, 2, [fl[f] for f in p], idx, col, val, [len(o[3]) for o in ops]))
+        names += [o[0] for o in ops]
+    ref = np.concatenate(refs)
+    assert open(os.path.join(d, f"tetra_{prec}.names")).read().split() == names
+    a = np.fromfile(os.path.join(d, f"tetra_{prec}.bin"), dtype=np.float64)
+    got = (a[0::2] + 1j * a[1::2]).reshape(ref.shape)
+    assert G.rel_err(got, ref, axis=-1) < tol

@@ -136,3 +136,92 @@ def test_tetra_window_views(props):
     ref = _oracle(props, tab, hosts=hosts, views=[(nt * V3, tf, nt)] * 4)
     got = _run(views, tab)
     assert G.rel_err(got, ref, axis=-1) < 1e-12
+
+
+def test_tetra_fused_prep(props):
+    """rotation to the physical basis (both signs) and the boundary sign of
+    the four views applied on load, against the oracle on prepared copies"""
+    tab = _table(np.random.default_rng(9), nops=4, nterm=40)
+    hosts, views = [], []
+    for (t, v, h), sign in zip(props, (1, -1, 1, -1)):
+        w = capi.plg_prop(v.data, v.precision, v.t_first, v.nt, 0, v.comp_stride)
+        w.rotate_sign = sign
+        w.bc_t0 = SRC[3]
+        views.append(w)
+        hp = h.copy().reshape(144, -1, 2)
+        O.prop_rotate(hp, L, sign)
+        O.prop_boundaries(hp, L, SRC[3])
+        hosts.append(hp.reshape(-1))
+    ref = _oracle(props, tab, hosts=hosts)
+    got = _run(views, tab)
+    assert G.rel_err(got, ref, axis=-1) < 1e-12
+
+
+def _builtin_ref(fl, moms_q2=Q2, only_st=False, only_ch=False):
+    """the oracle's 81-term sums over the generated tables of every product
+    whose flavours are present (None = absent), datasets in order"""
+    from plegma_amd import gen_tetra
+    names, refs = [], []
+    for p, ops in gen_tetra.tables():
+        if any(fl[f] is None for f in p) or (only_st and "s" not in p) or (only_ch and "c" not in p):
+            continue
+        idx = np.concatenate([o[1] for o in ops])
+        col = np.concatenate([o[2] for o in ops])
+        val = np.concatenate([o[3] for o in ops]).astype(np.complex128)
+        refs.append(O.tetra_correlator(L, SRC, TOTALT, moms_q2, [fl[f] for f in p], idx, col, val,
+                                       [len(o[3]) for o in ops]))
+        names += [o[0] for o in ops]
+    return names, (np.concatenate(refs) if refs else np.zeros((0, TOTALT, len(O.momlist(moms_q2)))))
+
+
+def _run_builtin(lt, st, ch, bt, mode=capi.MODE_F64, only_st=False, only_ch=False):
+    torch = G.torch_cuda()
+    moms = capi.momlist(Q2)
+    n = capi.tetraquarks_size(lt, st, ch, bt, SRC, TOTALT, len(moms), only_st=only_st, only_ch=only_ch)
+    out = torch.full((max(n, 1), 2), float("nan"), dtype=torch.float64, device="cuda")
+    print(f"[gpu] built-in tetraquarks: {n} elements, mode {mode}", flush=True)
+    capi.tetraquarks(lt, st, ch, bt, SRC, TOTALT, moms, mode, capi.MOMENTUM_SPACE, capi.F64, out,
+                     only_st=only_st, only_ch=only_ch)
+    h = G.host(out).reshape(-1, 2)[:n]
+    return (h[:, 0] + 1j * h[:, 1]).reshape(-1, TOTALT, len(moms))
+
+
+def test_builtin_tetraquarks_products():
+    prods = capi.tetra_products()
+    from plegma_amd import gen_tetra
+    assert [(p, [o[0] for o in ops]) for p, ops in gen_tetra.tables()] == prods
+    assert [p for p, _ in prods] == ["udbb", "usbb"]
+
+
+@pytest.mark.parametrize("mode,tol", [(capi.MODE_F64, 1e-12), (capi.MODE_MIXED, 1e-6)])
+def test_builtin_tetraquarks(props, mode, tol):
+    """plg_tetraquarks with LT, ST and BT present and CH absent: both
+    products ("udbb" with LT twice, "usbb"), 32 datasets, against the oracle
+    over the generated tables (which equal the reference's)"""
+    lt, st, bt = props[0], props[1], props[3]
+    fl = {"u": lt[2], "d": lt[2], "s": st[2], "c": None, "b": bt[2]}
+    names, ref = _builtin_ref(fl)
+    assert len(names) == 32
+    if mode == capi.MODE_F64:
+        views = [lt[1], st[1], None, bt[1]]
+    else:
+        keep = [G.cast_f32(p[0], p[1], scaled=True) for p in (lt, st, bt)]
+        views = [keep[0][1], keep[1][1], None, keep[2][1]]
+    got = _run_builtin(*views, mode=mode)
+    assert got.shape == ref.shape
+    assert G.rel_err(got, ref, axis=-1) < tol
+
+
+def test_builtin_tetraquarks_selection(props):
+    """the reference's product selection (lib/PLEGMA_Correlator.cu:209-233):
+    ST absent drops "usbb", only_st keeps it alone, BT absent or only_ch
+    (no charm product) leaves nothing"""
+    lt, st, bt = props[0][1], props[1][1], props[3][1]
+    assert capi.tetraquarks_select(lt, None, None, bt) == ([0], 16)
+    assert capi.tetraquarks_select(lt, st, None, bt, only_st=True) == ([1], 16)
+    assert capi.tetraquarks_select(lt, st, None, None) == ([], 0)
+    assert capi.tetraquarks_select(lt, st, None, bt, only_ch=True) == ([], 0)
+    fl = {"u": props[0][2], "d": props[0][2], "s": None, "c": None, "b": props[3][2]}
+    names, ref = _builtin_ref(fl)
+    got = _run_builtin(lt, None, None, bt)
+    assert len(names) == 16 and G.rel_err(got, ref, axis=-1) < 1e-12
