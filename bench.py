@@ -271,16 +271,34 @@ def rel_err_block(A, B):
     return (np.abs(A - B) / den).max(axis=(1, 3))
 
 
+def shell_pools(q2, minsize=24):
+    """|n|^2 shells pooled in increasing order until each pool holds at least
+    `minsize` momenta (a remainder joins the last pool): boolean masks"""
+    q2 = np.asarray(q2)
+    pools, cur = [], np.zeros(len(q2), bool)
+    for q in np.unique(q2):
+        cur = cur | (q2 == q)
+        if cur.sum() >= minsize:
+            pools.append(cur)
+            cur = np.zeros(len(q2), bool)
+    if cur.any():
+        if pools:
+            pools[-1] = pools[-1] | cur
+        else:
+            pools.append(cur)
+    return pools
+
+
 def shell_err_block(A, B, moms):
     """[group, momentum] max over time slices and channels of |A - B| / rms
-    of |B| over the momenta of the same |n|^2 shell: the error per channel and
-    momentum at the scale of a correlator at that |n| (the n = 0 value does
-    not set the scale at |n| > 0, an accidental zero at one momentum does not
-    either); A, B: [group, row, momentum, channel]"""
+    of |B| over the momenta of similar |n| (|n|^2 shells pooled until a pool
+    holds at least 24 momenta): the error per channel and momentum at the
+    scale of a correlator at that |n| (the n = 0 value does not set the scale
+    at |n| > 0; an accidental zero at one momentum, or a shell of one or six
+    momenta, does not either); A, B: [group, row, momentum, channel]"""
     q2 = (np.asarray(moms).reshape(-1, 3) ** 2).sum(1)
     den = np.empty(B.shape)
-    for q in np.unique(q2):
-        sel = q2 == q
+    for sel in shell_pools(q2):
         den[:, :, sel, :] = np.sqrt((np.abs(B[:, :, sel, :]) ** 2).mean(axis=2, keepdims=True))
     den = np.where(den == 0, 1.0, den)
     return (np.abs(A - B) / den).max(axis=(1, 3))
@@ -306,7 +324,7 @@ def err_summary(A, B, moms):
 
 ERR_DEFS = {"normwise": "|C - C_ref| / max over momenta |C_ref| per (group, row, channel)",
             "per_channel_momentum": "|C - C_ref| per (group, row, channel, momentum) / rms of |C_ref| "
-                                    "over the momenta of the same |n|^2 shell",
+                                    "over the momenta of similar |n| (|n|^2 shells pooled to >= 24 momenta)",
             "pointwise": "|C - C_ref| / |C_ref| per element (unbounded at accidental zeros of C_ref)"}
 
 

@@ -591,6 +591,7 @@ static ProjGeom plan_geom(const Plan &P) {
   g.mom_pair = P.d_mom_pair;
   g.pidx = P.d_pidx;
   g.zero_pair = P.zero_pair;
+  g.shifted = P.T == PLG_F32 && P.npairs > 0;
   return g;
 }
 
@@ -764,17 +765,17 @@ static int autotune(Plan &P, hipStream_t st) {
   // projection: tile code and K splits, timed together with the finalize
   // (which sums the splits) into a scratch output
   ProjGeom g = plan_geom(P);
-  void *scratch = nullptr;
-  if (hipMalloc(&scratch, std::max<size_t>(1, (size_t)P.out_elems) * sizeof(cx<double>)) != hipSuccess)
-    return fail(PLG_ERR_HIP, "autotune scratch allocation failed");
+  void *tune_out = nullptr;
+  if (hipMalloc(&tune_out, std::max<size_t>(1, (size_t)P.out_elems) * sizeof(cx<double>)) != hipSuccess)
+    return fail(PLG_ERR_HIP, "autotune output allocation failed");
   const int nout = (int)P.csel.size();
   std::vector<int> codes = P.T == PLG_F64 ? std::vector<int>{32, 64} : proj_candidates(std::max(P.npairs - P.zero_pair, 1));
-  const size_t per_split = part_bytes(P) / std::max(P.nsplit, 1);
   double best_p = 1e30;
   int best_code = P.T == PLG_F64 ? P.mtile : P.mbw, best_ns = P.nsplit;
   for (int code : codes)
     for (int ns : {1, 2, 4, 8, 16, 32}) {
-      if ((int64_t)ns * 256 > g_ctx.V3 || per_split * ns > P.part_bytes) continue;
+      const size_t need = proj_part_bytes(ns, (int64_t)P.rows.size() * P.nch_c, std::max(P.nmom, 1), P.npairs);
+      if ((int64_t)ns * 256 > g_ctx.V3 || need > P.part_bytes) continue;
       g.nsplit = ns;
       int lrc = PLG_OK;
       const double ms = time_ms(
@@ -784,7 +785,7 @@ static int autotune(Plan &P, hipStream_t st) {
               lrc = launch_project_bf16(g, (const cx<float> *)P.c, (double *)P.part, P.terms, g_ctx.d_wtab, code, st);
             if (lrc == PLG_OK)
               lrc = launch_finalize(g, (const double *)P.part, P.d_orows, (int)P.orows.size(), P.nch_c, nout,
-                                    P.d_csel, P.d_choff, P.m_stride, scratch, PLG_F64, st);
+                                    P.d_csel, P.d_choff, P.m_stride, tune_out, PLG_F64, st);
           },
           st);
       if (lrc != PLG_OK) {  // a tile this geometry does not provide (LDS size)
@@ -801,7 +802,7 @@ static int autotune(Plan &P, hipStream_t st) {
   else P.mbw = best_code;
   P.nsplit = best_ns;
   hipStreamSynchronize(st);
-  hipFree(scratch);
+  hipFree(tune_out);
   if (hipGetLastError() != hipSuccess) return fail(PLG_ERR_HIP, "autotune launch failed");
   tune_store(key, best_code, best_ns, best_tpb, best_c + best_p);
   return PLG_OK;
@@ -1608,6 +1609,7 @@ int plg_ft_apply(const void *field, int precision, int dof, int nt, const int *m
     rc = ensure(&g_ctx.ws_c, &g_ctx.ws_c_bytes, (size_t)g.ncols * ctile_sites(g_ctx.V3) * sizeof(cx<float>));
     if (!rc) rc = launch_to_tiles((const cx<float> *)field, g.ncols, g_ctx.V3, (cx<float> *)g_ctx.ws_c, st);
     g.ctile = 1;
+    g.shifted = 1;
   }
   if (!rc) {
     if (precision == PLG_F64)

@@ -160,6 +160,10 @@ struct ProjGeom {
   const int *mom_pair = nullptr;  // device, nmom: 2j (+1 for the -n member)
   const uint16_t *pidx = nullptr;  // device, phase indices n_j.x (M/L) mod M of the pairs
   int zero_pair = 0;  // 1: the last pair is n = 0, summed directly instead of a GEMM row
+  // 1: the partials hold sums of c - a (a: per-column shift stored after
+  // them, see proj_shift); the mixed projection, whose FP32 accumulation of
+  // a column's mean value would otherwise leave a coherent error at n != 0
+  int shifted = 0;
   // layout of the K x N operand: 1 = site correlator, tile-major (below);
   // 0 = a field, column-major (column n, site x at n*V3 + x).  The mixed
   // projection reads the tile-major layout only.
@@ -198,6 +202,16 @@ std::vector<int> proj_candidates(int npairs);
 // momentum pair is moved to the end (*zero_last = 1).
 int make_pairs(const std::vector<int> &ms, int M, std::vector<int> &pair_ms, std::vector<int> &mom_pair,
                int *zero_last);
+// Per-column shift of the mixed projection: complex FP32 a[n] after the
+// paired partials.  The projection sums phase x (c - a); since the phases of
+// a momentum n != 0 (mod L) sum to zero over the volume, only momenta whose
+// scaled components are all zero get V3 a added back (part_value).
+__host__ __device__ __forceinline__ cx<float> *proj_shift(const ProjGeom &g, double *part) {
+  return (cx<float> *)(part + (int64_t)g.nsplit * 2 * g.ncols * 2 * g.npairs);
+}
+__host__ __device__ __forceinline__ const cx<float> *proj_shift(const ProjGeom &g, const double *part) {
+  return (const cx<float> *)(part + (int64_t)g.nsplit * 2 * g.ncols * 2 * g.npairs);
+}
 // the projection sum_s part[s] of column n, momentum m, in either layout
 __device__ __forceinline__ cx<double> part_value(const ProjGeom &g, const double *part, int64_t n, int m) {
   double re = 0, im = 0;
@@ -212,6 +226,12 @@ __device__ __forceinline__ cx<double> part_value(const ProjGeom &g, const double
       sr += p[1];
       ci += p[R];
       si += p[R + 1];
+    }
+    const int *pj = g.pair_ms + 3 * (pm >> 1);
+    if (g.shifted && pj[0] == 0 && pj[1] == 0 && pj[2] == 0) {
+      const cx<float> a = proj_shift(g, part)[n];
+      cr += (double)g.V3 * (double)a.re;
+      ci += (double)g.V3 * (double)a.im;
     }
     // P(n) = (Cr + Si) + i (Ci - Sr), P(-n) = (Cr - Si) + i (Ci + Sr)
     if (pm & 1) {
@@ -232,7 +252,8 @@ __device__ __forceinline__ cx<double> part_value(const ProjGeom &g, const double
 }
 // bytes of the partial sums of a projection
 inline size_t proj_part_bytes(int nsplit, int64_t ncols, int nmom, int npairs) {
-  return npairs > 0 ? (size_t)nsplit * ncols * npairs * 32 : (size_t)nsplit * ncols * nmom * 16;
+  return npairs > 0 ? (size_t)nsplit * ncols * npairs * 32 + (size_t)ncols * sizeof(cx<float>)
+                    : (size_t)nsplit * ncols * nmom * 16;
 }
 int launch_ft_out(const ProjGeom &g, const double *part, int dof, int nt, int accumulate, void *out,
                   int out_prec, hipStream_t st);

@@ -124,6 +124,58 @@ __device__ __forceinline__ void split3(float a, float b, uint32_t *w) {
   w[2] = __builtin_bit_cast(uint32_t, h2);
 }
 
+// Per-column shift a[n] of the mixed projection.  Any value is exact (the
+// phases of n != 0 sum to zero over the volume); a value near the column's
+// mean keeps the FP32 accumulators from carrying its coherent part, which
+// otherwise leaves an error of order 2^-24 |C(0)| at momenta with some
+// n_k = 0 (a large mean: the pion's sum of |S|^2).  From 256 sites spread
+// through the volume, per real part: mean m and spread s in FP64; a = 0
+// unless |m| > s / 4 (four standard errors), else m rounded to 8
+// significant bits, so that c - a is exact in FP32 for every value within
+// 2^16 |a| of a (a rounding of c - a that depends on a's low bits would be
+// the same at every site, a coherent error of V3 ulp).
+__device__ __forceinline__ float col_shift_part(double m, double s2) {
+  if (m * m * 16.0 <= s2) return 0.0f;
+  uint32_t u = __float_as_uint((float)m);
+  u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
+  return __uint_as_float(u);
+}
+// 256 threads per 16 columns: thread (sample lane l, column j) sums the
+// samples l, l + 16, ...; the 16 lanes of a column are combined in LDS.
+constexpr int CS_SAMPLES = 256, CS_COLS = 16, CS_LANES = 16;
+__global__ __launch_bounds__(256) void k_col_shift(const cx<float> *__restrict__ c, int64_t ncols, int64_t V3,
+                                                   cx<float> *__restrict__ a) {
+  __shared__ double red[4][CS_LANES][CS_COLS];
+  const int j = threadIdx.x % CS_COLS, l = threadIdx.x / CS_COLS;
+  const int64_t n = (int64_t)blockIdx.x * CS_COLS + j;
+  const int64_t ns = V3 < CS_SAMPLES ? V3 : CS_SAMPLES;
+  double re = 0, im = 0, re2 = 0, im2 = 0;
+  if (n < ncols)
+    for (int64_t i = l; i < ns; i += CS_LANES) {
+      const cx<float> v = c[ctile_idx(n, i * V3 / ns, ncols)];
+      re += v.re;
+      im += v.im;
+      re2 += (double)v.re * v.re;
+      im2 += (double)v.im * v.im;
+    }
+  red[0][l][j] = re;
+  red[1][l][j] = im;
+  red[2][l][j] = re2;
+  red[3][l][j] = im2;
+  __syncthreads();
+  if (l == 0 && n < ncols) {
+    for (int k = 1; k < CS_LANES; k++) {
+      re += red[0][k][j];
+      im += red[1][k][j];
+      re2 += red[2][k][j];
+      im2 += red[3][k][j];
+    }
+    const double mr = re / (double)ns, mi = im / (double)ns;
+    a[n] = cmake<float>(col_shift_part(mr, re2 / (double)ns - mr * mr),
+                        col_shift_part(mi, im2 / (double)ns - mi * mi));
+  }
+}
+
 // Warp-specialised workgroup of 512 threads: waves 0-3 run the MFMAs of
 // step k on one stage of the LDS tiles while waves 4-7 (the producers) split
 // step k + 1 into the other stage; one barrier per step.  The producers keep
@@ -136,6 +188,7 @@ __device__ __forceinline__ void split3(float a, float b, uint32_t *w) {
 template <int WR, int WC, int GR, int TERMS, int NCW>
 __global__ __launch_bounds__((NCW + 4) * 64, 1) void k_project_pair(ProjGeom g, const cx<float> *__restrict__ c,
                                                                     const uint32_t *__restrict__ wtab, double *__restrict__ part,
+                                                                    const cx<float> *__restrict__ shift,
                                                                     int nrg, int ncg) {
   constexpr int NP = TERMS == 3 ? 2 : 3;
   constexpr int GC = NCW / GR;            // consumer waves along the columns
@@ -203,10 +256,13 @@ __global__ __launch_bounds__((NCW + 4) * 64, 1) void k_project_pair(ProjGeom g, 
       asrc[q] = g.pidx + (xb / PQ_KT * PP + pr) * PQ_KT + 8 * ao[q];
     }
     // column item q: complex column bn of the tile, sites 8 bo .. 8 bo + 7;
-    // reads clamped to the last column, items past the tile store into junk
+    // reads clamped to the last column, items past the tile store into junk.
+    // The column's shift is subtracted from every site value as it is loaded
+    // (sites past the split stay zero).
     int boff[BPI], bo[BPI];
     bool bin[BPI];
     const cx<float> *bsrc[BPI];
+    cx<float> bsh[BPI];
     const int64_t bstep = 2 * (int64_t)g.ncols * 16;  // two site tiles
 #pragma unroll
     for (int q = 0; q < BPI; q++) {
@@ -215,7 +271,9 @@ __global__ __launch_bounds__((NCW + 4) * 64, 1) void k_project_pair(ProjGeom g, 
       bo[q] = (it >> 4) & 3;
       boff[q] = (BPI * PQ_THREADS == BIT || it < BIT) ? lds_q(bn, bo[q]) : -1;
       bin[q] = cx0 + bn < g.ncols;
-      bsrc[q] = c + ctile_idx(min(cx0 + bn, (int64_t)g.ncols - 1), xb + 8 * bo[q], g.ncols);
+      const int64_t col = min(cx0 + bn, (int64_t)g.ncols - 1);
+      bsrc[q] = c + ctile_idx(col, xb + 8 * bo[q], g.ncols);
+      bsh[q] = shift[col];
     }
     constexpr int NQ = PQ_INFLIGHT;
     double zre[BPI], zim[BPI];  // column sums of the staged steps (n = 0)
@@ -235,8 +293,8 @@ __global__ __launch_bounds__((NCW + 4) * 64, 1) void k_project_pair(ProjGeom g, 
 #pragma unroll
         for (int s = 0; s < 4; s++) {
           const f4 v = src[s];
-          bvr[SL][q][2 * s] = cmake<float>(v[0], v[1]);
-          bvr[SL][q][2 * s + 1] = cmake<float>(v[2], v[3]);
+          bvr[SL][q][2 * s] = cmake<float>(v[0] - bsh[q].re, v[1] - bsh[q].im);
+          bvr[SL][q][2 * s + 1] = cmake<float>(v[2] - bsh[q].re, v[3] - bsh[q].im);
         }
       }
     };
@@ -249,8 +307,15 @@ __global__ __launch_bounds__((NCW + 4) * 64, 1) void k_project_pair(ProjGeom g, 
       for (int q = 0; q < BPI; q++) {
         const int64_t x0 = xt + 8 * bo[q];
 #pragma unroll
-        for (int s = 0; s < 8; s++)
-          bvr[0][q][s] = (bin[q] && x0 + s < xe) ? bsrc[q][k * bstep + s] : cmake<float>(0, 0);
+        for (int s = 0; s < 8; s++) {
+          cx<float> v = cmake<float>(0, 0);
+          if (bin[q] && x0 + s < xe) {
+            v = bsrc[q][k * bstep + s];
+            v.re -= bsh[q].re;
+            v.im -= bsh[q].im;
+          }
+          bvr[0][q][s] = v;
+        }
       }
     };
     // register set SL -> tiles of one stage (column values split into bf16 pieces)
@@ -481,10 +546,13 @@ int launch_project_bf16(const ProjGeom &g, const cx<float> *c, double *part, int
   if ((ncg + 7) / 8 * 8 * nrg > INT32_MAX) return PLG_ERR_UNSUPPORTED;
   dim3 block((t.ncw + 4) * 64);
   dim3 grid((unsigned)((ncg + 7) / 8 * 8 * nrg), (unsigned)g.nsplit);
+  if (!g.shifted) return PLG_ERR_INVALID;
+  cx<float> *shift = proj_shift(g, part);
+  k_col_shift<<<dim3((unsigned)((g.ncols + CS_COLS - 1) / CS_COLS)), dim3(256), 0, st>>>(c, g.ncols, g.V3, shift);
 #define PLG_PQ(WR, WC, GR, NCW)                                                                            \
   if (t.wr == WR && t.wc == WC && t.gr == GR && t.ncw == NCW) {                                          \
-    if (terms == 6) k_project_pair<WR, WC, GR, 6, NCW><<<grid, block, 0, st>>>(g, c, wtab, part, nrg, (int)ncg); \
-    else k_project_pair<WR, WC, GR, 3, NCW><<<grid, block, 0, st>>>(g, c, wtab, part, nrg, (int)ncg);            \
+    if (terms == 6) k_project_pair<WR, WC, GR, 6, NCW><<<grid, block, 0, st>>>(g, c, wtab, part, shift, nrg, (int)ncg); \
+    else k_project_pair<WR, WC, GR, 3, NCW><<<grid, block, 0, st>>>(g, c, wtab, part, shift, nrg, (int)ncg);            \
     return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;                                       \
   }
   PLG_PQ_CODES(PLG_PQ)

@@ -75,21 +75,40 @@ def oracle_H(kind, L, src, totalT, q2, P1, P2, mask=O.G_NUCL, rows=None, nthread
     return O.correlator(kind, L, src, totalT, q2, P1, P2, mask=mask, rows=rows, nthreads=nthreads)
 
 
+def shell_pools(q2, minsize=24):
+    """|n|^2 shells pooled in increasing order until each pool holds at least
+    `minsize` momenta (a remainder below that joins the last pool): boolean
+    masks over the momenta"""
+    q2 = np.asarray(q2)
+    pools, cur = [], np.zeros(len(q2), bool)
+    for q in np.unique(q2):
+        cur = cur | (q2 == q)
+        if cur.sum() >= minsize:
+            pools.append(cur)
+            cur = np.zeros(len(q2), bool)
+    if cur.any():
+        if pools:
+            pools[-1] = pools[-1] | cur
+        else:
+            pools.append(cur)
+    return pools
+
+
 def shell_rel_err(got, ref, moms, axis=-2):
     """error per (channel, momentum): |got - ref| / rms of |ref| over the
-    momenta of the same |n|^2 shell (same group, dataset, row and channel).
-    The shell rms is the scale of a correlator at that |n|, so the n = 0 value
-    (which dominates a row's maximum) does not set the scale at |n| > 0, and
-    an accidental near-zero value at one momentum does not blow the error up.
-    Returns the max over all elements."""
+    momenta of similar |n|: the |n|^2 shells pooled until a pool holds at
+    least 24 momenta (same group, dataset, row and channel).  The pool rms is
+    the scale of a correlator at that |n|, so the n = 0 value (which dominates
+    a row's maximum) does not set the scale at |n| > 0, and an accidental
+    near-zero value at one momentum (n = 0 of a zero-mean channel, a shell of
+    6) does not blow the error up.  Returns the max over all elements."""
     moms = np.asarray(moms).reshape(-1, 3)
     q2 = (moms ** 2).sum(1)
     got = np.moveaxis(got, axis, -1)
     ref = np.moveaxis(ref, axis, -1)
     err = np.abs(got - ref)
     den = np.empty(err.shape)
-    for q in np.unique(q2):
-        sel = q2 == q
+    for sel in shell_pools(q2):
         den[..., sel] = np.sqrt((np.abs(ref[..., sel]) ** 2).mean(-1, keepdims=True))
     den = np.where(den == 0, 1.0, den)
     return float((err / den).max())

@@ -13,7 +13,8 @@ check compares sampled rows (time slices) in full: every group, channel and
 momentum of the row.  Tolerances:
   FP64:  normwise over momenta (per group, dataset, row and channel) 1e-12;
   mixed: normwise 1e-6, and per (channel, momentum) against the rms of the
-         reference over the momentum's |n|^2 shell 5e-6 (the pointwise
+         reference over the momenta of similar |n| (|n|^2 shells pooled to at
+         least 24 momenta) 5e-6 (the pointwise
          relative error is printed; it is unbounded at accidental zeros).
 """
 import os

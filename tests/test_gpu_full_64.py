@@ -8,7 +8,8 @@ plg_correlator (mixed path: FP32 contraction, split-bf16 MFMA projection, FP64
 time-slice sums).  The oracle recomputes sampled time slices in FP64 from the
 FP64 propagator values the FP32 data were rounded from.  Tolerances: normwise
 over momenta 1e-6; per (channel, momentum) against the rms of the reference
-over the momentum's |n|^2 shell 5e-6.
+over the momenta of similar |n| (|n|^2 shells pooled to at least 24
+momenta) 5e-6.
 """
 import os
 
