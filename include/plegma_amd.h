@@ -124,6 +124,23 @@ int plg_kind_shape(int kind, int group_mask, int *ngroups, int *ndatasets, int *
 int plg_correlator(int kind, int group_mask, const plg_prop *p1, const plg_prop *p2,
                    const int source[4], int totalT, const int *moms, int nmom, int mode,
                    int space, int out_precision, void *out, void *stream);
+/* plg_correlator without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory while they run
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernels and results as plg_correlator; the
+ * plan tables, the site correlator and the projection partials of the call
+ * live in one static device array of PLG_STATIC_SCRATCH_BYTES per GPU, set up
+ * by plg_init and held until the process ends.  A correlator that needs more
+ * (site correlator: rows x channels x sites x 8 B in FP32, 16 B in FP64)
+ * returns PLG_ERR_UNSUPPORTED before any work is queued.  Tiles come from the
+ * tuning cache when it holds the geometry, else from the built-in choice (no
+ * tuning run).  Calls of one process are serialised; calls of several
+ * processes, or replays of a captured call, that overlap on one GPU race on
+ * the array.  Synchronises `stream` before returning. */
+#define PLG_STATIC_SCRATCH_BYTES 134217728LL /* 128 MiB */
+int plg_correlator_static(int kind, int group_mask, const plg_prop *p1, const plg_prop *p2,
+                          const int source[4], int totalT, const int *moms, int nmom, int mode,
+                          int space, int out_precision, void *out, void *stream);
 /* number of complex elements plg_correlator writes (getTotalSize) */
 int plg_correlator_size(int kind, int group_mask, const int source[4], int totalT, int nmom,
                         int space, int64_t *elems);

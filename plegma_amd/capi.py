@@ -28,7 +28,7 @@ STATUS = {0: "PLG_OK", -1: "PLG_ERR_INVALID", -2: "PLG_ERR_NOT_INIT", -3: "PLG_E
 
 EXPORTS = ["plg_init", "plg_finalize", "plg_last_error", "plg_version", "plg_momlist",
            "plg_window", "plg_row_time", "plg_kind_shape", "plg_correlator",
-           "plg_correlator_size", "plg_batch_create", "plg_batch_run", "plg_batch_info",
+           "plg_correlator_static", "plg_correlator_size", "plg_batch_create", "plg_batch_run", "plg_batch_info",
            "plg_batch_destroy", "plg_batch_profile", "plg_synth_prop", "plg_synth_prop_perturbed", "plg_cast_prop",
            "plg_mlp_create", "plg_mlp_nparams", "plg_mlp_set_params", "plg_mlp_get_params",
            "plg_mlp_forward", "plg_mlp_gradient", "plg_mlp_train", "plg_mlp_destroy", "plg_ft_apply",
@@ -76,6 +76,7 @@ def lib():
         L.plg_correlator.argtypes = [C.c_int, C.c_int, C.POINTER(plg_prop), C.POINTER(plg_prop), ip,
                                      C.c_int, ip, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                      C.c_void_p]
+        L.plg_correlator_static.argtypes = L.plg_correlator.argtypes
         L.plg_correlator_size.argtypes = [C.c_int, C.c_int, ip, C.c_int, C.c_int, C.c_int,
                                           C.POINTER(C.c_int64)]
         L.plg_batch_create.argtypes = [C.c_int, C.c_int, C.POINTER(plg_prop), C.c_int,
@@ -198,6 +199,18 @@ def correlator(kind, mask, p1, p2, source, totalT, moms, mode, space, out_prec, 
     check(lib().plg_correlator(kind, mask, C.byref(p1), C.byref(p2) if p2 is not None else None,
                                ivec(source), totalT, ivec(flat) if flat else None, len(moms), mode,
                                space, out_prec, C.c_void_p(out.data_ptr()), _stream(stream)))
+
+
+def correlator_static(kind, mask, p1, p2, source, totalT, moms, mode, space, out_prec, out,
+                      stream=None):
+    """plg_correlator without device allocation (the library's static scratch)"""
+    flat = [int(v) for m in moms for v in m]
+    check(lib().plg_correlator_static(kind, mask, C.byref(p1), C.byref(p2) if p2 is not None else None,
+                                      ivec(source), totalT, ivec(flat) if flat else None, len(moms), mode,
+                                      space, out_prec, C.c_void_p(out.data_ptr()), _stream(stream)))
+
+
+STATIC_SCRATCH_BYTES = 128 << 20
 
 
 def ft_apply(field, precision, dof, nt, moms, sign, out, out_prec=F64, accumulate=False, stream=None):

@@ -65,6 +65,19 @@ static Context g_ctx;
 static std::mutex g_mu;
 static std::mutex g_ws_mu;
 
+// Device scratch of plg_correlator_static: one file-scope array per device,
+// held until the process ends.  Host code never passes `scratch` itself (its
+// host-side name is no device address): plg_init takes the array's device
+// address on every device it sets up (g_scratch_dev), and only that address
+// is used.  g_static_mu serialises the calls of one process; calls whose work
+// overlaps on one GPU from several processes, or replays of a captured call,
+// race on the array.
+constexpr int MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[PLG_STATIC_SCRATCH_BYTES];
+static void *g_scratch_dev[MAX_DEVICES];
+static std::string g_scratch_err[MAX_DEVICES];
+static std::mutex g_static_mu;
+
 static uint16_t host_bf16_rn(float f) {
   uint32_t u;
   std::memcpy(&u, &f, 4);
@@ -882,6 +895,88 @@ static int tune_single(Plan &P, hipStream_t st) {
   return rc;
 }
 
+// ------------------------------------------------------------------ no-allocation correlator
+// Carves 256-byte aligned pieces out of the device's static scratch.
+struct ScratchCarver {
+  unsigned char *base;
+  size_t used = 0;
+  void *take(size_t bytes) {
+    void *p = base + used;
+    used += (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
+    return p;
+  }
+};
+
+template <typename A>
+static int put_async(ScratchCarver &sc, A **dst, const std::vector<A> &v, hipStream_t st) {
+  *dst = (A *)sc.take(v.size() * sizeof(A));
+  if (!v.empty()) HIPCHK(hipMemcpyAsync(*dst, v.data(), v.size() * sizeof(A), hipMemcpyHostToDevice, st));
+  return PLG_OK;
+}
+
+// the device tables of a plan that lives in the static scratch: pointers
+// into the array, cleared before the plan is destroyed (it owns none of them)
+static void release_static(Plan &P) {
+  P.d_props = nullptr;
+  P.d_rows = nullptr;
+  P.d_orows = nullptr;
+  P.d_csel = nullptr;
+  P.d_choff = nullptr;
+  P.d_ms = P.d_pair_ms = P.d_mom_pair = nullptr;
+  P.d_pidx = nullptr;
+}
+
+static int correlator_static(Plan &P, ScratchCarver &sc, void *out, int out_precision, hipStream_t st) {
+  int rc;
+  // sizes first: nothing is copied or launched unless the whole call fits
+  size_t need = 0;
+  {
+    ScratchCarver probe{nullptr};
+    probe.take(2 * (P.T == PLG_F64 ? sizeof(PropDev<double>) : sizeof(PropDev<float>)));
+    probe.take(P.rows.size() * sizeof(RowDev));
+    probe.take(P.orows.size() * sizeof(OutRowDev));
+    probe.take(P.csel.size() * sizeof(int));
+    probe.take(P.ch_off.size() * sizeof(int64_t));
+    probe.take(P.ms.size() * sizeof(int));
+    probe.take(P.pair_ms.size() * sizeof(int));
+    probe.take(P.mom_pair.size() * sizeof(int));
+    if (P.npairs > 0 && P.T == PLG_F32) probe.take(phase_index_bytes(g_ctx.V3, P.npairs));
+    probe.take(c_bytes(P));
+    if (P.space == PLG_MOMENTUM_SPACE) probe.take(part_bytes(P));
+    need = probe.used;
+  }
+  if (need > PLG_STATIC_SCRATCH_BYTES)
+    return fail(PLG_ERR_UNSUPPORTED, "plg_correlator_static: the correlator needs " + std::to_string(need) +
+                                         " bytes of device scratch, above the static maximum of " +
+                                         std::to_string((size_t)PLG_STATIC_SCRATCH_BYTES));
+  if (P.T == PLG_F64) {
+    std::vector<PropDev<double>> pd;
+    fill_propdev(P.props, pd);
+    PropDev<double> *d = nullptr;
+    if ((rc = put_async(sc, &d, pd, st))) return rc;
+    P.d_props = d;
+  } else {
+    std::vector<PropDev<float>> pd;
+    fill_propdev(P.props, pd);
+    PropDev<float> *d = nullptr;
+    if ((rc = put_async(sc, &d, pd, st))) return rc;
+    P.d_props = d;
+  }
+  if ((rc = put_async(sc, &P.d_rows, P.rows, st)) || (rc = put_async(sc, &P.d_orows, P.orows, st)) ||
+      (rc = put_async(sc, &P.d_csel, P.csel, st)) || (rc = put_async(sc, &P.d_choff, P.ch_off, st)) ||
+      (rc = put_async(sc, &P.d_ms, P.ms, st)) || (rc = put_async(sc, &P.d_pair_ms, P.pair_ms, st)) ||
+      (rc = put_async(sc, &P.d_mom_pair, P.mom_pair, st)))
+    return rc;
+  if (P.npairs > 0 && P.T == PLG_F32) {
+    P.d_pidx = (uint16_t *)sc.take(phase_index_bytes(g_ctx.V3, P.npairs));
+    ProjGeom g = plan_geom(P);
+    if ((rc = launch_phase_index(g, P.d_pidx, st))) return fail(rc, "phase index launch failed");
+  }
+  void *c = sc.take(c_bytes(P));
+  void *part = P.space == PLG_MOMENTUM_SPACE ? sc.take(part_bytes(P)) : nullptr;
+  return enqueue(P, c, part, out, out_precision, st);
+}
+
 }  // namespace plg
 
 using namespace plg;
@@ -910,6 +1005,17 @@ int plg_init(const int L[4], int device) {
   if (device < 0 || device >= ndev) return fail(PLG_ERR_INVALID, "no such device");
   HIPCHK(hipSetDevice(device));
   if (g_ctx.init) finalize_locked();
+  // device address of the static scratch on this device (once per device)
+  if (device < MAX_DEVICES && !g_scratch_dev[device]) {
+    void *ptr = nullptr;
+    const hipError_t se = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (se == hipSuccess && ptr) {
+      g_scratch_dev[device] = ptr;
+    } else {
+      g_scratch_err[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(se);
+      (void)hipGetLastError();
+    }
+  }
   std::memcpy(g_ctx.L, L, sizeof(int) * 4);
   g_ctx.V3 = (int64_t)L[0] * L[1] * L[2];
   g_ctx.device = device;
@@ -1047,6 +1153,51 @@ int plg_correlator(int kind, int group_mask, const plg_prop *p1, const plg_prop 
   // the workspace is shared by single calls: finish this correlator before
   // the next call may use it, also after an error
   const hipError_t se = hipStreamSynchronize(st);
+  if (rc) return rc;
+  if (se != hipSuccess) return fail(PLG_ERR_HIP, std::string("correlator sync: ") + hipGetErrorString(se));
+  return PLG_OK;
+}
+
+// ------------------------------------------------------------------ no-allocation correlator
+int plg_correlator_static(int kind, int group_mask, const plg_prop *p1, const plg_prop *p2, const int source[4],
+                          int totalT, const int *moms, int nmom, int mode, int space, int out_precision, void *out,
+                          void *stream) {
+  if (!g_ctx.init) return fail(PLG_ERR_NOT_INIT, "plg_init not called");
+  if (!p1 || !out || !source) return fail(PLG_ERR_INVALID, "NULL argument");
+  if (out_precision != PLG_F64 && out_precision != PLG_F32) return fail(PLG_ERR_INVALID, "bad out precision");
+  int lt;
+  int rc = plg_window(source, totalT, nullptr, nullptr, &lt);
+  if (rc) return rc;
+  const int dev = g_ctx.device;
+  if (dev >= MAX_DEVICES) return fail(PLG_ERR_UNSUPPORTED, "device index above the static scratch table");
+  if (!g_scratch_dev[dev])
+    return fail(PLG_ERR_HIP, g_scratch_err[dev].empty() ? "static scratch unavailable" : g_scratch_err[dev]);
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<plg_prop> props = {*p1, p2 ? *p2 : *p1};
+  if (!same_view(props[0], props[1]) && (kind == PLG_TWOPT))
+    return fail(PLG_ERR_INVALID, "twopt takes one propagator");
+  std::lock_guard<std::mutex> sl(g_static_mu);
+  Plan P;
+  std::vector<JobIn> jobs = {{{source[0], source[1], source[2], source[3]}, totalT, 0, 1, 0}};
+  if ((rc = build_plan(P, kind, group_mask, props, jobs, moms, nmom, mode, PLG_SPLIT_BF16X6, space))) return rc;
+  choose_tiles(P);
+  // tiles of an earlier tuning run of this geometry, if any (no tuning here:
+  // it needs buffers of its own)
+  if (!std::getenv("PLG_PROJ_CODE") && space == PLG_MOMENTUM_SPACE) {
+    int a, b, c;
+    if (tune_lookup(tune_key(P), a, b, c)) {
+      if (P.T == PLG_F64) P.mtile = a;
+      else P.mbw = a;
+      P.nsplit = b;
+      P.tpb = c;
+    }
+  }
+  ScratchCarver sc{(unsigned char *)g_scratch_dev[dev]};
+  rc = correlator_static(P, sc, out, out_precision, st);
+  // the host tables of the copies and the scratch are this call's until its
+  // work has finished, also after an error
+  const hipError_t se = hipStreamSynchronize(st);
+  release_static(P);
   if (rc) return rc;
   if (se != hipSuccess) return fail(PLG_ERR_HIP, std::string("correlator sync: ") + hipGetErrorString(se));
   return PLG_OK;
