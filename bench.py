@@ -777,8 +777,13 @@ def run_tetra(args, torch, capi, dev, stream):
         mixed()
     ms_mx, _ = timed(torch, stream, mixed, args.steps)
     log(f"mixed: {ms_mx:.3f} ms/step")
-    # implemented complex MACs per site: 19 per direct term, 63 per crossed, 1 per coefficient
-    cmac = 19 * ndirect + 63 * (len(idxs) - ndirect) + len(idxs)
+    # implemented complex MACs per site: the spin-pair tables (256 entries of
+    # <B1,B3>, B3^dag B1, <B2,B4>, B4^dag B2: 2 x 256 x 36), then 1 per direct
+    # term, 9 per crossed term and 1 per coefficient
+    cmac = 2 * 256 * 36 + ndirect + 9 * (len(idxs) - ndirect) + len(idxs)
+    # the colour sums of the table terms as the reference writes them: 81
+    # products of four propagator entries (3 complex multiplications) per term
+    cmac_ref = 81 * 3 * len(idxs)
     res = {"metric": "tetraquark table 2pt sites/s (16 operators x 768 terms), mixed precision",
            "value": nsites / (ms_mx / 1e3), "unit": "sites/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms_mx, "higher_is_better": True, "scaling": "weak",
@@ -789,9 +794,10 @@ def run_tetra(args, torch, capi, dev, stream):
                                   "slices, |n|^2<=9 (123 momenta)",
                       "lattice": L, "sites": nsites, "operators": nops, "terms": int(len(idxs)),
                       "nmom": len(moms), "parallelism": "single GPU"},
-           "contraction_valu": {"flops_per_site": 8 * cmac,
+           "contraction_valu": {"flops_per_site": 8 * cmac, "reference_body_flops_per_site": 8 * cmac_ref,
                                 "achieved_upper_bound_TFLOPs": nsites * 8 * cmac / (ms_mx / 1e3) / 1e12,
-                                "note": "whole step (contraction + projection) in the denominator"}}
+                                "note": "whole step (contraction + projection) in the denominator; the kernel "
+                                        "reads the spin-pair tables from LDS (about 1 MB per site), which bounds it"}}
     if not args.no_fp64:
         plain()
         ms_64, _ = timed(torch, stream, plain, max(1, args.steps // 2))

@@ -1431,12 +1431,14 @@ static int tetra_run(const plg_prop *const views[4], const int16_t *idxs, const 
   for (int i = 0; i < 4; i++)
     if (!views[i]) return fail(PLG_ERR_INVALID, "NULL propagator view");
   std::vector<TetraTerm> terms;
-  std::vector<int> first, count;
-  rc = tetra_pack(idxs, col_contr, vals, counts, nops, terms, first, count);
+  std::vector<float> coef;
+  std::vector<int> first, count, ndirect;
+  rc = tetra_pack(idxs, col_contr, vals, counts, nops, terms, coef, first, count, ndirect);
   if (rc == PLG_ERR_INVALID) return fail(rc, "spin or colour index outside 0..3");
   if (rc)
     return fail(rc, "table outside the supported form (32 operators; colour slots a permutation of the four "
-                    "sums, closing as <P1,P3><P2,P4> or Tr[(P3^dag P1)(P4^dag P2)])");
+                    "sums, closing as <P1,P3><P2,P4> or Tr[(P3^dag P1)(P4^dag P2)]; 65536 distinct "
+                    "coefficients)");
   int lt;
   if ((rc = plg_window(source, totalT, nullptr, nullptr, &lt))) return rc;
   const Window w = window_of(g_ctx.L, source, totalT);
@@ -1486,13 +1488,21 @@ static int tetra_run(const plg_prop *const views[4], const int16_t *idxs, const 
   P.nsites = (int64_t)lt * g_ctx.V3;
   P.launches.push_back({Launch::TETRA, 0, false, false, 0, 0});
   TetraTerm *d_terms = nullptr;
+  float *d_coef = nullptr;
   if ((rc = upload(&d_terms, terms, hs))) return rc;
+  if ((rc = upload(&d_coef, coef, hs))) {
+    hipFree(d_terms);
+    return rc;
+  }
+  P.tetra.ncoef = (int)coef.size() / 2;
   P.tetra.nops = nops;
   for (int k = 0; k < nops; k++) {
     P.tetra.first[k] = first[k];
     P.tetra.count[k] = count[k];
+    P.tetra.ndirect[k] = ndirect[k];
   }
   P.tetra.terms = d_terms;
+  P.tetra.coef = d_coef;
   choose_tiles(P);
   P.tpb = 64;  // sites per workgroup of the kernel
   if (!(rc = upload_plan(P, hs)) && !(rc = upload(&P.d_rows4, P.rows4, hs))) {
@@ -1508,6 +1518,7 @@ static int tetra_run(const plg_prop *const views[4], const int16_t *idxs, const 
   }
   hipStreamSynchronize(hs);  // no launch may still read the terms
   hipFree(d_terms);
+  hipFree(d_coef);
   return rc;
 }
 

@@ -102,22 +102,28 @@ int udsc_schedule_terms(const UdscTerm *terms, const int *first, const int *coun
 
 // four-propagator table contractions (tetraquark table format)
 constexpr int TETRA_MAXOPS = 32;
-struct TetraTerm {
-  uint32_t spins;  // 4 bits per block: P1 (i0 i1), P2 (i2 i3), P3 (i5+2, i4+2), P4 (i7+2, i6+2)
-  int kind;        // 0 direct <B1,B3><B2,B4>, 1 crossed Tr[(B3^dag B1)(B4^dag B2)]
-  float re, im;
-};
+// A term, 32 bits: the (P1, P3) spin pair p = 16 s1 + s3 (bits 0-7), the
+// (P2, P4) pair q = 16 s2 + s4 (bits 8-15) and the coefficient index (bits
+// 16-31), with the 4-bit spin pairs s1 = 4 i0 + i1, s2 = 4 i2 + i3,
+// s3 = 4 (i5+2) + (i4+2), s4 = 4 (i7+2) + (i6+2) (indices mod 4).  An
+// operator's terms are its direct ones (<P1,P3><P2,P4>) first, then its
+// crossed ones (Tr[(P3^dag P1)(P4^dag P2)]).
+typedef uint32_t TetraTerm;
+constexpr int TETRA_MAXCOEF = 65536;
 struct TetraSched {
   int nops;
-  int first[TETRA_MAXOPS], count[TETRA_MAXOPS];
+  int first[TETRA_MAXOPS], count[TETRA_MAXOPS], ndirect[TETRA_MAXOPS];
   const TetraTerm *terms;
+  const float *coef;  // re, im of each distinct coefficient
+  int ncoef;
 };
 struct RowDev4 {
   int t;
   int p[4];
 };
 int tetra_pack(const int16_t *idxs, const int16_t *cols, const float *vals, const int16_t *counts, int nops,
-               std::vector<TetraTerm> &terms, std::vector<int> &first, std::vector<int> &count);
+               std::vector<TetraTerm> &terms, std::vector<float> &coef, std::vector<int> &first,
+               std::vector<int> &count, std::vector<int> &ndirect);
 template <typename T>
 int launch_tetra(const TetraSched &S, const PropDev<T> *props, const RowDev4 *rows, int nrows, cx<T> *out,
                  int64_t V3, int T_lat, int nch_total, int spw, hipStream_t st);

@@ -128,9 +128,9 @@ __device__ __forceinline__ void split3(float a, float b, uint32_t *w) {
 // phases of n != 0 sum to zero over the volume); a value near the column's
 // mean keeps the FP32 accumulators from carrying its coherent part, which
 // otherwise leaves an error of order 2^-24 |C(0)| at momenta with some
-// n_k = 0 (a large mean: the pion's sum of |S|^2).  From 256 sites spread
-// through the volume, per real part: mean m and spread s in FP64; a = 0
-// unless |m| > s / 4 (four standard errors), else m rounded to 8
+// n_k = 0 (a large mean: the pion's sum of |S|^2).  From 256 sites (16 site
+// tiles spread through the volume), per real part: mean m and spread s in
+// FP64; a = 0 unless |m| > s / 4 (four standard errors), else m rounded to 8
 // significant bits, so that c - a is exact in FP32 for every value within
 // 2^16 |a| of a (a rounding of c - a that depends on a's low bits would be
 // the same at every site, a coherent error of V3 ulp).
@@ -140,39 +140,49 @@ __device__ __forceinline__ float col_shift_part(double m, double s2) {
   u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
   return __uint_as_float(u);
 }
-// 256 threads per 16 columns: thread (sample lane l, column j) sums the
-// samples l, l + 16, ...; the 16 lanes of a column are combined in LDS.
-constexpr int CS_SAMPLES = 256, CS_COLS = 16, CS_LANES = 16;
+// 256 threads per 16 columns: thread (column j, site s of a tile) reads site
+// s of 16 site tiles spread through the volume (the 16 columns of one tile
+// are 2 KB contiguous, read by the workgroup at once); the 16 sites of a
+// column are combined by shuffles within the wave.
+constexpr int CS_TILES = 16, CS_COLS = 16;
 __global__ __launch_bounds__(256) void k_col_shift(const cx<float> *__restrict__ c, int64_t ncols, int64_t V3,
                                                    cx<float> *__restrict__ a) {
-  __shared__ double red[4][CS_LANES][CS_COLS];
-  const int j = threadIdx.x % CS_COLS, l = threadIdx.x / CS_COLS;
+  const int s = threadIdx.x & 15, j = threadIdx.x >> 4;
   const int64_t n = (int64_t)blockIdx.x * CS_COLS + j;
-  const int64_t ns = V3 < CS_SAMPLES ? V3 : CS_SAMPLES;
+  const int64_t ntiles = V3 / 16;  // whole tiles only
   double re = 0, im = 0, re2 = 0, im2 = 0;
-  if (n < ncols)
-    for (int64_t i = l; i < ns; i += CS_LANES) {
-      const cx<float> v = c[ctile_idx(n, i * V3 / ns, ncols)];
-      re += v.re;
-      im += v.im;
-      re2 += (double)v.re * v.re;
-      im2 += (double)v.im * v.im;
+  int64_t cnt = 0;
+  if (n < ncols) {
+    if (ntiles > 0) {
+      const int nt = ntiles < CS_TILES ? (int)ntiles : CS_TILES;
+      for (int i = 0; i < nt; i++) {
+        const cx<float> v = c[ctile_idx(n, (int64_t)i * ntiles / nt * 16 + s, ncols)];
+        re += v.re;
+        im += v.im;
+        re2 += (double)v.re * v.re;
+        im2 += (double)v.im * v.im;
+      }
+      cnt = (int64_t)nt * 16;
+    } else if (s < V3) {  // fewer than 16 sites
+      const cx<float> v = c[ctile_idx(n, s, ncols)];
+      re = v.re;
+      im = v.im;
+      re2 = (double)v.re * v.re;
+      im2 = (double)v.im * v.im;
+      cnt = V3;
     }
-  red[0][l][j] = re;
-  red[1][l][j] = im;
-  red[2][l][j] = re2;
-  red[3][l][j] = im2;
-  __syncthreads();
-  if (l == 0 && n < ncols) {
-    for (int k = 1; k < CS_LANES; k++) {
-      re += red[0][k][j];
-      im += red[1][k][j];
-      re2 += red[2][k][j];
-      im2 += red[3][k][j];
-    }
-    const double mr = re / (double)ns, mi = im / (double)ns;
-    a[n] = cmake<float>(col_shift_part(mr, re2 / (double)ns - mr * mr),
-                        col_shift_part(mi, im2 / (double)ns - mi * mi));
+  }
+#pragma unroll
+  for (int off = 8; off >= 1; off >>= 1) {  // the 16 lanes of a column
+    re += __shfl_xor(re, off, 16);
+    im += __shfl_xor(im, off, 16);
+    re2 += __shfl_xor(re2, off, 16);
+    im2 += __shfl_xor(im2, off, 16);
+  }
+  if (s == 0 && n < ncols) {
+    const double ns = (double)cnt;
+    const double mr = re / ns, mi = im / ns;
+    a[n] = cmake<float>(col_shift_part(mr, re2 / ns - mr * mr), col_shift_part(mi, im2 / ns - mi * mi));
   }
 }
 

@@ -11,39 +11,45 @@
 // colour traces of the 3x3 blocks B1 = P1[i0][i1], ..., B4 = P4[i7+2][i6+2]:
 //   direct  (c0 = c5, c1 = c4, c2 = c7, c3 = c6):  <B1, B3> <B2, B4>,   <X, Y> = sum X conj(Y)
 //   crossed (c0 = c5, c1 = c6, c2 = c7, c3 = c4):  Tr[(B3^dag B1)(B4^dag B2)]
-// (19 and 63 complex products instead of 243).  The host classifies every
-// term; a table with another structure is refused.
+// The host classifies every term; a table with another structure is refused.
 //
-// A workgroup of 4 waves stages 128-byte lines of the four propagators (16
-// FP32 or 8 FP64 consecutive sites) in LDS; each wave contracts a site at a
-// time, its lanes splitting an operator's terms, then a wave reduction.
+// Both structures factor through the spin pairs p = (B1, B3) and
+// q = (B2, B4), 256 of each per site: per site the workgroup builds the
+// tables <B1,B3>, B3^dag B1 (over p) and <B2,B4>, B4^dag B2 (over q), 36
+// complex products per entry, and then a direct term costs one complex
+// product of two table entries and a crossed term the trace of a product of
+// two (9 products), instead of 19 and 63 from the blocks.  The terms of the
+// reference's tables use every p and q, about 50 terms per entry.
+//
+// A workgroup of 4 waves stages 64-byte lines of the four propagators (8
+// FP32 or 4 FP64 consecutive sites) in LDS; per site, each thread builds one
+// p entry and one q entry of the tables, then each wave takes every fourth
+// operator, its lanes splitting the operator's terms, and a wave reduction.
 #include "plg_internal.h"
 
 #include <algorithm>
+#include <cstring>
+#include <map>
 
 namespace plg {
 
-template <typename T> struct TetraGeom {
-  static constexpr int G = 128 / (int)sizeof(cx<T>);  // sites per line
-  static constexpr int SPAD = 577;                    // LDS stride of a site (576 + 1)
-};
+constexpr int TETRA_TB = 12;      // terms per lane loaded together
+constexpr int TETRA_LDS_COEF = 256;  // coefficients kept in LDS
 
-// block pointers of a term inside a staged site (each 9 complex, [a][b])
-struct TetraBlocks {
-  int b1, b2, b3, b4;
+template <typename T> struct TetraGeom {
+  static constexpr int G = 64 / (int)sizeof(cx<T>);  // sites per staged line
+  static constexpr int SPAD = 577;                   // LDS stride of a site (576 + 1)
 };
-__device__ __forceinline__ TetraBlocks tetra_blocks(uint32_t sp) {
-  // sp: 4 bits per propagator: P1 i0 i1, P2 i2 i3, P3 (i5+2, i4+2), P4 (i7+2, i6+2)
-  return TetraBlocks{(int)(sp & 15) * 9, 144 + (int)((sp >> 4) & 15) * 9, 288 + (int)((sp >> 8) & 15) * 9,
-                     432 + (int)((sp >> 12) & 15) * 9};
-}
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_tetra(TetraSched S, const PropDev<T> *__restrict__ props,
                                                const RowDev4 *__restrict__ rows, cx<T> *__restrict__ out,
                                                int64_t V3, int T_lat, int nch_total, int groups_per_block) {
   constexpr int G = TetraGeom<T>::G, SPAD = TetraGeom<T>::SPAD;
-  __shared__ cx<T> sp[G * SPAD];  // [site][f][m01][a b]
+  __shared__ cx<T> sp[G * SPAD];                 // [site][f][m01][a b]
+  __shared__ cx<T> d13[256], d24[256];           // <B1,B3> over p, <B2,B4> over q
+  __shared__ cx<T> m13[256 * 9], n24[256 * 9];   // (B3^dag B1)[z][y] over p, (B4^dag B2)[y][z] over q, at z*3+y
+  __shared__ cx<T> scoef[TETRA_LDS_COEF];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const RowDev4 row = rows[blockIdx.y];
   const PropDev<T> P0 = props[row.p[0]], P1 = props[row.p[1]], P2 = props[row.p[2]], P3 = props[row.p[3]];
@@ -56,6 +62,10 @@ __global__ __launch_bounds__(256) void k_tetra(TetraSched S, const PropDev<T> *_
   const T sg = prop_sign(P0, row.t) * prop_sign(P1, row.t) * prop_sign(P2, row.t) * prop_sign(P3, row.t);
   const int rot_any = P0.rot | P1.rot | P2.rot | P3.rot;
   const int64_t ncols = (int64_t)gridDim.y * nch_total, col0 = (int64_t)blockIdx.y * nch_total;
+  // the coefficients in LDS when they fit (the reference's tables have six)
+  const bool coef_lds = S.ncoef <= TETRA_LDS_COEF;
+  if (coef_lds)
+    for (int i = tid; i < S.ncoef; i += 256) scoef[i] = cmake<T>((T)S.coef[2 * i], (T)S.coef[2 * i + 1]);
 
   for (int gi = 0; gi < groups_per_block; gi++) {
     const int64_t x0 = ((int64_t)blockIdx.x * groups_per_block + gi) * G;
@@ -86,56 +96,116 @@ __global__ __launch_bounds__(256) void k_tetra(TetraSched S, const PropDev<T> *_
       }
       __syncthreads();
     }
-    for (int s = wave; s < G; s += 4) {
-      const cx<T> *spx = sp + s * SPAD;
+    for (int s = 0; s < G; s++) {
       const int64_t x = x0 + s;
-      for (int k = 0; k < S.nops; k++) {
-        cx<T> acc = cmake<T>(0, 0);
-        const int n = S.count[k];
-        const TetraTerm *E = S.terms + S.first[k];
-        for (int t = lane; t < n; t += 64) {
-          const TetraTerm u = E[t];
-          const TetraBlocks bl = tetra_blocks(u.spins);
-          const cx<T> *B1 = spx + bl.b1, *B2 = spx + bl.b2, *B3 = spx + bl.b3, *B4 = spx + bl.b4;
-          cx<T> v;
-          if (u.kind == 0) {  // direct: <B1, B3> <B2, B4>
-            cx<T> x13 = cmake<T>(0, 0), x24 = cmake<T>(0, 0);
+      if (x >= V3) break;  // uniform
+      const cx<T> *spx = sp + s * SPAD;
+      // tables of the site: thread tid = 16 hi + lo builds entry tid of both
+      {
+        const int hi = tid >> 4, lo = tid & 15;
+        cx<T> a[9], b[9];
 #pragma unroll
-            for (int j = 0; j < 9; j++) {
-              cfma_conj(x13, B1[j], B3[j]);
-              cfma_conj(x24, B2[j], B4[j]);
-            }
-            v = x13 * x24;
-          } else {  // crossed: Tr[(B3^dag B1)(B4^dag B2)]
-            cx<T> m[9], nn[9];
-#pragma unroll
-            for (int z = 0; z < 3; z++)
-#pragma unroll
-              for (int y = 0; y < 3; y++) {
-                cx<T> a = cmake<T>(0, 0), b = cmake<T>(0, 0);
-#pragma unroll
-                for (int w = 0; w < 3; w++) {
-                  cfma_conj(a, B1[w * 3 + y], B3[w * 3 + z]);  // (B3^dag B1)[z][y]
-                  cfma_conj(b, B2[w * 3 + z], B4[w * 3 + y]);  // (B4^dag B2)[y][z]
-                }
-                m[z * 3 + y] = a;
-                nn[y * 3 + z] = b;
-              }
-            v = cmake<T>(0, 0);
-#pragma unroll
-            for (int z = 0; z < 3; z++)
-#pragma unroll
-              for (int y = 0; y < 3; y++) cfma(v, m[z * 3 + y], nn[y * 3 + z]);
-          }
-          cfma(acc, cmake<T>((T)u.re, (T)u.im), v);
+        for (int j = 0; j < 9; j++) {
+          a[j] = spx[hi * 9 + j];          // B1 = P1 block s1 = hi
+          b[j] = spx[288 + lo * 9 + j];    // B3 = P3 block s3 = lo
         }
+        cx<T> d = cmake<T>(0, 0);
+#pragma unroll
+        for (int j = 0; j < 9; j++) cfma_conj(d, a[j], b[j]);
+        d13[tid] = d;
+#pragma unroll
+        for (int z = 0; z < 3; z++)
+#pragma unroll
+          for (int y = 0; y < 3; y++) {
+            cx<T> m = cmake<T>(0, 0);
+#pragma unroll
+            for (int w = 0; w < 3; w++) cfma_conj(m, a[w * 3 + y], b[w * 3 + z]);
+            m13[tid * 9 + z * 3 + y] = m;
+          }
+#pragma unroll
+        for (int j = 0; j < 9; j++) {
+          a[j] = spx[144 + hi * 9 + j];    // B2 = P2 block s2 = hi
+          b[j] = spx[432 + lo * 9 + j];    // B4 = P4 block s4 = lo
+        }
+        d = cmake<T>(0, 0);
+#pragma unroll
+        for (int j = 0; j < 9; j++) cfma_conj(d, a[j], b[j]);
+        d24[tid] = d;
+#pragma unroll
+        for (int z = 0; z < 3; z++)
+#pragma unroll
+          for (int y = 0; y < 3; y++) {
+            cx<T> m = cmake<T>(0, 0);
+#pragma unroll
+            for (int w = 0; w < 3; w++) cfma_conj(m, a[w * 3 + z], b[w * 3 + y]);
+            n24[tid * 9 + z * 3 + y] = m;
+          }
+      }
+      __syncthreads();
+      // one term of the tables: direct <B1,B3><B2,B4> or crossed trace
+      auto term = [&](uint32_t u, bool crossed) {
+        const int p = u & 255, q = (u >> 8) & 255, ci = (int)(u >> 16);
+        cx<T> v;
+        if (!crossed) {
+          v = d13[p] * d24[q];
+        } else {
+          v = cmake<T>(0, 0);
+#pragma unroll
+          for (int j = 0; j < 9; j++) cfma(v, m13[p * 9 + j], n24[q * 9 + j]);
+        }
+        const cx<T> c = coef_lds ? scoef[ci] : cmake<T>((T)S.coef[2 * ci], (T)S.coef[2 * ci + 1]);
+        return c * v;
+      };
+      auto emit = [&](int k, cx<T> acc) {
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
           acc.re += __shfl_xor(acc.re, off);
           acc.im += __shfl_xor(acc.im, off);
         }
-        if (lane == 0 && x < V3) gstore(out + ctile_idx(col0 + k, x, ncols), cmake<T>(sg * acc.re, sg * acc.im));
+        if (lane == 0) gstore(out + ctile_idx(col0 + k, x, ncols), cmake<T>(sg * acc.re, sg * acc.im));
+      };
+      // wave w takes the operators k = w, w + 4, ...; its lanes split an
+      // operator's terms, TETRA_TB terms per lane loaded together, the next
+      // batch (of this operator or the wave's next one) in flight while one
+      // is summed
+      int k = wave, base = 0;
+      uint32_t u[TETRA_TB];
+      auto fetch = [&](int kk, int b, uint32_t *dst) {
+        const int n = kk < S.nops ? S.count[kk] : 0;
+        const TetraTerm *E = S.terms + (kk < S.nops ? S.first[kk] : 0);
+#pragma unroll
+        for (int i = 0; i < TETRA_TB; i++) {
+          const int t = b + lane + 64 * i;
+          dst[i] = t < n ? E[t] : 0u;
+        }
+      };
+      fetch(k, 0, u);
+      cx<T> acc = cmake<T>(0, 0);
+      while (k < S.nops) {
+        const int n = S.count[k], nd = S.ndirect[k];
+        // the batch after this one
+        int k2 = k, b2 = base + 64 * TETRA_TB;
+        if (b2 >= n) {
+          k2 = k + 4;
+          b2 = 0;
+        }
+        uint32_t v[TETRA_TB];
+        fetch(k2, b2, v);
+#pragma unroll
+        for (int i = 0; i < TETRA_TB; i++) {
+          const int t = base + lane + 64 * i;
+          if (t < n) acc = acc + term(u[i], t >= nd);
+        }
+        if (k2 != k) {
+          emit(k, acc);
+          acc = cmake<T>(0, 0);
+        }
+        k = k2;
+        base = b2;
+#pragma unroll
+        for (int i = 0; i < TETRA_TB; i++) u[i] = v[i];
       }
+      __syncthreads();  // the tables may be rebuilt for the next site
     }
   }
 }
@@ -158,11 +228,18 @@ template int launch_tetra<float>(const TetraSched &, const PropDev<float> *, con
 
 // host: classify and pack the terms of a table (reference format)
 int tetra_pack(const int16_t *idxs, const int16_t *cols, const float *vals, const int16_t *counts, int nops,
-               std::vector<TetraTerm> &terms, std::vector<int> &first, std::vector<int> &count) {
+               std::vector<TetraTerm> &terms, std::vector<float> &coef, std::vector<int> &first,
+               std::vector<int> &count, std::vector<int> &ndirect) {
   if (nops <= 0 || nops > TETRA_MAXOPS) return PLG_ERR_UNSUPPORTED;
   terms.clear();
+  coef.clear();
   first.assign(nops, 0);
   count.assign(nops, 0);
+  ndirect.assign(nops, 0);
+  std::map<std::pair<uint32_t, uint32_t>, uint32_t> cidx;  // coefficient bits -> index
+  cidx.emplace(std::make_pair(0u, 0u), 0u);                 // entry 0: zero
+  coef.push_back(0.0f);
+  coef.push_back(0.0f);
   size_t t = 0;
   for (int k = 0; k < nops; k++) {
     if (counts[k] < 0) return PLG_ERR_INVALID;
@@ -176,25 +253,30 @@ int tetra_pack(const int16_t *idxs, const int16_t *cols, const float *vals, cons
       for (int a = 0; a < 4; a++)
         for (int b = a + 1; b < 4; b++) perm &= c[a] != c[b];
       if (!perm) return PLG_ERR_UNSUPPORTED;
-      TetraTerm u;
-      u.spins = (uint32_t)((m[0] * 4 + m[1]) | ((m[2] * 4 + m[3]) << 4) |
-                           ((((m[5] + 2) & 3) * 4 + ((m[4] + 2) & 3)) << 8) |
-                           ((((m[7] + 2) & 3) * 4 + ((m[6] + 2) & 3)) << 12));
-      u.re = vals[2 * t];
-      u.im = vals[2 * t + 1];
-      if (c[0] == c[5] && c[1] == c[4] && c[2] == c[7] && c[3] == c[6]) {
-        u.kind = 0;
-        direct.push_back(u);
-      } else if (c[0] == c[5] && c[1] == c[6] && c[2] == c[7] && c[3] == c[4]) {
-        u.kind = 1;
-        crossed.push_back(u);
+      const uint32_t s1 = m[0] * 4 + m[1], s2 = m[2] * 4 + m[3];
+      const uint32_t s3 = ((m[5] + 2) & 3) * 4 + ((m[4] + 2) & 3), s4 = ((m[7] + 2) & 3) * 4 + ((m[6] + 2) & 3);
+      uint32_t re, im;
+      std::memcpy(&re, vals + 2 * t, 4);
+      std::memcpy(&im, vals + 2 * t + 1, 4);
+      auto it = cidx.find({re, im});
+      uint32_t ci;
+      if (it != cidx.end()) {
+        ci = it->second;
       } else {
-        return PLG_ERR_UNSUPPORTED;
+        ci = (uint32_t)cidx.size();
+        if (ci >= (uint32_t)TETRA_MAXCOEF) return PLG_ERR_UNSUPPORTED;
+        cidx.emplace(std::make_pair(re, im), ci);
+        coef.push_back(vals[2 * t]);
+        coef.push_back(vals[2 * t + 1]);
       }
+      const TetraTerm u = (s1 * 16 + s3) | ((s2 * 16 + s4) << 8) | (ci << 16);
+      if (c[0] == c[5] && c[1] == c[4] && c[2] == c[7] && c[3] == c[6]) direct.push_back(u);
+      else if (c[0] == c[5] && c[1] == c[6] && c[2] == c[7] && c[3] == c[4]) crossed.push_back(u);
+      else return PLG_ERR_UNSUPPORTED;
     }
-    // one colour structure after the other: lanes of a wave take the same branch
     terms.insert(terms.end(), direct.begin(), direct.end());
     terms.insert(terms.end(), crossed.begin(), crossed.end());
+    ndirect[k] = (int)direct.size();
     count[k] = (int)terms.size() - first[k];
   }
   return PLG_OK;
