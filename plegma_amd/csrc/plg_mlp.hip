@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -44,6 +45,10 @@ struct GemmArgs {
   uint16_t *Cb;
   const float *bias;
   int silu;  // Cb = bf16(silu(C)) when set, else bf16(C)
+  // split K: ks > 1 workgroup layers (blockIdx.z) sum K ranges of kper into
+  // Cs[z][m][n]; k_gemm_reduce adds them and applies the epilogue
+  int ks, kper;
+  float *Cs;
 };
 
 __device__ __forceinline__ float silu(float z) { return z / (1.0f + expf(-z)); }
@@ -60,14 +65,15 @@ __global__ __launch_bounds__(MG_THREADS) void k_gemm_bf16(GemmArgs g) {
   // and B[k0 + 8(l/16) + i][n0 + l%16], i < 8
   const int am = m0 + (lane & 15), bn = n0 + (lane & 15);
   const int kq = 8 * (lane >> 4);
+  const int kb = blockIdx.z * g.kper, ke = min(g.K, kb + g.kper);
   f4 acc = {0, 0, 0, 0};
-  for (int k0 = 0; k0 < g.K; k0 += 32) {
+  for (int k0 = kb; k0 < ke; k0 += 32) {
     u16x8 a, b;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
       const int k = k0 + kq + i;
-      a[i] = (am < g.M && k < g.K) ? g.A[am * g.sam + k * g.sak] : (uint16_t)0;
-      b[i] = (bn < g.N && k < g.K) ? g.B[k * g.sbk + bn * g.sbn] : (uint16_t)0;
+      a[i] = (am < g.M && k < ke) ? g.A[am * g.sam + k * g.sak] : (uint16_t)0;
+      b[i] = (bn < g.N && k < ke) ? g.B[k * g.sbk + bn * g.sbn] : (uint16_t)0;
     }
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), acc,
                                                  0, 0, 0);
@@ -76,12 +82,28 @@ __global__ __launch_bounds__(MG_THREADS) void k_gemm_bf16(GemmArgs g) {
 #pragma unroll
   for (int v = 0; v < 4; v++) {
     const int m = m0 + 4 * (lane >> 4) + v, n = n0 + (lane & 15);
+    if (g.ks > 1) {
+      if (m < g.M && n < g.N) g.Cs[((int64_t)blockIdx.z * g.M + m) * g.N + n] = acc[v];
+      continue;
+    }
     if (m < g.M && n < g.N) {
       float z = acc[v] + (g.bias ? g.bias[n] : 0.0f);
       if (g.C) g.C[m * g.ldc + n] = z;
       if (g.Cb) g.Cb[m * g.ldc + n] = bf16_rne(g.silu ? silu(z) : z);
     }
   }
+}
+
+// the split-K partials summed in order of z, then the GEMM's epilogue
+__global__ void k_gemm_reduce(GemmArgs g) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)g.M * g.N) return;
+  const int m = (int)(e / g.N), n = (int)(e % g.N);
+  float z = 0;
+  for (int k = 0; k < g.ks; k++) z += g.Cs[(int64_t)k * g.M * g.N + e];
+  z += g.bias ? g.bias[n] : 0.0f;
+  if (g.C) g.C[m * g.ldc + n] = z;
+  if (g.Cb) g.Cb[m * g.ldc + n] = bf16_rne(g.silu ? silu(z) : z);
 }
 
 __global__ void k_to_bf16(const float *__restrict__ x, uint16_t *__restrict__ y, int64_t n) {
@@ -122,21 +144,40 @@ __global__ void k_silu_back(const float *__restrict__ dx, const float *__restric
   }
 }
 
-// db[n] = sum_b dZ[b][n]
-__global__ void k_bias_grad(const float *__restrict__ dz, float *__restrict__ db, int B, int N) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+// db[n] = sum_b dZ[b][n]: 256 threads per 16 columns, thread (row lane r,
+// column j) sums the rows r, r + 16, ...; the 16 row lanes of a column are
+// added in LDS in a fixed order
+__global__ __launch_bounds__(256) void k_bias_grad(const float *__restrict__ dz, float *__restrict__ db, int B,
+                                                   int N) {
+  __shared__ float red[16][17];
+  const int j = threadIdx.x & 15, r = threadIdx.x >> 4;
+  const int n = blockIdx.x * 16 + j;
   float s = 0;
-  for (int b = 0; b < B; b++) s += dz[(int64_t)b * N + n];
-  db[n] = s;
+  if (n < N)
+    for (int b = r; b < B; b += 16) s += dz[(int64_t)b * N + n];
+  red[r][j] = s;
+  __syncthreads();
+  if (r == 0 && n < N) {
+    float t = 0;
+    for (int k = 0; k < 16; k++) t += red[k][j];
+    db[n] = t;
+  }
 }
 
+// the Adam step counter and its bias corrections 1 / (1 - b^step), on the
+// device so that a captured training step replays unchanged
+__global__ void k_adam_step(int *__restrict__ step, float *__restrict__ coef, float b1, float b2) {
+  const int s = ++step[0];
+  coef[0] = 1.0f / (1.0f - powf(b1, (float)s));
+  coef[1] = 1.0f / (1.0f - powf(b2, (float)s));
+}
 // Adam (Kingma & Ba 2015) on FP32 master weights; wb: bf16 copy (NULL for biases)
 __global__ void k_adam(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v,
                        const float *__restrict__ g, uint16_t *__restrict__ wb, int64_t n, float lr, float b1,
-                       float b2, float eps, float c1, float c2) {
+                       float b2, float eps, const float *__restrict__ coef) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const float c1 = coef[0], c2 = coef[1];
   const float gi = g[i];
   const float mi = b1 * m[i] + (1.0f - b1) * gi;
   const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
@@ -167,8 +208,21 @@ struct plg_mlp_s {
   float *dz = nullptr, *dx = nullptr, *part = nullptr;
   uint16_t *dzb = nullptr;
   int64_t step = 0;
+  float *ksw = nullptr;     // split-K partials of the products
+  int *d_step = nullptr;    // Adam step on the device
+  float *d_coef = nullptr;  // its bias corrections
+  // one training step captured as a graph, for the arguments it was captured with
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const float *gx = nullptr, *gt = nullptr;
+  int gB = 0;
+  float glr = 0;
   std::vector<void *> allocs;
   ~plg_mlp_s() {
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    if (cap) hipStreamDestroy(cap);
     for (void *q : allocs) hipFree(q);
   }
 };
@@ -179,9 +233,22 @@ static int mlp_alloc(plg_mlp_s *net, void **dst, size_t bytes) {
   return PLG_OK;
 }
 
-static void gemm(const GemmArgs &a, hipStream_t st) {
-  dim3 grid((unsigned)((a.N + 31) / 32), (unsigned)((a.M + 31) / 32));
+// K split for long, narrow products (the weight gradients sum over the
+// batch): up to MG_KSPLIT ranges of at least 128, when the tile grid is small
+constexpr int MG_KSPLIT = 16;
+static int gemm_ksplit(int M, int N, int K) {
+  const int64_t tiles = (int64_t)((N + 31) / 32) * ((M + 31) / 32);
+  if (K < 512 || tiles >= 512) return 1;
+  return std::max(1, std::min(MG_KSPLIT, K / 128));
+}
+static void gemm(GemmArgs a, float *ws, hipStream_t st) {
+  a.ks = ws ? gemm_ksplit(a.M, a.N, a.K) : 1;
+  a.kper = a.ks > 1 ? ((a.K + a.ks - 1) / a.ks + 31) / 32 * 32 : a.K;
+  a.ks = a.ks > 1 ? (a.K + a.kper - 1) / a.kper : 1;
+  a.Cs = ws;
+  dim3 grid((unsigned)((a.N + 31) / 32), (unsigned)((a.M + 31) / 32), (unsigned)a.ks);
   k_gemm_bf16<<<grid, MG_THREADS, 0, st>>>(a);
+  if (a.ks > 1) k_gemm_reduce<<<blocks((int64_t)a.M * a.N), 256, 0, st>>>(a);
 }
 
 static int mlp_forward(plg_mlp_s *net, const float *x, int B, hipStream_t st) {
@@ -198,7 +265,7 @@ static int mlp_forward(plg_mlp_s *net, const float *x, int B, hipStream_t st) {
     a.Cb = last ? nullptr : net->X[l + 1];
     a.bias = net->p + net->b_off[l];
     a.silu = 1;
-    gemm(a, st);
+    gemm(a, net->ksw, st);
   }
   return hipGetLastError() == hipSuccess ? PLG_OK : fail(PLG_ERR_HIP, "MLP forward launch failed");
 }
@@ -217,8 +284,8 @@ static int mlp_backward(plg_mlp_s *net, const float *t, int B, hipStream_t st) {
     a.A = net->dzb; a.sam = 1; a.sak = No;
     a.B = net->X[l]; a.sbk = Ni; a.sbn = 1;
     a.C = net->g + net->w_off[l]; a.ldc = Ni;
-    gemm(a, st);
-    k_bias_grad<<<blocks(No), 256, 0, st>>>(net->dz, net->g + net->b_off[l], B, No);
+    gemm(a, net->ksw, st);
+    k_bias_grad<<<blocks(No, 16), 256, 0, st>>>(net->dz, net->g + net->b_off[l], B, No);
     if (l > 0) {
       // dX_l[b][i] = sum_o dZ[b][o] W_l[o][i], then dZ_{l-1} = dX_l silu'(Z_{l-1})
       GemmArgs d{};
@@ -226,7 +293,7 @@ static int mlp_backward(plg_mlp_s *net, const float *t, int B, hipStream_t st) {
       d.A = net->dzb; d.sam = No; d.sak = 1;
       d.B = net->pb + net->w_off[l]; d.sbk = Ni; d.sbn = 1;
       d.C = net->dx; d.ldc = Ni;
-      gemm(d, st);
+      gemm(d, net->ksw, st);
       const int64_t n = (int64_t)B * Ni;
       k_silu_back<<<blocks(n), 256, 0, st>>>(net->dx, net->Z[l - 1], net->dz, net->dzb, n);
     }
@@ -245,6 +312,24 @@ static int mlp_loss(plg_mlp_s *net, int B, hipStream_t st, double *loss) {
   for (float v : h) s += v;
   *loss = s / (double)ny;
   return PLG_OK;
+}
+
+// one Adam step: forward, backward, step counter, updates (no allocation,
+// no memset: capturable)
+static int train_step(plg_mlp_s *net, const float *x, const float *t, int B, float lr, hipStream_t st) {
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  int rc;
+  if ((rc = mlp_forward(net, x, B, st)) || (rc = mlp_backward(net, t, B, st))) return rc;
+  k_adam_step<<<1, 1, 0, st>>>(net->d_step, net->d_coef, b1, b2);
+  for (int l = 0; l < net->L; l++) {
+    const int64_t nw = (int64_t)net->dims[l] * net->dims[l + 1], nb = net->dims[l + 1];
+    const int64_t ow = net->w_off[l], ob = net->b_off[l];
+    k_adam<<<blocks(nw), 256, 0, st>>>(net->p + ow, net->m + ow, net->v + ow, net->g + ow, net->pb + ow, nw, lr,
+                                      b1, b2, eps, net->d_coef);
+    k_adam<<<blocks(nb), 256, 0, st>>>(net->p + ob, net->m + ob, net->v + ob, net->g + ob, nullptr, nb, lr, b1,
+                                      b2, eps, net->d_coef);
+  }
+  return hipGetLastError() == hipSuccess ? PLG_OK : fail(PLG_ERR_HIP, "MLP update launch failed");
 }
 
 static int check_batch(plg_mlp_s *net, int B) {
@@ -294,6 +379,21 @@ int plg_mlp_create(const int *dims, int nlayers, int max_batch, plg_mlp *out) {
   rc = rc ? rc : mlp_alloc(net, (void **)&net->dx, wb * sizeof(float));
   rc = rc ? rc : mlp_alloc(net, (void **)&net->dzb, wb * sizeof(uint16_t));
   rc = rc ? rc : mlp_alloc(net, (void **)&net->part, (size_t)blocks((int64_t)wb) * sizeof(float));
+  {
+    // split-K partials: a bound of ks x M x N over the network's products
+    // at any batch up to max_batch (the split ignoring the tile count)
+    size_t kw = 1;
+    for (int l = 0; l < nlayers; l++) {
+      const int Ni = dims[l], No = dims[l + 1];
+      const int64_t shapes[3][3] = {{max_batch, No, Ni}, {No, Ni, max_batch}, {max_batch, Ni, No}};
+      for (auto &sh : shapes)
+        if (sh[2] >= 512) kw = std::max(kw, (size_t)std::min<int64_t>(MG_KSPLIT, sh[2] / 128) * sh[0] * sh[1]);
+    }
+    rc = rc ? rc : mlp_alloc(net, (void **)&net->ksw, kw * sizeof(float));
+  }
+  rc = rc ? rc : mlp_alloc(net, (void **)&net->d_step, sizeof(int));
+  rc = rc ? rc : mlp_alloc(net, (void **)&net->d_coef, 2 * sizeof(float));
+  if (!rc && hipMemset(net->d_step, 0, sizeof(int)) != hipSuccess) rc = fail(PLG_ERR_HIP, "MLP step clear failed");
   if (!rc && (hipMemset(net->p, 0, pbytes) != hipSuccess || hipMemset(net->pb, 0, off * sizeof(uint16_t)) != hipSuccess))
     rc = fail(PLG_ERR_HIP, "MLP parameter clear failed");
   if (rc) {
@@ -319,6 +419,7 @@ int plg_mlp_set_params(plg_mlp net, const float *host) {
     return fail(PLG_ERR_HIP, "MLP parameter upload failed");
   k_to_bf16<<<blocks(net->nparams), 256>>>(net->p, net->pb, net->nparams);
   net->step = 0;
+  if (hipMemset(net->d_step, 0, sizeof(int)) != hipSuccess) return fail(PLG_ERR_HIP, "MLP step clear failed");
   if (hipDeviceSynchronize() != hipSuccess) return fail(PLG_ERR_HIP, "MLP parameter conversion failed");
   return PLG_OK;
 }
@@ -366,22 +467,51 @@ int plg_mlp_train(plg_mlp net, const float *x, const float *t, int batch, int st
   if (rc) return rc;
   if (!x || !t || steps < 0 || !(lr > 0)) return fail(PLG_ERR_INVALID, "plg_mlp_train: bad arguments");
   hipStream_t st = (hipStream_t)stream;
-  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  for (int s = 0; s < steps; s++) {
-    if ((rc = mlp_forward(net, x, batch, st)) || (rc = mlp_backward(net, t, batch, st))) return rc;
-    net->step++;
-    const float c1 = 1.0f / (1.0f - powf(b1, (float)net->step));
-    const float c2 = 1.0f / (1.0f - powf(b2, (float)net->step));
-    for (int l = 0; l < net->L; l++) {
-      const int64_t nw = (int64_t)net->dims[l] * net->dims[l + 1], nb = net->dims[l + 1];
-      const int64_t ow = net->w_off[l], ob = net->b_off[l];
-      k_adam<<<blocks(nw), 256, 0, st>>>(net->p + ow, net->m + ow, net->v + ow, net->g + ow, net->pb + ow, nw, lr,
-                                        b1, b2, eps, c1, c2);
-      k_adam<<<blocks(nb), 256, 0, st>>>(net->p + ob, net->m + ob, net->v + ob, net->g + ob, nullptr, nb, lr, b1,
-                                        b2, eps, c1, c2);
+  // Steps are replayed from a graph of one step (captured again when the
+  // inputs, batch or rate change) unless PLG_MLP_EAGER=1 or the call is
+  // short; both forms launch the same kernels with the same arguments.
+  const char *ev = std::getenv("PLG_MLP_EAGER");
+  const bool eager = steps < 4 || (ev && std::atoi(ev) != 0);
+  if (eager) {
+    for (int s = 0; s < steps; s++)
+      if ((rc = train_step(net, x, t, batch, lr, st))) return rc;
+  } else {
+    if (!net->exec || net->gx != x || net->gt != t || net->gB != batch || net->glr != lr) {
+      if (net->exec) hipGraphExecDestroy(net->exec);
+      if (net->graph) hipGraphDestroy(net->graph);
+      net->exec = nullptr;
+      net->graph = nullptr;
+      if (!net->cap && hipStreamCreateWithFlags(&net->cap, hipStreamNonBlocking) != hipSuccess)
+        return fail(PLG_ERR_HIP, "MLP capture stream creation failed");
+      // the capture stream starts after the work queued on `stream`
+      hipEvent_t ev0;
+      if (hipEventCreateWithFlags(&ev0, hipEventDisableTiming) != hipSuccess) return fail(PLG_ERR_HIP, "event");
+      hipEventRecord(ev0, st);
+      hipStreamWaitEvent(net->cap, ev0, 0);
+      hipEventDestroy(ev0);
+      if (hipStreamBeginCapture(net->cap, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return fail(PLG_ERR_HIP, "MLP step capture failed");
+      rc = train_step(net, x, t, batch, lr, net->cap);
+      hipGraph_t g = nullptr;
+      const hipError_t ce = hipStreamEndCapture(net->cap, &g);
+      if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+      }
+      if (ce != hipSuccess || !g || hipGraphInstantiate(&net->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        return fail(PLG_ERR_HIP, "MLP step graph instantiation failed");
+      }
+      net->graph = g;
+      net->gx = x;
+      net->gt = t;
+      net->gB = batch;
+      net->glr = lr;
     }
-    if (hipGetLastError() != hipSuccess) return fail(PLG_ERR_HIP, "MLP update launch failed");
+    for (int s = 0; s < steps; s++)
+      if (hipGraphLaunch(net->exec, st) != hipSuccess) return fail(PLG_ERR_HIP, "MLP step graph launch failed");
   }
+  net->step += steps;
   if (loss) {
     // loss of the parameters after the last update
     if ((rc = mlp_forward(net, x, batch, st)) || (rc = mlp_backward(net, t, batch, st))) return rc;

@@ -110,6 +110,35 @@ def test_training_fits_a_smooth_map():
     net.close()
 
 
+def test_training_graph_matches_eager(monkeypatch):
+    """plg_mlp_train replays a captured step graph; with PLG_MLP_EAGER=1 it
+    launches the same kernels one by one.  Both must leave identical
+    parameters, over several calls (the step counter and the Adam moments
+    carry over), with eager work (forward passes, a gradient) and
+    synchronisations between the graph replays, and after new inputs (the
+    graph is captured again)."""
+    torch = G.torch_cuda()
+    rng = np.random.default_rng(31)
+    dims = [17, 64, 64, 11]
+    x = [_dev(rng.normal(size=(48, 17))) for _ in range(2)]
+    t = [_dev(rng.normal(size=(48, 11))) for _ in range(2)]
+    params = []
+    for eager in ("1", "0"):
+        monkeypatch.setenv("PLG_MLP_EAGER", eager)
+        net, _ = _net(dims, 32)
+        y = torch.empty((48, 11), dtype=torch.float32, device="cuda")
+        losses = []
+        for call, (xi, ti, n) in enumerate([(x[0], t[0], 40), (x[0], t[0], 25), (x[1], t[1], 30)]):
+            losses.append(net.train(xi, ti, n, 2e-3))
+            net.forward(xi, y)
+            torch.cuda.synchronize()
+            net.gradient(xi, ti)
+        params.append(net.get_params())
+        print(f"[gpu] training eager={eager}: losses {losses}", flush=True)
+        net.close()
+    assert np.array_equal(params[0], params[1])
+
+
 def test_bias_corrected_estimator_gains():
     """With 64 labelled sources of 256 (32 train, 32 held out) and inputs that
     carry information about the outputs, the bias-corrected mean over all
