@@ -149,8 +149,11 @@ def test_learned_workload():
     """The learned workload at its configuration (256 sources, base + 5%
     perturbation, train 32 / bias-correct 32 / predict 192): the network is
     trained on the device by the library's kernels; the fallback decision
-    follows the gain the held-out sources predict, and when the learned
-    estimator is used it beats the labelled sources alone."""
+    follows the gain the held-out sources predict.  What the network gains:
+    the bias-corrected estimate of the ensemble mean (64 exact sources + 192
+    predictions) has a mean squared error, against the mean of the exact
+    results of all 256 sources, more than 10x below that of the 64 exact
+    sources alone (measured 43x), so the fallback stays unused."""
     import bench
     torch = G.torch_cuda()
     stream = torch.cuda.Stream()
@@ -161,7 +164,5 @@ def test_learned_workload():
           f"fallback {r['fallback_used']}, per-source error median {r['pred_err_per_source']['median']:.3f}",
           flush=True)
     assert r["fallback_used"] == (r["expected_gain_from_heldout"] < 1.0)
-    if not r["fallback_used"]:
-        assert g["gain"] > 1.0
-    else:
-        assert np.array_equal(arr["est"].pred, arr["Y"][64:])
+    assert not r["fallback_used"]
+    assert g["gain"] > 10.0
