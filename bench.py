@@ -185,17 +185,35 @@ def stage_ms(batch, out, stream, reps=3):
     return {name: float(np.median([dict(r)[name] for r in runs])) for name, _ in runs[0]}
 
 
-def csrc_sha():
-    """hash of the kernel sources and build recipe (plegma_amd/csrc, the udsc
-    table generator, the build flags): counter readings taken on another
-    build are not used"""
+# the sources that make each measured kernel (besides the shared headers and
+# the build recipe): counter readings are tied to a hash of these files
+KERNEL_SOURCES = {
+    "k_baryon": ["csrc/plg_baryon.hip", "csrc/plg_tile.h"],
+    "k_mesmes": ["csrc/plg_mesmes.hip", "csrc/plg_tile.h"],
+    "k_project_pair": ["csrc/plg_project_bf16.hip"],
+    "k_udsc": ["csrc/plg_udsc.hip", "gen_udsc.py"],
+    "k_tetra": ["csrc/plg_tetra.hip"],
+}
+COMMON_SOURCES = ["csrc/plg_internal.h", "csrc/plg_common.h", "../include/plegma_amd.h", "build.py"]
+
+
+def kernel_family(kernel):
+    """'plg::k_baryon<float, ...>' -> 'k_baryon'"""
+    name = kernel.split("<")[0].split("(")[0]
+    return name.split("::")[-1]
+
+
+def sources_sha(kernel):
+    """hash of the files a kernel is built from (its own sources, the shared
+    headers, the build recipe), or None for a kernel without a list"""
+    fam = KERNEL_SOURCES.get(kernel_family(kernel))
+    if fam is None:
+        return None
     h = hashlib.sha256()
     d = os.path.join(ROOT, "plegma_amd")
-    files = sorted(f for f in os.listdir(os.path.join(d, "csrc"))
-                   if f.endswith((".hip", ".h")) and f != "plg_udsc_tables.h")
-    for f in [os.path.join("csrc", f) for f in files] + ["gen_udsc.py", "build.py"]:
+    for f in sorted(fam) + COMMON_SOURCES:
         h.update(f.encode())
-        with open(os.path.join(d, f), "rb") as fh:
+        with open(os.path.normpath(os.path.join(d, f)), "rb") as fh:
             h.update(fh.read())
     return h.hexdigest()[:16]
 
@@ -203,7 +221,7 @@ def csrc_sha():
 def measured_traffic(workload, stage):
     """HBM bytes per launch of a stage from the committed counter summary
     (profiles/traffic.json): (bytes, kernel, note), bytes None when the
-    summary was taken on other kernel sources than these"""
+    counters were taken on other sources of that kernel than these"""
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
             d = json.load(f)
@@ -212,11 +230,12 @@ def measured_traffic(workload, stage):
     e = d.get(workload, {}).get(stage)
     if not e:
         return None, None, "stage not in the counter summary"
-    if d.get("csrc_sha") != csrc_sha():
-        return None, e.get("kernel"), (f"stale: counters of kernel sources {d.get('csrc_sha')}, "
-                                       f"this build is {csrc_sha()}")
+    now = sources_sha(e["kernel"])
+    if now is None or e.get("sources_sha") != now:
+        return None, e.get("kernel"), (f"stale: counters of {e['kernel']} sources {e.get('sources_sha')}, "
+                                       f"this build is {now}")
     return e["bytes"], e["kernel"], f"rocprofv3 FETCH_SIZE/WRITE_SIZE of {e['kernel']} " \
-                                    f"(profiles/traffic.json, kernel sources {d.get('csrc_sha')})"
+                                    f"(profiles/traffic.json, kernel sources {now})"
 
 
 def roof(kernel, t_s, nsites, flops_site, bytes_site, bound_compute, cpeak, workload, algorithmic):
