@@ -464,6 +464,67 @@ def run_batching(args, torch, capi, dev, stream):
     res["roofline"] = roofline(st_ms, nsites, len(moms), 80, args.split, CONTRACT_FLOPS_PER_SITE,
                                144 * 8, "batching")
     mixed.close()
+    del keep, v64, v32
+    if not args.no_cpu:
+        try:
+            res["config_8x8x8x16"] = run_small(torch, capi, dev, stream)
+        except Exception as e:  # the headline line is printed regardless
+            res["config_8x8x8x16"] = {"error": repr(e)}
+    return res
+
+
+def run_small(torch, capi, dev, stream, reps=20):
+    """The reference's CPU-runnable case: lattice 8^3x16, one point source at
+    the origin, 16 mesons + the proton (16 spin projections), |n|^2 <= 4 (33
+    momenta): the oracle (FP64 reference bodies + per-site DFT) on all 16 time
+    slices and the GPU paths through plg_correlator (the drop-in call), with
+    the errors of both GPU paths against the oracle"""
+    from oracle import oracle as O
+    Ls = [8, 8, 8, 16]
+    capi.init(Ls, 0)
+    V3, T = Ls[0] * Ls[1] * Ls[2], Ls[3]
+    src = [0, 0, 0, 0]
+    moms = capi.momlist(4)
+    kind, mask = capi.TWOPT, capi.G_NUCL
+    t64 = torch.empty((144, T * V3, 2), dtype=torch.float64, device=dev)
+    v64 = capi.plg_prop(t64.data_ptr(), capi.F64, 0, T, 0, T * V3)
+    capi.synth_prop(v64, SEED, 0, 0, LAMBDA)
+    t32 = torch.empty((144, T * V3, 2), dtype=torch.float32, device=dev)
+    e32 = torch.empty(T, dtype=torch.int32, device=dev)
+    v32 = capi.plg_prop(t32.data_ptr(), capi.F32, 0, T, 0, T * V3, e32.data_ptr())
+    capi.cast_prop(v32, v64)
+    n = capi.correlator_size(kind, mask, src, T, len(moms))
+    out = {}
+    res = {"workload": "lattice 8^3x16, source at the origin, 16 mesons + proton, |n|^2<=4 (33 momenta)",
+           "sites": V3 * T}
+    for name, v, mode in (("fp64", v64, capi.MODE_F64), ("mixed", v32, capi.MODE_MIXED)):
+        o = torch.empty((n, 2), dtype=torch.float64, device=dev)
+        capi.correlator(kind, mask, v, v, src, T, moms, mode, capi.MOMENTUM_SPACE, capi.F64, o, stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            capi.correlator(kind, mask, v, v, src, T, moms, mode, capi.MOMENTUM_SPACE, capi.F64, o, stream)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        h = o.cpu().numpy()
+        out[name] = (h[:, 0] + 1j * h[:, 1]).reshape(2, T, len(moms), 16)
+        res[f"gpu_{name}"] = {"value": V3 * T / (ms / 1e3), "unit": "sites/s", "ms_per_call": ms,
+                              "note": "plg_correlator, one call per correlator (synchronising)"}
+    P = np.ascontiguousarray(t64.cpu().numpy().reshape(-1))
+    nth = baseline_threads()
+    runs, t_cpu = 0, 0.0
+    while t_cpu < 1.0:  # repeated to about a second of CPU time
+        t0 = time.perf_counter()
+        R = O.correlator(O.TWOPT, Ls, src, T, 4, P, P, mask=O.G_NUCL, nthreads=nth)
+        t_cpu += time.perf_counter() - t0
+        runs += 1
+    R = R[:, 0]
+    res["cpu_baseline"] = {"value": runs * V3 * T / t_cpu, "unit": "sites/s", "cores": nth, "kind": "port",
+                           "sample": f"the whole workload ({V3 * T} sites) {runs} times, {t_cpu:.2f} s"}
+    for name in ("fp64", "mixed"):
+        A = out[name][:, :, :, :]
+        res[f"rel_err_{name}_vs_oracle"] = err_summary(A, R, moms)
+    capi.finalize()
     return res
 
 
