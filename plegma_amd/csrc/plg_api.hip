@@ -61,7 +61,10 @@ static Context g_ctx;
 // g_ws_mu is held by a single call (plg_correlator, the udsc and table
 // contractions, plg_ft_apply) from the moment it sizes the shared workspace
 // until its stream has finished, so two threads' single calls never share
-// or free each other's workspace; lock order g_ws_mu, then g_mu.
+// or free each other's workspace.  g_static_mu (below) is held by
+// plg_correlator_static for the whole call.  plg_init and plg_finalize take
+// all three, so the context is never rebuilt under a running call; lock
+// order g_static_mu, g_ws_mu, g_mu.
 static std::mutex g_mu;
 static std::mutex g_ws_mu;
 
@@ -993,6 +996,7 @@ const char *plg_last_error(void) { return g_err.c_str(); }
 static void finalize_locked();
 
 int plg_init(const int L[4], int device) {
+  std::lock_guard<std::mutex> sl(g_static_mu);
   std::lock_guard<std::mutex> wl(g_ws_mu);
   std::lock_guard<std::mutex> lk(g_mu);
   if (!L) return fail(PLG_ERR_INVALID, "L is NULL");
@@ -1049,6 +1053,7 @@ int plg_init(const int L[4], int device) {
 }
 
 int plg_finalize(void) {
+  std::lock_guard<std::mutex> sl(g_static_mu);
   std::lock_guard<std::mutex> wl(g_ws_mu);
   std::lock_guard<std::mutex> lk(g_mu);
   finalize_locked();
@@ -1162,6 +1167,7 @@ int plg_correlator(int kind, int group_mask, const plg_prop *p1, const plg_prop 
 int plg_correlator_static(int kind, int group_mask, const plg_prop *p1, const plg_prop *p2, const int source[4],
                           int totalT, const int *moms, int nmom, int mode, int space, int out_precision, void *out,
                           void *stream) {
+  std::lock_guard<std::mutex> sl(g_static_mu);  // the whole call (plg_init waits for it)
   if (!g_ctx.init) return fail(PLG_ERR_NOT_INIT, "plg_init not called");
   if (!p1 || !out || !source) return fail(PLG_ERR_INVALID, "NULL argument");
   if (out_precision != PLG_F64 && out_precision != PLG_F32) return fail(PLG_ERR_INVALID, "bad out precision");
@@ -1176,7 +1182,6 @@ int plg_correlator_static(int kind, int group_mask, const plg_prop *p1, const pl
   std::vector<plg_prop> props = {*p1, p2 ? *p2 : *p1};
   if (!same_view(props[0], props[1]) && (kind == PLG_TWOPT))
     return fail(PLG_ERR_INVALID, "twopt takes one propagator");
-  std::lock_guard<std::mutex> sl(g_static_mu);
   Plan P;
   std::vector<JobIn> jobs = {{{source[0], source[1], source[2], source[3]}, totalT, 0, 1, 0}};
   if ((rc = build_plan(P, kind, group_mask, props, jobs, moms, nmom, mode, PLG_SPLIT_BF16X6, space))) return rc;
