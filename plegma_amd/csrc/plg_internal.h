@@ -68,17 +68,23 @@ __device__ __forceinline__ double row_scale(const OutRowDev &R) {
 
 // udsc baryons: chunked schedule of one flavour product's term tables.
 // Chunk c holds up to 64 (m0m1, m2m3) quads, one per lane; lane l's quad has
-// up to UD_TPQ distinct terms, kept in the chunk-local slots l*UD_TPQ + e.
+// up to UD_TPQ distinct terms, kept in the chunk-local slots e*64 + l.
+// Operator k owns the lanes [op_lo[k], op_lo[k] + op_n[k]) (all 64 lanes are
+// owned, more lanes to operators with more entries); in chunk c each lane sums
+// its share of its operator's entries, nblk[c] blocks of 4 entries, laid out
+// as [block][lane][4] from block blk0[c] (zero-coefficient padding).
 constexpr int UD_MAXC = 4;        // chunks
 constexpr int UD_TPQ = 4;         // distinct terms per quad
 constexpr int UD_MAXOPS = 16;     // operators of a product
-constexpr int UD_MAXENT = 2816;   // operator entries of a product (padded)
+constexpr int UD_MAXBLK = 16;     // entry blocks of a product (4 per lane each)
+constexpr int UD_MAXENT = UD_MAXBLK * 256;
 constexpr int UD_MAXCOEF = 64;    // distinct coefficients of a product
 struct UdscSched {
   int nops, nchunks, nent, ncoef;
-  // entries of operator k in chunk c: four sublists of ent_len entries (a
-  // multiple of 4) from ent_off (a multiple of 4), one per lane of the operator
-  int ent_off[UD_MAXC][UD_MAXOPS], ent_len[UD_MAXC][UD_MAXOPS];
+  int real_coef;                    // every coefficient has a zero imaginary part
+  int ntp[UD_MAXC];                 // most distinct terms of a quad in chunk c
+  int blk0[UD_MAXC], nblk[UD_MAXC];
+  int op_lo[UD_MAXOPS], op_n[UD_MAXOPS];
   const short *quads;       // [chunk][lane]: m0m1 | m2m3 << 4, -1 = none
   const signed char *tp;    // [chunk][lane][UD_TPQ]: m4m5 of each term, -1 = none
   const uint16_t *ents;     // chunk-local term slot | coefficient index << 8

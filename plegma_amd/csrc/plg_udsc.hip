@@ -58,7 +58,11 @@ int udsc_schedule_terms(const UdscTerm *terms, const int *first, const int *coun
     if ((int)kv.second.size() > UD_TPQ) return PLG_ERR_UNSUPPORTED;
     quads.push_back(kv.first);
   }
-  std::sort(quads.begin(), quads.end(), [](int a, int b) {
+  // quads with the most distinct terms first: a wave runs the term loop of a
+  // chunk as often as its fullest quad needs, so the later chunks run fewer
+  std::sort(quads.begin(), quads.end(), [&](int a, int b) {
+    const size_t na = by_quad[a].size(), nb = by_quad[b].size();
+    if (na != nb) return na > nb;
     return (a & 15) != (b & 15) ? (a & 15) < (b & 15) : (a >> 4) < (b >> 4);
   });
   const int nch = std::max(1, ((int)quads.size() + 63) / 64);
@@ -73,44 +77,100 @@ int udsc_schedule_terms(const UdscTerm *terms, const int *first, const int *coun
     const int c = (int)i / 64, l = (int)i % 64;
     h.quads[i] = (short)quads[i];
     const auto &v = by_quad[quads[i]];
+    h.s.ntp[c] = std::max(h.s.ntp[c], (int)v.size());
     for (size_t e = 0; e < v.size(); e++) {
       h.tp[i * UD_TPQ + e] = (signed char)((v[e] >> 2 & 3) * 4 + (v[e] & 3));
-      slot[v[e]] = {c, l * UD_TPQ + (int)e};
+      slot[v[e]] = {c, (int)e * 64 + l};  // term-major: a wave's 64 writes hit distinct banks
     }
   }
-  // each operator's entries, per chunk: slot | coefficient index << 8, in
-  // four sublists padded with zero-coefficient entries
+  // each operator's entries per chunk: slot | coefficient index << 8
   std::map<std::pair<float, float>, int> cix;
   h.coef = {0.f, 0.f};
   cix[{0.f, 0.f}] = 0;
-  for (int c = 0; c < nch; c++)
-    for (int k = 0; k < nops; k++) {
-      std::vector<uint16_t> v;
-      for (int i = 0; i < count[k]; i++) {
-        const UdscTerm &t = terms[first[k] + i];
-        const auto &sl = slot[key6(t)];
-        if (sl.first != c) continue;
-        auto it = cix.find({t.re, t.im});
-        if (it == cix.end()) {
-          if ((int)h.coef.size() / 2 >= UD_MAXCOEF) return PLG_ERR_UNSUPPORTED;
-          it = cix.insert({{t.re, t.im}, (int)h.coef.size() / 2}).first;
-          h.coef.push_back(t.re);
-          h.coef.push_back(t.im);
-        }
-        v.push_back((uint16_t)(sl.second | (it->second << 8)));
+  h.s.real_coef = 1;
+  std::vector<std::vector<std::vector<uint16_t>>> lists(nch, std::vector<std::vector<uint16_t>>(nops));
+  for (int k = 0; k < nops; k++)
+    for (int i = 0; i < count[k]; i++) {
+      const UdscTerm &t = terms[first[k] + i];
+      const auto &sl = slot[key6(t)];
+      auto it = cix.find({t.re, t.im});
+      if (it == cix.end()) {
+        if ((int)h.coef.size() / 2 >= UD_MAXCOEF) return PLG_ERR_UNSUPPORTED;
+        it = cix.insert({{t.re, t.im}, (int)h.coef.size() / 2}).first;
+        h.coef.push_back(t.re);
+        h.coef.push_back(t.im);
+        if (t.im != 0.f) h.s.real_coef = 0;
       }
-      const int len = (((int)v.size() + 3) / 4 + 3) / 4 * 4;
-      h.s.ent_off[c][k] = (int)h.ents.size();
-      h.s.ent_len[c][k] = len;
-      for (int r = 0; r < 4; r++)
-        for (int j = 0; j < len; j++) {
-          const size_t e = (size_t)r * len + j;  // contiguous quarters of the list
-          h.ents.push_back(e < v.size() ? v[e] : (uint16_t)0);
-        }
+      lists[sl.first][k].push_back((uint16_t)(sl.second | (it->second << 8)));
     }
+  // lanes per operator: one each, then one at a time to the operator whose
+  // extra lane lowers the blocks summed over the chunks the most (a chunk
+  // takes as many blocks as its longest lane list)
+  auto blocks = [&](const std::vector<int> &L) {
+    int tot = 0;
+    for (int c = 0; c < nch; c++) {
+      int m = 0;
+      for (int k = 0; k < nops; k++) {
+        const int per = ((int)lists[c][k].size() + L[k] - 1) / L[k];
+        m = std::max(m, (per + 3) / 4);
+      }
+      tot += m;
+    }
+    return tot;
+  };
+  std::vector<int> L(nops, 1), tot_ent(nops, 0);
+  for (int c = 0; c < nch; c++)
+    for (int k = 0; k < nops; k++) tot_ent[k] += (int)lists[c][k].size();
+  for (int extra = 0; extra < 64 - nops; extra++) {
+    int best_k = 0, best_b = 1 << 30;
+    double best_load = -1;
+    for (int k = 0; k < nops; k++) {
+      L[k]++;
+      const int bl = blocks(L);
+      const double load = (double)tot_ent[k] / L[k];
+      L[k]--;
+      if (bl < best_b || (bl == best_b && load > best_load)) {
+        best_b = bl;
+        best_load = load;
+        best_k = k;
+      }
+    }
+    L[best_k]++;
+  }
+  for (int k = 0, lo = 0; k < nops; k++) {
+    h.s.op_lo[k] = lo;
+    h.s.op_n[k] = L[k];
+    lo += L[k];
+  }
+  // blocks: [block][lane][4]; operator k's list split into L[k] contiguous
+  // shares of near-equal length, one per lane
+  int nblk_tot = 0;
+  for (int c = 0; c < nch; c++) {
+    std::vector<std::vector<uint16_t>> lane(64);
+    int nb = 0;
+    for (int k = 0; k < nops; k++) {
+      const auto &v = lists[c][k];
+      const int n = (int)v.size();
+      for (int r = 0; r < L[k]; r++) {
+        auto &ln = lane[h.s.op_lo[k] + r];
+        for (int i = r * n / L[k]; i < (r + 1) * n / L[k]; i++) ln.push_back(v[i]);
+        nb = std::max(nb, ((int)ln.size() + 3) / 4);
+      }
+    }
+    h.s.blk0[c] = nblk_tot;
+    h.s.nblk[c] = nb;
+    nblk_tot += nb;
+    if (nblk_tot > UD_MAXBLK) return PLG_ERR_UNSUPPORTED;
+    for (int b = 0; b < nb; b++)
+      for (int l = 0; l < 64; l++)
+        for (int j = 0; j < 4; j++) {
+          const size_t e = (size_t)b * 4 + j;
+          h.ents.push_back(e < lane[l].size() ? lane[l][e] : (uint16_t)0);
+        }
+  }
   h.s.nent = (int)h.ents.size();
   h.s.ncoef = (int)h.coef.size() / 2;
-  if (h.s.nent > UD_MAXENT || h.s.ncoef > UD_MAXCOEF || 64 * UD_TPQ > 256) return PLG_ERR_UNSUPPORTED;
+  if (h.s.nent > UD_MAXENT || h.s.ncoef > UD_MAXCOEF) return PLG_ERR_UNSUPPORTED;
   return PLG_OK;
 }
 
@@ -132,9 +192,13 @@ int udsc_nops(int prod) { return UDSC_PRODS[prod].nops; }
 const char *udsc_op_name(int prod, int k) { return UDSC_OPS[UDSC_PRODS[prod].first_op + k].name; }
 
 // ------------------------------------------------------------------ kernel
-// A workgroup of 4 waves stages the three propagators of G consecutive sites
+// A workgroup of W waves stages the three propagators of G consecutive sites
 // (one 128-byte line of every component) into LDS with coalesced loads; each
-// wave then contracts G/4 of those sites, one at a time.
+// wave then contracts G/W of those sites, one at a time.  The staged sites
+// take 55.4 KB of LDS in either precision; FP32 runs 8 waves on them (two
+// workgroups of 78 KB fill a CU's 160 KB with 4 waves per SIMD), FP64 4
+// waves (its 159 VGPRs allow 3 waves per SIMD, and 8 waves would double
+// the term slots to 94 KB, one workgroup per CU).
 // LDS written and read by lanes of one wave only: a wavefront-scope fence
 // orders the accesses (a wave's LDS operations are performed in order)
 __device__ __forceinline__ void wave_sync() {
@@ -145,17 +209,23 @@ __device__ __forceinline__ void wave_sync() {
 template <typename T> struct UdscGeom {
   static constexpr int G = 128 / (int)sizeof(cx<T>);  // sites per line
   static constexpr int SPAD = 433;                    // LDS stride of a site (432 + 1)
+  static constexpr int W = sizeof(T) == 4 ? 8 : 4;    // waves per workgroup
+  static constexpr int NT = 64 * W;
 };
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_udsc(UdscSched S, const PropDev<T> *__restrict__ props,
+// RC: every coefficient is real (the built-in tables), an entry costs two
+// FMAs instead of a complex product
+template <typename T, bool RC>
+__global__ __launch_bounds__(64 * UdscGeom<T>::W) void k_udsc(UdscSched S, const PropDev<T> *__restrict__ props,
                                               const RowDev *__restrict__ rows, cx<T> *__restrict__ out, int64_t V3,
                                               int T_lat, int nch_total, int ch0, int groups_per_block) {
-  constexpr int G = UdscGeom<T>::G, SPAD = UdscGeom<T>::SPAD;
+  constexpr int G = UdscGeom<T>::G, SPAD = UdscGeom<T>::SPAD, W = UdscGeom<T>::W, NT = UdscGeom<T>::NT;
+  constexpr int CPP = NT / G;                   // components one pass of the workgroup stages
+  constexpr int NPASS = (144 + CPP - 1) / CPP;  // passes over a propagator's 144 components
   __shared__ cx<T> sp[G * SPAD];              // [site][f][m01][c c']
-  __shared__ cx<T> sterm[4][64 * UD_TPQ];     // each wave's chunk terms
+  __shared__ cx<T> sterm[W][64 * UD_TPQ];     // each wave's chunk terms, then its lanes' sums
   __shared__ __align__(16) uint16_t sent[UD_MAXENT];
-  __shared__ float scoef[2 * UD_MAXCOEF];
+  __shared__ T scoef[2 * UD_MAXCOEF];         // RC: the real parts only
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const RowDev row = rows[blockIdx.y];
   const PropDev<T> P0 = props[row.pa], P1 = props[row.pb], P2 = props[row.pc];
@@ -168,53 +238,68 @@ __global__ __launch_bounds__(256) void k_udsc(UdscSched S, const PropDev<T> *__r
   const T sg = prop_sign(P0, row.t) * prop_sign(P1, row.t) * prop_sign(P2, row.t);
   const int rot_any = P0.rot | P1.rot | P2.rot;
   const int64_t ncols = (int64_t)gridDim.y * nch_total, col0 = (int64_t)blockIdx.y * nch_total + ch0;
-  const int kop = lane >> 2, rr = lane & 3;
-  const bool op_lane = kop < S.nops;
-  // the lane's schedule in registers; the entries and coefficients in LDS
-  int qv[UD_MAXC], tpv[UD_MAXC], eoff[UD_MAXC], en[UD_MAXC];
+  // the lane's quads and terms in registers; the entries and coefficients in LDS
+  int qv[UD_MAXC], tpv[UD_MAXC];
 #pragma unroll
   for (int c = 0; c < UD_MAXC; c++) {
     const bool live = c < S.nchunks;
     qv[c] = live ? S.quads[c * 64 + lane] : -1;
     tpv[c] = live ? *(const int *)(S.tp + (c * 64 + lane) * UD_TPQ) : -1;
-    en[c] = live && op_lane ? S.ent_len[c][kop] : 0;
-    eoff[c] = live && op_lane ? S.ent_off[c][kop] + rr * en[c] : 0;
   }
-  for (int i = tid; i < S.nent; i += 256) sent[i] = S.ents[i];
-  for (int i = tid; i < 2 * S.ncoef; i += 256) scoef[i] = S.coef[i];
+  for (int i = tid; i < S.nent; i += NT) sent[i] = S.ents[i];
+  if (RC)
+    for (int i = tid; i < S.ncoef; i += NT) scoef[i] = (T)S.coef[2 * i];
+  else
+    for (int i = tid; i < 2 * S.ncoef; i += NT) scoef[i] = (T)S.coef[i];
   cx<T> *st = sterm[wave];
+  // staging: thread (site s_l, component c_l) loads components c_l + i CPP of
+  // the three propagators at site x0 + s_l; the G threads of a component read
+  // one 128-byte line
+  const int s_l = tid % G, c_l = tid / G;
+  const int64_t o0 = (int64_t)c_l * P0.cs + s_l, o1 = (int64_t)c_l * P1.cs + s_l, o2 = (int64_t)c_l * P2.cs + s_l;
+  const int64_t d0 = (int64_t)CPP * P0.cs, d1 = (int64_t)CPP * P1.cs, d2 = (int64_t)CPP * P2.cs;
+  cx<T> *dst = sp + s_l * SPAD + c_l;
 
   for (int gi = 0; gi < groups_per_block; gi++) {
     const int64_t x0 = ((int64_t)blockIdx.x * groups_per_block + gi) * G;
     if (x0 >= V3) break;  // uniform over the workgroup
-    __syncthreads();      // the previous group's sites are done
-    // stage: element idx = comp * G + s, lanes of a component read G consecutive sites
-    for (int idx = tid; idx < 432 * G; idx += 256) {
-      const int comp = idx / G, s = idx - comp * G;
-      const int f = comp / 144, cl = comp - f * 144;
-      const cx<T> *b = f == 0 ? b0 : (f == 1 ? b1 : b2);
-      const int64_t cs = f == 0 ? P0.cs : (f == 1 ? P1.cs : P2.cs);
-      const int64_t x = x0 + s;
-      sp[s * SPAD + comp] = x < V3 ? gload(b + (int64_t)cl * cs + x) : cmake<T>(0, 0);
+    // all loads of the group in flight at once, issued before the wait for
+    // the previous group's sites
+    const bool xin = x0 + s_l < V3;
+    cx<T> v[3][NPASS];
+#pragma unroll
+    for (int i = 0; i < NPASS; i++) {
+      const bool ok = xin && c_l + i * CPP < 144;
+      v[0][i] = ok ? gload(b0 + x0 + o0 + i * d0) : cmake<T>(0, 0);
+      v[1][i] = ok ? gload(b1 + x0 + o1 + i * d1) : cmake<T>(0, 0);
+      v[2][i] = ok ? gload(b2 + x0 + o2 + i * d2) : cmake<T>(0, 0);
     }
+    __syncthreads();  // the previous group's sites are done
+#pragma unroll
+    for (int i = 0; i < NPASS; i++)
+      if (c_l + i * CPP < 144) {
+        dst[i * CPP] = v[0][i];
+        dst[144 + i * CPP] = v[1][i];
+        dst[288 + i * CPP] = v[2][i];
+      }
     __syncthreads();
     if (rot_any) {  // rotation to the physical basis, per (site, propagator, colour pair)
-      for (int k = tid; k < G * 27; k += 256) {
+      for (int k = tid; k < G * 27; k += NT) {
         const int s = k / 27, f = (k % 27) / 9, cc = k % 9;
         const int rot = f == 0 ? P0.rot : (f == 1 ? P1.rot : P2.rot);
         if (rot) {
           cx<T> *b = sp + s * SPAD + f * 144 + cc;
-          cx<T> v[16];
+          cx<T> w[16];
 #pragma unroll
-          for (int m = 0; m < 16; m++) v[m] = b[m * 9];
-          rot_spins(v, rot);
+          for (int m = 0; m < 16; m++) w[m] = b[m * 9];
+          rot_spins(w, rot);
 #pragma unroll
-          for (int m = 0; m < 16; m++) b[m * 9] = v[m];
+          for (int m = 0; m < 16; m++) b[m * 9] = w[m];
         }
       }
       __syncthreads();
     }
-    for (int s = wave; s < G; s += 4) {
+    for (int s = wave; s < G; s += W) {
       const cx<T> *spx = sp + s * SPAD;
       cx<T> acc = cmake<T>(0, 0);
 #pragma unroll
@@ -261,37 +346,49 @@ __global__ __launch_bounds__(256) void k_udsc(UdscSched S, const PropDev<T> *__r
           // its distinct terms T = sum_cc' D[c][c'] S3[m4m5][c][c']
 #pragma unroll
           for (int e = 0; e < UD_TPQ; e++) {
+            if (e >= S.ntp[c]) break;  // uniform: the chunk's fullest quad is done
             const int p = (signed char)(tpv[c] >> (8 * e));
             if (p >= 0) {
               const cx<T> *C3 = spx + 288 + p * 9;
-              cx<T> v = cmake<T>(0, 0);
+              cx<T> t = cmake<T>(0, 0);
 #pragma unroll
-              for (int j = 0; j < 9; j++) cfma(v, d[j], C3[j]);
-              st[lane * UD_TPQ + e] = v;
+              for (int j = 0; j < 9; j++) cfma(t, d[j], C3[j]);
+              st[e * 64 + lane] = t;
             }
           }
         }
         wave_sync();
-        // operator entries: 4 lanes per operator, each a quarter of the list,
-        // four entries per LDS read
-        for (int e = 0; e < en[c]; e += 4) {
-          const uint2 w = *(const uint2 *)(sent + eoff[c] + e);
+        // the lane's share of its operator's entries: nblk[c] blocks of four
+        const uint16_t *ec = sent + S.blk0[c] * 256 + lane * 4;
+        for (int bk = 0; bk < S.nblk[c]; bk++) {
+          const uint2 w = *(const uint2 *)(ec + bk * 256);
           const int u[4] = {(int)(w.x & 0xffff), (int)(w.x >> 16), (int)(w.y & 0xffff), (int)(w.y >> 16)};
 #pragma unroll
           for (int j = 0; j < 4; j++) {
+            const cx<T> t = st[u[j] & 255];
             const int ci = u[j] >> 8;
-            cfma(acc, cmake<T>((T)scoef[2 * ci], (T)scoef[2 * ci + 1]), st[u[j] & 255]);
+            if (RC) {
+              const T cr = scoef[ci];
+              acc.re = fma(cr, t.re, acc.re);
+              acc.im = fma(cr, t.im, acc.im);
+            } else {
+              cfma(acc, cmake<T>(scoef[2 * ci], scoef[2 * ci + 1]), t);
+            }
           }
         }
         wave_sync();
       }
-      acc.re += __shfl_xor(acc.re, 1);
-      acc.im += __shfl_xor(acc.im, 1);
-      acc.re += __shfl_xor(acc.re, 2);
-      acc.im += __shfl_xor(acc.im, 2);
+      // each operator sums the shares of its lanes
+      st[lane] = acc;
+      wave_sync();
       const int64_t x = x0 + s;
-      if (rr == 0 && op_lane && x < V3)
-        gstore(out + ctile_idx(col0 + kop, x, ncols), cmake<T>(sg * acc.re, sg * acc.im));
+      if (lane < S.nops && x < V3) {
+        const int lo = S.op_lo[lane], n = S.op_n[lane];
+        cx<T> sum = cmake<T>(0, 0);
+        for (int r = 0; r < n; r++) sum = sum + st[lo + r];
+        gstore(out + ctile_idx(col0 + lane, x, ncols), cmake<T>(sg * sum.re, sg * sum.im));
+      }
+      wave_sync();
     }
   }
 }
@@ -303,8 +400,11 @@ int launch_udsc(const UdscSched &S, const PropDev<T> *props, const RowDev *rows,
   constexpr int G = UdscGeom<T>::G;
   const int gpb = std::max(1, spw / G);
   const int64_t ngroups = (V3 + G - 1) / G;
-  dim3 block(256), grid((unsigned)((ngroups + gpb - 1) / gpb), (unsigned)nrows);
-  k_udsc<T><<<grid, block, 0, st>>>(S, props, rows, out, V3, T_lat, nch_total, ch0, gpb);
+  dim3 block(UdscGeom<T>::NT), grid((unsigned)((ngroups + gpb - 1) / gpb), (unsigned)nrows);
+  if (S.real_coef)
+    k_udsc<T, true><<<grid, block, 0, st>>>(S, props, rows, out, V3, T_lat, nch_total, ch0, gpb);
+  else
+    k_udsc<T, false><<<grid, block, 0, st>>>(S, props, rows, out, V3, T_lat, nch_total, ch0, gpb);
   return hipGetLastError() == hipSuccess ? PLG_OK : PLG_ERR_HIP;
 }
 
