@@ -295,6 +295,40 @@ public:
   void contractTetraquarks(PLEGMA_Propagator<Float2> &propLT, PLEGMA_Propagator<Float2> &propST,
                            PLEGMA_Propagator<Float2> &propCH, PLEGMA_Propagator<Float2> &propBT,
                            bool only_st = false, bool only_ch = false) {
+    static const TetraFamily hl = {plg_tetraquarks_select, plg_tetra_product, plg_tetra_op_name,
+                                   plg_tetraquarks_size, plg_tetraquarks, "plg_tetraquarks"};
+    tetraquarks(hl, propLT, propST, propCH, propBT, only_st, only_ch);
+  }
+
+  // lib/PLEGMA_Correlator.cu:252-310: the bcud tetraquark product ("udcb",
+  // J = 1 and J = 0 operators), same selection rules and storage
+  template <typename Float2>
+  void contractTetraquarksBCUD(PLEGMA_Propagator<Float2> &propLT, PLEGMA_Propagator<Float2> &propST,
+                               PLEGMA_Propagator<Float2> &propCH, PLEGMA_Propagator<Float2> &propBT,
+                               bool only_st = false, bool only_ch = false) {
+    static const TetraFamily bcud = {plg_tetraquarks_bcud_select, plg_tetra_bcud_product, plg_tetra_bcud_op_name,
+                                     plg_tetraquarks_bcud_size, plg_tetraquarks_bcud, "plg_tetraquarks_bcud"};
+    tetraquarks(bcud, propLT, propST, propCH, propBT, only_st, only_ch);
+  }
+
+ private:
+  // the C entry points of one family of built-in tetraquark tables
+  struct TetraFamily {
+    int (*select)(const plg_prop *, const plg_prop *, const plg_prop *, const plg_prop *, int, int, int *, int,
+                  int *, int *);
+    int (*product)(int, char *, int *);
+    int (*op_name)(int, int, char *, int);
+    int (*size)(const plg_prop *, const plg_prop *, const plg_prop *, const plg_prop *, int, int, const int *,
+                int, int, int, int64_t *);
+    int (*run)(const plg_prop *, const plg_prop *, const plg_prop *, const plg_prop *, int, int, const int *, int,
+               const int *, int, int, int, int, void *, void *);
+    const char *what;
+  };
+
+  template <typename Float2>
+  void tetraquarks(const TetraFamily &f, PLEGMA_Propagator<Float2> &propLT, PLEGMA_Propagator<Float2> &propST,
+                   PLEGMA_Propagator<Float2> &propCH, PLEGMA_Propagator<Float2> &propBT, bool only_st,
+                   bool only_ch) {
     shape = {};
     description = "";
     groups = {};
@@ -310,14 +344,13 @@ public:
       }
     }
     int prods[32], np = 0, nd = 0;
-    plg_check(plg_tetraquarks_select(vp[0], vp[1], vp[2], vp[3], only_st, only_ch, prods, 32, &np, &nd),
-              "plg_tetraquarks_select");
+    plg_check(f.select(vp[0], vp[1], vp[2], vp[3], only_st, only_ch, prods, 32, &np, &nd), f.what);
     for (int i = 0; i < np; i++) {
       int nops = 0;
-      plg_check(plg_tetra_product(prods[i], nullptr, &nops), "plg_tetra_product");
+      plg_check(f.product(prods[i], nullptr, &nops), f.what);
       for (int k = 0; k < nops; k++) {
         char name[64];
-        plg_check(plg_tetra_op_name(prods[i], k, name, sizeof(name)), "plg_tetra_op_name");
+        plg_check(f.op_name(prods[i], k, name, sizeof(name)), f.what);
         datasets.push_back(name);
       }
     }
@@ -327,9 +360,7 @@ public:
     const int nmom = (int)moms.size() / 3;
     const int space = corr_space == MOMENTUM_SPACE ? PLG_MOMENTUM_SPACE : PLG_POSITION_SPACE;
     int64_t n = 0;
-    plg_check(plg_tetraquarks_size(vp[0], vp[1], vp[2], vp[3], only_st, only_ch, source.data(), totalT, nmom, space,
-                                   &n),
-              "plg_tetraquarks_size");
+    plg_check(f.size(vp[0], vp[1], vp[2], vp[3], only_st, only_ch, source.data(), totalT, nmom, space, &n), f.what);
     if ((size_t)n != getTotalSize()) PLEGMA_error("tetraquark correlator size mismatch (%lld vs %zu)", (long long)n,
                                                   getTotalSize());
     const size_t bytes = (size_t)n * 2 * sizeof(Float);
@@ -339,13 +370,14 @@ public:
       d_bytes = bytes;
     }
     const int mode = sizeof(Float2) == 8 ? PLG_MODE_F64 : PLG_MODE_MIXED;
-    plg_check(plg_tetraquarks(vp[0], vp[1], vp[2], vp[3], only_st, only_ch, source.data(), totalT,
-                              moms.empty() ? nullptr : moms.data(), nmom, mode, space, plg_precision_of<Float>(),
-                              d_out, nullptr),
-              "plg_tetraquarks");
+    plg_check(f.run(vp[0], vp[1], vp[2], vp[3], only_st, only_ch, source.data(), totalT,
+                    moms.empty() ? nullptr : moms.data(), nmom, mode, space, plg_precision_of<Float>(), d_out,
+                    nullptr),
+              f.what);
     hip_check(hipMemcpy(H_elem(), d_out, bytes, hipMemcpyDeviceToHost), "result copy");
   }
 
+ public:
   // ---- output (lib/PLEGMA_Correlator.cu:721-806, momentum space)
   void writeASCII(std::string filename) const {
     if (corr_space != MOMENTUM_SPACE) PLEGMA_error("WriteASCII do not support writing in position space.");

@@ -191,9 +191,12 @@ int plg_baryons_table(const plg_prop *p1, const plg_prop *p2, const plg_prop *p3
  *         conj(P3[i5+2][i4+2][a[c5]][a[c4]]) conj(P4[i7+2][i6+2][a[c7]][a[c6]])
  * (spin indices + 2 mod 4), operator k the next counts[k] terms; idxs i0..i7
  * and col_contr c0..c7: 8 int16 per term; vals: re, im (float) per term.
- * Supported: up to 32 operators; c0..c3 a permutation of 0..3 closing either
- * as (c0,c1) = (c5,c4), (c2,c3) = (c7,c6) or as c0 = c5, c1 = c6, c2 = c7,
- * c3 = c4 (the forms of the reference's tables); PLG_ERR_UNSUPPORTED else.
+ * Supported: up to 64 operators; c0..c3 a permutation of 0..3 closing as
+ * (c0,c1) = (c5,c4), (c2,c3) = (c7,c6); as c0 = c5, c1 = c6, c2 = c7,
+ * c3 = c4 (the forms of the heavy-light tables); or as those two with P3 and
+ * P4 exchanged, (c0,c1) = (c7,c6), (c2,c3) = (c5,c4) and c0 = c7, c1 = c4,
+ * c2 = c5, c3 = c6 (the further forms of the bcud tables);
+ * PLG_ERR_UNSUPPORTED else.
  * out: [k][j][m] (or [k][j][x]) as plg_baryons_udsc. */
 int plg_tetraquark_table(const plg_prop *p1, const plg_prop *p2, const plg_prop *p3, const plg_prop *p4,
                          const int16_t *idxs, const int16_t *col_contr, const float *vals, const int16_t *counts,
@@ -225,6 +228,27 @@ int plg_tetraquarks_size(const plg_prop *lt, const plg_prop *st, const plg_prop 
 int plg_tetraquarks(const plg_prop *lt, const plg_prop *st, const plg_prop *ch, const plg_prop *bt, int only_st,
                     int only_ch, const int source[4], int totalT, const int *moms, int nmom, int mode, int space,
                     int out_precision, void *out, void *stream);
+/* ---- bcud tetraquarks ---------------------------------------------------
+ * Replaces contractTetraquarksBCUD / contract_tetraquarks_bcud
+ * (lib/PLEGMA_Correlator.cu:252-310,
+ * lib/kernels/PLEGMA_bcud_tetraquarks.cu:159-212): the J = 1 (48) and J = 0
+ * (9) two-point functions of the bcud tetraquark, flavour content
+ * (bbar u cbar d) - (bbar d cbar u), product "udcb" (u, d read lt, c reads ch,
+ * b reads bt; selection as plg_tetraquarks).  Datasets in the reference's
+ * order ("bcud_J1/g5_gj_g5_gj_ll", ..., "bcud_J0/g5C_Cg5_g5C_Cg5_ll"); out as
+ * plg_baryons_udsc.  Term tables: generated from the interpolating fields
+ * (python/tetraquarks_v4.py of the reference makes the same tables). */
+int plg_tetra_bcud_count(int *nprod);
+int plg_tetra_bcud_product(int prod, char *flavours, int *nops);
+int plg_tetra_bcud_op_name(int prod, int k, char *buf, int cap);
+int plg_tetraquarks_bcud_select(const plg_prop *lt, const plg_prop *st, const plg_prop *ch, const plg_prop *bt,
+                                int only_st, int only_ch, int *prods, int cap, int *nprods, int *ndatasets);
+int plg_tetraquarks_bcud_size(const plg_prop *lt, const plg_prop *st, const plg_prop *ch, const plg_prop *bt,
+                              int only_st, int only_ch, const int source[4], int totalT, int nmom, int space,
+                              int64_t *elems);
+int plg_tetraquarks_bcud(const plg_prop *lt, const plg_prop *st, const plg_prop *ch, const plg_prop *bt,
+                         int only_st, int only_ch, const int source[4], int totalT, const int *moms, int nmom,
+                         int mode, int space, int out_precision, void *out, void *stream);
 /* the same, with HIP events between the stages of every product: nstages
  * stage names (32 bytes each) and their milliseconds summed over the products */
 int plg_baryons_udsc_profile(const plg_prop *up, const plg_prop *dn, const plg_prop *st, const plg_prop *ch,

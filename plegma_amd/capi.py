@@ -36,7 +36,9 @@ EXPORTS = ["plg_init", "plg_finalize", "plg_last_error", "plg_version", "plg_mom
            "plg_udsc_op_name", "plg_baryons_udsc_select", "plg_baryons_udsc_size", "plg_baryons_udsc",
            "plg_baryons_udsc_profile", "plg_baryons_table",
            "plg_tetraquark_table", "plg_tetra_count", "plg_tetra_product", "plg_tetra_op_name",
-           "plg_tetraquarks_select", "plg_tetraquarks_size", "plg_tetraquarks"]
+           "plg_tetraquarks_select", "plg_tetraquarks_size", "plg_tetraquarks",
+           "plg_tetra_bcud_count", "plg_tetra_bcud_product", "plg_tetra_bcud_op_name",
+           "plg_tetraquarks_bcud_select", "plg_tetraquarks_bcud_size", "plg_tetraquarks_bcud"]
 
 
 class PlgError(RuntimeError):
@@ -121,6 +123,12 @@ def lib():
                                            C.POINTER(C.c_int64)]
         L.plg_tetraquarks.argtypes = [pp, pp, pp, pp, C.c_int, C.c_int, ip, C.c_int, ip, C.c_int, C.c_int,
                                       C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        L.plg_tetra_bcud_count.argtypes = L.plg_tetra_count.argtypes
+        L.plg_tetra_bcud_product.argtypes = L.plg_tetra_product.argtypes
+        L.plg_tetra_bcud_op_name.argtypes = L.plg_tetra_op_name.argtypes
+        L.plg_tetraquarks_bcud_select.argtypes = L.plg_tetraquarks_select.argtypes
+        L.plg_tetraquarks_bcud_size.argtypes = L.plg_tetraquarks_size.argtypes
+        L.plg_tetraquarks_bcud.argtypes = L.plg_tetraquarks.argtypes
         L.plg_mlp_create.argtypes = [ip, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         L.plg_mlp_nparams.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         L.plg_mlp_set_params.argtypes = [C.c_void_p, fp]
@@ -306,48 +314,62 @@ def tetraquark_table(p1, p2, p3, p4, idxs, cols, vals, counts, source, totalT, m
                                      out_prec, C.c_void_p(out.data_ptr()), _stream(stream)))
 
 
-def tetra_products():
-    """[(flavour letters, [dataset names])] of the heavy-light tetraquark
-    products, in order"""
+_TETRA_FAMILY = {"heavy-light": "", "bcud": "_bcud"}
+
+
+def _tf(name, family):
+    """the C entry point of a tetraquark family: plg_tetra_count ->
+    plg_tetra_bcud_count, plg_tetraquarks -> plg_tetraquarks_bcud, ..."""
+    sfx = _TETRA_FAMILY[family]
+    if name.startswith("plg_tetraquarks"):
+        return getattr(lib(), "plg_tetraquarks" + sfx + name[len("plg_tetraquarks"):])
+    return getattr(lib(), "plg_tetra" + sfx + name[len("plg_tetra"):])
+
+
+def tetra_products(family="heavy-light"):
+    """[(flavour letters, [dataset names])] of the tetraquark products of a
+    family ("heavy-light" or "bcud"), in order"""
     n = C.c_int()
-    check(lib().plg_tetra_count(C.byref(n)))
+    check(_tf("plg_tetra_count", family)(C.byref(n)))
     out = []
     for p in range(n.value):
         fl = C.create_string_buffer(5)
         k = C.c_int()
-        check(lib().plg_tetra_product(p, fl, C.byref(k)))
+        check(_tf("plg_tetra_product", family)(p, fl, C.byref(k)))
         names = []
         for i in range(k.value):
             b = C.create_string_buffer(64)
-            check(lib().plg_tetra_op_name(p, i, b, 64))
+            check(_tf("plg_tetra_op_name", family)(p, i, b, 64))
             names.append(b.value.decode())
         out.append((fl.value.decode(), names))
     return out
 
 
-def tetraquarks_select(lt, st, ch, bt, only_st=False, only_ch=False):
+def tetraquarks_select(lt, st, ch, bt, only_st=False, only_ch=False, family="heavy-light"):
     """(product indices, number of datasets) that tetraquarks computes"""
     prods = (C.c_int * 32)()
     n, nd = C.c_int(), C.c_int()
-    check(lib().plg_tetraquarks_select(_ref(lt), _ref(st), _ref(ch), _ref(bt), int(only_st), int(only_ch),
-                                       prods, 32, C.byref(n), C.byref(nd)))
+    check(_tf("plg_tetraquarks_select", family)(_ref(lt), _ref(st), _ref(ch), _ref(bt), int(only_st),
+                                                int(only_ch), prods, 32, C.byref(n), C.byref(nd)))
     return list(prods[:n.value]), nd.value
 
 
-def tetraquarks_size(lt, st, ch, bt, source, totalT, nmom, space=MOMENTUM_SPACE, only_st=False, only_ch=False):
+def tetraquarks_size(lt, st, ch, bt, source, totalT, nmom, space=MOMENTUM_SPACE, only_st=False, only_ch=False,
+                     family="heavy-light"):
     n = C.c_int64()
-    check(lib().plg_tetraquarks_size(_ref(lt), _ref(st), _ref(ch), _ref(bt), int(only_st), int(only_ch),
-                                     ivec(source), totalT, nmom, space, C.byref(n)))
+    check(_tf("plg_tetraquarks_size", family)(_ref(lt), _ref(st), _ref(ch), _ref(bt), int(only_st), int(only_ch),
+                                              ivec(source), totalT, nmom, space, C.byref(n)))
     return n.value
 
 
 def tetraquarks(lt, st, ch, bt, source, totalT, moms, mode, space, out_prec, out, only_st=False, only_ch=False,
-                stream=None):
-    """contractTetraquarks: views lt/st/ch/bt (None = absent) -> out [dataset][j][m]"""
+                stream=None, family="heavy-light"):
+    """contractTetraquarks (family "heavy-light") or contractTetraquarksBCUD
+    (family "bcud"): views lt/st/ch/bt (None = absent) -> out [dataset][j][m]"""
     flat = [int(v) for m in moms for v in m]
-    check(lib().plg_tetraquarks(_ref(lt), _ref(st), _ref(ch), _ref(bt), int(only_st), int(only_ch),
-                                ivec(source), totalT, ivec(flat) if flat else None, len(moms), mode, space,
-                                out_prec, C.c_void_p(out.data_ptr()), _stream(stream)))
+    check(_tf("plg_tetraquarks", family)(_ref(lt), _ref(st), _ref(ch), _ref(bt), int(only_st), int(only_ch),
+                                         ivec(source), totalT, ivec(flat) if flat else None, len(moms), mode,
+                                         space, out_prec, C.c_void_p(out.data_ptr()), _stream(stream)))
 
 
 def baryons_udsc_size(up, dn, st, ch, source, totalT, nmom, space=MOMENTUM_SPACE, only_st=False, only_ch=False):

@@ -227,7 +227,7 @@ struct Plan {
   // udsc baryons: the flavour product's schedule (device arrays owned by the context)
   UdscSched udsc{};
   // four-propagator tables: terms (owned by the caller of the plan) and rows
-  TetraSched tetra{};
+  TetraSched tetra{}, tetra2{};  // the table's two term sets (tetra2: views P1, P2, P4, P3)
   std::vector<RowDev4> rows4;
   RowDev4 *d_rows4 = nullptr;
   // batch-owned workspace (single calls use the context workspace)
@@ -644,8 +644,9 @@ template <typename T> static int launch_contract_t(const Plan &P, const Launch &
     return launch_mesmes<T>(pd, P.d_rows, nrows, (cx<T> *)c, g_ctx.V3, g_ctx.L[3], P.nch_c, l.ch0, P.tpb, st);
   case Launch::UDSC:
     return launch_udsc<T>(P.udsc, pd, P.d_rows, nrows, (cx<T> *)c, g_ctx.V3, g_ctx.L[3], P.nch_c, l.ch0, P.tpb, st);
-  case Launch::TETRA:
-    return launch_tetra<T>(P.tetra, pd, P.d_rows4, nrows, (cx<T> *)c, g_ctx.V3, g_ctx.L[3], P.nch_c, P.tpb, st);
+  case Launch::TETRA:  // set 1 runs on the second half of the rows (views P1, P2, P4, P3)
+    return launch_tetra<T>(l.set ? P.tetra2 : P.tetra, pd, P.d_rows4 + (l.set ? nrows : 0), nrows, (cx<T> *)c,
+                           g_ctx.V3, g_ctx.L[3], P.nch_c, P.tpb, st);
   default:
     return launch_baryon<T>(l.two, l.mes, l.gmask, pd, P.d_rows, nrows, (cx<T> *)c, g_ctx.V3, g_ctx.L[3],
                             P.nch_c, l.ch0, P.tpb, st);
@@ -1435,15 +1436,14 @@ static int tetra_run(const plg_prop *const views[4], const int16_t *idxs, const 
   if (!idxs || !col_contr || !vals || !counts || nops <= 0) return fail(PLG_ERR_INVALID, "empty table");
   for (int i = 0; i < 4; i++)
     if (!views[i]) return fail(PLG_ERR_INVALID, "NULL propagator view");
-  std::vector<TetraTerm> terms;
-  std::vector<float> coef;
-  std::vector<int> first, count, ndirect;
-  rc = tetra_pack(idxs, col_contr, vals, counts, nops, terms, coef, first, count, ndirect);
+  TetraPacked pk;
+  rc = tetra_pack(idxs, col_contr, vals, counts, nops, pk);
   if (rc == PLG_ERR_INVALID) return fail(rc, "spin or colour index outside 0..3");
   if (rc)
-    return fail(rc, "table outside the supported form (32 operators; colour slots a permutation of the four "
-                    "sums, closing as <P1,P3><P2,P4> or Tr[(P3^dag P1)(P4^dag P2)]; 65536 distinct "
-                    "coefficients)");
+    return fail(rc, "table outside the supported form (64 operators; colour slots a permutation of the four "
+                    "sums, closing as <P1,P3><P2,P4>, Tr[(P3^dag P1)(P4^dag P2)], <P1,P4><P2,P3> or "
+                    "Tr[(P4^dag P1)(P3^dag P2)]; 65536 distinct coefficients)");
+  const bool two_sets = !pk.terms[1].empty();
   int lt;
   if ((rc = plg_window(source, totalT, nullptr, nullptr, &lt))) return rc;
   const Window w = window_of(g_ctx.L, source, totalT);
@@ -1469,6 +1469,7 @@ static int tetra_run(const plg_prop *const views[4], const int16_t *idxs, const 
     P.rows.push_back({t, 0, 1, 0, 2});
     P.rows4.push_back({t, {0, 1, 2, 3}});
   }
+  for (int j = 0; j < lt; j++) P.rows4.push_back({P.rows[j].t, {0, 1, 3, 2}});  // set 1's views
   for (int k = 0; k < nops; k++)
     for (int j = 0; j < lt; j++) {
       const int t = P.rows[j].t;
@@ -1491,23 +1492,35 @@ static int tetra_run(const plg_prop *const views[4], const int16_t *idxs, const 
   P.m_stride = 1;
   P.out_elems = (int64_t)(d0 + nops) * lt * inner;
   P.nsites = (int64_t)lt * g_ctx.V3;
+  // set 0 writes every operator (zero where it has no terms), set 1 adds
   P.launches.push_back({Launch::TETRA, 0, false, false, 0, 0});
-  TetraTerm *d_terms = nullptr;
+  if (two_sets) P.launches.push_back({Launch::TETRA, 1, false, false, 0, 0});
+  TetraTerm *d_terms[2] = {nullptr, nullptr};
   float *d_coef = nullptr;
-  if ((rc = upload(&d_terms, terms, hs))) return rc;
-  if ((rc = upload(&d_coef, coef, hs))) {
-    hipFree(d_terms);
+  auto free_tables = [&] {
+    for (TetraTerm *p : d_terms) hipFree(p);
+    hipFree(d_coef);
+  };
+  for (int h = 0; h < (two_sets ? 2 : 1) && !rc; h++) rc = upload(&d_terms[h], pk.terms[h], hs);
+  if (!rc) rc = upload(&d_coef, pk.coef, hs);
+  if (rc) {
+    hipStreamSynchronize(hs);
+    free_tables();
     return rc;
   }
-  P.tetra.ncoef = (int)coef.size() / 2;
-  P.tetra.nops = nops;
-  for (int k = 0; k < nops; k++) {
-    P.tetra.first[k] = first[k];
-    P.tetra.count[k] = count[k];
-    P.tetra.ndirect[k] = ndirect[k];
+  for (int h = 0; h < 2; h++) {
+    TetraSched &S = h ? P.tetra2 : P.tetra;
+    S.ncoef = (int)pk.coef.size() / 2;
+    S.nops = nops;
+    for (int k = 0; k < nops; k++) {
+      S.first[k] = pk.first[h][k];
+      S.count[k] = pk.count[h][k];
+      S.ndirect[k] = pk.ndirect[h][k];
+    }
+    S.terms = d_terms[h];
+    S.coef = d_coef;
+    S.accumulate = h;
   }
-  P.tetra.terms = d_terms;
-  P.tetra.coef = d_coef;
   choose_tiles(P);
   P.tpb = 64;  // sites per workgroup of the kernel
   if (!(rc = upload_plan(P, hs)) && !(rc = upload(&P.d_rows4, P.rows4, hs))) {
@@ -1522,8 +1535,7 @@ static int tetra_run(const plg_prop *const views[4], const int16_t *idxs, const 
     if (!rc && se != hipSuccess) rc = fail(PLG_ERR_HIP, "tetraquark table sync failed");
   }
   hipStreamSynchronize(hs);  // no launch may still read the terms
-  hipFree(d_terms);
-  hipFree(d_coef);
+  free_tables();
   return rc;
 }
 
@@ -1536,18 +1548,32 @@ int plg_tetraquark_table(const plg_prop *p1, const plg_prop *p2, const plg_prop 
                    out_precision, out, stream);
 }
 
-// ------------------------------------------------------------------ heavy-light tetraquarks
-// lib/PLEGMA_Correlator.cu:192-250 (product selection, datasets) and
-// lib/kernels/PLEGMA_heavy_light_tetraquarks.cu:163-212 (one product after the
-// other, propagator per flavour letter: u, d -> LT, s -> ST, c -> CH, b -> BT)
+// ------------------------------------------------------------------ built-in tetraquark tables
+// heavy-light: lib/PLEGMA_Correlator.cu:192-250 (product selection, datasets)
+// and lib/kernels/PLEGMA_heavy_light_tetraquarks.cu:163-212; bcud:
+// lib/PLEGMA_Correlator.cu:252-310 and lib/kernels/PLEGMA_bcud_tetraquarks.cu:
+// 159-212.  One product after the other, propagator per flavour letter:
+// u, d -> LT, s -> ST, c -> CH, b -> BT.
 static const char TT_LETTERS[5] = {'u', 'd', 's', 'c', 'b'};
 static const int TT_VIEW_OF[5] = {0, 0, 1, 2, 3};  // letter -> LT, ST, CH, BT argument
 
-static void tetra_select(const plg_prop *const fl[4], int only_st, int only_ch, std::vector<int> &prods, int &nd) {
+struct TetraSet {
+  int nprod;
+  const char *const *prod;
+  const int *nops;
+  const char *const *const *names;
+  const int16_t *const *count, *const *idx, *const *col;
+  const float *const *val;
+};
+static const TetraSet TETRA_HL = {TT_NPROD, TT_PROD, TT_NOPS, TT_NAMES, TT_COUNT, TT_IDX, TT_COL, TT_VAL};
+static const TetraSet TETRA_BCUD = {TB_NPROD, TB_PROD, TB_NOPS, TB_NAMES, TB_COUNT, TB_IDX, TB_COL, TB_VAL};
+
+static void tetra_select(const TetraSet &ts, const plg_prop *const fl[4], int only_st, int only_ch,
+                         std::vector<int> &prods, int &nd) {
   prods.clear();
   nd = 0;
-  for (int i = 0; i < TT_NPROD; i++) {
-    const std::string f = TT_PROD[i];
+  for (int i = 0; i < ts.nprod; i++) {
+    const std::string f = ts.prod[i];
     bool ok = true;
     for (int k = 0; k < 5; k++)
       if (!fl[TT_VIEW_OF[k]] && f.find(TT_LETTERS[k]) != std::string::npos) ok = false;
@@ -1555,39 +1581,40 @@ static void tetra_select(const plg_prop *const fl[4], int only_st, int only_ch, 
     if (only_ch && f.find('c') == std::string::npos) ok = false;
     if (ok) {
       prods.push_back(i);
-      nd += TT_NOPS[i];
+      nd += ts.nops[i];
     }
   }
 }
 
-int plg_tetra_count(int *nprod) {
+static int tetra_count(const TetraSet &ts, int *nprod) {
   if (!nprod) return fail(PLG_ERR_INVALID, "NULL argument");
-  *nprod = TT_NPROD;
+  *nprod = ts.nprod;
   return PLG_OK;
 }
 
-int plg_tetra_product(int prod, char *flavours, int *nops) {
-  if (prod < 0 || prod >= TT_NPROD) return fail(PLG_ERR_INVALID, "no such flavour product");
-  if (flavours) std::memcpy(flavours, TT_PROD[prod], 5);
-  if (nops) *nops = TT_NOPS[prod];
+static int tetra_product(const TetraSet &ts, int prod, char *flavours, int *nops) {
+  if (prod < 0 || prod >= ts.nprod) return fail(PLG_ERR_INVALID, "no such flavour product");
+  if (flavours) std::memcpy(flavours, ts.prod[prod], 5);
+  if (nops) *nops = ts.nops[prod];
   return PLG_OK;
 }
 
-int plg_tetra_op_name(int prod, int k, char *buf, int cap) {
-  if (prod < 0 || prod >= TT_NPROD || k < 0 || k >= TT_NOPS[prod] || !buf || cap <= 0)
+static int tetra_op_name(const TetraSet &ts, int prod, int k, char *buf, int cap) {
+  if (prod < 0 || prod >= ts.nprod || k < 0 || k >= ts.nops[prod] || !buf || cap <= 0)
     return fail(PLG_ERR_INVALID, "bad product, operator or buffer");
-  const char *n = TT_NAMES[prod][k];
+  const char *n = ts.names[prod][k];
   if ((int)std::strlen(n) >= cap) return fail(PLG_ERR_RANGE, "name buffer too small");
   std::strcpy(buf, n);
   return PLG_OK;
 }
 
-int plg_tetraquarks_select(const plg_prop *lt, const plg_prop *st, const plg_prop *ch, const plg_prop *bt,
-                           int only_st, int only_ch, int *prods, int cap, int *nprods, int *ndatasets) {
+static int tetra_select_api(const TetraSet &ts, const plg_prop *lt, const plg_prop *st, const plg_prop *ch,
+                            const plg_prop *bt, int only_st, int only_ch, int *prods, int cap, int *nprods,
+                            int *ndatasets) {
   const plg_prop *fl[4] = {lt, st, ch, bt};
   std::vector<int> v;
   int nd;
-  tetra_select(fl, only_st, only_ch, v, nd);
+  tetra_select(ts, fl, only_st, only_ch, v, nd);
   if (prods && (int)v.size() > cap) return fail(PLG_ERR_RANGE, "product buffer too small");
   if (prods) std::copy(v.begin(), v.end(), prods);
   if (nprods) *nprods = (int)v.size();
@@ -1595,39 +1622,88 @@ int plg_tetraquarks_select(const plg_prop *lt, const plg_prop *st, const plg_pro
   return PLG_OK;
 }
 
-int plg_tetraquarks_size(const plg_prop *lt, const plg_prop *st, const plg_prop *ch, const plg_prop *bt,
-                         int only_st, int only_ch, const int source[4], int totalT, int nmom, int space,
-                         int64_t *elems) {
+static int tetra_size(const TetraSet &ts, const plg_prop *lt, const plg_prop *st, const plg_prop *ch,
+                      const plg_prop *bt, int only_st, int only_ch, const int source[4], int totalT, int nmom,
+                      int space, int64_t *elems) {
   int ltT, nd, rc;
   if (!elems) return fail(PLG_ERR_INVALID, "NULL argument");
   if ((rc = plg_window(source, totalT, nullptr, nullptr, &ltT))) return rc;
   const plg_prop *fl[4] = {lt, st, ch, bt};
   std::vector<int> v;
-  tetra_select(fl, only_st, only_ch, v, nd);
+  tetra_select(ts, fl, only_st, only_ch, v, nd);
   *elems = (int64_t)nd * ltT * (space == PLG_MOMENTUM_SPACE ? nmom : g_ctx.V3);
   return PLG_OK;
 }
 
-int plg_tetraquarks(const plg_prop *lt, const plg_prop *st, const plg_prop *ch, const plg_prop *bt, int only_st,
-                    int only_ch, const int source[4], int totalT, const int *moms, int nmom, int mode, int space,
-                    int out_precision, void *out, void *stream) {
+static int tetra_products(const TetraSet &ts, const plg_prop *lt, const plg_prop *st, const plg_prop *ch,
+                          const plg_prop *bt, int only_st, int only_ch, const int source[4], int totalT,
+                          const int *moms, int nmom, int mode, int space, int out_precision, void *out,
+                          void *stream) {
   int rc;
   if ((rc = udsc_check_args(out, source, out_precision, mode, space, moms, nmom))) return rc;
   const plg_prop *fl[4] = {lt, st, ch, bt};
   std::vector<int> prods;
   int nd_total;
-  tetra_select(fl, only_st, only_ch, prods, nd_total);
+  tetra_select(ts, fl, only_st, only_ch, prods, nd_total);
   int d0 = 0;
   for (int prod : prods) {
-    const char *f = TT_PROD[prod];
+    const char *f = ts.prod[prod];
     const plg_prop *views[4];
     for (int i = 0; i < 4; i++) views[i] = fl[TT_VIEW_OF[std::find(TT_LETTERS, TT_LETTERS + 5, f[i]) - TT_LETTERS]];
-    if ((rc = tetra_run(views, TT_IDX[prod], TT_COL[prod], TT_VAL[prod], TT_COUNT[prod], TT_NOPS[prod], d0, source,
+    if ((rc = tetra_run(views, ts.idx[prod], ts.col[prod], ts.val[prod], ts.count[prod], ts.nops[prod], d0, source,
                         totalT, moms, nmom, mode, space, out_precision, out, stream)))
       return rc;
-    d0 += TT_NOPS[prod];
+    d0 += ts.nops[prod];
   }
   return PLG_OK;
+}
+
+int plg_tetra_count(int *nprod) { return tetra_count(TETRA_HL, nprod); }
+int plg_tetra_product(int prod, char *flavours, int *nops) { return tetra_product(TETRA_HL, prod, flavours, nops); }
+int plg_tetra_op_name(int prod, int k, char *buf, int cap) { return tetra_op_name(TETRA_HL, prod, k, buf, cap); }
+
+int plg_tetraquarks_select(const plg_prop *lt, const plg_prop *st, const plg_prop *ch, const plg_prop *bt,
+                           int only_st, int only_ch, int *prods, int cap, int *nprods, int *ndatasets) {
+  return tetra_select_api(TETRA_HL, lt, st, ch, bt, only_st, only_ch, prods, cap, nprods, ndatasets);
+}
+
+int plg_tetraquarks_size(const plg_prop *lt, const plg_prop *st, const plg_prop *ch, const plg_prop *bt,
+                         int only_st, int only_ch, const int source[4], int totalT, int nmom, int space,
+                         int64_t *elems) {
+  return tetra_size(TETRA_HL, lt, st, ch, bt, only_st, only_ch, source, totalT, nmom, space, elems);
+}
+
+int plg_tetraquarks(const plg_prop *lt, const plg_prop *st, const plg_prop *ch, const plg_prop *bt, int only_st,
+                    int only_ch, const int source[4], int totalT, const int *moms, int nmom, int mode, int space,
+                    int out_precision, void *out, void *stream) {
+  return tetra_products(TETRA_HL, lt, st, ch, bt, only_st, only_ch, source, totalT, moms, nmom, mode, space,
+                        out_precision, out, stream);
+}
+
+int plg_tetra_bcud_count(int *nprod) { return tetra_count(TETRA_BCUD, nprod); }
+int plg_tetra_bcud_product(int prod, char *flavours, int *nops) {
+  return tetra_product(TETRA_BCUD, prod, flavours, nops);
+}
+int plg_tetra_bcud_op_name(int prod, int k, char *buf, int cap) {
+  return tetra_op_name(TETRA_BCUD, prod, k, buf, cap);
+}
+
+int plg_tetraquarks_bcud_select(const plg_prop *lt, const plg_prop *st, const plg_prop *ch, const plg_prop *bt,
+                                int only_st, int only_ch, int *prods, int cap, int *nprods, int *ndatasets) {
+  return tetra_select_api(TETRA_BCUD, lt, st, ch, bt, only_st, only_ch, prods, cap, nprods, ndatasets);
+}
+
+int plg_tetraquarks_bcud_size(const plg_prop *lt, const plg_prop *st, const plg_prop *ch, const plg_prop *bt,
+                              int only_st, int only_ch, const int source[4], int totalT, int nmom, int space,
+                              int64_t *elems) {
+  return tetra_size(TETRA_BCUD, lt, st, ch, bt, only_st, only_ch, source, totalT, nmom, space, elems);
+}
+
+int plg_tetraquarks_bcud(const plg_prop *lt, const plg_prop *st, const plg_prop *ch, const plg_prop *bt,
+                         int only_st, int only_ch, const int source[4], int totalT, const int *moms, int nmom,
+                         int mode, int space, int out_precision, void *out, void *stream) {
+  return tetra_products(TETRA_BCUD, lt, st, ch, bt, only_st, only_ch, source, totalT, moms, nmom, mode, space,
+                        out_precision, out, stream);
 }
 
 int plg_baryons_table(const plg_prop *p1, const plg_prop *p2, const plg_prop *p3, const int16_t *idxs,

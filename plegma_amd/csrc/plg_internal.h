@@ -107,7 +107,7 @@ struct UdscTerm {
 int udsc_schedule_terms(const UdscTerm *terms, const int *first, const int *count, int nops, UdscSchedHost &h);
 
 // four-propagator table contractions (tetraquark table format)
-constexpr int TETRA_MAXOPS = 32;
+constexpr int TETRA_MAXOPS = 64;
 // A term, 32 bits: the (P1, P3) spin pair p = 16 s1 + s3 (bits 0-7), the
 // (P2, P4) pair q = 16 s2 + s4 (bits 8-15) and the coefficient index (bits
 // 16-31), with the 4-bit spin pairs s1 = 4 i0 + i1, s2 = 4 i2 + i3,
@@ -122,14 +122,23 @@ struct TetraSched {
   const TetraTerm *terms;
   const float *coef;  // re, im of each distinct coefficient
   int ncoef;
+  int accumulate;     // add the operators to the site correlator instead of writing them
 };
 struct RowDev4 {
   int t;
   int p[4];
 };
+// The terms of a table in two sets.  Set 0 pairs the colours of P1 with P3
+// and of P2 with P4 (the two structures above); set 1 pairs P1 with P4 and
+// P2 with P3 (<P1,P4><P2,P3> and Tr[(P4^dag P1)(P3^dag P2)]), which are the
+// same structures on the views P1, P2, P4, P3, and is packed for those views.
+struct TetraPacked {
+  std::vector<TetraTerm> terms[2];
+  std::vector<int> first[2], count[2], ndirect[2];
+  std::vector<float> coef;  // shared by both sets
+};
 int tetra_pack(const int16_t *idxs, const int16_t *cols, const float *vals, const int16_t *counts, int nops,
-               std::vector<TetraTerm> &terms, std::vector<float> &coef, std::vector<int> &first,
-               std::vector<int> &count, std::vector<int> &ndirect);
+               TetraPacked &out);
 template <typename T>
 int launch_tetra(const TetraSched &S, const PropDev<T> *props, const RowDev4 *rows, int nrows, cx<T> *out,
                  int64_t V3, int T_lat, int nch_total, int spw, hipStream_t st);

@@ -7,10 +7,13 @@
 // per term).
 //
 // The reference sums the 81 colour assignments of every term.  Every term of
-// the reference's tables has one of two colour structures, which close into
+// its heavy-light tables has one of two colour structures, which close into
 // colour traces of the 3x3 blocks B1 = P1[i0][i1], ..., B4 = P4[i7+2][i6+2]:
 //   direct  (c0 = c5, c1 = c4, c2 = c7, c3 = c6):  <B1, B3> <B2, B4>,   <X, Y> = sum X conj(Y)
 //   crossed (c0 = c5, c1 = c6, c2 = c7, c3 = c4):  Tr[(B3^dag B1)(B4^dag B2)]
+// Terms that pair P1 with P4 and P2 with P3 instead (the bcud tables) close
+// into the same two structures on the views P1, P2, P4, P3; the host packs them
+// as a second set, contracted by a second launch that adds to the first.
 // The host classifies every term; a table with another structure is refused.
 //
 // Both structures factor through the spin pairs p = (B1, B3) and
@@ -162,7 +165,12 @@ __global__ __launch_bounds__(256) void k_tetra(TetraSched S, const PropDev<T> *_
           acc.re += __shfl_xor(acc.re, off);
           acc.im += __shfl_xor(acc.im, off);
         }
-        if (lane == 0) gstore(out + ctile_idx(col0 + k, x, ncols), cmake<T>(sg * acc.re, sg * acc.im));
+        if (lane == 0) {
+          cx<T> *o = out + ctile_idx(col0 + k, x, ncols);
+          cx<T> v = cmake<T>(sg * acc.re, sg * acc.im);
+          if (S.accumulate) v = v + gload(o);  // the set written by the previous launch
+          gstore(o, v);
+        }
       };
       // wave w takes the operators k = w, w + 4, ...; its lanes split an
       // operator's terms, TETRA_TB terms per lane loaded together, the next
@@ -255,23 +263,22 @@ static void spread_banks(std::vector<TetraTerm> &v) {
 
 // host: classify and pack the terms of a table (reference format)
 int tetra_pack(const int16_t *idxs, const int16_t *cols, const float *vals, const int16_t *counts, int nops,
-               std::vector<TetraTerm> &terms, std::vector<float> &coef, std::vector<int> &first,
-               std::vector<int> &count, std::vector<int> &ndirect) {
+               TetraPacked &out) {
   if (nops <= 0 || nops > TETRA_MAXOPS) return PLG_ERR_UNSUPPORTED;
-  terms.clear();
-  coef.clear();
-  first.assign(nops, 0);
-  count.assign(nops, 0);
-  ndirect.assign(nops, 0);
+  out = TetraPacked{};
+  for (int h = 0; h < 2; h++) {
+    out.first[h].assign(nops, 0);
+    out.count[h].assign(nops, 0);
+    out.ndirect[h].assign(nops, 0);
+  }
   std::map<std::pair<uint32_t, uint32_t>, uint32_t> cidx;  // coefficient bits -> index
   cidx.emplace(std::make_pair(0u, 0u), 0u);                 // entry 0: zero
-  coef.push_back(0.0f);
-  coef.push_back(0.0f);
+  out.coef.push_back(0.0f);
+  out.coef.push_back(0.0f);
   size_t t = 0;
   for (int k = 0; k < nops; k++) {
     if (counts[k] < 0) return PLG_ERR_INVALID;
-    first[k] = (int)terms.size();
-    std::vector<TetraTerm> direct, crossed;
+    std::vector<TetraTerm> direct[2], crossed[2];
     for (int i = 0; i < counts[k]; i++, t++) {
       const int16_t *m = idxs + 8 * t, *c = cols + 8 * t;
       for (int j = 0; j < 8; j++)
@@ -293,20 +300,29 @@ int tetra_pack(const int16_t *idxs, const int16_t *cols, const float *vals, cons
         ci = (uint32_t)cidx.size();
         if (ci >= (uint32_t)TETRA_MAXCOEF) return PLG_ERR_UNSUPPORTED;
         cidx.emplace(std::make_pair(re, im), ci);
-        coef.push_back(vals[2 * t]);
-        coef.push_back(vals[2 * t + 1]);
+        out.coef.push_back(vals[2 * t]);
+        out.coef.push_back(vals[2 * t + 1]);
       }
-      const TetraTerm u = (s1 * 16 + s3) | ((s2 * 16 + s4) << 8) | (ci << 16);
-      if (c[0] == c[5] && c[1] == c[4] && c[2] == c[7] && c[3] == c[6]) direct.push_back(u);
-      else if (c[0] == c[5] && c[1] == c[6] && c[2] == c[7] && c[3] == c[4]) crossed.push_back(u);
+      // set 0 on (P1, P2, P3, P4); set 1 on (P1, P2, P4, P3): P4's spin pair
+      // takes the third place and P3's the fourth
+      const TetraTerm u0 = (s1 * 16 + s3) | ((s2 * 16 + s4) << 8) | (ci << 16);
+      const TetraTerm u1 = (s1 * 16 + s4) | ((s2 * 16 + s3) << 8) | (ci << 16);
+      if (c[0] == c[5] && c[1] == c[4] && c[2] == c[7] && c[3] == c[6]) direct[0].push_back(u0);
+      else if (c[0] == c[5] && c[1] == c[6] && c[2] == c[7] && c[3] == c[4]) crossed[0].push_back(u0);
+      else if (c[0] == c[7] && c[1] == c[6] && c[2] == c[5] && c[3] == c[4]) direct[1].push_back(u1);
+      else if (c[0] == c[7] && c[1] == c[4] && c[2] == c[5] && c[3] == c[6]) crossed[1].push_back(u1);
       else return PLG_ERR_UNSUPPORTED;
     }
-    spread_banks(direct);
-    spread_banks(crossed);
-    terms.insert(terms.end(), direct.begin(), direct.end());
-    terms.insert(terms.end(), crossed.begin(), crossed.end());
-    ndirect[k] = (int)direct.size();
-    count[k] = (int)terms.size() - first[k];
+    for (int h = 0; h < 2; h++) {
+      std::vector<TetraTerm> &terms = out.terms[h];
+      out.first[h][k] = (int)terms.size();
+      spread_banks(direct[h]);
+      spread_banks(crossed[h]);
+      terms.insert(terms.end(), direct[h].begin(), direct[h].end());
+      terms.insert(terms.end(), crossed[h].begin(), crossed[h].end());
+      out.ndirect[h][k] = (int)direct[h].size();
+      out.count[h][k] = (int)terms.size() - out.first[h][k];
+    }
   }
   return PLG_OK;
 }

@@ -1,7 +1,11 @@
-"""Contraction tables of the heavy-light tetraquark two-point functions,
-derived from the interpolating fields (the reference ships the same tables as
-generated data: lib/kernels/PLEGMA_heavy_light_tetraquarks_arrays.cu, made by
-python/tetraquarks_v3.py).
+"""Contraction tables of the tetraquark two-point functions, derived from the
+interpolating fields (the reference ships the same tables as generated data):
+- heavy-light bbud / bbus: lib/kernels/PLEGMA_heavy_light_tetraquarks_arrays.cu,
+  made by python/tetraquarks_v3.py;
+- bcud (J = 1 and J = 0, flavour content (bbar u cbar d) - (bbar d cbar u)):
+  lib/kernels/PLEGMA_bcud_tetraquarks_arrays.cu, made by
+  python/tetraquarks_v4.py (operator list :105-545, flavour contents
+  :549-558, table assembly :1100-1235).
 
 A tetraquark (bbar G1 q1)(bbar G2 q2) -- "meson" form -- or its
 diquark-antidiquark regrouping (bbar bbar)(q1 q2) -- "diquark" form -- at the
@@ -21,6 +25,11 @@ Operator list and gamma tuples: python/tetraquarks_v3.py:31-446; flavour
 contents: :451-462; Wick rule and canonical ("default") view: :519-628;
 colour slots: :795-842; table assembly: :1040-1120.  Coefficients are stored as
 float32, as the reference's float2 arrays.
+
+A field with several flavour orderings is their sum with the given
+coefficients; <J Jbar> then sums the Wick terms of every (sink ordering,
+source ordering) pair, sink-major, each with the product of the two
+coefficients (the order the bcud generator visits them).
 
 Run as a script to write the C header that the library compiles in:
     python -m plegma_amd.gen_tetra plegma_amd/csrc/plg_tetra_tables.h
@@ -102,8 +111,109 @@ def _diquark_diquark():
 OPERATORS = [("meson", "meson", _meson_meson()), ("meson", "diquark", _meson_diquark()),
              ("diquark", "meson", _diquark_meson()), ("diquark", "diquark", _diquark_diquark())]
 
-# tetraquark name -> flavour string of its meson-meson sink field (bbar u bbar d)
-TETRAQUARKS = [("bbud", "bt^-up-bt^-dn"), ("bbus", "bt^-up-bt^-sr")]
+
+# ---------------------------------------------------------------- bcud operators
+# The bcud generator's spin-1 set: every spin-1 form summed over j = x, y, z
+# (no suffix), over the (x, y) pairing (suffix _12) and for z alone (suffix _z);
+# an odd permutation of the epsilon tensor enters as a minus sign on one gamma.
+def _bcud_j1():
+    x, y, z = GX, GY, GZ
+    d = _d
+    mm, md, dm, dd = {}, {}, {}, {}
+    for a, b in (("g5", "gj"), ("gj", "g5")):
+        for c, e in (("g5", "gj"), ("gj", "g5")):
+            def tup(j, k, a=a, b=b, c=c, e=e):
+                G1 = G5 if a == "g5" else j
+                G2 = G5 if b == "g5" else j
+                G3 = G5 if c == "g5" else k
+                G4 = G5 if e == "g5" else k
+                return (G1, G2, d(G3), d(G4))
+            base = f"{a}-{b}-{c}-{e}_ll"
+            mm[base] = [tup(g, g) for g in (x, y, z)]
+            mm[base + "_12"] = [tup(x, y), tup(y, x)]
+            mm[base.replace("gj", "g3") + "_z"] = [tup(z, z)]
+    m5 = -G5
+    mm["gi-gj-g5-gk_ll"] = [(y, z, d(G5), d(x)), (z, y, d(m5), d(x)), (x, z, d(m5), d(y)),
+                            (z, x, d(G5), d(y)), (x, y, d(G5), d(z)), (y, x, d(m5), d(z))]
+    mm["gi-gj-g5-gk_ll_12"] = [(y, z, d(G5), d(y)), (z, y, d(m5), d(y)), (x, z, d(m5), d(x)),
+                               (z, x, d(G5), d(x))]
+    mm["gi-gj-g5-gk_ll_z"] = [(x, y, d(G5), d(z)), (y, x, d(m5), d(z))]
+    mm["gi-gj-gk-g5_ll"] = [(y, z, d(x), d(G5)), (z, y, d(x), d(m5)), (x, z, d(y), d(m5)),
+                            (z, x, d(y), d(G5)), (x, y, d(z), d(G5)), (y, x, d(z), d(m5))]
+    mm["gi-gj-gk-g5_ll_12"] = [(y, z, d(y), d(G5)), (z, y, d(y), d(m5)), (x, z, d(x), d(m5)),
+                               (z, x, d(x), d(G5))]
+    mm["gi-gj-gk-g5_ll_z"] = [(x, y, d(z), d(G5)), (y, x, d(z), d(m5))]
+    mm["g5-gi-gj-gk_ll"] = [(G5, x, d(y), d(z)), (m5, x, d(z), d(y)), (m5, y, d(x), d(z)),
+                            (G5, y, d(z), d(x)), (G5, z, d(x), d(y)), (m5, z, d(y), d(x))]
+    mm["g5-gi-gj-gk_ll_12"] = [(m5, x, d(x), d(z)), (G5, x, d(z), d(x)), (G5, y, d(y), d(z)),
+                               (m5, y, d(z), d(y))]
+    mm["g5-gi-gj-gk_ll_z"] = [(G5, z, d(x), d(y)), (m5, z, d(y), d(x))]
+    mm["gi-g5-gj-gk_ll"] = [(x, G5, d(y), d(z)), (x, m5, d(z), d(y)), (y, m5, d(x), d(z)),
+                            (y, G5, d(z), d(x)), (z, G5, d(x), d(y)), (z, m5, d(y), d(x))]
+    mm["gi-g5-gj-gk_ll_12"] = [(x, m5, d(x), d(z)), (x, G5, d(z), d(x)), (y, G5, d(y), d(z)),
+                               (y, m5, d(z), d(y))]
+    mm["gi-g5-gj-gk_ll_z"] = [(z, G5, d(x), d(y)), (z, m5, d(y), d(x))]
+    mm["gi-gj-gi-gj_ll"] = [(y, z, d(y), d(z)), (-y, z, d(z), d(y)), (z, y, d(z), d(y)), (-z, y, d(y), d(z)),
+                            (x, z, d(x), d(z)), (-x, z, d(z), d(x)), (z, x, d(z), d(x)), (-z, x, d(x), d(z)),
+                            (x, y, d(x), d(y)), (-x, y, d(y), d(x)), (y, x, d(y), d(x)), (-y, x, d(x), d(y))]
+    mm["gi-gj-gi-gj_ll_12"] = [(y, z, d(z), d(x)), (-y, z, d(x), d(z)), (-z, y, d(z), d(x)), (z, y, d(x), d(z)),
+                               (z, x, d(y), d(z)), (-z, x, d(z), d(y)), (-x, z, d(y), d(z)), (x, z, d(z), d(y))]
+    mm["gi-gj-gi-gj_ll_z"] = [(x, y, d(x), d(y)), (-x, y, d(y), d(x)), (y, x, d(y), d(x)), (-y, x, d(x), d(y))]
+    # g4 (g_k C)^dag g4 and -C g5 close the diquark of the source
+    for a, b in (("g5", "gj"), ("gj", "g5")):
+        def tup(j, k, a=a, b=b):
+            return (G5 if a == "g5" else GI[j], G5 if b == "g5" else GI[j], BARC[k], -CG5)
+        base = f"{a}-{b}-gjC-Cg5_ll"
+        md[base] = [tup(k, k) for k in "xyz"]
+        md[base + "_12"] = [tup("x", "y"), tup("y", "x")]
+        md[base.replace("gj", "g3") + "_z"] = [tup("z", "z")]
+    md["gi-gj-gkC-Cg5_ll"] = [(y, z, BARC["x"], -CG5), (z, -y, BARC["x"], -CG5), (x, -z, BARC["y"], -CG5),
+                              (z, x, BARC["y"], -CG5), (x, y, BARC["z"], -CG5), (y, -x, BARC["z"], -CG5)]
+    md["gi-gj-gkC-Cg5_ll_12"] = [(y, z, BARC["y"], -CG5), (z, -y, BARC["y"], -CG5), (x, -z, BARC["x"], -CG5),
+                                 (z, x, BARC["x"], -CG5)]
+    md["gi-gj-gkC-Cg5_ll_z"] = [(x, y, BARC["z"], -CG5), (y, -x, BARC["z"], -CG5)]
+    for c, e in (("g5", "gj"), ("gj", "g5")):
+        def tup(j, k, c=c, e=e):
+            return (GIC[j], CG5, d(G5 if c == "g5" else GI[k]), d(G5 if e == "g5" else GI[k]))
+        base = f"gjC-Cg5-{c}-{e}_ll"
+        dm[base] = [tup(k, k) for k in "xyz"]
+        dm[base + "_12"] = [tup("x", "y"), tup("y", "x")]
+        dm[base.replace("gj", "g3") + "_z"] = [tup("z", "z")]
+    dm["giC-Cg5-gj-gk_ll"] = [(GIC["x"], CG5, d(y), d(z)), (GIC["x"], CG5, d(-z), d(y)),
+                              (GIC["y"], CG5, d(-x), d(z)), (GIC["y"], CG5, d(z), d(x)),
+                              (GIC["z"], CG5, d(x), d(y)), (GIC["z"], CG5, d(-y), d(x))]
+    dm["giC-Cg5-gj-gk_ll_12"] = [(GIC["x"], CG5, d(z), d(x)), (GIC["x"], CG5, d(-x), d(z)),
+                                 (GIC["y"], CG5, d(y), d(z)), (GIC["y"], CG5, d(-z), d(y))]
+    dm["giC-Cg5-gj-gk_ll_z"] = [(GIC["z"], CG5, d(x), d(y)), (GIC["z"], CG5, d(-y), d(x))]
+    dd["gjC-Cg5-gjC-Cg5_ll"] = [(GIC[k], CG5, BARC[k], -CG5) for k in "xyz"]
+    dd["gjC-Cg5-gjC-Cg5_ll_12"] = [(GIC["x"], CG5, BARC["y"], -CG5), (GIC["y"], CG5, BARC["x"], -CG5)]
+    dd["g3C-Cg5-g3C-Cg5_ll_z"] = [(GIC["z"], CG5, BARC["z"], -CG5)]
+    return [("meson", "meson", mm), ("meson", "diquark", md), ("diquark", "meson", dm), ("diquark", "diquark", dd)]
+
+
+def _bcud_j0():
+    """the bcud generator's spin-0 set"""
+    d = _d
+    G5C = G5 @ C
+    BAR5 = GT @ G5C.conj().T @ GT       # g4 (g5 C)^dag g4
+    gs = (GX, GY, GZ)
+    mm = {"g5-g5-g5-g5_ll": [(G5, G5, d(G5), d(G5))],
+          "g5-g5-gj-gj_ll": [(G5, G5, d(g), d(g)) for g in gs],
+          "gj-gj-g5-g5_ll": [(g, g, d(G5), d(G5)) for g in gs],
+          "gj-gj-gk-gk_ll": [(j, j, d(k), d(k)) for k in gs for j in gs]}
+    md = {"g5-g5-g5C-Cg5_ll": [(G5, G5, BAR5, -CG5)],
+          "gj-gj-g5C-Cg5_ll": [(g, g, BAR5, -CG5) for g in gs]}
+    dm = {"g5C-Cg5-g5-g5_ll": [(G5C, CG5, d(G5), d(G5))],
+          "g5C-Cg5-gj-gj_ll": [(G5C, CG5, d(g), d(g)) for g in gs]}
+    dd = {"g5C-Cg5-g5C-Cg5_ll": [(G5C, CG5, BAR5, -CG5)]}
+    return [("meson", "meson", mm), ("meson", "diquark", md), ("diquark", "meson", dm), ("diquark", "diquark", dd)]
+
+
+# tetraquark name, flavour orderings of its meson-meson sink field and their
+# coefficients, operator sets
+TETRAQUARKS = [("bbud", ["bt^-up-bt^-dn"], [1], OPERATORS), ("bbus", ["bt^-up-bt^-sr"], [1], OPERATORS)]
+BCUD = [("bcud_J1", ["bt^-up-ch^-dn", "bt^-dn-ch^-up"], [1, -1], _bcud_j1()),
+        ("bcud_J0", ["bt^-up-ch^-dn", "bt^-dn-ch^-up"], [1, -1], _bcud_j0())]
 ORDER = {"up": 0, "dn": 1, "sr": 2, "ch": 3, "bt": 4}
 
 
@@ -199,22 +309,21 @@ def _nonzero(mats):
     return out
 
 
-def tables():
-    """[(product, [(dataset name, idxs [N, 8], cols [N, 8], vals [N] complex64)])]
-    in the reference's product and dataset order"""
+def _tables(tetraquarks):
     prods = {}
-    for name, flavs in TETRAQUARKS:
+    for name, flavs_list, coeffs, opsets in tetraquarks:
         ops_out = []
         key = None
-        for sink_kind, source_kind, ops in OPERATORS:
-            sink = flavs if sink_kind == "meson" else _diquark_form(flavs)
-            source = _bar(flavs) if source_kind == "meson" else _bar(_diquark_form(flavs))
-            terms = _operator_terms(sink, source)
+        for sink_kind, source_kind, ops in opsets:
             maps = []
-            for sign, props, fl in terms:
-                ms, mc, canon = _default_maps(props, fl)
-                maps.append((sign, ms, _colour_slots(props, mc, sink_kind, source_kind)))
-                key = "".join(f[0] for f in canon)
+            for fs, cs in zip(flavs_list, coeffs):
+                sink = fs if sink_kind == "meson" else _diquark_form(fs)
+                for fb, cb in zip(flavs_list, coeffs):
+                    source = _bar(fb) if source_kind == "meson" else _bar(_diquark_form(fb))
+                    for sign, props, fl in _operator_terms(sink, source):
+                        ms, mc, canon = _default_maps(props, fl)
+                        maps.append((sign * cs * cb, ms, _colour_slots(props, mc, sink_kind, source_kind)))
+                        key = "".join(f[0] for f in canon)
             for opname, gammas in ops.items():
                 acc = {}
                 for g in gammas:
@@ -235,30 +344,48 @@ def tables():
     return list(prods.items())
 
 
-def write_header(path):
-    """C header with the tables: per product the flavours, the operator names,
-    counts, and the terms as int16 spins, int16 colour slots and float pairs"""
-    lines = ["// generated by plegma_amd/gen_tetra.py from the tetraquark interpolating fields;",
-             "// do not edit", "#pragma once", "#include <cstdint>", "namespace plg {"]
-    prods = tables()
-    lines.append(f"constexpr int TT_NPROD = {len(prods)};")
-    lines.append("static const char *const TT_PROD[TT_NPROD] = {" + ", ".join(f'"{p}"' for p, _ in prods) + "};")
-    lines.append("static const int TT_NOPS[TT_NPROD] = {" + ", ".join(str(len(o)) for _, o in prods) + "};")
+def tables():
+    """heavy-light tables: [(product, [(dataset name, idxs [N, 8], cols [N, 8],
+    vals [N] complex64)])] in the reference's product and dataset order"""
+    return _tables(TETRAQUARKS)
+
+
+def bcud_tables():
+    """bcud tables (product "udcb"), same layout"""
+    return _tables(BCUD)
+
+
+def _emit(lines, tag, prods):
+    """the tables of one set as C arrays with names TT_* (heavy-light) or
+    TB_* (bcud)"""
+    lines.append(f"constexpr int {tag}_NPROD = {len(prods)};")
+    lines.append(f"static const char *const {tag}_PROD[{tag}_NPROD] = {{" + ", ".join(f'"{p}"' for p, _ in prods) + "};")
+    lines.append(f"static const int {tag}_NOPS[{tag}_NPROD] = {{" + ", ".join(str(len(o)) for _, o in prods) + "};")
     for pi, (p, ops) in enumerate(prods):
         idx = np.concatenate([o[1] for o in ops]).reshape(-1)
         col = np.concatenate([o[2] for o in ops]).reshape(-1)
         val = np.concatenate([o[3] for o in ops])
         fv = np.stack([val.real, val.imag], -1).astype(np.float32).reshape(-1)
-        lines.append(f"static const char *const TT_NAMES_{pi}[] = {{" + ", ".join(f'"{o[0]}"' for o in ops) + "};")
-        lines.append(f"static const int16_t TT_COUNT_{pi}[] = {{" + ", ".join(str(len(o[3])) for o in ops) + "};")
-        lines.append(f"static const int16_t TT_IDX_{pi}[] = {{" + ",".join(str(int(v)) for v in idx) + "};")
-        lines.append(f"static const int16_t TT_COL_{pi}[] = {{" + ",".join(str(int(v)) for v in col) + "};")
-        lines.append(f"static const float TT_VAL_{pi}[] = {{" + ",".join(repr(float(v)) + "f" for v in fv) + "};")
+        lines.append(f"static const char *const {tag}_NAMES_{pi}[] = {{" + ", ".join(f'"{o[0]}"' for o in ops) + "};")
+        lines.append(f"static const int16_t {tag}_COUNT_{pi}[] = {{" + ", ".join(str(len(o[3])) for o in ops) + "};")
+        lines.append(f"static const int16_t {tag}_IDX_{pi}[] = {{" + ",".join(str(int(v)) for v in idx) + "};")
+        lines.append(f"static const int16_t {tag}_COL_{pi}[] = {{" + ",".join(str(int(v)) for v in col) + "};")
+        lines.append(f"static const float {tag}_VAL_{pi}[] = {{" + ",".join(repr(float(v)) + "f" for v in fv) + "};")
     for what in ("NAMES", "COUNT", "IDX", "COL", "VAL"):
         typ = {"NAMES": "const char *const *", "COUNT": "const int16_t *", "IDX": "const int16_t *",
                "COL": "const int16_t *", "VAL": "const float *"}[what]
-        lines.append(f"static {typ} const TT_{what}[TT_NPROD] = {{" +
-                     ", ".join(f"TT_{what}_{i}" for i in range(len(prods))) + "};")
+        lines.append(f"static {typ} const {tag}_{what}[{tag}_NPROD] = {{" +
+                     ", ".join(f"{tag}_{what}_{i}" for i in range(len(prods))) + "};")
+
+
+def write_header(path):
+    """C header with the tables: per product the flavours, the operator names,
+    counts, and the terms as int16 spins, int16 colour slots and float pairs;
+    TT_* the heavy-light set, TB_* the bcud set"""
+    lines = ["// generated by plegma_amd/gen_tetra.py from the tetraquark interpolating fields;",
+             "// do not edit", "#pragma once", "#include <cstdint>", "namespace plg {"]
+    _emit(lines, "TT", tables())
+    _emit(lines, "TB", bcud_tables())
     lines.append("}  // namespace plg")
     text = "\n".join(lines) + "\n"
     try:

@@ -112,6 +112,22 @@ template <typename Float> static int run(const std::string &out) {
     FILE *f = std::fopen((out + "/tetra_" + tag + ".names").c_str(), "w");
     for (auto &n : c.getDatasets()) std::fprintf(f, "%s\n", n.c_str());
     std::fclose(f);
+    // bcud tetraquarks: u as LT, a charm propagator (stream 5, decay 1.0)
+    // as CH, the bottom as BT, the strange one absent
+    PLEGMA_Propagator<Float> ch;
+    if (sizeof(Float) == 8) {
+      ch.synthesize(1234, 5, 0, 1.0);
+    } else {
+      PLEGMA_Propagator<double> c64;
+      c64.synthesize(1234, 5, 0, 1.0);
+      ch.castFrom(c64);
+    }
+    PLEGMA_Correlator<double> cb(MOMENTUM_SPACE, src, 2, 6);
+    cb.contractTetraquarksBCUD(u, none, ch, b);
+    dump(cb, out + "/bcud_" + tag + ".bin");
+    f = std::fopen((out + "/bcud_" + tag + ".names").c_str(), "w");
+    for (auto &n : cb.getDatasets()) std::fprintf(f, "%s\n", n.c_str());
+    std::fclose(f);
   }
   {
     // a propagator as a 144-component field through PLEGMA_FT::apply

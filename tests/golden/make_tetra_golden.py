@@ -1,16 +1,20 @@
-"""Writes tetra_golden.npz: the reference's heavy-light tetraquark operators
-evaluated on seeded site propagators, and a fingerprint of each table.
+"""Writes tetra_golden.npz and tetra_bcud_golden.npz: the reference's
+heavy-light and bcud tetraquark operators evaluated on seeded site
+propagators, and a fingerprint of each table.
 
 The reference's tables (lib/kernels/PLEGMA_heavy_light_tetraquarks_arrays.cuh:
 124-129 for the product list, dataset names and counts;
 PLEGMA_heavy_light_tetraquarks_arrays.cu for the float2 coefficients, the
 eight spin indices and the eight colour slots of every term) are read as text,
-data only, and evaluated here with the colour sums of
+data only (the bcud tables likewise from
+lib/kernels/PLEGMA_bcud_tetraquarks_arrays.cuh:199-204 and
+PLEGMA_bcud_tetraquarks_arrays.cu, the same format), and evaluated here with
+the colour sums of
 lib/kernels/PLEGMA_heavy_light_tetraquarks.cu:29-47:
     O = sum_t v_t sum_{a0..a3} P1[m0][m1][a_c0][a_c1] P2[m2][m3][a_c2][a_c3]
             conj(P3[(m5+2)%4][(m4+2)%4][a_c5][a_c4]) conj(P4[(m7+2)%4][(m6+2)%4][a_c7][a_c6])
-on two sites of random propagators per flavour u, d, s, b (a product "udbb"
-takes P1..P4 from its letters).  The fixture holds the propagators, the
+on two sites of random propagators per flavour u, d, s, b (c for bcud; a
+product "udbb" takes P1..P4 from its letters).  The fixture holds the propagators, the
 names, the counts, a SHA-256 fingerprint of each table (int16 spins, int16
 colour slots, float32 coefficient pairs, in the reference's term order) and the
 values; the tables themselves are not kept.
@@ -32,15 +36,15 @@ def _strings(text, name):
     return body[:body.index("};")]
 
 
-def parse(ref):
+def parse(ref, stem="PLEGMA_heavy_light_tetraquarks_arrays", var="TETRA_prop_prods"):
     kd = os.path.join(ref, "lib", "kernels")
-    head = open(os.path.join(kd, "PLEGMA_heavy_light_tetraquarks_arrays.cuh")).read()
-    data = open(os.path.join(kd, "PLEGMA_heavy_light_tetraquarks_arrays.cu")).read()
-    prods = re.findall(r'"(\w+)"', _strings(head, "TETRA_prop_prods"))
+    head = open(os.path.join(kd, stem + ".cuh")).read()
+    data = open(os.path.join(kd, stem + ".cu")).read()
+    prods = re.findall(r'"(\w+)"', _strings(head, var))
     names = [re.findall(r'"([^"]+)"', g) for g in
-             re.findall(r"\{([^{}]*)\}", _strings(head, "TETRA_prop_prods_names"))]
+             re.findall(r"\{([^{}]*)\}", _strings(head, var + "_names"))]
     idx_names = [re.findall(r"(\w+)", g) for g in
-                 re.findall(r"\{([^{}]*)\}", _strings(head, "TETRA_prop_prods_idxs"))]
+                 re.findall(r"\{([^{}]*)\}", _strings(head, var + "_idxs"))]
     out = []
     for p, ns, ins in zip(prods, names, idx_names):
         ops = []
@@ -84,21 +88,29 @@ def evaluate(S, letters, idx, col, vals):
     return (t1 * t2 * t3 * t4).sum(-1) @ v
 
 
-def main(ref):
-    rng = np.random.default_rng(20261203)
-    S = {f: rng.standard_normal((2, 4, 4, 3, 3)) + 1j * rng.standard_normal((2, 4, 4, 3, 3)) for f in "udsb"}
+def _write(path, tables, S, letters):
     prods, names, counts, prints, values = [], [], [], [], []
-    for p, ops in parse(ref):
+    for p, ops in tables:
         prods.append(p)
         for n, idx, col, vals in ops:
             names.append(p + ":" + n)
             counts.append(len(idx))
             prints.append(fingerprint(idx, col, vals))
             values.append(evaluate(S, p, idx, col, vals))
-    np.savez_compressed(os.path.join(HERE, "tetra_golden.npz"), props=np.stack([S[f] for f in "udsb"]),
-                        products=np.array(prods), names=np.array(names), counts=np.array(counts),
-                        fingerprints=np.array(prints), values=np.array(values))
-    print(len(prods), "products,", len(names), "operators")
+    np.savez_compressed(path, props=np.stack([S[f] for f in letters]), products=np.array(prods),
+                        names=np.array(names), counts=np.array(counts), fingerprints=np.array(prints),
+                        values=np.array(values))
+    print(os.path.basename(path), len(prods), "products,", len(names), "operators")
+
+
+def main(ref):
+    rng = np.random.default_rng(20261203)
+    S = {f: rng.standard_normal((2, 4, 4, 3, 3)) + 1j * rng.standard_normal((2, 4, 4, 3, 3)) for f in "udsb"}
+    _write(os.path.join(HERE, "tetra_golden.npz"), parse(ref), S, "udsb")
+    rng = np.random.default_rng(20261204)
+    S = {f: rng.standard_normal((2, 4, 4, 3, 3)) + 1j * rng.standard_normal((2, 4, 4, 3, 3)) for f in "udcb"}
+    _write(os.path.join(HERE, "tetra_bcud_golden.npz"),
+           parse(ref, "PLEGMA_bcud_tetraquarks_arrays", "TETRA_bcud_prop_prods"), S, "udcb")
 
 
 if __name__ == "__main__":

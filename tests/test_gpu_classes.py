@@ -231,3 +231,26 @@ def test_tetraquarks_class(outputs, prec, tol):
     a = np.fromfile(os.path.join(d, f"tetra_{prec}.bin"), dtype=np.float64)
     got = (a[0::2] + 1j * a[1::2]).reshape(ref.shape)
     assert G.rel_err(got, ref, axis=-1) < tol
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-6)])
+def test_tetraquarks_bcud_class(outputs, prec, tol):
+    """contractTetraquarksBCUD(LT = u, ST absent, CH = c, BT = b) into a double
+    correlator: the 57 datasets of "udcb" (reference order, names), H_elem
+    [dataset][t][mom], against the oracle's 81-term sums over the generated
+    bcud tables (terms pairing the colours of P1 with P4 and P2 with P3 run
+    as a second launch)"""
+    from plegma_amd import gen_tetra
+    d, u, dn = outputs
+    c = O.synth_prop(L, 1234, 5, 0, 1.0)
+    b = O.synth_prop(L, 1234, 4, 0, 1.2)
+    fl = {"u": u, "d": u, "c": c, "b": b}
+    (p, ops), = gen_tetra.bcud_tables()
+    idx = np.concatenate([o[1] for o in ops])
+    col = np.concatenate([o[2] for o in ops])
+    val = np.concatenate([o[3] for o in ops]).astype(np.complex128)
+    ref = O.tetra_correlator(L, SRC, 6, 2, [fl[f] for f in p], idx, col, val, [len(o[3]) for o in ops])
+    assert open(os.path.join(d, f"bcud_{prec}.names")).read().split() == [o[0] for o in ops]
+    a = np.fromfile(os.path.join(d, f"bcud_{prec}.bin"), dtype=np.float64)
+    got = (a[0::2] + 1j * a[1::2]).reshape(ref.shape)
+    assert G.rel_err(got, ref, axis=-1) < tol

@@ -17,6 +17,11 @@ SRC = [2, 1, 3, 6]
 TOTALT = 5
 Q2 = 2
 PATTERNS = [(0, 2, 1, 3, 2, 0, 3, 1), (0, 3, 1, 2, 3, 0, 2, 1), (0, 2, 1, 3, 3, 0, 2, 1), (0, 3, 1, 2, 2, 0, 3, 1)]
+# the further patterns of the bcud tables: P1-P3 / P2-P4 pairings with other
+# colour labels, and the P1-P4 / P2-P3 pairings (a second launch that adds)
+SWAPPED = [(0, 2, 1, 3, 3, 1, 2, 0), (0, 3, 1, 2, 2, 1, 3, 0), (0, 3, 1, 2, 3, 1, 2, 0), (0, 2, 1, 3, 2, 1, 3, 0)]
+BCUD_PATTERNS = PATTERNS + [(1, 3, 0, 2, 3, 1, 2, 0), (1, 2, 0, 3, 2, 1, 3, 0), (1, 2, 0, 3, 3, 1, 2, 0),
+                            (1, 3, 0, 2, 2, 1, 3, 0)] + SWAPPED
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -37,7 +42,7 @@ def props(ctx):
     return out
 
 
-def _table(rng, nops=5, nterm=60):
+def _table(rng, nops=5, nterm=60, patterns=PATTERNS):
     idxs, cols, vals, counts = [], [], [], []
     coefs = [1, -1, 1j, -0.5, 0.25 + 0.5j, 1 / 3]
     for k in range(nops):
@@ -45,7 +50,7 @@ def _table(rng, nops=5, nterm=60):
         counts.append(n)
         for _ in range(n):
             idxs.append(rng.integers(0, 4, size=8))
-            cols.append(PATTERNS[rng.integers(4)])
+            cols.append(patterns[rng.integers(len(patterns))])
             v = complex(coefs[rng.integers(len(coefs))])
             vals.append([np.float32(v.real), np.float32(v.imag)])
     return (np.array(idxs, dtype=np.int16), np.array(cols, dtype=np.int16), np.array(vals, dtype=np.float32),
@@ -225,3 +230,76 @@ def test_builtin_tetraquarks_selection(props):
     names, ref = _builtin_ref(fl)
     got = _run_builtin(lt, None, None, bt)
     assert len(names) == 16 and G.rel_err(got, ref, axis=-1) < 1e-12
+
+
+@pytest.mark.parametrize("patterns", [BCUD_PATTERNS, SWAPPED], ids=["all", "swapped-only"])
+def test_tetra_swapped_pairings_f64(props, patterns):
+    """tables with terms that pair the colours of P1 with P4 and P2 with P3:
+    packed for the views P1, P2, P4, P3 and added by a second launch; with
+    swapped terms only the first launch writes zeros"""
+    tab = _table(np.random.default_rng(10), nops=6, nterm=50, patterns=patterns)
+    ref = _oracle(props, tab)
+    got = _run([p[1] for p in props], tab)
+    assert G.rel_err(got, ref, axis=-1) < 1e-12
+
+
+def test_tetra_swapped_pairings_mixed_position(props):
+    tab = _table(np.random.default_rng(11), nops=5, nterm=40, patterns=BCUD_PATTERNS)
+    keep = [G.cast_f32(p[0], p[1], scaled=True) for p in props]
+    ref = _oracle(props, tab)
+    got = _run([k[1] for k in keep], tab, mode=capi.MODE_MIXED)
+    assert G.rel_err(got, ref, axis=-1) < 1e-6
+    ref = _oracle(props, tab, space="position")
+    got = _run([p[1] for p in props], tab, space=capi.POSITION_SPACE)
+    assert G.rel_err(got, ref, axis=-1) < 1e-12
+
+
+def _bcud_ref(fl):
+    from plegma_amd import gen_tetra
+    (p, ops), = gen_tetra.bcud_tables()
+    idx = np.concatenate([o[1] for o in ops])
+    col = np.concatenate([o[2] for o in ops])
+    val = np.concatenate([o[3] for o in ops]).astype(np.complex128)
+    ref = O.tetra_correlator(L, SRC, TOTALT, Q2, [fl[f] for f in p], idx, col, val, [len(o[3]) for o in ops])
+    return [o[0] for o in ops], ref
+
+
+def test_builtin_bcud_products():
+    from plegma_amd import gen_tetra
+    prods = capi.tetra_products("bcud")
+    assert [(p, [o[0] for o in ops]) for p, ops in gen_tetra.bcud_tables()] == prods
+    assert prods[0][0] == "udcb" and len(prods[0][1]) == 57
+
+
+@pytest.mark.parametrize("mode,tol", [(capi.MODE_F64, 1e-12), (capi.MODE_MIXED, 1e-6)])
+def test_builtin_bcud(props, mode, tol):
+    """plg_tetraquarks_bcud with LT, CH and BT (ST absent): the 57 datasets of
+    "udcb" against the oracle over the generated tables (which equal the
+    reference's)"""
+    torch = G.torch_cuda()
+    lt, ch, bt = props[0], props[2], props[3]
+    names, ref = _bcud_ref({"u": lt[2], "d": lt[2], "c": ch[2], "b": bt[2]})
+    if mode == capi.MODE_F64:
+        views = [lt[1], None, ch[1], bt[1]]
+    else:
+        keep = [G.cast_f32(p[0], p[1], scaled=True) for p in (lt, ch, bt)]
+        views = [keep[0][1], None, keep[1][1], keep[2][1]]
+    moms = capi.momlist(Q2)
+    n = capi.tetraquarks_size(*views, SRC, TOTALT, len(moms), family="bcud")
+    assert n == ref.size
+    out = torch.full((n, 2), float("nan"), dtype=torch.float64, device="cuda")
+    print(f"[gpu] bcud tetraquarks: {n} elements, mode {mode}", flush=True)
+    capi.tetraquarks(*views, SRC, TOTALT, moms, mode, capi.MOMENTUM_SPACE, capi.F64, out, family="bcud")
+    h = G.host(out).reshape(-1, 2)
+    got = (h[:, 0] + 1j * h[:, 1]).reshape(ref.shape)
+    assert G.rel_err(got, ref, axis=-1) < tol
+
+
+def test_builtin_bcud_selection(props):
+    """the reference's selection (lib/PLEGMA_Correlator.cu:270-293): the
+    product needs LT, CH and BT; only_st drops it (no strange quark)"""
+    lt, st, ch, bt = (p[1] for p in props)
+    assert capi.tetraquarks_select(lt, None, ch, bt, family="bcud") == ([0], 57)
+    assert capi.tetraquarks_select(lt, st, None, bt, family="bcud") == ([], 0)
+    assert capi.tetraquarks_select(lt, st, ch, bt, only_st=True, family="bcud") == ([], 0)
+    assert capi.tetraquarks_select(lt, st, ch, bt, only_ch=True, family="bcud") == ([0], 57)
