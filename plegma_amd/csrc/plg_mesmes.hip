@@ -11,7 +11,7 @@
 // Lane (mu, nu) of a site holds the colour block
 //   M'^{ab}_{mu nu} = sum_rho phi(rho) sum_c S_L^{ac}_{mu rho} conj(S_H^{bc}_{nu^x rho^x})
 // (M = M' times the lane phase phi(nu^x)); Tr[M M] pairs lane (mu, nu) with
-// the transposed lane (nu, mu) through LDS.  g5/g3 share X's xor class 0 and
+// the transposed lane (nu, mu) by lane permutes.  g5/g3 share X's xor class 0 and
 // g1/g2 class 1; within a class the row phases differ by +-i^RA, so one pass
 // over rho fills both operators (the same split as the baryon blocks).
 // Cost per site: 16 lanes x (2 x 108 + 36 + 36) complex MACs.
@@ -21,17 +21,24 @@ namespace plg {
 
 __constant__ MesonX c_mesonx_mm = make_mesonx();
 
+// w holds the meson Gram rows (4 x 16 per site) and then the output stage
+// (20 per site); 68 = 64 + 4 keeps the sites' rows on different banks.  The
+// crossed terms exchange their blocks by lane permutes, not through LDS: a
+// workgroup needs 47.9 KB (FP32), and the kernel is held at two waves per
+// SIMD by its registers (the next tile's prefetch among them), which measured
+// faster than three waves with a few spilled registers.
+constexpr int MM_WS = 68;
 template <typename T> struct MesMesShared {
   cx<T> s1[144][BT_SP];
   cx<T> s2[144][BT_SP];
-  cx<T> w[BT_SITES][BT_WS];
+  cx<T> w[BT_SITES][MM_WS];
 };
 
 // mesmes of operators OPA, OPB (one xor class of X = g5 G); tr_a / tr_b are
 // Tr M of the two operators, held by every lane of the site
 template <typename T, int OPA, int OPB>
-__device__ __forceinline__ void mesmes_pair(const cx<T> (*SL)[BT_SP], const cx<T> (*SH)[BT_SP],
-                                            cx<T> (*Wsh)[16][9], int s, int lane, cx<T> &mm_a, cx<T> &mm_b) {
+__device__ __forceinline__ void mesmes_pair(const cx<T> (*SL)[BT_SP], const cx<T> (*SH)[BT_SP], int s, int lane,
+                                            cx<T> &mm_a, cx<T> &mm_b) {
   constexpr SPerm XA = sp_mul(G_5, gamma_op(OPA)), XB = sp_mul(G_5, gamma_op(OPB));
   constexpr int x = sp_class(XA);
   static_assert(sp_class(XB) == x, "one xor class per pair");
@@ -65,21 +72,22 @@ __device__ __forceinline__ void mesmes_pair(const cx<T> (*SL)[BT_SP], const cx<T
 #pragma unroll
   for (int i = 0; i < 9; i++) {
     const cx<T> u0 = U[0][i], u1 = U[1][i];
-    Wsh[0][lane][i] = u0 + u1;
-    Wsh[1][lane][i] = cmul_iph(u0 - u1, RA);
+    U[0][i] = u0 + u1;
+    U[1][i] = cmul_iph(u0 - u1, RA);
   }
-  wave_lds_sync();
-  // Tr[M M] = sum_lanes phi(nu^x) phi(mu^x) sum_ab M'^{ab}_{mu nu} M'^{ba}_{nu mu}
-  const int tl = lo * 4 + hi;
+  // Tr[M M] = sum_lanes phi(nu^x) phi(mu^x) sum_ab M'^{ab}_{mu nu} M'^{ba}_{nu mu}:
+  // the transposed lane (nu, mu) of the same site's row sends its blocks
+  const int partner = (int)(threadIdx.x & 48) + lo * 4 + hi;
   cx<T> sa = cmake<T>(0, 0), sb = cmake<T>(0, 0);
 #pragma unroll
   for (int ca = 0; ca < 3; ca++)
 #pragma unroll
     for (int cb = 0; cb < 3; cb++) {
-      cfma(sa, Wsh[0][lane][ca * 3 + cb], Wsh[0][tl][cb * 3 + ca]);
-      cfma(sb, Wsh[1][lane][ca * 3 + cb], Wsh[1][tl][cb * 3 + ca]);
+      const cx<T> ta = cmake<T>(__shfl(U[0][cb * 3 + ca].re, partner, 64), __shfl(U[0][cb * 3 + ca].im, partner, 64));
+      const cx<T> tb = cmake<T>(__shfl(U[1][cb * 3 + ca].re, partner, 64), __shfl(U[1][cb * 3 + ca].im, partner, 64));
+      cfma(sa, U[0][ca * 3 + cb], ta);
+      cfma(sb, U[1][ca * 3 + cb], tb);
     }
-  wave_lds_sync();  // Wsh is rewritten by the next pair
   const int pha = pick4(XA.ph, lo ^ x) + pick4(XA.ph, hi ^ x);
   const int phb = pick4(XB.ph, lo ^ x) + pick4(XB.ph, hi ^ x);
   sa = cmul_iph_rt(sa, pha);
@@ -153,7 +161,6 @@ void k_mesmes(const PropDev<T> *__restrict__ props, const RowDev *__restrict__ r
     if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
     const cx<T>(*SL)[BT_SP] = sh.s1;
     const cx<T>(*SH)[BT_SP] = sh.s2;
-    cx<T>(*Wsh)[16][9] = reinterpret_cast<cx<T>(*)[16][9]>(&sh.w[s][0]);
 
     // ---- 16 meson traces: lane p = (alpha, beta) forms the colour-traced
     // Gram entries S_L[p] . conj(S_H[p ^ (d,d)]), lane op sums them with phases
@@ -182,8 +189,8 @@ void k_mesmes(const PropDev<T> *__restrict__ props, const RowDev *__restrict__ r
     }
     // ---- crossed terms of the two class pairs
     cx<T> mm5, mm3, mm1, mm2;
-    mesmes_pair<T, 5, 3>(SL, SH, Wsh, s, lane, mm5, mm3);
-    mesmes_pair<T, 1, 2>(SL, SH, Wsh, s, lane, mm1, mm2);
+    mesmes_pair<T, 5, 3>(SL, SH, s, lane, mm5, mm3);
+    mesmes_pair<T, 1, 2>(SL, SH, s, lane, mm1, mm2);
     // mesmes(G) on lane k of the site (k = 0..3 for G = g5, g1, g2, g3)
     cx<T> mm;
     {
@@ -197,8 +204,8 @@ void k_mesmes(const PropDev<T> *__restrict__ props, const RowDev *__restrict__ r
     }
     // ---- stage [k][lane] per site in w, then coalesced stores along x
     cx<T> *stage = &sh.w[0][0];
-    stage[s * BT_WS + lane * 2 + 0] = tr;
-    stage[s * BT_WS + lane * 2 + 1] = mm;
+    stage[s * MM_WS + lane * 2 + 0] = tr;
+    stage[s * MM_WS + lane * 2 + 1] = mm;
     __syncthreads();
     {
       const int ls = tid & 15;
@@ -206,7 +213,7 @@ void k_mesmes(const PropDev<T> *__restrict__ props, const RowDev *__restrict__ r
       if (x < V3) {
         const int64_t ncols = (int64_t)gridDim.y * nch_total, col0 = (int64_t)blockIdx.y * nch_total + ch0;
         for (int ch = tid >> 4; ch < NCH; ch += BT_THREADS / BT_SITES)
-          gstore(out + ctile_idx(col0 + ch, x, ncols), stage[ls * BT_WS + (ch & 15) * 2 + (ch >> 4)]);
+          gstore(out + ctile_idx(col0 + ch, x, ncols), stage[ls * MM_WS + (ch & 15) * 2 + (ch >> 4)]);
       }
     }
     __syncthreads();
