@@ -144,24 +144,42 @@ __global__ void k_silu_back(const float *__restrict__ dx, const float *__restric
   }
 }
 
-// db[n] = sum_b dZ[b][n]: 256 threads per 16 columns, thread (row lane r,
-// column j) sums the rows r, r + 16, ...; the 16 row lanes of a column are
-// added in LDS in a fixed order
-__global__ __launch_bounds__(256) void k_bias_grad(const float *__restrict__ dz, float *__restrict__ db, int B,
-                                                   int N) {
+// db[n] = sum_b dZ[b][n] in two deterministic stages.  k_bias_part: block
+// (column group cx, row block rb) sums the rows [rb*rpb, (rb+1)*rpb) of 16
+// columns; thread (row lane r, column j) takes the rows r, r + 16, ... with
+// four independent accumulators (loads in flight together), and the 16 row
+// lanes are added in LDS in a fixed order into part[rb][n].  k_bias_sum adds
+// the row blocks' partials in order.  (One block per 16 columns walking all
+// rows took about 35 us per call at B = 2048, most of a training step.)
+constexpr int MG_BROWS = 16;  // row blocks of the bias gradient
+__global__ __launch_bounds__(256) void k_bias_part(const float *__restrict__ dz, float *__restrict__ part, int B,
+                                                   int N, int rpb) {
   __shared__ float red[16][17];
   const int j = threadIdx.x & 15, r = threadIdx.x >> 4;
   const int n = blockIdx.x * 16 + j;
-  float s = 0;
-  if (n < N)
-    for (int b = r; b < B; b += 16) s += dz[(int64_t)b * N + n];
-  red[r][j] = s;
+  const int b0 = blockIdx.y * rpb, b1 = min(B, b0 + rpb);
+  float s[4] = {0.f, 0.f, 0.f, 0.f};
+  if (n < N) {
+    int b = b0 + r;
+    for (; b + 48 < b1; b += 64)
+#pragma unroll
+      for (int u = 0; u < 4; u++) s[u] += dz[(int64_t)(b + 16 * u) * N + n];
+    for (int u = 0; b < b1; b += 16, u++) s[u & 3] += dz[(int64_t)b * N + n];
+  }
+  red[r][j] = (s[0] + s[1]) + (s[2] + s[3]);
   __syncthreads();
   if (r == 0 && n < N) {
     float t = 0;
     for (int k = 0; k < 16; k++) t += red[k][j];
-    db[n] = t;
+    part[(int64_t)blockIdx.y * N + n] = t;
   }
+}
+__global__ void k_bias_sum(const float *__restrict__ part, float *__restrict__ db, int nrb, int N) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  float t = 0;
+  for (int k = 0; k < nrb; k++) t += part[(int64_t)k * N + n];
+  db[n] = t;
 }
 
 // the Adam step counter and its bias corrections 1 / (1 - b^step), on the
@@ -206,6 +224,7 @@ struct plg_mlp_s {
   std::vector<uint16_t *> X;
   std::vector<float *> Z;
   float *dz = nullptr, *dx = nullptr, *part = nullptr;
+  float *bpart = nullptr;  // bias-gradient partials [MG_BROWS][widest]
   uint16_t *dzb = nullptr;
   int64_t step = 0;
   float *ksw = nullptr;     // split-K partials of the products
@@ -285,7 +304,11 @@ static int mlp_backward(plg_mlp_s *net, const float *t, int B, hipStream_t st) {
     a.B = net->X[l]; a.sbk = Ni; a.sbn = 1;
     a.C = net->g + net->w_off[l]; a.ldc = Ni;
     gemm(a, net->ksw, st);
-    k_bias_grad<<<blocks(No, 16), 256, 0, st>>>(net->dz, net->g + net->b_off[l], B, No);
+    {
+      const int rpb = (B + MG_BROWS - 1) / MG_BROWS;
+      k_bias_part<<<dim3(blocks(No, 16), MG_BROWS), 256, 0, st>>>(net->dz, net->bpart, B, No, rpb);
+      k_bias_sum<<<blocks(No), 256, 0, st>>>(net->bpart, net->g + net->b_off[l], MG_BROWS, No);
+    }
     if (l > 0) {
       // dX_l[b][i] = sum_o dZ[b][o] W_l[o][i], then dZ_{l-1} = dX_l silu'(Z_{l-1})
       GemmArgs d{};
@@ -321,14 +344,10 @@ static int train_step(plg_mlp_s *net, const float *x, const float *t, int B, flo
   int rc;
   if ((rc = mlp_forward(net, x, B, st)) || (rc = mlp_backward(net, t, B, st))) return rc;
   k_adam_step<<<1, 1, 0, st>>>(net->d_step, net->d_coef, b1, b2);
-  for (int l = 0; l < net->L; l++) {
-    const int64_t nw = (int64_t)net->dims[l] * net->dims[l + 1], nb = net->dims[l + 1];
-    const int64_t ow = net->w_off[l], ob = net->b_off[l];
-    k_adam<<<blocks(nw), 256, 0, st>>>(net->p + ow, net->m + ow, net->v + ow, net->g + ow, net->pb + ow, nw, lr,
-                                      b1, b2, eps, net->d_coef);
-    k_adam<<<blocks(nb), 256, 0, st>>>(net->p + ob, net->m + ob, net->v + ob, net->g + ob, nullptr, nb, lr, b1,
-                                      b2, eps, net->d_coef);
-  }
+  // the parameters (W_0, b_0, W_1, ...) are one contiguous array: one update
+  // launch for all of them (the biases' bf16 copies are written but unused)
+  k_adam<<<blocks(net->nparams), 256, 0, st>>>(net->p, net->m, net->v, net->g, net->pb, net->nparams, lr, b1, b2,
+                                               eps, net->d_coef);
   return hipGetLastError() == hipSuccess ? PLG_OK : fail(PLG_ERR_HIP, "MLP update launch failed");
 }
 
@@ -379,6 +398,7 @@ int plg_mlp_create(const int *dims, int nlayers, int max_batch, plg_mlp *out) {
   rc = rc ? rc : mlp_alloc(net, (void **)&net->dx, wb * sizeof(float));
   rc = rc ? rc : mlp_alloc(net, (void **)&net->dzb, wb * sizeof(uint16_t));
   rc = rc ? rc : mlp_alloc(net, (void **)&net->part, (size_t)blocks((int64_t)wb) * sizeof(float));
+  rc = rc ? rc : mlp_alloc(net, (void **)&net->bpart, (size_t)MG_BROWS * widest * sizeof(float));
   {
     // split-K partials: a bound of ks x M x N over the network's products
     // at any batch up to max_batch (the split ignoring the tile count)
