@@ -1426,6 +1426,57 @@ static int udsc_run(const plg_prop *up, const plg_prop *dn, const plg_prop *st, 
   return PLG_OK;
 }
 
+// Packed tables by content (the term order is a search over the table, a few
+// milliseconds per thousand terms): the last few tables packed, keyed by a
+// 64-bit hash of the counts, spins, colour slots and coefficients.
+static std::mutex g_pack_mu;
+static std::vector<std::pair<uint64_t, std::shared_ptr<const TetraPacked>>> g_pack_cache;
+constexpr size_t PACK_CACHE = 8;
+
+static uint64_t hash_bytes(const void *data, size_t n, uint64_t h) {
+  const unsigned char *b = (const unsigned char *)data;
+  size_t i = 0;
+  for (; i + 8 <= n; i += 8) {
+    uint64_t w;
+    std::memcpy(&w, b + i, 8);
+    h = (h ^ w) * 0x100000001b3ull;
+    h ^= h >> 29;
+  }
+  for (; i < n; i++) h = (h ^ b[i]) * 0x100000001b3ull;
+  return h;
+}
+
+static int tetra_pack_cached(const int16_t *idxs, const int16_t *cols, const float *vals, const int16_t *counts,
+                             int nops, TetraPacked &out) {
+  if (nops <= 0 || nops > TETRA_MAXOPS) return PLG_ERR_UNSUPPORTED;
+  size_t nt = 0;
+  for (int k = 0; k < nops; k++) {
+    if (counts[k] < 0) return PLG_ERR_INVALID;
+    nt += (size_t)counts[k];
+  }
+  uint64_t h = 0xcbf29ce484222325ull ^ (uint64_t)nops;
+  h = hash_bytes(counts, nops * sizeof(int16_t), h);
+  h = hash_bytes(idxs, nt * 8 * sizeof(int16_t), h);
+  h = hash_bytes(cols, nt * 8 * sizeof(int16_t), h);
+  h = hash_bytes(vals, nt * 2 * sizeof(float), h);
+  {
+    std::lock_guard<std::mutex> lk(g_pack_mu);
+    for (auto &e : g_pack_cache)
+      if (e.first == h) {
+        out = *e.second;
+        return PLG_OK;
+      }
+  }
+  auto pk = std::make_shared<TetraPacked>();
+  const int rc = tetra_pack(idxs, cols, vals, counts, nops, *pk);
+  if (rc) return rc;
+  out = *pk;
+  std::lock_guard<std::mutex> lk(g_pack_mu);
+  if (g_pack_cache.size() >= PACK_CACHE) g_pack_cache.erase(g_pack_cache.begin());
+  g_pack_cache.emplace_back(h, std::move(pk));
+  return PLG_OK;
+}
+
 // one four-propagator table (views P1..P4), its operators written as datasets
 // d0 .. d0 + nops - 1 of the output
 static int tetra_run(const plg_prop *const views[4], const int16_t *idxs, const int16_t *col_contr, const float *vals,
@@ -1437,7 +1488,7 @@ static int tetra_run(const plg_prop *const views[4], const int16_t *idxs, const 
   for (int i = 0; i < 4; i++)
     if (!views[i]) return fail(PLG_ERR_INVALID, "NULL propagator view");
   TetraPacked pk;
-  rc = tetra_pack(idxs, col_contr, vals, counts, nops, pk);
+  rc = tetra_pack_cached(idxs, col_contr, vals, counts, nops, pk);
   if (rc == PLG_ERR_INVALID) return fail(rc, "spin or colour index outside 0..3");
   if (rc)
     return fail(rc, "table outside the supported form (64 operators; colour slots a permutation of the four "
