@@ -18,7 +18,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libplegma_amd.so")
-SOURCES = ["plg_api.hip", "plg_contract.hip", "plg_baryon.hip", "plg_project.hip", "plg_project_bf16.hip",
+SOURCES = ["plg_api.hip", "plg_contract.hip", "plg_baryon.hip", "plg_project.hip", "plg_project_bf16.hip", "plg_project_sep.hip",
            "plg_synth.hip", "plg_mesmes.hip", "plg_mlp.hip", "plg_prep.hip", "plg_udsc.hip",
            "plg_tetra.hip"]
 # generated from the udsc baryon and the heavy-light tetraquark interpolating

@@ -224,6 +224,9 @@ struct Plan {
   int64_t *d_choff = nullptr;
   int *d_ms = nullptr, *d_pair_ms = nullptr, *d_mom_pair = nullptr;
   uint16_t *d_pidx = nullptr;  // phase indices of the mixed projection
+  // separable projection (both precisions): host tables and their device copy
+  SepHost sep;
+  void *d_sep = nullptr;
   // udsc baryons: the flavour product's schedule (device arrays owned by the context)
   UdscSched udsc{};
   // four-propagator tables: terms (owned by the caller of the plan) and rows
@@ -245,7 +248,7 @@ struct Plan {
     if (graph) hipGraphDestroy(graph);
     if (cap_stream) hipStreamDestroy(cap_stream);
     for (void *p : {(void *)d_props, (void *)d_rows, (void *)d_orows, (void *)d_csel, (void *)d_choff, (void *)d_ms,
-                    (void *)d_pair_ms, (void *)d_mom_pair, (void *)d_pidx, (void *)d_rows4})
+                    (void *)d_pair_ms, (void *)d_mom_pair, (void *)d_pidx, (void *)d_rows4, d_sep})
       if (p) hipFree(p);
     if (own_ws) {
       if (c) hipFree(c);
@@ -544,6 +547,16 @@ static void choose_tiles(Plan &P) {
   int64_t ns = (1024 + tiles - 1) / tiles;
   ns = std::max<int64_t>(1, std::min<int64_t>(ns, g_ctx.V3 / 512));
   P.nsplit = (int)std::max<int64_t>(1, ns);
+  // the separable projection where the momenta suit it (PLG_PROJ_SEP=0
+  // keeps the phase-matrix GEMM; a pinned PLG_PROJ_CODE wins for FP32)
+  P.sep = SepHost{};
+  const char *se = std::getenv("PLG_PROJ_SEP");
+  if (P.space == PLG_MOMENTUM_SPACE && P.nmom > 0 && !(se && std::string(se) == "0"))
+    sep_build(g_ctx.L, g_ctx.M, P.ms, ncols, P.T == PLG_F64, P.sep);
+  if (P.sep.ok && !(P.T == PLG_F32 && std::getenv("PLG_PROJ_CODE"))) {
+    if (P.T == PLG_F64) P.mtile = PROJ_SEP;
+    else P.mbw = PROJ_SEP;
+  }
   const int64_t btiles = (g_ctx.V3 + 15) / 16;
   int64_t tpb = (btiles * (int64_t)P.rows.size()) / 8192;
   P.tpb = (int)std::max<int64_t>(1, std::min<int64_t>(tpb, 64));
@@ -567,6 +580,12 @@ static int upload_plan(Plan &P, hipStream_t st) {
   if ((rc = upload(&P.d_ms, P.ms, st))) return rc;
   if ((rc = upload(&P.d_pair_ms, P.pair_ms, st))) return rc;
   if ((rc = upload(&P.d_mom_pair, P.mom_pair, st))) return rc;
+  if (P.sep.ok) {
+    unsigned char *d = nullptr;
+    if ((rc = upload(&d, P.sep.blob, st))) return rc;
+    if (P.d_sep) hipFree(P.d_sep);
+    P.d_sep = d;
+  }
   if (P.npairs > 0 && P.T == PLG_F32) {  // phase indices of the mixed projection
     if (P.d_pidx) hipFree(P.d_pidx);
     P.d_pidx = nullptr;
@@ -585,8 +604,19 @@ static int upload_plan(Plan &P, hipStream_t st) {
 static size_t c_bytes(const Plan &P) {
   return (size_t)P.rows.size() * P.nch_c * ctile_sites(g_ctx.V3) * (P.T == PLG_F64 ? 16 : 8);
 }
+static int proj_code(const Plan &P) { return P.T == PLG_F64 ? P.mtile : P.mbw; }
+static bool plan_sep(const Plan &P) { return P.sep.ok && proj_code(P) == PROJ_SEP; }
 static size_t part_bytes(const Plan &P) {
+  if (plan_sep(P)) return sep_part_bytes(P.sep.g);
   return proj_part_bytes(P.nsplit, (int64_t)P.rows.size() * P.nch_c, std::max(P.nmom, 1), P.npairs);
+}
+// room for every candidate of a tuning run: the GEMM with the most K splits
+// and the separable form
+static size_t part_bytes_tune(const Plan &P) {
+  const int ns = (int)std::max<int64_t>(P.nsplit, std::min<int64_t>(32, g_ctx.V3 / 256));
+  size_t b = proj_part_bytes(ns, (int64_t)P.rows.size() * P.nch_c, std::max(P.nmom, 1), P.npairs);
+  if (P.sep.ok) b = std::max(b, sep_part_bytes(P.sep.g));
+  return b;
 }
 
 // projection geometry of a plan
@@ -608,6 +638,15 @@ static ProjGeom plan_geom(const Plan &P) {
   g.pidx = P.d_pidx;
   g.zero_pair = P.zero_pair;
   g.shifted = P.T == PLG_F32 && P.npairs > 0;
+  return g;
+}
+// the geometry the finalize reads: the separable projection's sums are
+// momentum-major, one piece, unshifted
+static ProjGeom sep_proj_geom(ProjGeom g) {
+  g.sep = 1;
+  g.npairs = 0;
+  g.nsplit = 1;
+  g.shifted = 0;
   return g;
 }
 
@@ -673,11 +712,16 @@ static int enqueue(Plan &P, void *c, void *part, void *out, int out_prec, hipStr
                             P.m_stride, out, out_prec, st);
     return rc ? fail(rc, "position-space scatter failed") : PLG_OK;
   }
-  const ProjGeom g = plan_geom(P);
-  if (P.T == PLG_F64) rc = launch_project_f64(g, (const cx<double> *)c, (cx<double> *)part, P.mtile, st);
+  const bool sep = plan_sep(P);
+  const ProjGeom g = sep ? sep_proj_geom(plan_geom(P)) : plan_geom(P);
+  if (sep) rc = launch_project_sep(sep_geom(P.sep, P.d_sep), c, P.T, (double *)part, st);
+  else if (P.T == PLG_F64) rc = launch_project_f64(g, (const cx<double> *)c, (cx<double> *)part, P.mtile, st);
   else rc = launch_project_bf16(g, (const cx<float> *)c, (double *)part, P.terms, g_ctx.d_wtab, P.mbw, st);
   if (rc) return fail(rc, "projection launch failed");
-  if (log) log->mark(P.T == PLG_F64 ? "project_f64" : (P.terms == 6 ? "project_bf16x6" : "project_bf16x3"), st);
+  if (log)
+    log->mark(sep ? (P.T == PLG_F64 ? "project_sep_f64" : "project_sep_f32")
+                  : (P.T == PLG_F64 ? "project_f64" : (P.terms == 6 ? "project_bf16x6" : "project_bf16x3")),
+              st);
   rc = launch_finalize(g, (const double *)part, P.d_orows, (int)P.orows.size(), P.nch_c, nout, P.d_csel,
                        P.d_choff, P.m_stride, out, out_prec, st);
   if (log) log->mark("finalize", st);
@@ -703,7 +747,7 @@ static std::string tune_key(const Plan &P) {
   k << prop.gcnArchName << " L=" << g_ctx.L[0] << "x" << g_ctx.L[1] << "x" << g_ctx.L[2] << "x" << g_ctx.L[3]
     << " nmom=" << P.nmom << " cols=" << P.rows.size() * P.nch_c << " rows=" << P.rows.size() << " mode=" << P.mode
     << " npairs=" << P.npairs << " terms=" << P.terms << " kind=" << P.kind << " mask=" << P.mask
-    << " proj=pair32z";
+    << " proj=pair32z.sep" << (P.sep.ok ? 1 : 0);
   for (auto &l : P.launches) k << " L" << l.what << "." << l.set << "." << l.two << l.mes << "." << l.gmask;
   std::string s = k.str();
   for (char &ch : s)
@@ -815,6 +859,26 @@ static int autotune(Plan &P, hipStream_t st) {
         best_ns = ns;
       }
     }
+  if (P.sep.ok && sep_part_bytes(P.sep.g) <= P.part_bytes) {
+    const ProjGeom gs = sep_proj_geom(plan_geom(P));
+    const SepGeom sg = sep_geom(P.sep, P.d_sep);
+    int lrc = PLG_OK;
+    const double ms = time_ms(
+        [&] {
+          lrc = launch_project_sep(sg, P.c, P.T, (double *)P.part, st);
+          if (lrc == PLG_OK)
+            lrc = launch_finalize(gs, (const double *)P.part, P.d_orows, (int)P.orows.size(), P.nch_c, nout,
+                                  P.d_csel, P.d_choff, P.m_stride, tune_out, PLG_F64, st);
+        },
+        st);
+    if (lrc != PLG_OK) {
+      (void)hipGetLastError();
+    } else if (ms < best_p) {
+      best_p = ms;
+      best_code = PROJ_SEP;
+      best_ns = 1;
+    }
+  }
   if (P.T == PLG_F64) P.mtile = best_code;
   else P.mbw = best_code;
   P.nsplit = best_ns;
@@ -847,6 +911,7 @@ static std::string single_key(int kind, int mask, const std::vector<plg_prop> &p
     for (int i = 0; i < 3 * nmom; i++) k << ' ' << moms[i];
   // a pinned projection tile (PLG_PROJ_CODE) is part of the plan
   if (const char *e = std::getenv("PLG_PROJ_CODE")) k << " code=" << e;
+  if (const char *e = std::getenv("PLG_PROJ_SEP")) k << " sep=" << e;
   return k.str();
 }
 
@@ -881,15 +946,12 @@ static int tune_single(Plan &P, hipStream_t st) {
   }
   const char *e = std::getenv("PLG_TUNE_SINGLE");
   if (e && std::string(e) == "0") return PLG_OK;
-  const int ns0 = P.nsplit;
-  P.nsplit = (int)std::max<int64_t>(ns0, std::min<int64_t>(32, g_ctx.V3 / 256));
   int rc;
   {
     std::lock_guard<std::mutex> lk(g_mu);
     if ((rc = ensure(&g_ctx.ws_c, &g_ctx.ws_c_bytes, c_bytes(P)))) return rc;
-    if ((rc = ensure(&g_ctx.ws_part, &g_ctx.ws_part_bytes, part_bytes(P)))) return rc;
+    if ((rc = ensure(&g_ctx.ws_part, &g_ctx.ws_part_bytes, part_bytes_tune(P)))) return rc;
   }
-  P.nsplit = ns0;
   P.c = g_ctx.ws_c;
   P.part = g_ctx.ws_part;
   P.part_bytes = g_ctx.ws_part_bytes;
@@ -928,6 +990,7 @@ static void release_static(Plan &P) {
   P.d_choff = nullptr;
   P.d_ms = P.d_pair_ms = P.d_mom_pair = nullptr;
   P.d_pidx = nullptr;
+  P.d_sep = nullptr;
 }
 
 static int correlator_static(Plan &P, ScratchCarver &sc, void *out, int out_precision, hipStream_t st) {
@@ -944,7 +1007,8 @@ static int correlator_static(Plan &P, ScratchCarver &sc, void *out, int out_prec
     probe.take(P.ms.size() * sizeof(int));
     probe.take(P.pair_ms.size() * sizeof(int));
     probe.take(P.mom_pair.size() * sizeof(int));
-    if (P.npairs > 0 && P.T == PLG_F32) probe.take(phase_index_bytes(g_ctx.V3, P.npairs));
+    if (plan_sep(P)) probe.take(P.sep.blob.size());
+    else if (P.npairs > 0 && P.T == PLG_F32) probe.take(phase_index_bytes(g_ctx.V3, P.npairs));
     probe.take(c_bytes(P));
     if (P.space == PLG_MOMENTUM_SPACE) probe.take(part_bytes(P));
     need = probe.used;
@@ -971,7 +1035,11 @@ static int correlator_static(Plan &P, ScratchCarver &sc, void *out, int out_prec
       (rc = put_async(sc, &P.d_ms, P.ms, st)) || (rc = put_async(sc, &P.d_pair_ms, P.pair_ms, st)) ||
       (rc = put_async(sc, &P.d_mom_pair, P.mom_pair, st)))
     return rc;
-  if (P.npairs > 0 && P.T == PLG_F32) {
+  if (plan_sep(P)) {
+    unsigned char *d = nullptr;
+    if ((rc = put_async(sc, &d, P.sep.blob, st))) return rc;
+    P.d_sep = d;
+  } else if (P.npairs > 0 && P.T == PLG_F32) {
     P.d_pidx = (uint16_t *)sc.take(phase_index_bytes(g_ctx.V3, P.npairs));
     ProjGeom g = plan_geom(P);
     if ((rc = launch_phase_index(g, P.d_pidx, st))) return fail(rc, "phase index launch failed");
@@ -1949,11 +2017,8 @@ int plg_batch_create(int kind, int group_mask, const plg_prop *props, int nprops
     P.own_ws = true;
     rc = ensure(&P.c, &P.c_bytes, c_bytes(P));
     if (!rc && (flags & PLG_BATCH_TUNE)) {
-      // room for the largest K-split candidate
-      const int ns0 = P.nsplit;
-      P.nsplit = (int)std::max<int64_t>(ns0, std::min<int64_t>(32, g_ctx.V3 / 256));
-      rc = ensure(&P.part, &P.part_bytes, part_bytes(P));
-      P.nsplit = ns0;
+      // room for every candidate (the largest K split, the separable form)
+      rc = ensure(&P.part, &P.part_bytes, part_bytes_tune(P));
       if (!rc) rc = autotune(P, nullptr);
     }
     if (!rc) rc = ensure(&P.part, &P.part_bytes, part_bytes(P));

@@ -189,6 +189,9 @@ struct ProjGeom {
   // 0 = a field, column-major (column n, site x at n*V3 + x).  The mixed
   // projection reads the tile-major layout only.
   int ctile;
+  // 1: part holds the separable projection's sums, momentum-major
+  // (part[m][n], one complex double each, see plg_project_sep.hip)
+  int sep = 0;
 };
 // The site correlator is stored tile-major: the 16 sites of a site tile are
 // contiguous per column, and the columns of a tile are contiguous, so that a
@@ -236,6 +239,10 @@ __host__ __device__ __forceinline__ const cx<float> *proj_shift(const ProjGeom &
 // the projection sum_s part[s] of column n, momentum m, in either layout
 __device__ __forceinline__ cx<double> part_value(const ProjGeom &g, const double *part, int64_t n, int m) {
   double re = 0, im = 0;
+  if (g.sep) {
+    const double *p = part + 2 * ((int64_t)m * g.ncols + n);
+    return cmake<double>(p[0], p[1]);
+  }
   if (g.npairs > 0) {
     const int pm = g.mom_pair[m];
     const int R = 2 * g.npairs, rc = pm & ~1;
@@ -276,6 +283,52 @@ inline size_t proj_part_bytes(int nsplit, int64_t ncols, int nmom, int npairs) {
   return npairs > 0 ? (size_t)nsplit * ncols * npairs * 32 + (size_t)ncols * sizeof(cx<float>)
                     : (size_t)nsplit * ncols * nmom * 16;
 }
+// Separable projection (plg_project_sep.hip): the phase of a momentum
+// factorises over the three directions, so the sums run direction by
+// direction in FP64 -- over x per line and over y per plane (k_sep_xy), then
+// over z (k_sep_z) -- instead of as one phase-matrix GEMM over the volume.
+constexpr int PROJ_SEP = 1;  // projection code of the separable form (both precisions)
+struct SepGeom {
+  int Lx, Ly, Lz;
+  int64_t ncols;
+  int nmom;
+  int NA;     // distinct |n_x| classes a (cos and sin sums over x)
+  int NXR;    // distinct n_x residues
+  int NYR;    // distinct n_y residues
+  int NS;     // distinct (n_x, n_y) residue pairs: planes of the per-plane sums
+  int NPART;  // threads per line of a chunk (the line's sites split in parts)
+  int W;      // sites of a part (Lx / NPART)
+  int PS, LS, RS;  // LDS strides of a staged chunk: part, line, column (elements)
+  // 1: NA <= 5, NYR <= 9, at most 9 momenta per (n_x, n_y) pair; the
+  // tables are padded to these counts (else 8, 16, 16) with zero phases and
+  // slots -1, so that the kernels' loops run without bounds
+  int small;
+  const cx<double> *ph1;   // [W][NAP]: cos, sin (2 pi a i / Lx)
+  const cx<double> *ph1p;  // [NPART][NAP]: cos, sin (2 pi a p W / Lx)
+  const cx<double> *ph2;   // [Ly][NYP]: cos, sin (2 pi r_y y / Ly)
+  const cx<double> *ph3;   // [Lz][Lz]: cos, sin (2 pi r z / Lz) at r * Lz + z
+  const int *xs;           // [NAP][2]: residue slot of +a and of -a (or -1)
+  const int *slot;         // [NXR][NYP]: (n_x, n_y) pair index or -1
+  const int *sl_m, *sl_rz; // [NS][NZP]: momentum index (or -1) and n_z residue of a pair's momenta
+};
+// host description of a plan's separable projection
+struct SepHost {
+  bool ok = false;
+  SepGeom g{};                     // scalars (pointers set by sep_geom)
+  std::vector<unsigned char> blob;  // the device tables, uploaded as one piece
+  size_t off_ph1 = 0, off_ph1p = 0, off_ph2 = 0, off_ph3 = 0, off_xs = 0, off_slot = 0, off_slm = 0,
+         off_slrz = 0;
+};
+// tables of the separable projection of scaled momenta ms (3 per momentum)
+// on L; false (h.ok) when the momenta or the lattice do not suit it
+void sep_build(const int L[3], int M, const std::vector<int> &ms, int64_t ncols, bool f64, SepHost &h);
+SepGeom sep_geom(const SepHost &h, const void *d_blob);
+// bytes of part for the separable projection: the sums part[m][n], then
+// the per-plane sums of the (n_x, n_y) pairs
+inline size_t sep_part_bytes(const SepGeom &g) {
+  return ((size_t)g.nmom + (size_t)g.Lz * g.NS) * (size_t)g.ncols * sizeof(cx<double>);
+}
+int launch_project_sep(const SepGeom &g, const void *c, int c_prec, double *part, hipStream_t st);
 int launch_ft_out(const ProjGeom &g, const double *part, int dof, int nt, int accumulate, void *out,
                   int out_prec, hipStream_t st);
 // sum over splits, source phase, scatter into the H layout
