@@ -13,10 +13,13 @@ FP64 plain path and the error check run on 16 of the time slices
 (t = 120..127, 0..7) from FP64 copies of those slices.
 
 A step is one pass over all sources:
-  mixed (the metric): FP32 contraction + split-bf16 (bf16x6) MFMA projection,
-        one batched plan replayed as a hipGraph;
-  fp64 plain:        FP64 contraction + FP64 MFMA projection, one plan per
-        source, launched source after source.
+  mixed (the metric): FP32 contraction + the projection the autotuner picks
+        (the separable FP64 sums over x, y, z, or the split-bf16 (bf16x6)
+        phase-matrix GEMM on MFMA), one batched plan replayed as a hipGraph;
+  fp64 plain:        FP64 contraction + the same projection form in FP64, one
+        plan per source, launched source after source.
+The batching workload also times both paths with the projection pinned to the
+phase-matrix GEMM (PLG_PROJ_SEP=0) and reports its MFMA use.
 value = sites/s of the mixed path (site = (source, x, t) inside the window),
 inputs resident in HBM before the timed region.
 
@@ -30,7 +33,7 @@ oracle (reference per-site bodies, FP64, OpenMP) on a bounded sample.
 Workload "udsc" (--workload udsc): lattice 32^3 x 64, one source, four
 flavour propagators (u, d, s, c), all 20 flavour products of the udsc baryon
 interpolators (200 operators), all 64 time slices, |n|^2 <= 9 (123 momenta);
-value = sites/s of the mixed path (FP32 contraction, split-bf16 projection),
+value = sites/s of the mixed path (FP32 contraction, autotuned projection),
 FP64 path and the mixed path's error beside it.
 
 Workload "tetra" (--workload tetra): the four-propagator table engine on
@@ -191,6 +194,7 @@ KERNEL_SOURCES = {
     "k_baryon": ["csrc/plg_baryon.hip", "csrc/plg_tile.h"],
     "k_mesmes": ["csrc/plg_mesmes.hip", "csrc/plg_tile.h"],
     "k_project_pair": ["csrc/plg_project_bf16.hip"],
+    "k_sep_xy": ["csrc/plg_project_sep.hip"],
     "k_udsc": ["csrc/plg_udsc.hip", "gen_udsc.py"],
     "k_tetra": ["csrc/plg_tetra.hip"],
 }
@@ -266,12 +270,23 @@ def roof(kernel, t_s, nsites, flops_site, bytes_site, bound_compute, cpeak, work
     return r
 
 
+def proj_dtype(st_ms, split):
+    """the arithmetic of a mixed plan, from its stage names (the autotuner
+    picks the separable FP64 projection or the split-bf16 GEMM)"""
+    if any(k.startswith("project_sep") for k in st_ms):
+        return "f32 contraction + fp64 separable projection (fp64 sums)"
+    return f"f32 contraction + bf16x{split} MFMA projection (fp32 acc, fp64 sums)"
+
+
 def roofline(st_ms, nsites, nmom, nch, split, contract_flops, prop_bytes, workload):
     """the dominant stage against its roofline: a contraction (FP32 VALU; HBM:
     compulsory propagator read + site-correlator write) or the projection
     (BF16 MFMA, useful flops; HBM: site-correlator read)"""
     dom = max(st_ms, key=st_ms.get)
     t = st_ms[dom] / 1e3
+    if dom.startswith("project_sep"):
+        return roof(dom, t, nsites, 0, nch * 8, "hbm", 1.0, workload,
+                    f"{nch * 8} B FP32 site correlator read per site")
     if dom.startswith("contract"):
         return roof(dom, t, nsites, contract_flops, prop_bytes + nch * 8, "valu", FP32_VALU_PEAK_TFS, workload,
                     f"{contract_flops} implemented FP32 flop; {prop_bytes} B FP32 propagator read + "
@@ -347,6 +362,54 @@ ERR_DEFS = {"normwise": "|C - C_ref| / max over momenta |C_ref| per (group, row,
             "pointwise": "|C - C_ref| / |C_ref| per element (unbounded at accidental zeros of C_ref)"}
 
 
+def run_gemm_projection(args, torch, capi, stream, srcs, v32, v64, jobs, sizes, moms, kind, mask, dev,
+                        nsites, nout, out_ref):
+    """the same batch with the projection pinned to the phase-matrix GEMM
+    (PLG_PROJ_SEP=0: split-bf16 MFMA for the mixed path, FP64 MFMA for the
+    plain path): its step time, its MFMA use and the like-for-like ratio"""
+    os.environ["PLG_PROJ_SEP"] = "0"
+    try:
+        mixed = capi.Batch(kind, mask, v32, jobs, moms, capi.MODE_MIXED, split=args.split,
+                           flags=capi.BATCH_GRAPH | capi.BATCH_TUNE)
+        out = torch.zeros((nout, 2), dtype=torch.float64, device=dev)
+        mixed.run(out, stream)
+        ms_mx, _ = timed(torch, stream, lambda: mixed.run(out, stream), args.steps)
+        st = stage_ms(mixed, out, stream)
+        d = (out - out_ref).abs().max().item() / max(out_ref.abs().max().item(), 1e-300)
+        mixed.close()
+        plain = [capi.Batch(kind, mask, [v64[s]], [(x0, w, 0, 0)], moms, capi.MODE_F64, flags=capi.BATCH_TUNE)
+                 for s, (x0, w) in enumerate(srcs)]
+        outs = [torch.zeros((n, 2), dtype=torch.float64, device=dev) for n in sizes]
+
+        def run_plain():
+            for b, o in zip(plain, outs):
+                b.run(o, stream)
+
+        run_plain()
+        k64 = args.fp64_steps or max(1, args.steps // 2)
+        ms_64, _ = timed(torch, stream, run_plain, k64)
+        for b in plain:
+            b.close()
+    finally:
+        del os.environ["PLG_PROJ_SEP"]
+    pname = next((k for k in st if k.startswith("project")), None)
+    r = {"note": "PLG_PROJ_SEP=0: the projection as the phase-matrix GEMM on MFMA",
+         "mixed": {"value": nsites / (ms_mx / 1e3), "ms_per_step": ms_mx, "stages_ms": st,
+                   "dtype": proj_dtype(st, args.split),
+                   "max_abs_diff_vs_autotuned_rel": d},
+         "fp64_plain": {"value": nsites / (ms_64 / 1e3), "ms_per_step": ms_64, "steps": k64},
+         "speedup_vs_fp64": ms_64 / ms_mx}
+    if pname:
+        t = st[pname] / 1e3
+        r["projection_mfma"] = roof(pname, t, nsites, 8 * len(moms) * 80, 80 * 8, "mfma", BF16_PEAK_TFS,
+                                    "batching", f"useful 8 flop x {len(moms)} momenta x 80 channels per site; "
+                                                f"{80 * 8} B site correlator read")
+        pm = r["projection_mfma"]
+        comp = pm if pm["bound"] == "mfma" else pm["mfma"]
+        pm["mfma_executed_frac"] = comp["frac"] * args.split  # split products x useful flops
+    return r
+
+
 def run_batching(args, torch, capi, dev, stream):
     capi.init(L, 0)
     V3 = L[0] * L[1] * L[2]
@@ -394,7 +457,7 @@ def run_batching(args, torch, capi, dev, stream):
     res = {"metric": METRIC, "value": nsites / (ms_mx / 1e3), "unit": "sites/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_mx,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": f"f32 contraction + bf16x{args.split} MFMA projection (fp32 acc, fp64 sums)",
+           "dtype": proj_dtype(st_ms, args.split),
            "data": "synthetic",
            "config": {"workload": "batching: lattice 32^3x64, 16 seeded sources, windows 8-32, "
                                   "16 mesons + proton + Delta++ (3x16), |n|^2<=16 (257 momenta)",
@@ -431,14 +494,18 @@ def run_batching(args, torch, capi, dev, stream):
         res["fp64_plain"] = {"value": nsites / (ms_64 / 1e3), "unit": "sites/s",
                              "ms_per_step": ms_64, "steps": k64,
                              "stages_ms_source0": st64, "sites_source0": srcs[0][1] * V3,
-                             "algorithm": "the mixed path's: factorised contraction, projection on "
-                                          "(n, -n) momentum pairs; FP64 throughout"}
+                             "algorithm": "the mixed path's: factorised contraction, the autotuner's "
+                                          "projection (separable sums or the (n, -n)-paired GEMM); "
+                                          "FP64 throughout",
+                             "projection": "separable" if "project_sep_f64" in st64 else "gemm"}
         res["speedup_vs_fp64"] = ms_64 / ms_mx
         res["rel_err_vs_fp64"] = {**err_summary(A, B, moms),
                                   "per_group_max": [float(x) for x in rel_err_block(A, B).max(axis=1)],
                                   "scope": "every source, time slice, group, channel and momentum",
                                   "definitions": ERR_DEFS}
         del plain, A, B, As, Bs
+        res["gemm_projection"] = run_gemm_projection(args, torch, capi, stream, srcs, v32, v64, jobs, sizes,
+                                                     moms, kind, mask, dev, nsites, nout, out_mx)
         res["dropin"] = run_dropin(args, torch, capi, stream, srcs, v32, v64, sizes, moms, kind, mask, dev)
 
     res["stages_ms"] = st_ms
@@ -586,7 +653,7 @@ def run_largest(args, torch, capi, dev, stream):
     res = {"metric": METRIC, "value": nsites / (ms_mx / 1e3), "unit": "sites/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_mx,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": f"f32 contraction + bf16x{args.split} MFMA projection (fp32 acc, fp64 sums)",
+           "dtype": proj_dtype(st_ms, args.split),
            "data": "synthetic",
            "config": {"workload": "largest: lattice 64^3x128, one source, light + heavy FP32 "
                                   "propagators, 16 heavy-light mesons + 4 meson-meson, "
@@ -734,7 +801,7 @@ def run_udsc(args, torch, capi, dev, stream):
            "value": nsites / (ms_mx / 1e3), "unit": "sites/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_mx,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": f"f32 contraction + bf16x6 MFMA projection (fp32 acc, fp64 sums)",
+           "dtype": proj_dtype(st_ms, 6),
            "data": "synthetic",
            "config": {"workload": "udsc: lattice 32^3x64, one source, u/d/s/c propagators, all 20 flavour "
                                   "products (200 operators), 64 time slices, |n|^2<=9 (123 momenta)",
@@ -867,7 +934,8 @@ def run_tetra(args, torch, capi, dev, stream):
     res = {"metric": "tetraquark table 2pt sites/s (16 operators x 768 terms), mixed precision",
            "value": nsites / (ms_mx / 1e3), "unit": "sites/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms_mx, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f32 contraction + bf16x6 MFMA projection (fp32 acc, fp64 sums)",
+           "vs_baseline": None, "dtype": "f32 contraction + autotuned projection (fp64 separable sums "
+                                          "or bf16x6 MFMA GEMM), fp64 sums",
            "data": "synthetic",
            "config": {"workload": "tetra: lattice 32^3x64, one source, light/light/heavy/heavy propagators, "
                                   "synthetic 16 x 768-term table in the reference's colour patterns, 64 time "
@@ -1024,7 +1092,8 @@ def run_learned(args, torch, capi, dev, stream):
     res = {"metric": METRIC, "value": r["sites"] / (r["flow_ms"] / 1e3), "unit": "sites/s", "n_gpus": 1,
            "steps": 1, "warmup": 0, "ms_per_step": r["flow_ms"], "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None,
-           "dtype": "f64 two-point path (labels); f32 + bf16x6 at n = 0 (inputs); bf16 MLP (fp32 accumulation)",
+           "dtype": "f64 two-point path (labels); f32 contraction + autotuned projection at n = 0 (inputs); "
+                    "bf16 MLP (fp32 accumulation)",
            "data": "synthetic",
            "config": {"workload": "learned: lattice 32^3x64, 256 sources, propagators base + 5% "
                                   "perturbation, MLP 17-256-256-11 per time slice, train 32 / bias-correct 32 / "
