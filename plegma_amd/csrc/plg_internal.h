@@ -295,20 +295,21 @@ struct SepGeom {
   int NA;     // distinct |n_x| classes a (cos and sin sums over x)
   int NXR;    // distinct n_x residues
   int NYR;    // distinct n_y residues
+  int NB;     // distinct |n_y| classes b (cos and sin sums over y)
   int NS;     // distinct (n_x, n_y) residue pairs: planes of the per-plane sums
   int NPART;  // threads per line of a chunk (the line's sites split in parts)
   int W;      // sites of a part (Lx / NPART)
   int PS, LS, RS;  // LDS strides of a staged chunk: part, line, column (elements)
-  // 1: NA <= 5, NYR <= 9, at most 9 momenta per (n_x, n_y) pair; the
-  // tables are padded to these counts (else 8, 16, 16) with zero phases and
+  // 1: NA <= 5, NB <= 5, at most 9 momenta per (n_x, n_y) pair; the
+  // tables are padded to these counts (else 8, 8, 16) with zero phases and
   // slots -1, so that the kernels' loops run without bounds
   int small;
   const cx<double> *ph1;   // [W][NAP]: cos, sin (2 pi a i / Lx)
   const cx<double> *ph1p;  // [NPART][NAP]: cos, sin (2 pi a p W / Lx)
-  const cx<double> *ph2;   // [Ly][NYP]: cos, sin (2 pi r_y y / Ly)
+  const cx<double> *ph2;   // [Ly][NBP]: cos, sin (2 pi b y / Ly)
   const cx<double> *ph3;   // [Lz][Lz]: cos, sin (2 pi r z / Lz) at r * Lz + z
   const int *xs;           // [NAP][2]: residue slot of +a and of -a (or -1)
-  const int *slot;         // [NXR][NYP]: (n_x, n_y) pair index or -1
+  const int *slot;         // [NXR][NBP][2]: pair index of (n_x, +b) and (n_x, -b), or -1
   const int *sl_m, *sl_rz; // [NS][NZP]: momentum index (or -1) and n_z residue of a pair's momenta
 };
 // host description of a plan's separable projection

@@ -20,8 +20,9 @@
 //            i < W read as uniform (scalar) loads, rotated by the part's
 //            phase and combined by lane exchanges.  The cos and sin sums of
 //            a = |n_x| give both n_x = a and n_x = -a.  Thread (column,
-//            n_x residue) then accumulates the plane's sums over y for every
-//            n_y residue; the (n_x, n_y) pairs the momenta use are written.
+//            n_x residue) then accumulates the plane's cos and sin sums over y
+//            for every |n_y| class; the (n_x, n_y) pairs the momenta use are
+//            written.
 //  k_sep_z   thread = column, workgroup row = (n_x, n_y) pair: the sums over
 //            z of the pair's momenta, written momentum-major for the
 //            finalize kernel.
@@ -56,12 +57,7 @@ template <> struct SepVec<double, 1> {
   typedef double v __attribute__((ext_vector_type(2)));
 };
 
-__device__ __forceinline__ cx<double> cmul_conj_phase(cx<double> ph, cx<double> v) {
-  // (cos - i sin) v
-  return cmake<double>(ph.re * v.re + ph.im * v.im, ph.re * v.im - ph.im * v.re);
-}
-
-template <typename T, int NPART, int MAXA, int MAXR>
+template <typename T, int NPART, int MAXA, int MAXB>
 __global__ __launch_bounds__(256) void k_sep_xy(SepGeom s, const cx<T> *__restrict__ c,
                                                 cx<double> *__restrict__ H) {
   using Q = SepT<T>;
@@ -93,12 +89,13 @@ __global__ __launch_bounds__(256) void k_sep_xy(SepGeom s, const cx<T> *__restri
   // line role: part p1 of line l1 of column c1
   const int p1 = tid % NPART, c1 = (tid / NPART) & 15, l1 = tid / (NPART * 16);
   const int base1 = c1 * s.RS + l1 * s.LS + p1 * s.PS;
-  // plane role: column c2, n_x residue slot j2
+  // plane role: column c2, n_x residue slot j2; cos and sin sums over y for
+  // the |n_y| classes b
   const int c2 = tid & 15, j2 = tid >> 4;
   const bool role2 = j2 < s.NXR;
-  cx<double> h[MAXR];
+  cx<double> hc[MAXB], hs[MAXB];
 #pragma unroll
-  for (int r = 0; r < MAXR; r++) h[r] = cmake<double>(0, 0);
+  for (int b = 0; b < MAXB; b++) hc[b] = hs[b] = cmake<double>(0, 0);
 
   const int64_t tile_plane = (int64_t)z * (Lx * s.Ly) / 16;
   const int nchunk = s.Ly / CHL;
@@ -181,15 +178,27 @@ __global__ __launch_bounds__(256) void k_sep_xy(SepGeom s, const cx<T> *__restri
         const int y = q * CHL + l;
         const cx<double> g = gb[(l * s.NXR + j2) * 16 + c2];
 #pragma unroll
-        for (int r = 0; r < MAXR; r++) h[r] = h[r] + cmul_conj_phase(s.ph2[y * MAXR + r], g);
+        for (int b = 0; b < MAXB; b++) {  // (classes b >= NB have zero phases)
+          const cx<double> ph = s.ph2[y * MAXB + b];
+          hc[b].re += ph.re * g.re;
+          hc[b].im += ph.re * g.im;
+          hs[b].re += ph.im * g.re;
+          hs[b].im += ph.im * g.im;
+        }
       }
     }
   }
   if (role2 && col0 + c2 < s.ncols) {
+    // n_y = b: C - i S; n_y = -b (mod Ly): C + i S
 #pragma unroll
-    for (int r = 0; r < MAXR; r++) {
-      const int sl = s.slot[j2 * MAXR + r];
-      if (sl >= 0) gstore(H + ((int64_t)z * s.NS + sl) * s.ncols + col0 + c2, h[r]);
+    for (int b = 0; b < MAXB; b++) {
+      const int sp = s.slot[(j2 * MAXB + b) * 2], sm = s.slot[(j2 * MAXB + b) * 2 + 1];
+      if (sp >= 0)
+        gstore(H + ((int64_t)z * s.NS + sp) * s.ncols + col0 + c2,
+               cmake<double>(hc[b].re + hs[b].im, hc[b].im - hs[b].re));
+      if (sm >= 0)
+        gstore(H + ((int64_t)z * s.NS + sm) * s.ncols + col0 + c2,
+               cmake<double>(hc[b].re - hs[b].im, hc[b].im + hs[b].re));
     }
   }
 }
@@ -208,7 +217,13 @@ __global__ __launch_bounds__(256) void k_sep_z(SepGeom s, const cx<double> *__re
   for (int z = 0; z < s.Lz; z++) {
     const cx<double> h = gload(H + ((int64_t)z * s.NS + sl) * s.ncols + col);
 #pragma unroll
-    for (int j = 0; j < MAXZ; j++) acc[j] = acc[j] + cmul_conj_phase(s.ph3[rz[j] * s.Lz + z], h);
+    for (int j = 0; j < MAXZ; j++) {  // (cos - i sin) h
+      const cx<double> ph = s.ph3[rz[j] * s.Lz + z];
+      acc[j].re += ph.re * h.re;
+      acc[j].re += ph.im * h.im;
+      acc[j].im += ph.re * h.im;
+      acc[j].im -= ph.im * h.re;
+    }
   }
 #pragma unroll
   for (int j = 0; j < MAXZ; j++)
@@ -231,11 +246,12 @@ void sep_build(const int L[3], int M, const std::vector<int> &ms, int64_t ncols,
     ry[m] = ms[3 * m + 1] / (M / Ly);
     rz[m] = ms[3 * m + 2] / (M / Lz);
   }
-  std::map<int, int> amap, rxmap, rymap;
+  std::map<int, int> amap, rxmap, rymap, bmap;
   for (int m = 0; m < nmom; m++) {
     amap[std::min(rx[m], Lx - rx[m])] = 0;
     rxmap[rx[m]] = 0;
     rymap[ry[m]] = 0;
+    bmap[std::min(ry[m], Ly - ry[m])] = 0;
   }
   int k = 0;
   for (auto &e : amap) e.second = k++;
@@ -243,6 +259,8 @@ void sep_build(const int L[3], int M, const std::vector<int> &ms, int64_t ncols,
   for (auto &e : rxmap) e.second = k++;
   k = 0;
   for (auto &e : rymap) e.second = k++;
+  k = 0;
+  for (auto &e : bmap) e.second = k++;
   SepGeom &g = h.g;
   g.Lx = Lx;
   g.Ly = Ly;
@@ -252,7 +270,8 @@ void sep_build(const int L[3], int M, const std::vector<int> &ms, int64_t ncols,
   g.NA = (int)amap.size();
   g.NXR = (int)rxmap.size();
   g.NYR = (int)rymap.size();
-  if (g.NA > 8 || g.NXR > 16 || g.NYR > 16) return;
+  g.NB = (int)bmap.size();
+  if (g.NA > 8 || g.NXR > 16 || g.NB > 8) return;
   // chunk: CHL lines (a power of two dividing Ly and 16), whole tiles, the
   // most lines whose staged sites and line sums fit the LDS
   auto fits = [&](int cl) {
@@ -277,14 +296,18 @@ void sep_build(const int L[3], int M, const std::vector<int> &ms, int64_t ncols,
   int maxcnt = 0;
   for (auto &e : pairs) maxcnt = std::max(maxcnt, (int)e.second.size());
   if (maxcnt > 16) return;
-  g.small = g.NA <= 5 && g.NYR <= 9 && maxcnt <= 9;
-  // tables padded to the kernel's class, residue and momentum counts
-  const int NAP = g.small ? 5 : 8, NYP = g.small ? 9 : 16, NZP = g.small ? 9 : 16;
-  std::vector<int> slot(g.NXR * NYP, -1), slm(g.NS * NZP, -1), slrz(g.NS * NZP, 0);
+  g.small = g.NA <= 5 && g.NB <= 5 && maxcnt <= 9;
+  // tables padded to the kernel's class and momentum counts
+  const int NAP = g.small ? 5 : 8, NBP = g.small ? 5 : 8, NZP = g.small ? 9 : 16;
+  std::vector<int> ryres(g.NYR);
+  for (auto &e : rymap) ryres[e.second] = e.first;
+  std::vector<int> slot(g.NXR * NBP * 2, -1), slm(g.NS * NZP, -1), slrz(g.NS * NZP, 0);
   {
     int sidx = 0;
     for (auto &e : pairs) {
-      slot[e.first.first * NYP + e.first.second] = sidx;
+      // the pair's n_y residue as class b, sign + (r = b) or - (r = Ly - b != b)
+      const int r = ryres[e.first.second], b = std::min(r, Ly - r);
+      slot[(e.first.first * NBP + bmap[b]) * 2 + (r == b ? 0 : 1)] = sidx;
       for (size_t j = 0; j < e.second.size(); j++) {
         slm[sidx * NZP + j] = e.second[j];
         slrz[sidx * NZP + j] = rz[e.second[j]];
@@ -306,7 +329,7 @@ void sep_build(const int L[3], int M, const std::vector<int> &ms, int64_t ncols,
     const double t = 2.0 * M_PI * (double)(num % den) / (double)den;
     return std::make_pair(std::cos(t), std::sin(t));
   };
-  std::vector<double> ph1(2 * g.W * NAP, 0.0), ph1p(2 * g.NPART * NAP, 0.0), ph2(2 * Ly * NYP, 0.0), ph3;
+  std::vector<double> ph1(2 * g.W * NAP, 0.0), ph1p(2 * g.NPART * NAP, 0.0), ph2(2 * Ly * NBP, 0.0), ph3;
   for (auto &e : amap)
     for (int i = 0; i < g.W; i++) {
       auto cs = phase((int64_t)e.first * i, Lx);
@@ -319,11 +342,11 @@ void sep_build(const int L[3], int M, const std::vector<int> &ms, int64_t ncols,
       ph1p[2 * (p * NAP + e.second)] = cs.first;
       ph1p[2 * (p * NAP + e.second) + 1] = cs.second;
     }
-  for (auto &e : rymap)
+  for (auto &e : bmap)
     for (int y = 0; y < Ly; y++) {
       auto cs = phase((int64_t)e.first * y, Ly);
-      ph2[2 * (y * NYP + e.second)] = cs.first;
-      ph2[2 * (y * NYP + e.second) + 1] = cs.second;
+      ph2[2 * (y * NBP + e.second)] = cs.first;
+      ph2[2 * (y * NBP + e.second) + 1] = cs.second;
     }
   for (int r = 0; r < Lz; r++)
     for (int z = 0; z < Lz; z++) {
@@ -363,15 +386,15 @@ SepGeom sep_geom(const SepHost &h, const void *d_blob) {
   return g;
 }
 
-template <typename T, int MAXA, int MAXR>
+template <typename T, int MAXA, int MAXB>
 static int launch_xy(const SepGeom &g, const cx<T> *c, cx<double> *H, hipStream_t st) {
   dim3 block(256), grid((unsigned)((g.ncols + 15) / 16), (unsigned)g.Lz);
   switch (g.NPART) {
-  case 1: k_sep_xy<T, 1, MAXA, MAXR><<<grid, block, 0, st>>>(g, c, H); break;
-  case 2: k_sep_xy<T, 2, MAXA, MAXR><<<grid, block, 0, st>>>(g, c, H); break;
-  case 4: k_sep_xy<T, 4, MAXA, MAXR><<<grid, block, 0, st>>>(g, c, H); break;
-  case 8: k_sep_xy<T, 8, MAXA, MAXR><<<grid, block, 0, st>>>(g, c, H); break;
-  case 16: k_sep_xy<T, 16, MAXA, MAXR><<<grid, block, 0, st>>>(g, c, H); break;
+  case 1: k_sep_xy<T, 1, MAXA, MAXB><<<grid, block, 0, st>>>(g, c, H); break;
+  case 2: k_sep_xy<T, 2, MAXA, MAXB><<<grid, block, 0, st>>>(g, c, H); break;
+  case 4: k_sep_xy<T, 4, MAXA, MAXB><<<grid, block, 0, st>>>(g, c, H); break;
+  case 8: k_sep_xy<T, 8, MAXA, MAXB><<<grid, block, 0, st>>>(g, c, H); break;
+  case 16: k_sep_xy<T, 16, MAXA, MAXB><<<grid, block, 0, st>>>(g, c, H); break;
   default: return PLG_ERR_UNSUPPORTED;
   }
   return PLG_OK;
@@ -382,11 +405,11 @@ int launch_project_sep(const SepGeom &g, const void *c, int c_prec, double *part
   cx<double> *H = P + (int64_t)g.nmom * g.ncols;
   int rc;
   if (c_prec == PLG_F64)
-    rc = g.small ? launch_xy<double, 5, 9>(g, (const cx<double> *)c, H, st)
-                 : launch_xy<double, 8, 16>(g, (const cx<double> *)c, H, st);
+    rc = g.small ? launch_xy<double, 5, 5>(g, (const cx<double> *)c, H, st)
+                 : launch_xy<double, 8, 8>(g, (const cx<double> *)c, H, st);
   else
-    rc = g.small ? launch_xy<float, 5, 9>(g, (const cx<float> *)c, H, st)
-                 : launch_xy<float, 8, 16>(g, (const cx<float> *)c, H, st);
+    rc = g.small ? launch_xy<float, 5, 5>(g, (const cx<float> *)c, H, st)
+                 : launch_xy<float, 8, 8>(g, (const cx<float> *)c, H, st);
   if (rc) return rc;
   dim3 block(256), grid((unsigned)((g.ncols + 255) / 256), (unsigned)g.NS);
   if (g.small) k_sep_z<9><<<grid, block, 0, st>>>(g, H, P);
