@@ -216,6 +216,7 @@ struct Plan {
   int64_t nsites = 0;
   // tuning
   int mtile = 64, mbw = 9, nsplit = 1, tpb = 1;
+  int gemm_code = 0;  // the GEMM tile picked before the separable form (its fallback)
   // device tables (owned by the plan)
   void *d_props = nullptr;
   RowDev *d_rows = nullptr;
@@ -553,6 +554,7 @@ static void choose_tiles(Plan &P) {
   const char *se = std::getenv("PLG_PROJ_SEP");
   if (P.space == PLG_MOMENTUM_SPACE && P.nmom > 0 && !(se && std::string(se) == "0"))
     sep_build(g_ctx.L, g_ctx.M, P.ms, ncols, P.T == PLG_F64, P.sep);
+  P.gemm_code = P.T == PLG_F64 ? P.mtile : P.mbw;
   if (P.sep.ok && !(P.T == PLG_F32 && std::getenv("PLG_PROJ_CODE"))) {
     if (P.T == PLG_F64) P.mtile = PROJ_SEP;
     else P.mbw = PROJ_SEP;
@@ -832,7 +834,8 @@ static int autotune(Plan &P, hipStream_t st) {
   const int nout = (int)P.csel.size();
   std::vector<int> codes = P.T == PLG_F64 ? std::vector<int>{32, 64} : proj_candidates(std::max(P.npairs - P.zero_pair, 1));
   double best_p = 1e30;
-  int best_code = P.T == PLG_F64 ? P.mtile : P.mbw, best_ns = P.nsplit;
+  // the separable form wins only where it was timed
+  int best_code = P.gemm_code, best_ns = P.nsplit;
   for (int code : codes)
     for (int ns : {1, 2, 4, 8, 16, 32}) {
       const size_t need = proj_part_bytes(ns, (int64_t)P.rows.size() * P.nch_c, std::max(P.nmom, 1), P.npairs);
@@ -993,26 +996,35 @@ static void release_static(Plan &P) {
   P.d_sep = nullptr;
 }
 
+// device scratch a static call takes: its tables, the site correlator and
+// the projection's partial sums (ScratchCarver's 256-byte pieces)
+static size_t static_need(const Plan &P) {
+  ScratchCarver probe{nullptr};
+  probe.take(2 * (P.T == PLG_F64 ? sizeof(PropDev<double>) : sizeof(PropDev<float>)));
+  probe.take(P.rows.size() * sizeof(RowDev));
+  probe.take(P.orows.size() * sizeof(OutRowDev));
+  probe.take(P.csel.size() * sizeof(int));
+  probe.take(P.ch_off.size() * sizeof(int64_t));
+  probe.take(P.ms.size() * sizeof(int));
+  probe.take(P.pair_ms.size() * sizeof(int));
+  probe.take(P.mom_pair.size() * sizeof(int));
+  if (plan_sep(P)) probe.take(P.sep.blob.size());
+  else if (P.npairs > 0 && P.T == PLG_F32) probe.take(phase_index_bytes(g_ctx.V3, P.npairs));
+  probe.take(c_bytes(P));
+  if (P.space == PLG_MOMENTUM_SPACE) probe.take(part_bytes(P));
+  return probe.used;
+}
+
 static int correlator_static(Plan &P, ScratchCarver &sc, void *out, int out_precision, hipStream_t st) {
   int rc;
-  // sizes first: nothing is copied or launched unless the whole call fits
-  size_t need = 0;
-  {
-    ScratchCarver probe{nullptr};
-    probe.take(2 * (P.T == PLG_F64 ? sizeof(PropDev<double>) : sizeof(PropDev<float>)));
-    probe.take(P.rows.size() * sizeof(RowDev));
-    probe.take(P.orows.size() * sizeof(OutRowDev));
-    probe.take(P.csel.size() * sizeof(int));
-    probe.take(P.ch_off.size() * sizeof(int64_t));
-    probe.take(P.ms.size() * sizeof(int));
-    probe.take(P.pair_ms.size() * sizeof(int));
-    probe.take(P.mom_pair.size() * sizeof(int));
-    if (plan_sep(P)) probe.take(P.sep.blob.size());
-    else if (P.npairs > 0 && P.T == PLG_F32) probe.take(phase_index_bytes(g_ctx.V3, P.npairs));
-    probe.take(c_bytes(P));
-    if (P.space == PLG_MOMENTUM_SPACE) probe.take(part_bytes(P));
-    need = probe.used;
+  // sizes first: nothing is copied or launched unless the whole call fits;
+  // the separable projection's per-plane sums take more room than the GEMM's
+  // partials, so a call that fits only with the GEMM uses the GEMM
+  if (plan_sep(P) && static_need(P) > PLG_STATIC_SCRATCH_BYTES) {
+    if (P.T == PLG_F64) P.mtile = P.gemm_code;
+    else P.mbw = P.gemm_code;
   }
+  const size_t need = static_need(P);
   if (need > PLG_STATIC_SCRATCH_BYTES)
     return fail(PLG_ERR_UNSUPPORTED, "plg_correlator_static: the correlator needs " + std::to_string(need) +
                                          " bytes of device scratch, above the static maximum of " +
