@@ -57,8 +57,9 @@ template <> struct SepVec<double, 1> {
   typedef double v __attribute__((ext_vector_type(2)));
 };
 
+// small tables: three waves per SIMD
 template <typename T, int NPART, int MAXA, int MAXB>
-__global__ __launch_bounds__(256) void k_sep_xy(SepGeom s, const cx<T> *__restrict__ c,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MAXA <= 5 ? 3 : 1))) void k_sep_xy(SepGeom s, const cx<T> *__restrict__ c,
                                                 cx<double> *__restrict__ H) {
   using Q = SepT<T>;
   typedef typename SepVec<T, Q::EPU>::v vec;
@@ -99,7 +100,8 @@ __global__ __launch_bounds__(256) void k_sep_xy(SepGeom s, const cx<T> *__restri
 
   const int64_t tile_plane = (int64_t)z * (Lx * s.Ly) / 16;
   const int nchunk = s.Ly / CHL;
-  // the chunk's 16-byte units, loaded one chunk ahead of the sums
+  // the chunk's 16-byte units (loading the next chunk during the sums holds
+  // 32 more registers: two waves per SIMD instead of three, measured slower)
   vec v[KU];
   auto fetch = [&](int q) {
     const int64_t tile0 = tile_plane + (int64_t)q * (SCHa / 16);
@@ -113,8 +115,8 @@ __global__ __launch_bounds__(256) void k_sep_xy(SepGeom s, const cx<T> *__restri
         v[k] = vec{};
     }
   };
-  fetch(0);
   for (int q = 0; q < nchunk; q++) {
+    fetch(q);
     __syncthreads();  // the previous chunk's sums (which share the LDS) are read
 #pragma unroll
     for (int k = 0; k < KU; k++) {
@@ -125,7 +127,6 @@ __global__ __launch_bounds__(256) void k_sep_xy(SepGeom s, const cx<T> *__restri
       }
     }
     __syncthreads();
-    if (q + 1 < nchunk) fetch(q + 1);  // in flight while this chunk is summed
     // sums over the part's sites: C_a = sum cos(2 pi a i / Lx) c, S_a likewise
     cx<double> C[MAXA], S[MAXA];
 #pragma unroll
