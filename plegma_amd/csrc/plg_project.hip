@@ -15,7 +15,9 @@
 // in FP64 as a per-momentum constant phase (the finalize kernel).
 //
 //  k_project_f64   v_mfma_f64_16x16x4_f64, FP64 operands and accumulation.
-//  (the split-bf16 projection of the mixed path is in plg_project_bf16.hip)
+//  (the split-bf16 GEMM of the mixed path is in plg_project_bf16.hip; the
+//  separable sums that both paths use where the momenta suit them, in
+//  plg_project_sep.hip)
 #include "plg_internal.h"
 
 namespace plg {

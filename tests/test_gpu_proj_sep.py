@@ -101,3 +101,46 @@ def test_separable_is_the_batch_projection():
         assert np.abs(a - g).max() <= 1e-12 * np.abs(g).max()
     finally:
         capi.finalize()
+
+
+def test_separable_custom_momentum_list(monkeypatch):
+    """a momentum list in no particular order, with repeated momenta and
+    momenta aliased by +-L (the same phases): every entry equals the entry of
+    its momentum in the createMom result (checked against the oracle above),
+    and the GEMM (PLG_PROJ_SEP=0) agrees"""
+    torch = G.torch_cuda()
+    L, q2, src, totalT = CASES[0]
+    capi.init(L, 0)
+    try:
+        t, v = G.make_prop(L, 4242, 3, src[3], 0.5)
+        base = np.array(capi.momlist(q2)).reshape(-1, 3)
+        rng = np.random.default_rng(5)
+        pick = rng.permutation(len(base))[:40]
+        pick = np.concatenate([pick, pick[:5]])            # repeated momenta
+        custom = base[pick].copy()
+        custom[3] += np.array([L[0], 0, 0])                # aliased by +Lx
+        custom[7] -= np.array([0, L[1], L[2]])             # aliased by -Ly, -Lz
+        mom_list = custom.tolist()
+
+        def run(moms):
+            n = capi.correlator_size(capi.TWOPT, MASK, src, totalT, len(moms))
+            out = torch.full((n, 2), float("nan"), dtype=torch.float64, device="cuda")
+            capi.correlator(capi.TWOPT, MASK, v, v, src, totalT, moms, capi.MODE_F64, capi.MOMENTUM_SPACE,
+                            capi.F64, out)
+            h = G.host(out).reshape(n, 2)
+            ng, nd, nch = capi.kind_shape(capi.TWOPT, MASK)
+            _, _, lt = capi.window(src, totalT)
+            return (h[:, 0] + 1j * h[:, 1]).reshape(ng, nd, lt, len(moms), nch)
+
+        ref = run(base.tolist())
+        got = run(mom_list)
+        err = G.rel_err(got, ref[:, :, :, pick])
+        print(f"[gpu] custom list vs createMom entries {err:.2e}", flush=True)
+        assert err < 1e-12
+        monkeypatch.setenv("PLG_PROJ_SEP", "0")
+        gemm = run(mom_list)
+        err = G.rel_err(gemm, got)
+        print(f"[gpu] custom list, separable vs GEMM {err:.2e}", flush=True)
+        assert err < 1e-12
+    finally:
+        capi.finalize()
