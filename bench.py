@@ -194,7 +194,7 @@ KERNEL_SOURCES = {
     "k_baryon": ["csrc/plg_baryon.hip", "csrc/plg_tile.h"],
     "k_mesmes": ["csrc/plg_mesmes.hip", "csrc/plg_tile.h"],
     "k_project_pair": ["csrc/plg_project_bf16.hip"],
-    "k_sep_xy": ["csrc/plg_project_sep.hip"],
+    "k_sep_xy": ["csrc/plg_project_sep.hip", "csrc/plg_tile.h"],
     "k_udsc": ["csrc/plg_udsc.hip", "gen_udsc.py"],
     "k_tetra": ["csrc/plg_tetra.hip"],
 }

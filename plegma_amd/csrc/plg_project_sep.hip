@@ -18,7 +18,8 @@
 //            LDS; each line is summed by NPART threads (NPART * CHL = 16),
 //            each over W = Lx / NPART sites with the phases of its offset
 //            i < W read as uniform (scalar) loads, rotated by the part's
-//            phase and combined by lane exchanges.  The cos and sin sums of
+//            phase and combined by DPP row exchanges (the line's parts are
+//            neighbouring lanes).  The cos and sin sums of
 //            a = |n_x| give both n_x = a and n_x = -a.  Thread (column,
 //            n_x residue) then accumulates the plane's cos and sin sums over y
 //            for every |n_y| class; the (n_x, n_y) pairs the momenta use are
@@ -32,6 +33,7 @@
 // pass (16 for 8-byte reads, 8 for 16-byte reads), so the lanes of a pass
 // (consecutive parts, then columns) read distinct banks.
 #include "plg_internal.h"
+#include "plg_tile.h"
 
 #include <algorithm>
 #include <cmath>
@@ -56,6 +58,14 @@ template <> struct SepVec<float, 2> {
 template <> struct SepVec<double, 1> {
   typedef double v __attribute__((ext_vector_type(2)));
 };
+
+// v += the value of the lanes whose index differs in the low log2(N) bits
+template <int N> __device__ __forceinline__ void sep_rowsum(double &v) {
+  if (N > 1) v += dpp<0xB1>(v);   // quad_perm [1,0,3,2]
+  if (N > 2) v += dpp<0x4E>(v);   // quad_perm [2,3,0,1]
+  if (N > 4) v += dpp<0x141>(v);  // row_half_mirror
+  if (N > 8) v += dpp<0x140>(v);  // row_mirror
+}
 
 // small tables: three waves per SIMD
 template <typename T, int NPART, int MAXA, int MAXB>
@@ -153,15 +163,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MAXA <= 5 ?
         C[a] = cmake<double>(pp.re * c0.re - pp.im * s0.re, pp.re * c0.im - pp.im * s0.im);
         S[a] = cmake<double>(pp.im * c0.re + pp.re * s0.re, pp.im * c0.im + pp.re * s0.im);
       }
+      // sums over the line's NPART lanes (the low lane bits): quad
+      // exchanges, then the other quad, then the other half row
 #pragma unroll
-      for (int off = 1; off < NPART; off <<= 1)
-#pragma unroll
-        for (int a = 0; a < MAXA; a++) {
-          C[a].re += __shfl_xor(C[a].re, off);
-          C[a].im += __shfl_xor(C[a].im, off);
-          S[a].re += __shfl_xor(S[a].re, off);
-          S[a].im += __shfl_xor(S[a].im, off);
-        }
+      for (int a = 0; a < MAXA; a++) {
+        sep_rowsum<NPART>(C[a].re);
+        sep_rowsum<NPART>(C[a].im);
+        sep_rowsum<NPART>(S[a].re);
+        sep_rowsum<NPART>(S[a].im);
+      }
     }
     __syncthreads();  // every thread has read its staged sites
     // G[n_x] of the line: a = n_x gives C - i S, a = -n_x (mod Lx) gives C + i S
