@@ -104,7 +104,8 @@ __device__ __forceinline__ void baryon_group(const cx<T> (*P)[BT_SP], const cx<T
       D[k.value] = d;
     });
   }
-  // ---- step 2: lane (nu', mu') -> W = Abar-phase * sum_nu A-phase(nu) T(nu)
+  // ---- step 2: lane (nu', mu') -> W = sum_nu A-phase(nu) T(nu); the
+  // Abar-phase of row nu' is applied by the readers in step 3
   // The second block's row phases are A1.ph = A0.ph + RA or + RA + 2 (mod 4),
   // so U_0 and U_1 (the A0-phased T(nu) of the two kinds of rows) are
   // accumulated by FMA and W_0 = U_0 + U_1, W_1 = i^RA (U_0 - U_1).
@@ -135,14 +136,12 @@ __device__ __forceinline__ void baryon_group(const cx<T> (*P)[BT_SP], const cx<T
         W[1][i] = cmul_iph(u0 - u1, RA);
       }
     }
-    static_for<0, NB>([&](auto k) {
-      constexpr SPerm Ab = k.value == 0 ? Ab0 : Ab1;
-      int ph = 0;
+    // shared without this lane's Abar phase: the reader knows the row
+    // (nu' = its loop index) and folds the phase into its FMAs
 #pragma unroll
-      for (int i = 0; i < 4; i++) ph += (hi == i ? Ab.ph[i ^ c] : 0);
+    for (int b = 0; b < NB; b++)
 #pragma unroll
-      for (int i = 0; i < 9; i++) Wsh[k.value][lane][i] = cmul_iph_rt(W[k.value][i], ph);
-    });
+      for (int i = 0; i < 9; i++) Wsh[b][lane][i] = W[b][i];
   }
   wave_lds_sync();
   // ---- step 3: lane (mu, mu') -> X = sum_{nu'} P[mu][nu'] . W[nu'][mu']
@@ -150,14 +149,17 @@ __device__ __forceinline__ void baryon_group(const cx<T> (*P)[BT_SP], const cx<T
     cx<T> x[NB];
 #pragma unroll
     for (int b = 0; b < NB; b++) x[b] = cmake<T>(0, 0);
-#pragma unroll
-    for (int nup = 0; nup < 4; nup++)
+    static_for<0, 4>([&](auto nu) {
+      constexpr int nup = nu.value;
+      // the Abar phase of the writing lane (row nu')
+      constexpr int ph0 = Ab0.ph[nup ^ c], ph1 = Ab1.ph[nup ^ c];
 #pragma unroll
       for (int i = 0; i < 9; i++) {
         const cx<T> p = P[pidx(hi, nup, 0, 0) + i][s];
-        cfma(x[0], p, Wsh[0][nup * 4 + lo][i]);
-        if constexpr (NB == 2) cfma(x[1], p, Wsh[1][nup * 4 + lo][i]);
+        cfma_iph<ph0>(x[0], p, Wsh[0][nup * 4 + lo][i]);
+        if constexpr (NB == 2) cfma_iph<ph1>(x[1], p, Wsh[1][nup * 4 + lo][i]);
       }
+    });
 #pragma unroll
     for (int b = 0; b < NB; b++) X[b] = x[b];
   }
