@@ -144,3 +144,41 @@ def test_separable_custom_momentum_list(monkeypatch):
         assert err < 1e-12
     finally:
         capi.finalize()
+
+
+def test_static_entry_falls_back_to_the_gemm():
+    """plg_correlator_static on a plan whose separable sums would not fit the
+    static scratch (64 (n_x, n_y) pairs x 32 z planes per column) but whose
+    GEMM partials do: it runs with the GEMM and agrees with plg_correlator
+    (separable) to the mixed path's tolerance"""
+    torch = G.torch_cuda()
+    L, src, totalT = [8, 8, 32, 64], [3, 5, 17, 9], 48
+    moms = [[nx, ny, 0] for nx in range(-3, 5) for ny in range(-3, 5)]
+    cols = totalT * 80                                   # rows x channels of the two-point kind
+    V3 = L[0] * L[1] * L[2]
+    sep_bytes = (len(moms) + L[2] * 64) * cols * 16      # sums per momentum + per-plane sums
+    c_bytes = cols * V3 * 8                              # FP32 site correlator
+    assert sep_bytes + c_bytes > capi.STATIC_SCRATCH_BYTES
+    assert c_bytes + 16 * cols * 32 * 32 < capi.STATIC_SCRATCH_BYTES  # GEMM partials, up to 16 K splits
+    capi.init(L, 0)
+    try:
+        t, v = G.make_prop(L, 4242, 9, src[3], 0.5)
+        _, v32 = keep = G.cast_f32(t, v, scaled=True)
+        n = capi.correlator_size(capi.TWOPT, MASK, src, totalT, len(moms))
+        ref = torch.full((n, 2), float("nan"), dtype=torch.float64, device="cuda")
+        out = torch.full((n, 2), float("nan"), dtype=torch.float64, device="cuda")
+        print("[gpu] plg_correlator (separable) and plg_correlator_static (GEMM)", flush=True)
+        capi.correlator(capi.TWOPT, MASK, v32, v32, src, totalT, moms, capi.MODE_MIXED, capi.MOMENTUM_SPACE,
+                        capi.F64, ref)
+        capi.correlator_static(capi.TWOPT, MASK, v32, v32, src, totalT, moms, capi.MODE_MIXED,
+                               capi.MOMENTUM_SPACE, capi.F64, out)
+        ng, nd, nch = capi.kind_shape(capi.TWOPT, MASK)
+        h, r = G.host(out).reshape(n, 2), G.host(ref).reshape(n, 2)
+        got = (h[:, 0] + 1j * h[:, 1]).reshape(ng, nd, totalT, len(moms), nch)
+        exp = (r[:, 0] + 1j * r[:, 1]).reshape(ng, nd, totalT, len(moms), nch)
+        err = G.rel_err(got, exp)
+        print(f"[gpu] static (GEMM) vs separable {err:.2e}", flush=True)
+        assert np.isfinite(got).all() and err < 1e-6
+        del keep
+    finally:
+        capi.finalize()
