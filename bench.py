@@ -260,6 +260,8 @@ def roof(kernel, t_s, nsites, flops_site, bytes_site, bound_compute, cpeak, work
     tr, kname, note = measured_traffic(workload, kernel)
     if ai > ridge:
         r = {"kernel": kernel, "bound": bound_compute, **comp, "hbm": hbm}
+    elif bound_compute == "hbm":  # a kernel with no arithmetic to price
+        r = {"kernel": kernel, "bound": "hbm", **hbm}
     else:
         r = {"kernel": kernel, "bound": "hbm", **hbm, bound_compute: comp}
     r.update({"traffic": tr, "traffic_source": note,
@@ -1085,6 +1087,70 @@ def learned_flow(torch, capi, dev, stream, Lat=L, nsrc=256, split=(32, 32), step
     return res, {"X": X, "Y": Y, "est": est}
 
 
+def run_ft(args, torch, capi, dev, stream):
+    """the standalone projection of a field (PLEGMA_FT::apply,
+    lib/PLEGMA_FT.cu): a 4D field of 16 complex components on 32^3 x 64,
+    all momenta with |n|^2 <= 16, FP32 and FP64 fields, the separable sums
+    (the default) and the phase-matrix GEMM (PLG_PROJ_SEP=0)"""
+    capi.init(L, 0)
+    V3, T, dof = L[0] * L[1] * L[2], L[3], 16
+    moms = capi.momlist(Q2MAX)
+    g = torch.Generator(device=dev)
+    g.manual_seed(SEED)
+    f64 = torch.randn((dof, T, V3, 2), dtype=torch.float64, device=dev, generator=g)
+    f32 = f64.to(torch.float32)
+    out = {k: torch.zeros((T, dof, len(moms), 2), dtype=torch.float64, device=dev) for k in ("32", "64")}
+    ms = {}
+    for sep in ("1", "0"):
+        os.environ["PLG_PROJ_SEP"] = sep
+        try:
+            for prec, fld in (("32", f32), ("64", f64)):
+                o = out[prec] if sep == "1" else torch.zeros_like(out[prec])
+                fn = lambda: capi.ft_apply(fld, capi.F32 if prec == "32" else capi.F64, dof, T, moms, -1, o,
+                                           stream=stream)
+                for _ in range(args.warmup):
+                    fn()
+                ms[(sep, prec)], _ = timed(torch, stream, fn, args.steps)
+                log(f"ft {'separable' if sep == '1' else 'GEMM'} FP{prec}: {ms[(sep, prec)]:.3f} ms")
+                if sep == "0":
+                    ref = out[prec]
+                    ms[("diff", prec)] = ((o - ref).abs().max() / ref.abs().max()).item()
+        finally:
+            del os.environ["PLG_PROJ_SEP"]
+    a, b = out["32"].cpu().numpy(), out["64"].cpu().numpy()
+    A, B = a[..., 0] + 1j * a[..., 1], b[..., 0] + 1j * b[..., 1]
+    nsites = V3 * T
+    t32 = ms[("1", "32")] / 1e3
+    res = {"metric": "standalone FT sites/s (16 components, 257 momenta), FP32 field", "value": nsites / t32,
+           "unit": "sites/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": ms[("1", "32")], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 field, fp64 separable sums", "data": "synthetic",
+           "config": {"workload": "ft: lattice 32^3x64, a 4D field of 16 complex components (torch.randn), "
+                                  "|n|^2<=16 (257 momenta), sign -1",
+                      "lattice": L, "components": dof, "nmom": len(moms), "parallelism": "single GPU"},
+           "fp64_field": {"ms_per_step": ms[("1", "64")], "value": nsites / (ms[("1", "64")] / 1e3)},
+           "gemm": {"fp32_ms": ms[("0", "32")], "fp64_ms": ms[("0", "64")],
+                    "max_diff_vs_separable_rel": {"fp32": ms[("diff", "32")], "fp64": ms[("diff", "64")]}},
+           "rel_err_fp32_vs_fp64": float(np.abs(A - B).max() / np.abs(B).max()),
+           "roofline": roof("ft_apply_f32", t32, nsites, 0, dof * 8, "hbm", 1.0, "ft",
+                            f"{dof * 8} B FP32 field read per site; whole call (the sums over x and y, "
+                            "over z, the output layout, the stream synchronisation)")}
+    if not args.no_cpu:
+        # the oracle's per-site DFT of one time slice, all components
+        import oracle.oracle as O
+        nth = len(os.sched_getaffinity(0))
+        t0, n = time.perf_counter(), 0
+        while n < T and (n == 0 or time.perf_counter() - t0 < 10.0):
+            h = f64[:, n].cpu().numpy()
+            O.dft(L, [0, 0, 0], Q2MAX, np.ascontiguousarray((h[..., 0] + 1j * h[..., 1]).T), nth)
+            n += 1
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": n * V3 / dt, "unit": "sites/s", "cores": nth, "kind": "port",
+                               "sample": f"{n} time slices of {V3} sites x {dof} components, {len(moms)} momenta, "
+                                         f"FP64 per-site DFT (oracle), {dt:.1f} s"}
+    return res
+
+
 def run_learned(args, torch, capi, dev, stream):
     """32^3 x 64, 256 sources (config of the learned part); see learned_flow"""
     log("learned flow: exact FP64 and cheap mixed two-point results of 256 sources")
@@ -1110,7 +1176,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--split", type=int, default=6)
-    ap.add_argument("--workload", choices=["batching", "largest", "learned", "udsc", "tetra"], default="batching")
+    ap.add_argument("--workload", choices=["batching", "largest", "learned", "udsc", "tetra", "ft"],
+                    default="batching")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
@@ -1136,7 +1203,8 @@ def main():
         torch.cuda.synchronize()
         del a, b
     stream = torch.cuda.Stream()
-    run = {"largest": run_largest, "learned": run_learned, "udsc": run_udsc, "tetra": run_tetra}.get(args.workload, run_batching)
+    run = {"largest": run_largest, "learned": run_learned, "udsc": run_udsc, "tetra": run_tetra,
+           "ft": run_ft}.get(args.workload, run_batching)
     res = run(args, torch, capi, dev, stream)
     capi.finalize()
     print(json.dumps(res), flush=True)

@@ -901,6 +901,26 @@ static int autotune(Plan &P, hipStream_t st) {
 static std::list<std::pair<std::string, std::unique_ptr<Plan>>> g_single;
 constexpr size_t SINGLE_PLANS = 32;
 
+// plg_ft_apply keeps the device tables of its last momentum list (momenta,
+// pairs, separable tables, phase index), keyed by the list, the field's
+// shape and precision and the lattice: a caller that projects field after
+// field with one list uploads them once.  Used under g_ws_mu only; every call
+// synchronises its stream before it returns, so replacing them is safe.
+struct FtTables {
+  std::string key;
+  int *d_ms = nullptr, *d_pair_ms = nullptr, *d_mom_pair = nullptr;
+  uint16_t *d_pidx = nullptr;
+  unsigned char *d_sep = nullptr;
+  SepHost sh;
+  int npairs = 0, zero_pair = 0;
+  void release() {
+    for (void *p : {(void *)d_ms, (void *)d_pair_ms, (void *)d_mom_pair, (void *)d_pidx, (void *)d_sep})
+      if (p) hipFree(p);
+    *this = FtTables();
+  }
+};
+static FtTables g_ft;
+
 static std::string single_key(int kind, int mask, const std::vector<plg_prop> &props, const int src[4], int totalT,
                               const int *moms, int nmom, int mode, int space) {
   std::ostringstream k;
@@ -1146,6 +1166,7 @@ static void finalize_locked() {
   if (!g_ctx.init) return;
   hipDeviceSynchronize();
   g_single.clear();
+  g_ft.release();
   for (void *p : {(void *)g_ctx.d_cos, (void *)g_ctx.d_sin, (void *)g_ctx.d_wtab, g_ctx.ws_c, g_ctx.ws_part})
     if (p) hipFree(p);
   for (void *p : g_ctx.udsc_mem) hipFree(p);
@@ -1931,9 +1952,17 @@ int plg_ft_apply(const void *field, int precision, int dof, int nt, const int *m
   hipStream_t st = (hipStream_t)stream;
   // exp(sign i 2pi n.x/L): the GEMM applies exp(-i ...), so sign = +1 negates the momenta
   const std::vector<int> ms = scaled_moms(moms, nmom, -sign);
-  std::vector<int> pair_ms, mom_pair;
-  int zero_pair = 0;
-  const int npairs = make_pairs(ms, g_ctx.M, pair_ms, mom_pair, &zero_pair);
+  const char *sep_env = std::getenv("PLG_PROJ_SEP");
+  // the separable sums read the field with 16-byte loads
+  const bool sep_off = (sep_env && std::string(sep_env) == "0") || ((uintptr_t)field & 15);
+  std::ostringstream key;
+  key << precision << ' ' << dof << ' ' << nt << ' ' << sep_off;
+  for (int i = 0; i < 4; i++) key << ' ' << g_ctx.L[i];
+  for (int v : ms) key << ' ' << v;
+  // the tables and the shared workspace are this call's until the stream is
+  // synchronised below
+  std::unique_lock<std::mutex> wl(g_ws_mu);
+  FtTables &F = g_ft;
   ProjGeom g;
   g.V3 = g_ctx.V3;
   for (int i = 0; i < 3; i++) g.L[i] = g_ctx.L[i];
@@ -1942,6 +1971,27 @@ int plg_ft_apply(const void *field, int precision, int dof, int nt, const int *m
   g.ncols = dof * nt;
   g.cos_tab = g_ctx.d_cos;
   g.sin_tab = g_ctx.d_sin;
+  int rc = PLG_OK;
+  if (F.key != key.str()) {
+    F.release();
+    std::vector<int> pair_ms, mom_pair;
+    F.npairs = make_pairs(ms, g_ctx.M, pair_ms, mom_pair, &F.zero_pair);
+    // the separable sums where the momenta suit them (PLG_PROJ_SEP=0: the GEMM)
+    if (!sep_off) sep_build(g_ctx.L, g_ctx.M, ms, (int64_t)dof * nt, precision == PLG_F64, F.sh);
+    rc = upload(&F.d_ms, ms, st);
+    if (!rc) rc = upload(&F.d_pair_ms, pair_ms, st);
+    if (!rc) rc = upload(&F.d_mom_pair, mom_pair, st);
+    if (!rc && F.sh.ok) rc = upload(&F.d_sep, F.sh.blob, st);
+    F.key = key.str();
+  }
+  const int npairs = F.npairs, zero_pair = F.zero_pair;
+  const SepHost &sh = F.sh;
+  g.mom_scaled = F.d_ms;
+  g.npairs = npairs;
+  g.pair_ms = F.d_pair_ms;
+  g.mom_pair = F.d_mom_pair;
+  g.zero_pair = zero_pair;
+  g.ctile = 0;
   int code;
   int64_t tiles;
   if (precision == PLG_F64) {
@@ -1953,31 +2003,37 @@ int plg_ft_apply(const void *field, int precision, int dof, int nt, const int *m
     tiles = proj_workgroups(code, g.ncols, std::max(npairs - zero_pair, 1));
   }
   g.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>((1024 + tiles - 1) / tiles, g_ctx.V3 / 512));
-  int *d_ms = nullptr, *d_pair_ms = nullptr, *d_mom_pair = nullptr;
-  uint16_t *d_pidx = nullptr;
-  int rc = upload(&d_ms, ms, st);
-  if (!rc) rc = upload(&d_pair_ms, pair_ms, st);
-  if (!rc) rc = upload(&d_mom_pair, mom_pair, st);
-  g.mom_scaled = d_ms;
-  g.npairs = npairs;
-  g.pair_ms = d_pair_ms;
-  g.mom_pair = d_mom_pair;
-  g.zero_pair = zero_pair;
-  g.ctile = 0;
-  if (!rc && npairs > 0 && precision == PLG_F32) {
-    if (hipMalloc((void **)&d_pidx, phase_index_bytes(g_ctx.V3, npairs)) != hipSuccess)
-      rc = fail(PLG_ERR_HIP, "phase index allocation failed");
-    else if ((rc = launch_phase_index(g, d_pidx, st)))
-      rc = fail(rc, "phase index launch failed");
-    g.pidx = d_pidx;
+  if (!rc && !sh.ok && npairs > 0 && precision == PLG_F32) {
+    if (!F.d_pidx) {
+      if (hipMalloc((void **)&F.d_pidx, phase_index_bytes(g_ctx.V3, npairs)) != hipSuccess)
+        rc = fail(PLG_ERR_HIP, "phase index allocation failed");
+      else if ((rc = launch_phase_index(g, F.d_pidx, st)))
+        rc = fail(rc, "phase index launch failed");
+    }
+    g.pidx = F.d_pidx;
   }
-  // the shared workspace is this call's until the stream is synchronised below
-  std::unique_lock<std::mutex> wl(g_ws_mu);
   if (!rc) {
     std::lock_guard<std::mutex> lk(g_mu);
-    rc = ensure(&g_ctx.ws_part, &g_ctx.ws_part_bytes, proj_part_bytes(g.nsplit, g.ncols, nmom, npairs));
+    rc = ensure(&g_ctx.ws_part, &g_ctx.ws_part_bytes,
+                sh.ok ? sep_part_bytes(sh.g) : proj_part_bytes(g.nsplit, g.ncols, nmom, npairs));
   }
-  if (!rc && precision == PLG_F32) {
+  if (!rc && sh.ok) {
+    // the separable sums read the field where it lies: 16-site groups of a
+    // column are contiguous in its [column][site] layout too
+    SepGeom sg = sep_geom(sh, F.d_sep);
+    sg.gstride = 16;
+    sg.cstride = g_ctx.V3;
+    rc = launch_project_sep(sg, field, precision, (double *)g_ctx.ws_part, st);
+    if (!rc) {
+      ProjGeom gs = g;
+      gs.sep = 1;
+      gs.npairs = 0;
+      gs.nsplit = 1;
+      gs.ctile = 1;
+      rc = launch_ft_out(gs, (const double *)g_ctx.ws_part, dof, nt, accumulate, out, out_precision, st);
+    }
+    if (rc) rc = fail(rc, "FT launch failed");
+  } else if (!rc && precision == PLG_F32) {
     // the mixed projection reads the tile-major layout of the site correlator
     std::lock_guard<std::mutex> lk(g_mu);
     rc = ensure(&g_ctx.ws_c, &g_ctx.ws_c_bytes, (size_t)g.ncols * ctile_sites(g_ctx.V3) * sizeof(cx<float>));
@@ -1985,7 +2041,7 @@ int plg_ft_apply(const void *field, int precision, int dof, int nt, const int *m
     g.ctile = 1;
     g.shifted = 1;
   }
-  if (!rc) {
+  if (!rc && !sh.ok) {
     if (precision == PLG_F64)
       rc = launch_project_f64(g, (const cx<double> *)field, (cx<double> *)g_ctx.ws_part, code, st);
     else
@@ -1994,11 +2050,10 @@ int plg_ft_apply(const void *field, int precision, int dof, int nt, const int *m
     if (!rc) rc = launch_ft_out(g, (const double *)g_ctx.ws_part, dof, nt, accumulate, out, out_precision, st);
     if (rc) rc = fail(rc, "FT launch failed");
   }
-  // the tables are freed below: wait for every launch, also after an error
+  // the workspace is shared: wait for every launch, also after an error
   const hipError_t se = hipStreamSynchronize(st);
   if (!rc && se != hipSuccess) rc = fail(PLG_ERR_HIP, "FT synchronisation failed");
-  for (void *p : {(void *)d_ms, (void *)d_pair_ms, (void *)d_mom_pair, (void *)d_pidx})
-    if (p) hipFree(p);
+  if (rc) F.release();  // a table set that failed to build is not kept
   return rc;
 }
 

@@ -218,6 +218,7 @@ size_t phase_index_bytes(int64_t V3, int npairs);
 int launch_phase_index(const ProjGeom &g, uint16_t *tab, hipStream_t st);
 // column-major FP32 field (column n at n * V3) -> tile-major layout (ctile_idx)
 int launch_to_tiles(const cx<float> *f, int64_t ncols, int64_t V3, cx<float> *t, hipStream_t st);
+// the same for an FP64 field (the separable projection of plg_ft_apply)
 // (npairs here: the pairs of the GEMM, without a zero-momentum pair)
 int64_t proj_workgroups(int code, int64_t ncols, int npairs);
 int pick_proj(int npairs);
@@ -291,6 +292,10 @@ constexpr int PROJ_SEP = 1;  // projection code of the separable form (both prec
 struct SepGeom {
   int Lx, Ly, Lz;
   int64_t ncols;
+  // element strides of the input: of a group of 16 sites and of a column.
+  // The tile-major site correlator: 16 * ncols and 16; a field in its own
+  // [column][site] layout: 16 and V3
+  int64_t gstride, cstride;
   int nmom;
   int NA;     // distinct |n_x| classes a (cos and sin sums over x)
   int NXR;    // distinct n_x residues

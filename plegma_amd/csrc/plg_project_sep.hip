@@ -119,8 +119,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MAXA <= 5 ?
     for (int k = 0; k < KU; k++) {
       const int u = tid + 256 * k, tt = u / (16 * Q::UPC), j = u % Q::UPC;
       if (u < NU && col_ok)
-        v[k] = *(const __attribute__((address_space(1))) vec *)(c + ((tile0 + tt) * s.ncols + col0 + sc) * 16 +
-                                                               j * Q::EPU);
+        v[k] = *(const __attribute__((address_space(1))) vec *)(c + (tile0 + tt) * s.gstride +
+                                                               (col0 + sc) * s.cstride + j * Q::EPU);
       else
         v[k] = vec{};
     }
@@ -385,6 +385,8 @@ void sep_build(const int L[3], int M, const std::vector<int> &ms, int64_t ncols,
 
 SepGeom sep_geom(const SepHost &h, const void *d_blob) {
   SepGeom g = h.g;
+  g.gstride = 16 * g.ncols;  // the tile-major site correlator
+  g.cstride = 16;
   const unsigned char *b = (const unsigned char *)d_blob;
   g.ph1 = (const cx<double> *)(b + h.off_ph1);
   g.ph1p = (const cx<double> *)(b + h.off_ph1p);

@@ -374,8 +374,10 @@ def test_projection_tiles(props, code, monkeypatch):
 # ---- standalone projection of a field (PLEGMA_FT::apply)
 @pytest.mark.parametrize("prec,tol", [(capi.F64, 1e-12), (capi.F32, 1e-6)])
 @pytest.mark.parametrize("sign", [-1, 1])
-def test_ft_apply(prec, tol, sign):
+@pytest.mark.parametrize("sep", ["1", "0"])  # separable sums / phase-matrix GEMM
+def test_ft_apply(prec, tol, sign, sep, monkeypatch):
     import closed_forms as cf
+    monkeypatch.setenv("PLG_PROJ_SEP", sep)
     torch = G.torch_cuda()
     rng = np.random.default_rng(17)
     dof, T, V3 = 3, L1[3], L1[0] * L1[1] * L1[2]
@@ -403,8 +405,10 @@ IRREGULAR = [(0, 0, 0), (1, 0, 0), (1, 0, 0), (-1, 0, 0), (4, 0, 0), (2, 1, -3),
 
 
 @pytest.mark.parametrize("prec,tol", [(capi.F64, 1e-12), (capi.F32, 1e-6)])
-def test_ft_apply_irregular_momenta(prec, tol):
+@pytest.mark.parametrize("sep", ["1", "0"])  # separable sums / phase-matrix GEMM
+def test_ft_apply_irregular_momenta(prec, tol, sep, monkeypatch):
     import closed_forms as cf
+    monkeypatch.setenv("PLG_PROJ_SEP", sep)
     torch = G.torch_cuda()
     rng = np.random.default_rng(29)
     dof, V3 = 2, L1[0] * L1[1] * L1[2]
@@ -421,6 +425,24 @@ def test_ft_apply_irregular_momenta(prec, tol):
     # duplicates and aliases give the same numbers
     for a, b in ((1, 2), (3, 7)):
         assert np.abs(got[:, a] - got[:, b]).max() <= 1e-13 * np.abs(got[:, a]).max()
+
+
+def test_ft_apply_table_cache():
+    """the FT keeps the tables of its last momentum list: calls that switch
+    between lists, fields and field shapes each match the exact-phase DFT"""
+    import closed_forms as cf
+    torch = G.torch_cuda()
+    rng = np.random.default_rng(31)
+    V3 = L1[0] * L1[1] * L1[2]
+    lists = [capi.momlist(Q1), IRREGULAR, capi.momlist(Q1)]
+    for i, (moms, dof) in enumerate(zip(lists + lists[:1], (3, 2, 3, 3))):  # the last hits the cache
+        f = rng.normal(size=(dof, 1, V3)) + 1j * rng.normal(size=(dof, 1, V3))
+        fd = torch.tensor(np.stack([f.real, f.imag], -1), dtype=torch.float64, device="cuda")
+        out = torch.full((1, dof, len(moms), 2), float("nan"), dtype=torch.float64, device="cuda")
+        capi.ft_apply(fd, capi.F64, dof, 1, moms, -1, out)
+        h = G.host(out).reshape(dof, len(moms), 2)
+        ref = cf.dft(f[:, 0, :].T, L1, [0, 0, 0], np.array(moms))
+        assert G.rel_err((h[..., 0] + 1j * h[..., 1]).T, ref, axis=0) < 1e-12, i
 
 
 def test_mixed_correlator_irregular_momenta(props):
