@@ -262,9 +262,15 @@ int plg_baryons_udsc_profile(const plg_prop *up, const plg_prop *dn, const plg_p
  * with x the lattice coordinates (no source shift: PLEGMA_FT's
  * mulConstMomentumPhases applies that).  field: device, PLEGMA_Field layout,
  * dof components of nt*V3 complex (nt = T for a 4D field, 1 for a 3D field);
- * precision PLG_F64 (FP64 MFMA) or PLG_F32 (split-bf16 MFMA, FP64 sums).
+ * precision PLG_F64 or PLG_F32.  Where the momenta suit them (and the field
+ * is 16-byte aligned) the separable sums over x, y and z run in FP64 on the
+ * field in place; else the phase-matrix GEMM (FP64 MFMA, or split-bf16 MFMA
+ * with FP64 sums for an FP32 field); PLG_PROJ_SEP=0 pins the GEMM.
  * moms: host nmom*3 ints.  out: device [nt][dof][nmom] complex of
- * out_precision, overwritten, or added to when accumulate != 0 (IsAccum). */
+ * out_precision, overwritten, or added to when accumulate != 0 (IsAccum).
+ * The call synchronises the stream; the device tables of its momentum list
+ * are kept until a call with another list, field shape or precision, or
+ * plg_finalize. */
 int plg_ft_apply(const void *field, int precision, int dof, int nt, const int *moms, int nmom,
                  int sign, int accumulate, void *out, int out_precision, void *stream);
 
