@@ -7,8 +7,11 @@ Cases cover the chunk geometries the host picks: equal and unequal extents
 (lines split in 2 or 4 parts, a z extent of 6), momenta whose components
 reach L/2 (the residue that is its own negative) and alias (n and n + L), and
 the large tables (|n_i| up to 6: seven |n_x| classes, 13 n_y residues).
-Tolerances: FP64 1e-12 normwise over momenta; mixed (FP32 contraction)
-1e-6.
+The standalone FT (plg_ft_apply) runs the same sums on a field read in its
+own layout: pinned by the exact-phase DFT of the closed forms on the same
+lattices, with partial column groups, IsAccum and an FP32 output.
+Tolerances: FP64 1e-12 normwise over momenta; mixed (FP32 contraction) and
+FP32 fields 1e-6.
 """
 import numpy as np
 import pytest
@@ -180,5 +183,55 @@ def test_static_entry_falls_back_to_the_gemm():
         print(f"[gpu] static (GEMM) vs separable {err:.2e}", flush=True)
         assert np.isfinite(got).all() and err < 1e-6
         del keep
+    finally:
+        capi.finalize()
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_separable_ft_on_the_field_in_place(case, monkeypatch):
+    """plg_ft_apply's separable sums read the field in its own [column][site]
+    layout (no copy into tiles): FP64 and FP32 4D fields and a 3D field, on
+    the lattices above (unequal extents, components reaching L/2, aliases),
+    against the exact-phase DFT of the closed forms and against the GEMM"""
+    import closed_forms as cf
+    torch = G.torch_cuda()
+    L, q2, _, _ = CASES[case]
+    V3, T = L[0] * L[1] * L[2], L[3]
+    capi.init(L, 0)
+    try:
+        moms = capi.momlist(q2)
+        rng = np.random.default_rng(90 + case)
+        for prec, tol, nt, dof in ((capi.F64, 1e-12, T, 3), (capi.F32, 1e-6, T, 5), (capi.F64, 1e-12, 1, 17)):
+            f = rng.normal(size=(dof, nt, V3)) + 1j * rng.normal(size=(dof, nt, V3))
+            dt = torch.float64 if prec == capi.F64 else torch.float32
+            fd = torch.tensor(np.stack([f.real, f.imag], -1), dtype=dt, device="cuda")
+            fq = f if prec == capi.F64 else f.astype(np.complex64).astype(np.complex128)
+            got = {}
+            for sep in ("1", "0"):
+                monkeypatch.setenv("PLG_PROJ_SEP", sep)
+                out = torch.full((nt, dof, len(moms), 2), float("nan"), dtype=torch.float64, device="cuda")
+                print(f"[gpu] L={L} prec={prec} nt={nt} dof={dof} PLG_PROJ_SEP={sep}", flush=True)
+                capi.ft_apply(fd, prec, dof, nt, moms, 1, out)
+                h = G.host(out).reshape(nt, dof, len(moms), 2)
+                got[sep] = h[..., 0] + 1j * h[..., 1]
+            mm = -np.array(moms)                                  # exp(+i n.x) = exp(-i (-n).x)
+            for t in sorted({0, nt // 2, nt - 1}):
+                ref = cf.dft(fq[:, t, :].T, L, [0, 0, 0], mm)     # [mom][dof]
+                err = G.rel_err(got["1"][t].T, ref, axis=0)
+                print(f"[gpu]   t={t} separable vs DFT {err:.2e}", flush=True)
+                assert err < tol
+            assert G.rel_err(got["1"], got["0"], axis=2) < tol
+            if prec == capi.F64 and nt == T:
+                # IsAccum adds to the output; an FP32 output rounds the FP64 sums
+                monkeypatch.setenv("PLG_PROJ_SEP", "1")
+                g1 = got["1"]
+                acc = torch.tensor(np.stack([g1.real, g1.imag], -1), dtype=torch.float64, device="cuda")
+                capi.ft_apply(fd, prec, dof, nt, moms, 1, acc, accumulate=True)
+                h = G.host(acc).reshape(nt, dof, len(moms), 2)
+                assert G.rel_err(h[..., 0] + 1j * h[..., 1], 2 * g1, axis=2) < 1e-14
+                o32 = torch.full((nt, dof, len(moms), 2), float("nan"), dtype=torch.float32, device="cuda")
+                capi.ft_apply(fd, prec, dof, nt, moms, 1, o32, out_prec=capi.F32)
+                h = G.host(o32).reshape(nt, dof, len(moms), 2).astype(np.float64)
+                assert G.rel_err(h[..., 0] + 1j * h[..., 1], g1, axis=2) < 1e-6
     finally:
         capi.finalize()
