@@ -1095,6 +1095,7 @@ def run_ft(args, torch, capi, dev, stream):
     capi.init(L, 0)
     V3, T, dof = L[0] * L[1] * L[2], L[3], 16
     moms = capi.momlist(Q2MAX)
+    moms_i32 = np.ascontiguousarray(moms, dtype=np.int32)     # built once, passed as is
     g = torch.Generator(device=dev)
     g.manual_seed(SEED)
     f64 = torch.randn((dof, T, V3, 2), dtype=torch.float64, device=dev, generator=g)
@@ -1106,7 +1107,7 @@ def run_ft(args, torch, capi, dev, stream):
         try:
             for prec, fld in (("32", f32), ("64", f64)):
                 o = out[prec] if sep == "1" else torch.zeros_like(out[prec])
-                fn = lambda: capi.ft_apply(fld, capi.F32 if prec == "32" else capi.F64, dof, T, moms, -1, o,
+                fn = lambda: capi.ft_apply(fld, capi.F32 if prec == "32" else capi.F64, dof, T, moms_i32, -1, o,
                                            stream=stream)
                 for _ in range(args.warmup):
                     fn()

@@ -222,8 +222,17 @@ STATIC_SCRATCH_BYTES = 128 << 20
 
 
 def ft_apply(field, precision, dof, nt, moms, sign, out, out_prec=F64, accumulate=False, stream=None):
-    flat = [int(v) for m in moms for v in m]
-    check(lib().plg_ft_apply(C.c_void_p(field.data_ptr()), precision, dof, nt, ivec(flat), len(moms), sign,
+    """moms: triples, or a C-contiguous int32 numpy array of shape (nmom, 3),
+    which is passed without conversion (a caller applying one list to many
+    fields builds it once)"""
+    import numpy as np
+    if isinstance(moms, np.ndarray) and moms.dtype == np.int32 and moms.ndim == 2 and moms.shape[1] == 3 \
+            and moms.flags.c_contiguous:
+        mp, n = moms.ctypes.data_as(C.POINTER(C.c_int)), moms.shape[0]
+    else:
+        flat = [int(v) for m in moms for v in m]
+        mp, n = ivec(flat), len(flat) // 3
+    check(lib().plg_ft_apply(C.c_void_p(field.data_ptr()), precision, dof, nt, mp, n, sign,
                              int(accumulate), C.c_void_p(out.data_ptr()), out_prec, _stream(stream)))
 
 

@@ -211,7 +211,9 @@ def test_separable_ft_on_the_field_in_place(case, monkeypatch):
                 monkeypatch.setenv("PLG_PROJ_SEP", sep)
                 out = torch.full((nt, dof, len(moms), 2), float("nan"), dtype=torch.float64, device="cuda")
                 print(f"[gpu] L={L} prec={prec} nt={nt} dof={dof} PLG_PROJ_SEP={sep}", flush=True)
-                capi.ft_apply(fd, prec, dof, nt, moms, 1, out)
+                # the separable run passes the list as an int32 array (no conversion)
+                ml = np.array(moms, dtype=np.int32) if sep == "1" else moms
+                capi.ft_apply(fd, prec, dof, nt, ml, 1, out)
                 h = G.host(out).reshape(nt, dof, len(moms), 2)
                 got[sep] = h[..., 0] + 1j * h[..., 1]
             mm = -np.array(moms)                                  # exp(+i n.x) = exp(-i (-n).x)
